@@ -1,0 +1,206 @@
+// cgs_internal.hpp -- plan object and kernel launch interfaces of libcgsense_hip.
+//
+// Device data layouts (all complex64 as float2):
+//   maps      S[i*C + c]                 coil fastest ("coil-interleaved")
+//   grid      G[(f*ntot + l)*C + c]      l = l0 + n0*(l1 + n1*l2), oversampled k-space
+//   coil imgs X[(f*Ntot + i)*C + c]      per-coil images (E^H before the coil sum)
+//   samples   Y[s*C + c]                 s = tile-sorted sample position (all frames)
+//   images    x[f*Ntot + i]              i = i0 + N0*(i1 + N1*i2), codeare order
+// Samples are sorted by (frame, grid tile of the base cell); base[s*d+a] holds
+// (floor(n_a*x_a) - m) mod n_a and psi[(s*d+a)*taps+t] the window value of tap t.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/cgsense.h"
+
+namespace cgs {
+
+constexpr int kFftMaxElem = 8192;   // lines*n held in LDS by one FFT block
+constexpr int kFftThreads = 256;
+
+struct GridItem {       // one gridding work item: part of one tile's visit list
+  int frame, tile, vbeg, vend;
+  int slot;             // -1: write the tile to the grid; else partial slot
+  int pad0, pad1, pad2;
+};
+struct ReduceItem {     // sums the partial slots of one heavy tile
+  int frame, tile, slot0, nslots;
+};
+struct CgState {        // per-frame CG scalars, FP64
+  double rn, bn, xn, pq, alpha, beta, rel;
+  int it, done, nan, pad;
+};
+
+struct FftArgs {
+  const float2* in;
+  float2* out;
+  int n, nin, nout, lb, ldn, sign;
+  int64_t inner, outers, outer_per_frame;
+  const float2* tw;     // W_n^p = exp(-2 pi i p / n)
+  int nst, radix[8];
+  // fused operations of the first forward / last adjoint pass (axis 0)
+  const float2* x;      // image frames (load mode 1)
+  const float2* S;      // maps (load mode 1)
+  const float* dinv0;
+  const float* dinv1;
+  const float* dinv2;
+  int C, d, N0, N1, mask_nh;
+  int64_t Ntot;
+  const int* fdone;
+};
+
+struct SampArgs {
+  const int* base;
+  const float* psi;
+  const float* w;       // density weights (sorted), nullable
+  int d, taps, C;
+  int n0, n1, n2;
+  int64_t ntot;
+  const float2* grid;
+  float2* y;
+  const int4* items;    // interp: {frame, start, count, 0}
+  int nitems;
+  const int* fdone;
+};
+
+struct GridArgs {
+  const int* base;
+  const float* psi;
+  const float* w;
+  const float2* y;
+  float2* grid;
+  float2* partial;
+  const GridItem* items;
+  const int* visits;
+  const int* fdone;
+  int d, taps, C, T, CGW, rowstride;
+  int n0, n1, n2, nt0, nt1, nt2;
+  int64_t ntot, tilecells;
+};
+
+struct Plan {
+  // geometry
+  int d = 0, N[3] = {1, 1, 1}, n[3] = {1, 1, 1}, m = 0, taps = 0;
+  int C = 0, F = 0;
+  int64_t Mtot = 0, Ntot = 0, ntot = 0;
+  unsigned flags = 0;
+  double b[3] = {0, 0, 0};
+  std::vector<int64_t> foff;          // F+1 sample offsets
+  int T = 8, nt[3] = {1, 1, 1};       // tile side and tiles per axis
+  int64_t tiles_per_frame = 1;
+  int grid_cg = 8, grid_w = 4;        // gridding coil group / waves per block
+  int interp_cg = 32;
+  int visits_per_item = 256;
+
+  // stream / state
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  bool have_k = false, have_w = false, have_S = false;
+
+  // constant tables
+  float2* tw[3] = {nullptr, nullptr, nullptr};
+  int radix[3][8] = {{0}}, nst[3] = {0, 0, 0};
+  float* dinv[3] = {nullptr, nullptr, nullptr};
+  int64_t* d_foff = nullptr;
+
+  // maps
+  float2* S = nullptr;
+  float* Imap = nullptr;
+
+  // samples (sorted)
+  int* base = nullptr;
+  float* psi = nullptr;
+  float* w = nullptr;
+  int* perm = nullptr;                // sorted -> original sample index
+  int* sframe = nullptr;              // frame of each sorted sample
+  std::vector<int> h_perm;
+  int4* interp_items = nullptr;
+  int n_interp_items = 0;
+  GridItem* grid_items = nullptr;
+  int n_grid_items = 0;
+  ReduceItem* reduce_items = nullptr;
+  int n_reduce_items = 0;
+  int* visits = nullptr;
+  int64_t n_visits = 0;
+  int n_slots = 0;
+  float2* partial = nullptr;
+
+  // work buffers
+  float2* grid = nullptr;             // F*ntot*C
+  float2* t1 = nullptr;               // FFT intermediate
+  float2* t2 = nullptr;               // second intermediate (3D)
+  float2* Xc = nullptr;               // F*Ntot*C
+  float2* Xc2 = nullptr;              // CGNR: f_hat
+  float2* Xc3 = nullptr;              // CGNR: p_hat
+  float2* ybuf[4] = {nullptr, nullptr, nullptr, nullptr};   // Mtot*C each
+  float2* img[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // F*Ntot each
+  float2* ext_a = nullptr;            // staging for host I/O
+  float2* ext_b = nullptr;
+  size_t ext_a_bytes = 0, ext_b_bytes = 0;
+  double* partials = nullptr;         // reduction partials
+  size_t n_partials = 0;
+  CgState* cgst = nullptr;            // F
+  double* cnr = nullptr;              // CGNR per (frame, coil) scalars: 8*F*C
+  int* fdone = nullptr;               // F frame-done flags
+  int* fmask = nullptr;               // F scratch flags (single-frame NFFT calls)
+  double* hist = nullptr;             // residual history [F][maxit+1]
+  int hist_cap = 0;
+  CgState* h_cgst = nullptr;          // pinned mirror
+  int* h_done = nullptr;              // pinned
+
+  // profiling
+  struct Stat { int64_t launches = 0; double ms = 0, bytes = 0; };
+  Stat stat[CGS_K_COUNT];
+  struct Pending { hipEvent_t a, b; int kind; double bytes; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+};
+
+// ---- launchers (kernels_*.hip) ----------------------------------------------
+void launch_fft(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s);
+void launch_interp(const SampArgs& a, int cg, hipStream_t s);
+void launch_grid(const GridArgs& a, int cg, int waves, int nitems, int ncgroups, hipStream_t s);
+void launch_grid_reduce(const Plan& p, const float2* partial, float2* grid, hipStream_t s);
+void launch_coil_combine(const Plan& p, const float2* Xc, float2* x, const float2* pvec,
+                         float lambda, bool use_imap, double* partials, hipStream_t s);
+// vector / scalar kernels
+void launch_sqnorm_partials(const Plan& p, const float2* v, double* partials, hipStream_t s);
+void launch_cg_init(const Plan& p, const double* part_bn, const double* part_rn, int tolkind,
+                    double tol, int maxit, bool warm, hipStream_t s);
+void launch_cg_alpha(const Plan& p, const double* partials, hipStream_t s);
+void launch_cg_xr(const Plan& p, float2* x, float2* r, const float2* pv, const float2* q,
+                  double* partials, hipStream_t s);
+void launch_cg_beta(const Plan& p, const double* partials, int tolkind, double tol, int maxit,
+                    hipStream_t s);
+void launch_cg_p(const Plan& p, float2* pv, const float2* r, hipStream_t s);
+void launch_axpby_img(const Plan& p, float2* out, const float2* a, float alpha, const float2* b,
+                      float beta, hipStream_t s);
+// layout conversions
+void launch_y_ext2int(const Plan& p, const float2* ext, float2* in, int batch, int only_frame,
+                      hipStream_t s);
+void launch_y_int2ext(const Plan& p, const float2* in, float2* ext, int batch, int only_frame,
+                      hipStream_t s);
+void launch_img_ext2coil(const Plan& p, const float2* ext, float2* X, int batch, int frame,
+                         hipStream_t s);
+void launch_img_coil2ext(const Plan& p, const float2* X, float2* ext, int batch, int frame,
+                         hipStream_t s);
+// CGNR (reference E^H) kernels
+void launch_cnr_dot_samples(const Plan& p, const float2* v, double* out_slot, hipStream_t s);
+void launch_cnr_dot_image(const Plan& p, const float2* v, bool masked, double* out_slot,
+                          hipStream_t s);
+void launch_cnr_scalars(const Plan& p, int step, double eps, hipStream_t s);
+void launch_cnr_update_image(const Plan& p, float2* fhat, float2* phat, const float2* z,
+                             int step, hipStream_t s);
+void launch_cnr_update_samples(const Plan& p, float2* r, const float2* v, hipStream_t s);
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+void launch_fill_int(int* dst, int value, int n, hipStream_t s);
+
+// errors
+void set_error(const std::string& msg);
+
+}  // namespace cgs

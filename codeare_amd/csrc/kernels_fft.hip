@@ -1,0 +1,239 @@
+// kernels_fft.hip -- batched 1D FFT passes over one axis of the oversampled grid.
+//
+// One pass transforms every line along one axis of an array viewed as
+// [outer][P][inner] (inner contiguous).  A block stages `lb` adjacent lines in
+// LDS ([lb][n + pad]), so global loads and stores move lb*8 contiguous bytes per
+// position; the transform is a mixed-radix Stockham autosort (radix 2,3,4,8)
+// done in place in LDS with a barrier between the read and write halves of
+// every stage.
+//
+// Forward passes read image-index order (i in [0, nin), frequency i - nin/2,
+// zero-padded into bin (i - nin/2) mod n) and write grid order; adjoint passes
+// read grid order and write image-index order (crop).  Pass 1 of the forward
+// transform fuses the deapodisation 1/phi_hat, the zero padding and either the
+// coil multiply S_c .* x (E) or the damping mask w_hat (the CGNR's A(w_hat p));
+// the last adjoint pass fuses the crop and the deapodisation.
+#include "cgs_internal.hpp"
+
+namespace cgs {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// sign * i * z
+__device__ __forceinline__ float2 cmuli(float2 z, int sign) {
+  return sign > 0 ? make_float2(-z.y, z.x) : make_float2(z.y, -z.x);
+}
+
+template <int R>
+__device__ __forceinline__ void dft(float2* v, int sign);
+
+template <>
+__device__ __forceinline__ void dft<2>(float2* v, int) {
+  float2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <>
+__device__ __forceinline__ void dft<3>(float2* v, int sign) {
+  const float c = -0.5f, s = (sign > 0 ? 1.0f : -1.0f) * 0.86602540378443864676f;
+  float2 a = v[0], b = v[1], d = v[2];
+  float2 sum = cadd(b, d), dif = csub(b, d);
+  float2 m = make_float2(a.x + c * sum.x, a.y + c * sum.y);
+  float2 t = make_float2(-s * dif.y, s * dif.x);   // i*s*dif
+  v[0] = cadd(a, sum);
+  v[1] = cadd(m, t);
+  v[2] = csub(m, t);
+}
+
+template <>
+__device__ __forceinline__ void dft<4>(float2* v, int sign) {
+  float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+  float2 b0 = cadd(v[1], v[3]), b1 = cmuli(csub(v[1], v[3]), sign);
+  v[0] = cadd(a0, b0);
+  v[2] = csub(a0, b0);
+  v[1] = cadd(a1, b1);
+  v[3] = csub(a1, b1);
+}
+
+template <>
+__device__ __forceinline__ void dft<8>(float2* v, int sign) {
+  float2 e[4] = {v[0], v[2], v[4], v[6]};
+  float2 o[4] = {v[1], v[3], v[5], v[7]};
+  dft<4>(e, sign);
+  dft<4>(o, sign);
+  const float h = 0.70710678118654752440f;
+  const float sg = sign > 0 ? 1.0f : -1.0f;
+  // W8^1 = (1 + sg i)/sqrt2, W8^2 = sg i, W8^3 = (-1 + sg i)/sqrt2
+  float2 o1 = make_float2(h * (o[1].x - sg * o[1].y), h * (o[1].y + sg * o[1].x));
+  float2 o2 = cmuli(o[2], sign);
+  float2 o3 = make_float2(h * (-o[3].x - sg * o[3].y), h * (-o[3].y + sg * o[3].x));
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+
+// One Stockham stage of radix R over all lb lines held in LDS.
+template <int R>
+__device__ __forceinline__ void fft_stage(float2* lds, const FftArgs& a, int Ns, int tid) {
+  constexpr int MAXB = (kFftMaxElem / R + kFftThreads - 1) / kFftThreads;
+  const int n = a.n, nb = n / R, total = a.lb * nb;
+  const int tstep0 = n / (Ns * R);
+  float2 v[MAXB][R];
+  int dst[MAXB];
+#pragma unroll
+  for (int k = 0; k < MAXB; ++k) {
+    const int bidx = tid + k * kFftThreads;
+    if (bidx < total) {
+      const int q = bidx / nb, j = bidx - q * nb;
+      const int kk = j % Ns;
+      const float2* src = lds + q * a.ldn + j;
+      const int tstep = tstep0 * kk;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 x = src[r * nb];
+        if (r > 0 && kk > 0) {
+          float2 w = a.tw[r * tstep];
+          if (a.sign > 0) w.y = -w.y;
+          x = cmul(x, w);
+        }
+        v[k][r] = x;
+      }
+      dft<R>(v[k], a.sign);
+      dst[k] = q * a.ldn + (j / Ns) * Ns * R + kk;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < MAXB; ++k) {
+    const int bidx = tid + k * kFftThreads;
+    if (bidx < total) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[dst[k] + r * Ns] = v[k][r];
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ float deapod(const FftArgs& a, int i0, int64_t rest) {
+  float f = a.dinv0[i0];
+  const int i1 = (int)(rest % a.N1);
+  f *= a.dinv1[i1];
+  if (a.d == 3) f *= a.dinv2[(int)(rest / a.N1)];
+  return f;
+}
+
+__device__ __forceinline__ bool inside_mask(const FftArgs& a, int i0, int64_t rest) {
+  const int nh = a.mask_nh;
+  const int i1 = (int)(rest % a.N1);
+  int64_t s = (int64_t)(i0 - nh) * (i0 - nh) + (int64_t)(i1 - nh) * (i1 - nh);
+  if (a.d == 3) {
+    const int i2 = (int)(rest / a.N1);
+    s += (int64_t)(i2 - nh) * (i2 - nh);
+  }
+  return s <= (int64_t)nh * nh;
+}
+
+// LD: 0 plain, 1 E (x .* S_c .* D), 2 plain .* D, 3 plain .* D .* w_hat
+// ST: 0 plain, 1 .* D
+template <int LD, int ST>
+__global__ void __launch_bounds__(kFftThreads) k_fft(FftArgs a) {
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x;
+  const int64_t nib = (a.inner + a.lb - 1) / a.lb;
+  const int64_t o = blockIdx.x / nib;
+  const int64_t q0 = (blockIdx.x - o * nib) * a.lb;
+  const int64_t f = o / a.outer_per_frame;
+  if (a.fdone && a.fdone[f]) return;
+  const int64_t rest = o - f * a.outer_per_frame;
+  const int lb = a.lb, n = a.n;
+  const int nq = (int)min((int64_t)lb, a.inner - q0);
+
+  // ---- load (with zero padding for forward passes) ----
+  const bool pad = a.sign < 0;   // forward passes read image-index order
+  for (int idx = tid; idx < lb * n; idx += kFftThreads) {
+    const int q = idx % lb, pos = idx / lb;
+    float2 v = make_float2(0.f, 0.f);
+    int p = pos;
+    if (pad) {   // bin pos holds image index p = pos + nin/2 (mod n) when p < nin
+      p = pos + a.nin / 2;
+      if (p >= n) p -= n;
+    }
+    if (q < nq && p < a.nin) {
+      const int64_t qq = q0 + q;
+      if (LD == 1) {
+        const int64_t i = p + (int64_t)a.N0 * rest;
+        const float2 xv = a.x[f * a.Ntot + i];
+        const float2 sv = a.S[i * a.C + qq];
+        const float dv = deapod(a, p, rest);
+        float2 t = cmul(xv, sv);
+        v = make_float2(t.x * dv, t.y * dv);
+      } else {
+        v = a.in[(o * a.nin + p) * a.inner + qq];
+        if (LD >= 2) {
+          float dv = deapod(a, p, rest);
+          if (LD == 3 && !inside_mask(a, p, rest)) dv = 0.f;
+          v = make_float2(v.x * dv, v.y * dv);
+        }
+      }
+    }
+    lds[q * a.ldn + pos] = v;
+  }
+  __syncthreads();
+
+  // ---- Stockham stages ----
+  int Ns = 1;
+  for (int s = 0; s < a.nst; ++s) {
+    const int R = a.radix[s];
+    if (R == 8) fft_stage<8>(lds, a, Ns, tid);
+    else if (R == 4) fft_stage<4>(lds, a, Ns, tid);
+    else if (R == 2) fft_stage<2>(lds, a, Ns, tid);
+    else fft_stage<3>(lds, a, Ns, tid);
+    Ns *= R;
+  }
+
+  // ---- store (with crop for adjoint passes) ----
+  const bool crop = a.sign > 0;
+  for (int idx = tid; idx < lb * a.nout; idx += kFftThreads) {
+    const int q = idx % lb, p = idx / lb;
+    if (q >= nq) continue;
+    int pos = p;
+    if (crop) {
+      pos = p - a.nout / 2;
+      if (pos < 0) pos += n;
+    }
+    float2 v = lds[q * a.ldn + pos];
+    if (ST == 1) {
+      const float dv = deapod(a, p, rest);
+      v = make_float2(v.x * dv, v.y * dv);
+    }
+    a.out[(o * a.nout + p) * a.inner + q0 + q] = v;
+  }
+}
+
+void launch_fft(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s) {
+  const int64_t nib = (a.inner + a.lb - 1) / a.lb;
+  const dim3 grid((unsigned)(nib * a.outers));
+  const size_t shm = sizeof(float2) * (size_t)a.lb * a.ldn;
+#define CGS_FFT_CASE(L, S_)                                                   \
+  if (ld_mode == L && st_mode == S_) {                                        \
+    hipLaunchKernelGGL((k_fft<L, S_>), grid, dim3(kFftThreads), shm, s, a);  \
+    return;                                                                   \
+  }
+  CGS_FFT_CASE(0, 0)
+  CGS_FFT_CASE(1, 0)
+  CGS_FFT_CASE(2, 0)
+  CGS_FFT_CASE(3, 0)
+  CGS_FFT_CASE(0, 1)
+#undef CGS_FFT_CASE
+}
+
+}  // namespace cgs

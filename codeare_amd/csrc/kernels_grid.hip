@@ -1,0 +1,261 @@
+// kernels_grid.hip -- Kaiser-Bessel window gather (forward B) and tile-binned
+// window scatter (adjoint B^T) on the coil-interleaved oversampled grid.
+//
+// Both kernels put the coils of one sample across the lanes of a wave:
+// lane = cl + CG*rs with cl the coil within a group of CG coils and rs one of
+// RS = 64/CG tap rows handled side by side.  The window weights of a sample are
+// wave-uniform (scalar loads); every tap is one coalesced CG*8-byte access.
+//
+// Gridding is "owner computes": each grid tile (T^d cells) is owned by one
+// work item that visits every sample whose 2m+2 footprint reaches into the
+// tile and accumulates only the taps that land inside it, in LDS.  Each cell is
+// therefore written exactly once, with no atomics, no zero-fill of the grid and
+// a fixed summation order (deterministic).  Tiles whose visit lists are long
+// (the k-space centre of radial trajectories) are split across several items
+// that write partial tiles, summed afterwards in a fixed order.
+#include "cgs_internal.hpp"
+
+namespace cgs {
+
+__device__ __forceinline__ float2 shfl_xor2(float2 v, int m) {
+  return make_float2(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
+}
+
+// ---------------------------------------------------------------------------
+// forward B: y[s][c] = sum_taps psi * G[cell][c]
+template <int CG, int TAPS, int D>
+__global__ void __launch_bounds__(256) k_interp(SampArgs a) {
+  constexpr int RS = 64 / CG;
+  constexpr int ROWS = (D == 3) ? TAPS * TAPS : TAPS;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cl = lane % CG, rs = lane / CG;
+  const int item = blockIdx.x * 4 + wave;
+  if (item >= a.nitems) return;
+  const int4 it = a.items[item];
+  if (a.fdone && a.fdone[it.x]) return;
+  const float2* g = a.grid + (int64_t)it.x * a.ntot * a.C;
+  const int n0 = a.n0, n1 = a.n1, n2 = a.n2;
+  for (int s = it.y; s < it.y + it.z; ++s) {
+    const int* bs = a.base + (int64_t)s * D;
+    const float* ps = a.psi + (int64_t)s * D * TAPS;
+    const int b0 = bs[0], b1 = bs[1], b2 = (D == 3) ? bs[2] : 0;
+    for (int c0 = 0; c0 < a.C; c0 += CG) {
+      const int c = c0 + cl;
+      float2 acc = make_float2(0.f, 0.f);
+      if (c < a.C) {
+        for (int r = rs; r < ROWS; r += RS) {
+          const int s1 = (D == 3) ? r % TAPS : r;
+          int l1 = b1 + s1;
+          if (l1 >= n1) l1 -= n1;
+          float wr = ps[TAPS + s1];
+          int64_t row = l1;
+          if (D == 3) {
+            const int s2 = r / TAPS;
+            int l2 = b2 + s2;
+            if (l2 >= n2) l2 -= n2;
+            wr *= ps[2 * TAPS + s2];
+            row += (int64_t)n1 * l2;
+          }
+          const float2* gr = g + row * n0 * a.C + c;
+          float2 racc = make_float2(0.f, 0.f);
+#pragma unroll
+          for (int s0 = 0; s0 < TAPS; ++s0) {
+            int l0 = b0 + s0;
+            if (l0 >= n0) l0 -= n0;
+            const float2 v = gr[(int64_t)l0 * a.C];
+            const float p0 = ps[s0];
+            racc.x += p0 * v.x;
+            racc.y += p0 * v.y;
+          }
+          acc.x += wr * racc.x;
+          acc.y += wr * racc.y;
+        }
+      }
+#pragma unroll
+      for (int m = CG; m < 64; m <<= 1) {
+        const float2 o = shfl_xor2(acc, m);
+        acc.x += o.x;
+        acc.y += o.y;
+      }
+      if (rs == 0 && c < a.C) a.y[(int64_t)s * a.C + c] = acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint B^T, owner-computes tile accumulation.  Block = `waves` waves; wave
+// v handles coils [cbase + v*CG, +CG); LDS tile [T^d cells][CGW coils] with a
+// padded row stride.  Only one wave touches a given (cell, coil) and a wave
+// visits samples in order, so plain read-modify-write in LDS is race-free.
+template <int CG, int TAPS, int D>
+__global__ void __launch_bounds__(256) k_grid(GridArgs a) {
+  constexpr int RS = 64 / CG;
+  extern __shared__ float2 tile[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cl = lane % CG, rs = lane / CG;
+  const GridItem it = a.items[blockIdx.x];
+  if (a.fdone && a.fdone[it.frame]) return;
+  const int T = a.T, rowstride = a.rowstride;
+  const int cbase = blockIdx.y * a.CGW;
+  const int cw = wave * CG + cl;            // coil slot in the tile
+  const int c = cbase + cw;
+  const bool cok = (cw < a.CGW) && (c < a.C);
+
+  // tile coordinates
+  int t0 = it.tile % a.nt0, t1 = (it.tile / a.nt0) % a.nt1, t2 = (D == 3) ? it.tile / (a.nt0 * a.nt1) : 0;
+  const int ts0 = t0 * T, ts1 = t1 * T, ts2 = t2 * T;
+  const int te0 = min(T, a.n0 - ts0), te1 = min(T, a.n1 - ts1), te2 = (D == 3) ? min(T, a.n2 - ts2) : 1;
+  const int nrows_tile = (D == 3) ? T * T : T;
+
+  // zero the tile
+  for (int idx = threadIdx.x; idx < nrows_tile * rowstride; idx += blockDim.x)
+    tile[idx] = make_float2(0.f, 0.f);
+  __syncthreads();
+
+  for (int v = it.vbeg; v < it.vend; ++v) {
+    const int s = a.visits[v];
+    const int* bs = a.base + (int64_t)s * D;
+    const float* ps = a.psi + (int64_t)s * D * TAPS;
+    int rel0 = bs[0] - ts0; if (rel0 < 0) rel0 += a.n0; if (rel0 >= te0) rel0 -= a.n0;
+    int rel1 = bs[1] - ts1; if (rel1 < 0) rel1 += a.n1; if (rel1 >= te1) rel1 -= a.n1;
+    int rel2 = 0;
+    if (D == 3) { rel2 = bs[2] - ts2; if (rel2 < 0) rel2 += a.n2; if (rel2 >= te2) rel2 -= a.n2; }
+    const int lo0 = max(0, -rel0), hi0 = min(TAPS, te0 - rel0);
+    const int lo1 = max(0, -rel1), hi1 = min(TAPS, te1 - rel1);
+    const int lo2 = (D == 3) ? max(0, -rel2) : 0, hi2 = (D == 3) ? min(TAPS, te2 - rel2) : 1;
+    const int nr1 = hi1 - lo1;
+    const int nrows = nr1 * (hi2 - lo2);
+    if (nr1 <= 0 || hi0 <= lo0 || nrows <= 0) continue;
+    float2 yv = make_float2(0.f, 0.f);
+    if (cok) {
+      yv = a.y[(int64_t)s * a.C + c];
+      if (a.w) { const float ww = a.w[s]; yv.x *= ww; yv.y *= ww; }
+    }
+    for (int r = rs; r < nrows; r += RS) {
+      const int s1 = lo1 + r % nr1;
+      float wr = ps[TAPS + s1];
+      int trow = rel1 + s1;
+      if (D == 3) {
+        const int s2 = lo2 + r / nr1;
+        wr *= ps[2 * TAPS + s2];
+        trow += (rel2 + s2) * T;
+      }
+      const float vx = yv.x * wr, vy = yv.y * wr;
+      float2* dst = tile + trow * rowstride + cw;
+      for (int s0 = lo0; s0 < hi0; ++s0) {
+        const float p0 = ps[s0];
+        float2* cell = dst + (rel0 + s0) * a.CGW;
+        float2 cur = *cell;
+        cur.x += p0 * vx;
+        cur.y += p0 * vy;
+        *cell = cur;
+      }
+    }
+  }
+  __syncthreads();
+
+  // write the owned tile (all coils of the group), or a partial slot
+  const int ncell = te0 * te1 * te2;
+  for (int idx = threadIdx.x; idx < ncell * a.CGW; idx += blockDim.x) {
+    const int cc = idx % a.CGW, cell = idx / a.CGW;
+    const int cg = cbase + cc;
+    if (cg >= a.C) continue;
+    const int x0 = cell % te0, x1 = (cell / te0) % te1, x2 = cell / (te0 * te1);
+    const float2 v = tile[(x1 + x2 * T) * rowstride + x0 * a.CGW + cc];
+    if (it.slot < 0) {
+      const int64_t l = (ts0 + x0) + (int64_t)a.n0 * ((ts1 + x1) + (int64_t)a.n1 * (ts2 + x2));
+      a.grid[((int64_t)it.frame * a.ntot + l) * a.C + cg] = v;
+    } else {
+      const int64_t lc = x0 + (int64_t)T * (x1 + (int64_t)T * x2);
+      a.partial[((int64_t)it.slot * a.tilecells + lc) * a.C + cg] = v;
+    }
+  }
+}
+
+// sums the partial tiles of the heavy tiles in slot order
+__global__ void k_grid_reduce(const ReduceItem* items, const float2* partial, float2* grid,
+                              int C, int T, int d, int n0, int n1, int n2, int nt0, int nt1,
+                              int64_t ntot, int64_t tilecells, const int* fdone) {
+  const ReduceItem it = items[blockIdx.x];
+  if (fdone && fdone[it.frame]) return;
+  const int t0 = it.tile % nt0, t1 = (it.tile / nt0) % nt1, t2 = (d == 3) ? it.tile / (nt0 * nt1) : 0;
+  const int ts0 = t0 * T, ts1 = t1 * T, ts2 = t2 * T;
+  const int te0 = min(T, n0 - ts0), te1 = min(T, n1 - ts1), te2 = (d == 3) ? min(T, n2 - ts2) : 1;
+  const int ncell = te0 * te1 * te2;
+  for (int idx = threadIdx.x; idx < ncell * C; idx += blockDim.x) {
+    const int c = idx % C, cell = idx / C;
+    const int x0 = cell % te0, x1 = (cell / te0) % te1, x2 = cell / (te0 * te1);
+    const int64_t lc = x0 + (int64_t)T * (x1 + (int64_t)T * x2);
+    float2 acc = make_float2(0.f, 0.f);
+    for (int k = 0; k < it.nslots; ++k) {
+      const float2 v = partial[((int64_t)(it.slot0 + k) * tilecells + lc) * C + c];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    const int64_t l = (ts0 + x0) + (int64_t)n0 * ((ts1 + x1) + (int64_t)n1 * (ts2 + x2));
+    grid[((int64_t)it.frame * ntot + l) * C + c] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <int CG, int D>
+static void interp_taps(const SampArgs& a, dim3 g, hipStream_t s) {
+  switch (a.taps) {
+    case 4: hipLaunchKernelGGL((k_interp<CG, 4, D>), g, dim3(256), 0, s, a); break;
+    case 6: hipLaunchKernelGGL((k_interp<CG, 6, D>), g, dim3(256), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_interp<CG, 8, D>), g, dim3(256), 0, s, a); break;
+    case 10: hipLaunchKernelGGL((k_interp<CG, 10, D>), g, dim3(256), 0, s, a); break;
+    default: break;
+  }
+}
+
+void launch_interp(const SampArgs& a, int cg, hipStream_t s) {
+  const dim3 g((unsigned)((a.nitems + 3) / 4));
+  if (a.nitems == 0) return;
+  if (a.d == 2) {
+    if (cg == 8) interp_taps<8, 2>(a, g, s);
+    else if (cg == 16) interp_taps<16, 2>(a, g, s);
+    else interp_taps<32, 2>(a, g, s);
+  } else {
+    if (cg == 8) interp_taps<8, 3>(a, g, s);
+    else if (cg == 16) interp_taps<16, 3>(a, g, s);
+    else interp_taps<32, 3>(a, g, s);
+  }
+}
+
+template <int CG, int D>
+static void grid_taps(const GridArgs& a, dim3 g, dim3 b, size_t shm, hipStream_t s) {
+  switch (a.taps) {
+    case 4: hipLaunchKernelGGL((k_grid<CG, 4, D>), g, b, shm, s, a); break;
+    case 6: hipLaunchKernelGGL((k_grid<CG, 6, D>), g, b, shm, s, a); break;
+    case 8: hipLaunchKernelGGL((k_grid<CG, 8, D>), g, b, shm, s, a); break;
+    case 10: hipLaunchKernelGGL((k_grid<CG, 10, D>), g, b, shm, s, a); break;
+    default: break;
+  }
+}
+
+void launch_grid(const GridArgs& a, int cg, int waves, int nitems, int ncgroups, hipStream_t s) {
+  if (nitems == 0) return;
+  const dim3 g((unsigned)nitems, (unsigned)ncgroups), b((unsigned)(64 * waves));
+  const int nrows_tile = (a.d == 3) ? a.T * a.T : a.T;
+  const size_t shm = sizeof(float2) * (size_t)nrows_tile * a.rowstride;
+  if (a.d == 2) {
+    if (cg == 8) grid_taps<8, 2>(a, g, b, shm, s);
+    else if (cg == 16) grid_taps<16, 2>(a, g, b, shm, s);
+    else grid_taps<32, 2>(a, g, b, shm, s);
+  } else {
+    if (cg == 8) grid_taps<8, 3>(a, g, b, shm, s);
+    else if (cg == 16) grid_taps<16, 3>(a, g, b, shm, s);
+    else grid_taps<32, 3>(a, g, b, shm, s);
+  }
+}
+
+void launch_grid_reduce(const Plan& p, const float2* partial, float2* grid, hipStream_t s) {
+  if (p.n_reduce_items == 0) return;
+  const int64_t tilecells = (int64_t)p.T * p.T * (p.d == 3 ? p.T : 1);
+  hipLaunchKernelGGL(k_grid_reduce, dim3((unsigned)p.n_reduce_items), dim3(256), 0, s,
+                     p.reduce_items, partial, grid, p.C, p.T, p.d, p.n[0], p.n[1], p.n[2],
+                     p.nt[0], p.nt[1], p.ntot, tilecells, p.fdone);
+}
+
+}  // namespace cgs

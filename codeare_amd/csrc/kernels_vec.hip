@@ -1,0 +1,516 @@
+// kernels_vec.hip -- coil combine, CG / CGNR vector and scalar kernels, layout
+// conversions.
+//
+// Reductions are deterministic: every block writes an FP64 partial to a fixed
+// slot and a one-block kernel per frame sums the partials in a fixed tree order.
+// The CG scalars (rn, <p,q>, alpha, beta, residual) live on the device in FP64
+// (CgState); vector arithmetic stays FP32.  Every kernel skips frames whose
+// done flag is set, so finished frames cost only a flag read.
+#include "cgs_internal.hpp"
+
+namespace cgs {
+
+constexpr int kRedBlocks = 512;   // partial sums per frame for image-domain reductions
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int k = 0; k < nw; ++k) t += sh[k];
+  }
+  __syncthreads();
+  return t;   // valid in thread 0
+}
+
+// ---------------------------------------------------------------------------
+// x[f][i] = (sum_c conj(S[i][c]) X[f][i][c]) * I[i] + lambda p[f][i]; partial <p, x>
+// CP lanes per pixel (power of two >= C, at most 64).
+template <int CP>
+__global__ void __launch_bounds__(256) k_coil_combine(const float2* X, const float2* S,
+                                                      const float* Imap, float2* x,
+                                                      const float2* pv, float lambda,
+                                                      double* partials, int C, int64_t Ntot,
+                                                      const int* fdone) {
+  __shared__ double sh[4];
+  const int f = blockIdx.y;
+  if (fdone && fdone[f]) return;
+  constexpr int PPB = 256 / CP;
+  const int g = threadIdx.x / CP, cl = threadIdx.x % CP;
+  double dot = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * PPB + g; i < Ntot; i += (int64_t)gridDim.x * PPB) {
+    float2 acc = make_float2(0.f, 0.f);
+    for (int c = cl; c < C; c += CP) {
+      const float2 s = S[i * C + c];
+      const float2 v = X[((int64_t)f * Ntot + i) * C + c];
+      acc.x += s.x * v.x + s.y * v.y;
+      acc.y += s.x * v.y - s.y * v.x;
+    }
+#pragma unroll
+    for (int m = CP / 2; m > 0; m >>= 1) {
+      acc.x += __shfl_xor(acc.x, m);
+      acc.y += __shfl_xor(acc.y, m);
+    }
+    if (cl == 0) {
+      if (Imap) { const float iv = Imap[i]; acc.x *= iv; acc.y *= iv; }
+      const int64_t o = (int64_t)f * Ntot + i;
+      if (pv) {
+        const float2 p = pv[o];
+        acc.x += lambda * p.x;
+        acc.y += lambda * p.y;
+        dot += (double)p.x * acc.x + (double)p.y * acc.y;
+      }
+      x[o] = acc;
+    }
+  }
+  if (partials) {
+    const double t = block_sum(dot, sh);
+    if (threadIdx.x == 0) partials[(int64_t)f * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+void launch_coil_combine(const Plan& p, const float2* Xc, float2* x, const float2* pvec,
+                         float lambda, bool use_imap, double* partials, hipStream_t s) {
+  const dim3 g(kRedBlocks, p.F);
+  const float* I = use_imap ? p.Imap : nullptr;
+  if (p.C <= 8)
+    hipLaunchKernelGGL(k_coil_combine<8>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, p.fdone);
+  else if (p.C <= 16)
+    hipLaunchKernelGGL(k_coil_combine<16>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, p.fdone);
+  else if (p.C <= 32)
+    hipLaunchKernelGGL(k_coil_combine<32>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, p.fdone);
+  else
+    hipLaunchKernelGGL(k_coil_combine<64>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, p.fdone);
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_sqnorm_partials(const float2* v, int64_t Ntot, double* partials, const int* fdone) {
+  __shared__ double sh[4];
+  const int f = blockIdx.y;
+  if (fdone && fdone[f]) return;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Ntot; i += (int64_t)gridDim.x * blockDim.x) {
+    const float2 a = v[(int64_t)f * Ntot + i];
+    acc += (double)a.x * a.x + (double)a.y * a.y;
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) partials[(int64_t)f * gridDim.x + blockIdx.x] = t;
+}
+
+void launch_sqnorm_partials(const Plan& p, const float2* v, double* partials, hipStream_t s) {
+  hipLaunchKernelGGL(k_sqnorm_partials, dim3(kRedBlocks, p.F), dim3(256), 0, s, v, p.Ntot, partials, p.fdone);
+}
+
+__device__ double sum_partials(const double* part, int nb, double* sh) {
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += part[b];
+  return block_sum(acc, sh);
+}
+
+// CG start: bn = ||E^H y||^2, rn = ||r0||^2, residual measure, done test.
+__global__ void k_cg_init(CgState* st, const double* pbn, const double* prn, int nb, int tolkind,
+                          double tol, int maxit, double* hist, int hcap, int* fdone) {
+  __shared__ double sh[4];
+  const int f = blockIdx.x;
+  const double bn = sum_partials(pbn + (int64_t)f * nb, nb, sh);
+  const double rn = prn ? sum_partials(prn + (int64_t)f * nb, nb, sh) : bn;
+  if (threadIdx.x == 0) {
+    CgState c;
+    c.bn = bn;
+    c.rn = rn;
+    c.xn = rn;   // codeare: rn = xn = Re<r, p> (CGLS.hpp:35)
+    c.pq = 0; c.alpha = 0; c.beta = 0;
+    c.it = 0; c.nan = 0; c.pad = 0;
+    double rel;
+    if (tolkind == CGS_TOL_SQUARED_RATIO) { c.xn = bn; rel = bn > 0 ? rn / bn : 0.0; }
+    else rel = bn > 0 ? sqrt(rn / bn) : 0.0;
+    c.rel = rel;
+    c.nan = isnan(rel) ? 1 : 0;
+    c.done = (c.nan || rel <= tol || maxit <= 0) ? 1 : 0;
+    st[f] = c;
+    fdone[f] = c.done;
+    if (hist) hist[(int64_t)f * hcap] = rel;
+  }
+}
+
+void launch_cg_init(const Plan& p, const double* part_bn, const double* part_rn, int tolkind,
+                    double tol, int maxit, bool warm, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_init, dim3(p.F), dim3(256), 0, s, p.cgst, part_bn, warm ? part_rn : nullptr,
+                     kRedBlocks, tolkind, tol, maxit, p.hist, p.hist_cap, p.fdone);
+}
+
+__global__ void k_cg_alpha(CgState* st, const double* part, int nb, const int* fdone) {
+  __shared__ double sh[4];
+  const int f = blockIdx.x;
+  if (fdone[f]) return;
+  const double pq = sum_partials(part + (int64_t)f * nb, nb, sh);
+  if (threadIdx.x == 0) {
+    st[f].pq = pq;
+    st[f].alpha = st[f].rn / pq;   // CGLS.hpp:50
+  }
+}
+
+void launch_cg_alpha(const Plan& p, const double* partials, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_alpha, dim3(p.F), dim3(256), 0, s, p.cgst, partials, kRedBlocks, p.fdone);
+}
+
+// x += alpha p ; r -= alpha q ; partial ||r||^2   (CGLS.hpp:51-54)
+__global__ void k_cg_xr(float2* x, float2* r, const float2* pv, const float2* q, const CgState* st,
+                        double* partials, int64_t Ntot, const int* fdone) {
+  __shared__ double sh[4];
+  const int f = blockIdx.y;
+  if (fdone[f]) return;
+  const float al = (float)st[f].alpha;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Ntot; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = (int64_t)f * Ntot + i;
+    const float2 pp = pv[o], qq = q[o];
+    float2 xx = x[o], rr = r[o];
+    xx.x += al * pp.x; xx.y += al * pp.y;
+    rr.x -= al * qq.x; rr.y -= al * qq.y;
+    x[o] = xx;
+    r[o] = rr;
+    acc += (double)rr.x * rr.x + (double)rr.y * rr.y;
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) partials[(int64_t)f * gridDim.x + blockIdx.x] = t;
+}
+
+void launch_cg_xr(const Plan& p, float2* x, float2* r, const float2* pv, const float2* q,
+                  double* partials, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_xr, dim3(kRedBlocks, p.F), dim3(256), 0, s, x, r, pv, q, p.cgst, partials, p.Ntot, p.fdone);
+}
+
+__global__ void k_cg_beta(CgState* st, const double* part, int nb, int tolkind, double tol, int maxit,
+                          double* hist, int hcap, int* fdone) {
+  __shared__ double sh[4];
+  const int f = blockIdx.x;
+  if (fdone[f]) return;
+  const double rn_new = sum_partials(part + (int64_t)f * nb, nb, sh);
+  if (threadIdx.x == 0) {
+    CgState c = st[f];
+    c.beta = rn_new / c.rn;   // CGLS.hpp:53-55
+    c.rn = rn_new;
+    c.it += 1;
+    const double rel = (tolkind == CGS_TOL_SQUARED_RATIO) ? rn_new / c.xn : sqrt(rn_new / c.bn);
+    c.rel = rel;
+    if (isnan(rel) || isnan(c.alpha)) c.nan = 1;
+    c.done = (c.nan || rel <= tol || c.it >= maxit) ? 1 : 0;
+    st[f] = c;
+    if (hist && c.it < hcap) hist[(int64_t)f * hcap + c.it] = rel;
+    fdone[f] = c.done;
+  }
+}
+
+void launch_cg_beta(const Plan& p, const double* partials, int tolkind, double tol, int maxit,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_beta, dim3(p.F), dim3(256), 0, s, p.cgst, partials, kRedBlocks, tolkind, tol,
+                     maxit, p.hist, p.hist_cap, p.fdone);
+}
+
+// p = r + beta p  (CGLS.hpp:55-56)
+__global__ void k_cg_p(float2* pv, const float2* r, const CgState* st, int64_t Ntot, const int* fdone) {
+  const int f = blockIdx.y;
+  if (fdone[f]) return;
+  const float be = (float)st[f].beta;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Ntot; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = (int64_t)f * Ntot + i;
+    const float2 rr = r[o];
+    float2 pp = pv[o];
+    pp.x = rr.x + be * pp.x;
+    pp.y = rr.y + be * pp.y;
+    pv[o] = pp;
+  }
+}
+
+void launch_cg_p(const Plan& p, float2* pv, const float2* r, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_p, dim3(kRedBlocks, p.F), dim3(256), 0, s, pv, r, p.cgst, p.Ntot, p.fdone);
+}
+
+// out = alpha a + beta b over all frames (no done test)
+__global__ void k_axpby(float2* out, const float2* a, float al, const float2* b, float be, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float2 u = a ? a[i] : make_float2(0.f, 0.f);
+    float2 v = b ? b[i] : make_float2(0.f, 0.f);
+    out[i] = make_float2(al * u.x + be * v.x, al * u.y + be * v.y);
+  }
+}
+
+void launch_axpby_img(const Plan& p, float2* out, const float2* a, float alpha, const float2* b,
+                      float beta, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpby, dim3(1024), dim3(256), 0, s, out, a, alpha, b, beta, p.Ntot * p.F);
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversions between codeare's (M_f, C) frame blocks and the sorted,
+// coil-interleaved internal sample layout.
+__global__ void k_y_ext2int(const float2* ext, float2* in, const int* perm, const int* sframe,
+                            const int64_t* foff, int C, int batch, int only_frame, int64_t Mtot) {
+  const int64_t tot = Mtot * C;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = idx / C;
+    const int c = (int)(idx - s * C);
+    const int f = sframe[s];
+    if (only_frame >= 0 && f != only_frame) continue;
+    float2 v = make_float2(0.f, 0.f);
+    if (c < batch) {
+      const int64_t j = perm[s], o = foff[f], Mf = foff[f + 1] - o;
+      const int64_t e = (only_frame >= 0 ? 0 : (int64_t)batch * o) + (j - o) + Mf * c;
+      v = ext[e];
+    }
+    in[idx] = v;
+  }
+}
+
+__global__ void k_y_int2ext(const float2* in, float2* ext, const int* perm, const int* sframe,
+                            const int64_t* foff, int C, int batch, int only_frame, int64_t Mtot) {
+  const int64_t tot = Mtot * C;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = idx / C;
+    const int c = (int)(idx - s * C);
+    if (c >= batch) continue;
+    const int f = sframe[s];
+    if (only_frame >= 0 && f != only_frame) continue;
+    const int64_t j = perm[s], o = foff[f], Mf = foff[f + 1] - o;
+    const int64_t e = (only_frame >= 0 ? 0 : (int64_t)batch * o) + (j - o) + Mf * c;
+    ext[e] = in[idx];
+  }
+}
+
+void launch_y_ext2int(const Plan& p, const float2* ext, float2* in, int batch, int only_frame, hipStream_t s) {
+  hipLaunchKernelGGL(k_y_ext2int, dim3(2048), dim3(256), 0, s, ext, in, p.perm, p.sframe, p.d_foff, p.C, batch,
+                     only_frame, p.Mtot);
+}
+void launch_y_int2ext(const Plan& p, const float2* in, float2* ext, int batch, int only_frame, hipStream_t s) {
+  hipLaunchKernelGGL(k_y_int2ext, dim3(2048), dim3(256), 0, s, in, ext, p.perm, p.sframe, p.d_foff, p.C, batch,
+                     only_frame, p.Mtot);
+}
+
+// (N^d, batch) column-major <-> X[(f*Ntot + i)*C + c]
+__global__ void k_img_ext2coil(const float2* ext, float2* X, int C, int batch, int64_t Ntot, int f) {
+  const int64_t tot = Ntot * C;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx / C;
+    const int c = (int)(idx - i * C);
+    X[(int64_t)f * Ntot * C + idx] = (c < batch) ? ext[i + Ntot * c] : make_float2(0.f, 0.f);
+  }
+}
+__global__ void k_img_coil2ext(const float2* X, float2* ext, int C, int batch, int64_t Ntot, int f) {
+  const int64_t tot = Ntot * C;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx / C;
+    const int c = (int)(idx - i * C);
+    if (c < batch) ext[i + Ntot * c] = X[(int64_t)f * Ntot * C + idx];
+  }
+}
+void launch_img_ext2coil(const Plan& p, const float2* ext, float2* X, int batch, int frame, hipStream_t s) {
+  hipLaunchKernelGGL(k_img_ext2coil, dim3(2048), dim3(256), 0, s, ext, X, p.C, batch, p.Ntot, frame);
+}
+void launch_img_coil2ext(const Plan& p, const float2* X, float2* ext, int batch, int frame, hipStream_t s) {
+  hipLaunchKernelGGL(k_img_coil2ext, dim3(2048), dim3(256), 0, s, X, ext, p.C, batch, p.Ntot, frame);
+}
+
+// ---------------------------------------------------------------------------
+// CGNR (NFFT3 solver with PRECOMPUTE_WEIGHT | PRECOMPUTE_DAMP, NFFTTraits.hpp:122-138)
+// batched over (frame, coil) with one set of scalars per pair:
+//   cnr[k*F*C + f*C + c], k: 0 dot_r, 1 dot_z, 2 dot_v, 3 alpha, 4 beta, 5 active, 6 new value
+constexpr int kCnrBlocks = 256;
+
+// per-coil weighted sums over rows of a [rows][C] array; rows are samples of
+// each frame (weights w[s]) or pixels of each frame (damping mask)
+template <int CP>
+__global__ void __launch_bounds__(256) k_cnr_partials(const float2* v, const float* w, const int64_t* foff,
+                                                      int64_t Ntot, int mask_nh, int d, int N0, int N1,
+                                                      int C, int image, double* partials, const int* fmask) {
+  const int f = blockIdx.y;
+  if (fmask && fmask[f]) return;
+  constexpr int RPB = 256 / CP;
+  const int g = threadIdx.x / CP, cl = threadIdx.x % CP;
+  const int64_t r0 = image ? (int64_t)f * Ntot : foff[f];
+  const int64_t r1 = image ? r0 + Ntot : foff[f + 1];
+  __shared__ double sh[256];
+  for (int c0 = 0; c0 < C; c0 += CP) {
+    const int c = c0 + cl;
+    double acc = 0.0;
+    if (c < C) {
+      for (int64_t r = r0 + (int64_t)blockIdx.x * RPB + g; r < r1; r += (int64_t)gridDim.x * RPB) {
+        float wt;
+        if (image) {
+          const int64_t i = r - r0;
+          const int i0 = (int)(i % N0);
+          const int64_t rest = i / N0;
+          const int i1 = (int)(rest % N1);
+          int64_t q = (int64_t)(i0 - mask_nh) * (i0 - mask_nh) + (int64_t)(i1 - mask_nh) * (i1 - mask_nh);
+          if (d == 3) { const int i2 = (int)(rest / N1); q += (int64_t)(i2 - mask_nh) * (i2 - mask_nh); }
+          wt = (q <= (int64_t)mask_nh * mask_nh) ? 1.f : 0.f;
+        } else {
+          wt = w[r];
+        }
+        const float2 a = v[r * C + c];
+        acc += (double)wt * ((double)a.x * a.x + (double)a.y * a.y);
+      }
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < CP) {
+      double t = 0.0;
+      for (int k = 0; k < RPB; ++k) t += sh[k * CP + threadIdx.x];
+      if (c0 + threadIdx.x < C) partials[((int64_t)f * gridDim.x + blockIdx.x) * C + c0 + threadIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_cnr_reduce(const double* partials, int nb, int C, int F, double* cnr, int slot, const int* fmask) {
+  // one thread per (frame, coil), sums nb partials in order
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= F * C) return;
+  const int f = idx / C, c = idx % C;
+  if (fmask && fmask[f]) return;
+  double t = 0.0;
+  for (int b = 0; b < nb; ++b) t += partials[((int64_t)f * nb + b) * C + c];
+  cnr[(int64_t)slot * F * C + idx] = t;
+}
+
+static void cnr_dot(const Plan& p, const float2* v, int image, double* dst_cnr, int slot, hipStream_t s) {
+  const dim3 g(kCnrBlocks, p.F);
+  const int nh = (int)(0.5 * p.N[0]);
+  if (p.C <= 8)
+    hipLaunchKernelGGL(k_cnr_partials<8>, g, dim3(256), 0, s, v, p.w, p.d_foff, p.Ntot, nh, p.d, p.N[0], p.N[1], p.C, image, p.partials, p.fmask);
+  else if (p.C <= 16)
+    hipLaunchKernelGGL(k_cnr_partials<16>, g, dim3(256), 0, s, v, p.w, p.d_foff, p.Ntot, nh, p.d, p.N[0], p.N[1], p.C, image, p.partials, p.fmask);
+  else
+    hipLaunchKernelGGL(k_cnr_partials<32>, g, dim3(256), 0, s, v, p.w, p.d_foff, p.Ntot, nh, p.d, p.N[0], p.N[1], p.C, image, p.partials, p.fmask);
+  const int n = p.F * p.C;
+  hipLaunchKernelGGL(k_cnr_reduce, dim3((n + 255) / 256), dim3(256), 0, s, p.partials, kCnrBlocks, p.C, p.F, dst_cnr, slot, p.fmask);
+}
+
+void launch_cnr_dot_samples(const Plan& p, const float2* v, double* cnr_slot_base, hipStream_t s) {
+  // cnr_slot_base encodes the slot: pointer p.cnr + slot*F*C
+  const int slot = (int)((cnr_slot_base - p.cnr) / (p.F * p.C));
+  cnr_dot(p, v, 0, p.cnr, slot, s);
+}
+void launch_cnr_dot_image(const Plan& p, const float2* v, bool masked, double* cnr_slot_base, hipStream_t s) {
+  (void)masked;
+  const int slot = (int)((cnr_slot_base - p.cnr) / (p.F * p.C));
+  cnr_dot(p, v, 1, p.cnr, slot, s);
+}
+
+// step 0: active &= dot_r >= eps (NFFTTraits.hpp:131); step 1: alpha; step 2: beta,
+// step 3: initialise (active = 1).
+__global__ void k_cnr_scalars(double* cnr, int FC, int step, double eps, const int* fmask, int C) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= FC) return;
+  if (fmask && fmask[idx / C]) return;
+  double* dot_r = cnr + 0 * FC;
+  double* dot_z = cnr + 1 * FC;
+  double* dot_v = cnr + 2 * FC;
+  double* alpha = cnr + 3 * FC;
+  double* beta = cnr + 4 * FC;
+  double* active = cnr + 5 * FC;
+  double* nv = cnr + 6 * FC;
+  if (step == 3) {
+    active[idx] = 1.0;
+    alpha[idx] = 0.0;
+    beta[idx] = 0.0;
+  } else if (step == 0) {
+    if (!(dot_r[idx] >= eps)) active[idx] = 0.0;
+  } else if (step == 1) {
+    alpha[idx] = (active[idx] != 0.0) ? dot_z[idx] / dot_v[idx] : 0.0;
+  } else {
+    if (active[idx] != 0.0) {
+      beta[idx] = nv[idx] / dot_z[idx];
+      dot_z[idx] = nv[idx];
+    }
+  }
+}
+
+void launch_cnr_scalars(const Plan& p, int step, double eps, hipStream_t s) {
+  const int n = p.F * p.C;
+  hipLaunchKernelGGL(k_cnr_scalars, dim3((n + 255) / 256), dim3(256), 0, s, p.cnr, n, step, eps, p.fmask, p.C);
+}
+
+// step 0: f_hat += alpha w_hat p_hat ; step 1: p_hat = active ? beta p_hat + z : p_hat
+__global__ void k_cnr_update_image(float2* fhat, float2* phat, const float2* z, const double* cnr, int F, int C,
+                                   int64_t Ntot, int mask_nh, int d, int N0, int N1, int step, const int* fmask) {
+  const int64_t tot = (int64_t)F * Ntot * C;
+  const int FC = F * C;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % C);
+    const int64_t fi = idx / C;
+    const int f = (int)(fi / Ntot);
+    if (fmask && fmask[f]) continue;
+    const int fc = f * C + c;
+    if (step == 0) {
+      const float al = (float)cnr[3 * FC + fc];
+      const int64_t i = fi - (int64_t)f * Ntot;
+      const int i0 = (int)(i % N0);
+      const int64_t rest = i / N0;
+      const int i1 = (int)(rest % N1);
+      int64_t q = (int64_t)(i0 - mask_nh) * (i0 - mask_nh) + (int64_t)(i1 - mask_nh) * (i1 - mask_nh);
+      if (d == 3) { const int i2 = (int)(rest / N1); q += (int64_t)(i2 - mask_nh) * (i2 - mask_nh); }
+      if (q <= (int64_t)mask_nh * mask_nh) {
+        const float2 pp = phat[idx];
+        float2 ff = fhat[idx];
+        ff.x += al * pp.x;
+        ff.y += al * pp.y;
+        fhat[idx] = ff;
+      }
+    } else {
+      if (cnr[5 * FC + fc] != 0.0) {
+        const float be = (float)cnr[4 * FC + fc];
+        const float2 zz = z[idx];
+        float2 pp = phat[idx];
+        pp.x = be * pp.x + zz.x;
+        pp.y = be * pp.y + zz.y;
+        phat[idx] = pp;
+      }
+    }
+  }
+}
+
+void launch_cnr_update_image(const Plan& p, float2* fhat, float2* phat, const float2* z, int step, hipStream_t s) {
+  hipLaunchKernelGGL(k_cnr_update_image, dim3(2048), dim3(256), 0, s, fhat, phat, z, p.cnr, p.F, p.C, p.Ntot,
+                     (int)(0.5 * p.N[0]), p.d, p.N[0], p.N[1], step, p.fmask);
+}
+
+// r -= alpha v over the sorted samples
+__global__ void k_cnr_update_samples(float2* r, const float2* v, const double* cnr, const int* sframe, int F, int C,
+                                     int64_t Mtot, const int* fmask) {
+  const int64_t tot = Mtot * C;
+  const int FC = F * C;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = idx / C;
+    const int c = (int)(idx - s * C);
+    const int f = sframe[s];
+    if (fmask && fmask[f]) continue;
+    const float al = (float)cnr[3 * FC + f * C + c];
+    const float2 vv = v[idx];
+    float2 rr = r[idx];
+    rr.x -= al * vv.x;
+    rr.y -= al * vv.y;
+    r[idx] = rr;
+  }
+}
+
+void launch_cnr_update_samples(const Plan& p, float2* r, const float2* v, hipStream_t s) {
+  hipLaunchKernelGGL(k_cnr_update_samples, dim3(2048), dim3(256), 0, s, r, v, p.cnr, p.sframe, p.F, p.C, p.Mtot,
+                     p.fmask);
+}
+
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  (void)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
+}
+
+__global__ void k_fill_int(int* dst, int v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = v;
+}
+void launch_fill_int(int* dst, int value, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, s, dst, value, n);
+}
+
+}  // namespace cgs

@@ -1,0 +1,1146 @@
+// plan.hip -- host side of libcgsense_hip: plan construction, trajectory
+// presort, operator composition (E, E^H, CGNR, CG) and the C ABI.
+//
+// Reference counterparts (codeare source tree):
+//   plan        NFFT<T>::NFFT(Params)       src/matrix/ft/NFFT.hpp:66-191
+//               NCSENSE<T>::NCSENSE(Params) src/matrix/ft/NCSENSE.hpp:69-184
+//   trajectory  NFFT::KSpace / Weights      NFFT.hpp:258-296, NCSENSE.hpp:204-248
+//   E           NCSENSE::Trafo              NCSENSE.hpp:310-348
+//   E^H (ref)   NCSENSE::operator/          NCSENSE.hpp:251-301 -> NFFT::Adjoint (ITrafo)
+//   solve       NCSENSE::Adjoint -> CGLS    NCSENSE.hpp:408-414, CGLS.hpp:30-59
+// One plan replaces the C per-coil NFFT3 plans of NCSENSE: the trajectory and
+// window are stored once and every kernel is batched over coils and frames.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "cgs_internal.hpp"
+
+struct cgs_plan_s {
+  cgs::Plan p;
+};
+
+namespace cgs {
+
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+
+#define HIPCHK(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) {                                                          \
+      set_error(std::string(#x) + " failed: " + hipGetErrorString(e_));              \
+      return CGS_ERR_HIP;                                                            \
+    }                                                                                \
+  } while (0)
+
+#define CHK(x)                     \
+  do {                             \
+    int r_ = (x);                  \
+    if (r_ != CGS_OK) return r_;   \
+  } while (0)
+
+// ---- math for tables (FP64) --------------------------------------------------
+static double bessel_i0(double x) {
+  // power series sum_k (x^2/4)^k / (k!)^2
+  double q = 0.25 * x * x, term = 1.0, sum = 1.0;
+  for (int k = 1; k < 500; ++k) {
+    term *= q / ((double)k * k);
+    sum += term;
+    if (term < 1e-17 * sum) break;
+  }
+  return sum;
+}
+
+// Kaiser-Bessel window of NFFT3 in grid-cell units (NFFTTraits.hpp:665-673),
+// b = pi (2 - 1/sigma); continuous through |t| = m where it equals b/pi.
+static double kb_phi(double t, int m, double b) {
+  const double s2 = (double)m * m - t * t;
+  if (s2 > 0) { const double s = std::sqrt(s2); return std::sinh(b * s) / (M_PI * s); }
+  if (s2 < 0) { const double s = std::sqrt(-s2); return std::sin(b * s) / (M_PI * s); }
+  return b / M_PI;
+}
+
+static bool factorize(int n, int* radix, int* nst) {
+  int c2 = 0, r = n;
+  while (r % 2 == 0) { r /= 2; ++c2; }
+  int c3 = 0;
+  while (r % 3 == 0) { r /= 3; ++c3; }
+  if (r != 1) return false;
+  int k = 0;
+  while (c2 >= 3) { radix[k++] = 8; c2 -= 3; }
+  if (c2 == 2) radix[k++] = 4;
+  if (c2 == 1) radix[k++] = 2;
+  while (c3-- > 0) radix[k++] = 3;
+  if (k > 8) return false;
+  *nst = k;
+  return true;
+}
+
+template <typename T>
+static int dalloc(T** ptr, size_t count) {
+  *ptr = nullptr;
+  if (count == 0) return CGS_OK;
+  hipError_t e = hipMalloc((void**)ptr, sizeof(T) * count);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc of ") + std::to_string(sizeof(T) * count) + " bytes failed: " +
+              hipGetErrorString(e));
+    *ptr = nullptr;
+    return CGS_ERR_ALLOC;
+  }
+  return CGS_OK;
+}
+template <typename T>
+static void dfree(T*& ptr) {
+  if (ptr) (void)hipFree((void*)ptr);
+  ptr = nullptr;
+}
+
+// ---- profiling -----------------------------------------------------------------
+struct Prof {
+  Plan& p;
+  int kind;
+  double bytes;
+  hipEvent_t a = nullptr, b = nullptr;
+  Prof(Plan& p_, int kind_, double bytes_) : p(p_), kind(kind_), bytes(bytes_) {
+    if (!(p.flags & CGS_PROFILE)) return;
+    a = take();
+    b = take();
+    (void)hipEventRecord(a, p.stream);
+  }
+  ~Prof() {
+    if (!(p.flags & CGS_PROFILE)) return;
+    (void)hipEventRecord(b, p.stream);
+    p.pending.push_back({a, b, kind, bytes});
+  }
+  hipEvent_t take() {
+    if (!p.event_pool.empty()) {
+      hipEvent_t e = p.event_pool.back();
+      p.event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+};
+
+static int drain_profile(Plan& p) {
+  if (p.pending.empty()) return CGS_OK;
+  HIPCHK(hipStreamSynchronize(p.stream));
+  for (auto& q : p.pending) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, q.a, q.b);
+    p.stat[q.kind].launches += 1;
+    p.stat[q.kind].ms += ms;
+    p.stat[q.kind].bytes += q.bytes;
+    p.event_pool.push_back(q.a);
+    p.event_pool.push_back(q.b);
+  }
+  p.pending.clear();
+  return CGS_OK;
+}
+
+// ---- FFT passes ------------------------------------------------------------------
+static int pick_lb(int64_t inner, int n) {
+  int lb = 1;
+  while (lb * 2 <= kFftMaxElem / n && lb * 2 <= inner && lb * 2 <= 64) lb *= 2;
+  return lb;
+}
+
+static FftArgs fft_base(const Plan& p, int axis, int sign) {
+  FftArgs a{};
+  a.n = p.n[axis];
+  a.sign = sign;
+  a.tw = p.tw[axis];
+  a.nst = p.nst[axis];
+  for (int k = 0; k < 8; ++k) a.radix[k] = p.radix[axis][k];
+  a.dinv0 = p.dinv[0];
+  a.dinv1 = p.dinv[1];
+  a.dinv2 = p.d == 3 ? p.dinv[2] : p.dinv[1];
+  a.C = p.C;
+  a.d = p.d;
+  a.N0 = p.N[0];
+  a.N1 = p.N[1];
+  a.Ntot = p.Ntot;
+  a.S = p.S;
+  a.mask_nh = (int)(0.5 * p.N[0]);
+  return a;
+}
+
+static void fft_finish(FftArgs& a) {
+  a.lb = pick_lb(a.inner, a.n);
+  a.ldn = a.n + 1;
+}
+
+// image (mode 1: x .* S_c; 2: X; 3: X .* w_hat) -> grid, forward transform
+static int fft_forward(Plan& p, int mode, const float2* src, const int* fl) {
+  const int d = p.d, C = p.C, F = p.F;
+  const int64_t N0 = p.N[0], N1 = p.N[1], N2 = p.N[2], n0 = p.n[0], n1 = p.n[1], n2 = p.n[2];
+  double bytes = (mode == 1 ? (double)(1 + C) * p.Ntot * 8 : 2.0 * C * p.Ntot * 8) * F;
+  // pass 1, axis 0
+  {
+    FftArgs a = fft_base(p, 0, -1);
+    a.in = mode == 1 ? nullptr : src;
+    a.x = mode == 1 ? src : nullptr;
+    a.out = (d == 2) ? p.t1 : p.t1;
+    a.nin = (int)N0; a.nout = (int)n0;
+    a.inner = C;
+    a.outer_per_frame = (d == 2) ? N1 : N1 * N2;
+    a.outers = a.outer_per_frame * F;
+    a.fdone = fl;
+    fft_finish(a);
+    const double b = bytes + (double)C * n0 * a.outer_per_frame * F * 8;
+    Prof pr(p, CGS_K_FFT_FWD, b);
+    launch_fft(a, mode, 0, p.stream);
+  }
+  // pass 2, axis 1
+  {
+    FftArgs a = fft_base(p, 1, -1);
+    a.in = p.t1;
+    a.out = (d == 2) ? p.grid : p.t2;
+    a.nin = (int)N1; a.nout = (int)n1;
+    a.inner = C * n0;
+    a.outer_per_frame = (d == 2) ? 1 : N2;
+    a.outers = a.outer_per_frame * F;
+    a.fdone = fl;
+    fft_finish(a);
+    const double b = (double)C * n0 * (N1 + n1) * a.outer_per_frame * F * 8;
+    Prof pr(p, CGS_K_FFT_FWD, b);
+    launch_fft(a, 0, 0, p.stream);
+  }
+  if (d == 3) {
+    FftArgs a = fft_base(p, 2, -1);
+    a.in = p.t2;
+    a.out = p.grid;
+    a.nin = (int)N2; a.nout = (int)n2;
+    a.inner = C * n0 * n1;
+    a.outer_per_frame = 1;
+    a.outers = F;
+    a.fdone = fl;
+    fft_finish(a);
+    const double b = (double)C * n0 * n1 * (N2 + n2) * F * 8;
+    Prof pr(p, CGS_K_FFT_FWD, b);
+    launch_fft(a, 0, 0, p.stream);
+  }
+  return CGS_OK;
+}
+
+// grid -> per-coil images X (cropped, deapodised), adjoint transform
+static int fft_adjoint(Plan& p, float2* X, const int* fl) {
+  const int d = p.d, C = p.C, F = p.F;
+  const int64_t N0 = p.N[0], N1 = p.N[1], N2 = p.N[2], n0 = p.n[0], n1 = p.n[1], n2 = p.n[2];
+  if (d == 2) {
+    {
+      FftArgs a = fft_base(p, 1, +1);
+      a.in = p.grid; a.out = p.t1;
+      a.nin = (int)n1; a.nout = (int)N1;
+      a.inner = C * n0; a.outer_per_frame = 1; a.outers = F; a.fdone = fl;
+      fft_finish(a);
+      Prof pr(p, CGS_K_FFT_ADJ, (double)C * n0 * (n1 + N1) * F * 8);
+      launch_fft(a, 0, 0, p.stream);
+    }
+    {
+      FftArgs a = fft_base(p, 0, +1);
+      a.in = p.t1; a.out = X;
+      a.nin = (int)n0; a.nout = (int)N0;
+      a.inner = C; a.outer_per_frame = N1; a.outers = N1 * F; a.fdone = fl;
+      fft_finish(a);
+      Prof pr(p, CGS_K_FFT_ADJ, (double)C * (n0 + N0) * N1 * F * 8);
+      launch_fft(a, 0, 1, p.stream);
+    }
+  } else {
+    {
+      FftArgs a = fft_base(p, 2, +1);
+      a.in = p.grid; a.out = p.t2;
+      a.nin = (int)n2; a.nout = (int)N2;
+      a.inner = C * n0 * n1; a.outer_per_frame = 1; a.outers = F; a.fdone = fl;
+      fft_finish(a);
+      Prof pr(p, CGS_K_FFT_ADJ, (double)C * n0 * n1 * (n2 + N2) * F * 8);
+      launch_fft(a, 0, 0, p.stream);
+    }
+    {
+      FftArgs a = fft_base(p, 1, +1);
+      a.in = p.t2; a.out = p.t1;
+      a.nin = (int)n1; a.nout = (int)N1;
+      a.inner = C * n0; a.outer_per_frame = N2; a.outers = N2 * F; a.fdone = fl;
+      fft_finish(a);
+      Prof pr(p, CGS_K_FFT_ADJ, (double)C * n0 * (n1 + N1) * N2 * F * 8);
+      launch_fft(a, 0, 0, p.stream);
+    }
+    {
+      FftArgs a = fft_base(p, 0, +1);
+      a.in = p.t1; a.out = X;
+      a.nin = (int)n0; a.nout = (int)N0;
+      a.inner = C; a.outer_per_frame = N1 * N2; a.outers = N1 * N2 * F; a.fdone = fl;
+      fft_finish(a);
+      Prof pr(p, CGS_K_FFT_ADJ, (double)C * (n0 + N0) * N1 * N2 * F * 8);
+      launch_fft(a, 0, 1, p.stream);
+    }
+  }
+  return CGS_OK;
+}
+
+// ---- window gather / scatter --------------------------------------------------------
+static int interp(Plan& p, float2* y, const int* fl) {
+  SampArgs a{};
+  a.base = p.base; a.psi = p.psi; a.w = nullptr;
+  a.d = p.d; a.taps = p.taps; a.C = p.C;
+  a.n0 = p.n[0]; a.n1 = p.n[1]; a.n2 = p.n[2];
+  a.ntot = p.ntot;
+  a.grid = p.grid; a.y = y;
+  a.items = p.interp_items; a.nitems = p.n_interp_items;
+  a.fdone = fl;
+  const double bytes = (double)p.Mtot * p.d * 4 + (double)p.C * p.ntot * p.F * 8 + (double)p.C * p.Mtot * 8;
+  Prof pr(p, CGS_K_INTERP, bytes);
+  launch_interp(a, p.interp_cg, p.stream);
+  return CGS_OK;
+}
+
+static int gridding(Plan& p, const float2* y, bool weighted, const int* fl) {
+  GridArgs a{};
+  a.base = p.base; a.psi = p.psi; a.w = weighted ? p.w : nullptr;
+  a.y = y; a.grid = p.grid; a.partial = p.partial;
+  a.items = p.grid_items; a.visits = p.visits; a.fdone = fl;
+  a.d = p.d; a.taps = p.taps; a.C = p.C; a.T = p.T;
+  a.CGW = p.grid_cg * p.grid_w;
+  a.rowstride = p.T * a.CGW + ((p.grid_cg == 8) ? 8 : 0);
+  a.n0 = p.n[0]; a.n1 = p.n[1]; a.n2 = p.n[2];
+  a.nt0 = p.nt[0]; a.nt1 = p.nt[1]; a.nt2 = p.nt[2];
+  a.ntot = p.ntot;
+  a.tilecells = (int64_t)p.T * p.T * (p.d == 3 ? p.T : 1);
+  const int ngroups = (p.C + a.CGW - 1) / a.CGW;
+  const double bytes = (double)p.Mtot * p.d * 4 + (double)p.Mtot * 4 + (double)p.C * p.Mtot * 8 +
+                       (double)p.C * p.ntot * p.F * 8;
+  Prof pr(p, CGS_K_GRID, bytes);
+  launch_grid(a, p.grid_cg, p.grid_w, p.n_grid_items, ngroups, p.stream);
+  launch_grid_reduce(p, p.partial, p.grid, p.stream);
+  return CGS_OK;
+}
+
+static double coil_bytes(const Plan& p, bool lam) {
+  return ((double)2 * p.C * p.Ntot * 8 + (double)p.Ntot * 8 * (lam ? 2 : 1)) * p.F;
+}
+
+// ---- NCSENSE operators ----------------------------------------------------------------
+// E: y_int = B F D (S_c .* x) for all coils and frames
+static int op_E(Plan& p, const float2* x, float2* y, const int* fl) {
+  CHK(fft_forward(p, 1, x, fl));
+  CHK(interp(p, y, fl));
+  return CGS_OK;
+}
+
+// true E^H: x = sum_c conj(S_c) D F^H B^T (w .* y_c) [+ lambda pv, partial <pv, x>]
+static int op_EH_true(Plan& p, const float2* y, float2* x, const float2* pv, float lambda,
+                      double* partials, const int* fl) {
+  CHK(gridding(p, y, true, fl));
+  CHK(fft_adjoint(p, p.Xc, fl));
+  Prof pr(p, CGS_K_COIL, coil_bytes(p, pv != nullptr));
+  launch_coil_combine(p, p.Xc, x, pv, lambda, false, partials, p.stream);
+  return CGS_OK;
+}
+
+// NFFT3 weighted damped CGNR (NFFTTraits.hpp:122-138) for every (frame, coil):
+// f_hat = ITrafo(y), written to p.Xc2.  Frames flagged in fl are skipped.
+static int cgnr(Plan& p, const float2* y, int maxit, double eps, const int* fl) {
+  const size_t Fb = sizeof(int) * p.F;
+  if (fl && fl != p.fmask) HIPCHK(hipMemcpyAsync(p.fmask, fl, Fb, hipMemcpyDeviceToDevice, p.stream));
+  if (!fl) launch_fill_int(p.fmask, 0, p.F, p.stream);
+  const int* m = p.fmask;
+  const size_t ybytes = sizeof(float2) * (size_t)p.Mtot * p.C;
+  const size_t xbytes = sizeof(float2) * (size_t)p.F * p.Ntot * p.C;
+  const int FC = p.F * p.C;
+  float2* r = p.ybuf[2];
+  float2* v = p.ybuf[3];
+  {
+    Prof pr(p, CGS_K_CG, 2.0 * ybytes);
+    HIPCHK(hipMemcpyAsync(r, y, ybytes, hipMemcpyDeviceToDevice, p.stream));   // r = y - A 0
+    launch_cnr_scalars(p, 3, eps, p.stream);
+    launch_cnr_dot_samples(p, r, p.cnr + 0 * FC, p.stream);                    // dot_r
+  }
+  CHK(gridding(p, r, true, m));                                                 // z = A^H W r
+  CHK(fft_adjoint(p, p.Xc, m));
+  {
+    Prof pr(p, CGS_K_CG, 3.0 * xbytes);
+    launch_cnr_dot_image(p, p.Xc, true, p.cnr + 1 * FC, p.stream);             // dot_z
+    HIPCHK(hipMemcpyAsync(p.Xc3, p.Xc, xbytes, hipMemcpyDeviceToDevice, p.stream));   // p = z
+    HIPCHK(hipMemsetAsync(p.Xc2, 0, xbytes, p.stream));                          // f_hat = 0
+  }
+  for (int l = 0; l < maxit; ++l) {
+    launch_cnr_scalars(p, 0, eps, p.stream);                                   // dot_r < eps -> stop
+    CHK(fft_forward(p, 3, p.Xc3, m));                                           // v = A(w_hat p)
+    CHK(interp(p, v, m));
+    {
+      Prof pr(p, CGS_K_CG, 4.0 * xbytes + 4.0 * ybytes);
+      launch_cnr_dot_samples(p, v, p.cnr + 2 * FC, p.stream);                  // dot_v
+      launch_cnr_scalars(p, 1, eps, p.stream);                                 // alpha
+      launch_cnr_update_image(p, p.Xc2, p.Xc3, nullptr, 0, p.stream);          // f_hat += a w_hat p
+      launch_cnr_update_samples(p, r, v, p.stream);                            // r -= a v
+      launch_cnr_dot_samples(p, r, p.cnr + 0 * FC, p.stream);                  // dot_r
+    }
+    CHK(gridding(p, r, true, m));                                               // z = A^H W r
+    CHK(fft_adjoint(p, p.Xc, m));
+    {
+      Prof pr(p, CGS_K_CG, 4.0 * xbytes);
+      launch_cnr_dot_image(p, p.Xc, true, p.cnr + 6 * FC, p.stream);           // dot_z (new)
+      launch_cnr_scalars(p, 2, eps, p.stream);                                 // beta
+      launch_cnr_update_image(p, nullptr, p.Xc3, p.Xc, 1, p.stream);           // p = b p + z
+    }
+  }
+  return CGS_OK;
+}
+
+// reference E^H (NCSENSE::operator/): x = I .* sum_c conj(S_c) .* ITrafo_c(y_c)
+static int op_EH_ref(Plan& p, const float2* y, float2* x, int ftiter, double fteps, const float2* pv,
+                     float lambda, double* partials, const int* fl) {
+  CHK(cgnr(p, y, ftiter, fteps, fl));
+  Prof pr(p, CGS_K_COIL, coil_bytes(p, pv != nullptr));
+  launch_coil_combine(p, p.Xc2, x, pv, lambda, true, partials, p.stream);
+  return CGS_OK;
+}
+
+static int op_EH(Plan& p, int mode, const float2* y, float2* x, int ftiter, double fteps, const float2* pv,
+                 float lambda, double* partials, const int* fl) {
+  if (mode == CGS_EH_REF) return op_EH_ref(p, y, x, ftiter, fteps, pv, lambda, partials, fl);
+  return op_EH_true(p, y, x, pv, lambda, partials, fl);
+}
+
+// ---- trajectory presort -----------------------------------------------------------------
+static void free_samples(Plan& p) {
+  dfree(p.base); dfree(p.psi); dfree(p.perm); dfree(p.sframe);
+  dfree(p.interp_items); dfree(p.grid_items); dfree(p.reduce_items); dfree(p.visits); dfree(p.partial);
+  p.n_interp_items = p.n_grid_items = p.n_reduce_items = 0;
+  p.n_visits = 0;
+  p.n_slots = 0;
+}
+
+static int build_samples(Plan& p, const float* k) {
+  const int d = p.d, m = p.m, taps = p.taps, T = p.T;
+  const int64_t M = p.Mtot;
+  // base cell and fractional offset of every sample (FP64)
+  std::vector<int> cell((size_t)M * d);
+  std::vector<double> frac((size_t)M * d);
+  std::vector<int> frame((size_t)M);
+  for (int f = 0; f < p.F; ++f)
+    for (int64_t j = p.foff[f]; j < p.foff[f + 1]; ++j) frame[j] = f;
+  for (int64_t j = 0; j < M; ++j) {
+    for (int a = 0; a < d; ++a) {
+      const double x = (double)k[(d - 1 - a) + (int64_t)d * j];   // k(t,j) -> image axis d-1-t
+      if (!(x >= -0.5 && x < 0.5)) {
+        set_error("trajectory value outside [-0.5, 0.5) at sample " + std::to_string(j));
+        return CGS_ERR_ARG;
+      }
+      const double u = p.n[a] * x;
+      const double fl = std::floor(u);
+      int c = (int)((long)fl % p.n[a]);
+      if (c < 0) c += p.n[a];
+      cell[j * d + a] = c;
+      frac[j * d + a] = u - fl;
+    }
+  }
+  // counting sort by (frame, tile of the base cell)
+  const int64_t tpf = p.tiles_per_frame;
+  const int64_t nkeys = tpf * p.F;
+  std::vector<int64_t> key((size_t)M);
+  std::vector<int64_t> cnt((size_t)nkeys + 1, 0);
+  for (int64_t j = 0; j < M; ++j) {
+    int64_t t = 0, mul = 1;
+    for (int a = 0; a < d; ++a) { t += (int64_t)(cell[j * d + a] / T) * mul; mul *= p.nt[a]; }
+    key[j] = frame[j] * tpf + t;
+    cnt[key[j] + 1]++;
+  }
+  for (int64_t q = 0; q < nkeys; ++q) cnt[q + 1] += cnt[q];
+  std::vector<int> perm((size_t)M);
+  {
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t j = 0; j < M; ++j) perm[pos[key[j]]++] = (int)j;
+  }
+  // sorted base (cell - m mod n) and window values
+  std::vector<int> hbase((size_t)M * d);
+  std::vector<float> hpsi((size_t)M * d * taps);
+  std::vector<int> hframe((size_t)M);
+  {
+    const int nthr = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthr; ++t) {
+      th.emplace_back([&, t]() {
+        for (int64_t s = t; s < M; s += nthr) {
+          const int64_t j = perm[s];
+          hframe[s] = frame[j];
+          for (int a = 0; a < d; ++a) {
+            int b = cell[j * d + a] - m;
+            b %= p.n[a];
+            if (b < 0) b += p.n[a];
+            hbase[s * d + a] = b;
+            for (int q = 0; q < taps; ++q)
+              hpsi[(s * d + a) * taps + q] = (float)kb_phi(frac[j * d + a] + m - q, m, p.b[a]);
+          }
+        }
+      });
+    }
+    for (auto& x : th) x.join();
+  }
+  // interp items: runs of samples of one frame
+  const int ipw = (d == 3) ? 4 : 16;
+  std::vector<int4> items;
+  for (int f = 0; f < p.F; ++f)
+    for (int64_t s = p.foff[f]; s < p.foff[f + 1]; s += ipw)
+      items.push_back(make_int4(f, (int)s, (int)std::min<int64_t>(ipw, p.foff[f + 1] - s), 0));
+  // gridding visit lists: every (tile, sample) pair whose footprint meets the tile
+  std::vector<int64_t> vcnt((size_t)nkeys + 1, 0);
+  auto tiles_of = [&](int64_t s, int a, int* out) {
+    int nout = 0;
+    for (int q = 0; q < taps; ++q) {
+      int c = hbase[s * d + a] + q;
+      if (c >= p.n[a]) c -= p.n[a];
+      const int t = c / T;
+      bool seen = false;
+      for (int z = 0; z < nout; ++z) seen |= (out[z] == t);
+      if (!seen) out[nout++] = t;
+    }
+    return nout;
+  };
+  auto for_visits = [&](auto&& fn) {
+    int ta[3][16], na[3] = {1, 1, 1};
+    ta[1][0] = ta[2][0] = 0;
+    for (int64_t s = 0; s < M; ++s) {
+      for (int a = 0; a < d; ++a) na[a] = tiles_of(s, a, ta[a]);
+      const int64_t fbase = (int64_t)hframe[s] * tpf;
+      for (int z2 = 0; z2 < na[2]; ++z2)
+        for (int z1 = 0; z1 < na[1]; ++z1)
+          for (int z0 = 0; z0 < na[0]; ++z0) {
+            const int64_t t = ta[0][z0] + (int64_t)p.nt[0] * (ta[1][z1] + (int64_t)p.nt[1] * ta[2][z2]);
+            fn(fbase + t, s);
+          }
+    }
+  };
+  for_visits([&](int64_t kk, int64_t) { vcnt[kk + 1]++; });
+  for (int64_t q = 0; q < nkeys; ++q) vcnt[q + 1] += vcnt[q];
+  const int64_t nvis = vcnt[nkeys];
+  if (nvis > INT32_MAX) {
+    set_error("too many gridding visits for 32-bit indices");
+    return CGS_ERR_UNSUPPORTED;
+  }
+  std::vector<int> vis((size_t)nvis);
+  {
+    std::vector<int64_t> pos(vcnt.begin(), vcnt.end() - 1);
+    for_visits([&](int64_t kk, int64_t s) { vis[pos[kk]++] = (int)s; });
+  }
+  std::vector<GridItem> gitems;
+  std::vector<ReduceItem> ritems;
+  int slots = 0;
+  const int K = p.visits_per_item;
+  for (int64_t kk = 0; kk < nkeys; ++kk) {
+    const int f = (int)(kk / tpf), t = (int)(kk % tpf);
+    const int64_t b0 = vcnt[kk], b1 = vcnt[kk + 1], nv = b1 - b0;
+    if (nv <= K) {
+      gitems.push_back(GridItem{f, t, (int)b0, (int)b1, -1, 0, 0, 0});
+    } else {
+      const int nch = (int)((nv + K - 1) / K);
+      ritems.push_back(ReduceItem{f, t, slots, nch});
+      for (int c = 0; c < nch; ++c) {
+        const int64_t c0 = b0 + (int64_t)c * nv / nch, c1 = b0 + (int64_t)(c + 1) * nv / nch;
+        gitems.push_back(GridItem{f, t, (int)c0, (int)c1, slots + c, 0, 0, 0});
+      }
+      slots += nch;
+    }
+  }
+  // longest items first
+  std::stable_sort(gitems.begin(), gitems.end(),
+                   [](const GridItem& x, const GridItem& y) { return (x.vend - x.vbeg) > (y.vend - y.vbeg); });
+
+  free_samples(p);
+  CHK(dalloc(&p.base, hbase.size()));
+  CHK(dalloc(&p.psi, hpsi.size()));
+  CHK(dalloc(&p.perm, perm.size()));
+  CHK(dalloc(&p.sframe, hframe.size()));
+  CHK(dalloc(&p.interp_items, items.size()));
+  CHK(dalloc(&p.grid_items, gitems.size()));
+  CHK(dalloc(&p.reduce_items, ritems.size()));
+  CHK(dalloc(&p.visits, vis.size()));
+  const int64_t tilecells = (int64_t)T * T * (d == 3 ? T : 1);
+  CHK(dalloc(&p.partial, (size_t)slots * tilecells * p.C));
+  HIPCHK(hipMemcpy(p.base, hbase.data(), sizeof(int) * hbase.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.psi, hpsi.data(), sizeof(float) * hpsi.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.perm, perm.data(), sizeof(int) * perm.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.sframe, hframe.data(), sizeof(int) * hframe.size(), hipMemcpyHostToDevice));
+  if (!items.empty())
+    HIPCHK(hipMemcpy(p.interp_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
+  if (!gitems.empty())
+    HIPCHK(hipMemcpy(p.grid_items, gitems.data(), sizeof(GridItem) * gitems.size(), hipMemcpyHostToDevice));
+  if (!ritems.empty())
+    HIPCHK(hipMemcpy(p.reduce_items, ritems.data(), sizeof(ReduceItem) * ritems.size(), hipMemcpyHostToDevice));
+  if (!vis.empty()) HIPCHK(hipMemcpy(p.visits, vis.data(), sizeof(int) * vis.size(), hipMemcpyHostToDevice));
+  p.n_interp_items = (int)items.size();
+  p.n_grid_items = (int)gitems.size();
+  p.n_reduce_items = (int)ritems.size();
+  p.n_visits = nvis;
+  p.n_slots = slots;
+  p.h_perm = std::move(perm);
+  return CGS_OK;
+}
+
+// ---- plan life cycle --------------------------------------------------------------------
+static void destroy_plan(Plan& p) {
+  if (p.stream) (void)hipStreamSynchronize(p.stream);
+  free_samples(p);
+  for (int a = 0; a < 3; ++a) { dfree(p.tw[a]); dfree(p.dinv[a]); }
+  dfree(p.d_foff); dfree(p.S); dfree(p.Imap); dfree(p.w);
+  dfree(p.grid); dfree(p.t1); dfree(p.t2); dfree(p.Xc); dfree(p.Xc2); dfree(p.Xc3);
+  for (auto& b : p.ybuf) dfree(b);
+  for (auto& b : p.img) dfree(b);
+  dfree(p.ext_a); dfree(p.ext_b); dfree(p.partials); dfree(p.cgst); dfree(p.cnr);
+  dfree(p.fdone); dfree(p.fmask); dfree(p.hist);
+  if (p.h_cgst) (void)hipHostFree(p.h_cgst);
+  if (p.h_done) (void)hipHostFree(p.h_done);
+  p.h_cgst = nullptr; p.h_done = nullptr;
+  for (auto& q : p.pending) { p.event_pool.push_back(q.a); p.event_pool.push_back(q.b); }
+  p.pending.clear();
+  for (auto e : p.event_pool) (void)hipEventDestroy(e);
+  p.event_pool.clear();
+  if (p.own_stream && p.stream) (void)hipStreamDestroy(p.stream);
+  p.stream = nullptr;
+}
+
+static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma, int m, int C, int F,
+                       const int64_t* foff, unsigned flags) {
+  if (d != 2 && d != 3) { set_error("d must be 2 or 3"); return CGS_ERR_UNSUPPORTED; }
+  if (!N || Mtot < 1 || C < 1 || F < 1 || !(sigma >= 1.0)) { set_error("bad plan arguments"); return CGS_ERR_ARG; }
+  if (m < 1 || m > 4) { set_error("kernel cut-off m must be 1..4"); return CGS_ERR_UNSUPPORTED; }
+  if (C > 512) { set_error("at most 512 coils"); return CGS_ERR_UNSUPPORTED; }
+  p.d = d; p.m = m; p.taps = 2 * m + 2; p.C = C; p.F = F; p.Mtot = Mtot; p.flags = flags;
+  p.Ntot = 1; p.ntot = 1;
+  for (int a = 0; a < d; ++a) {
+    if (N[a] < 2 || (N[a] & 1)) { set_error("image sides must be even and >= 2"); return CGS_ERR_ARG; }
+    p.N[a] = N[a];
+    p.n[a] = (int)std::ceil(sigma * (double)N[a]);   // NFFT.hpp:108-109
+    if (!factorize(p.n[a], p.radix[a], &p.nst[a])) {
+      set_error("oversampled size " + std::to_string(p.n[a]) + " is not 2^a 3^b");
+      return CGS_ERR_UNSUPPORTED;
+    }
+    if (p.n[a] < 2 * p.taps) { set_error("oversampled grid smaller than two window widths"); return CGS_ERR_UNSUPPORTED; }
+    if (p.n[a] > 4096) { set_error("oversampled side above 4096"); return CGS_ERR_UNSUPPORTED; }
+    p.Ntot *= N[a];
+    p.ntot *= p.n[a];
+    const double s = (double)p.n[a] / (double)N[a];
+    p.b[a] = M_PI * (2.0 - 1.0 / s);
+  }
+  p.foff.assign(F + 1, 0);
+  if (F == 1) { p.foff[1] = Mtot; }
+  else {
+    if (!foff) { set_error("frame_offsets required for several frames"); return CGS_ERR_ARG; }
+    for (int f = 0; f <= F; ++f) p.foff[f] = foff[f];
+    if (p.foff[0] != 0 || p.foff[F] != Mtot) { set_error("frame_offsets must run from 0 to M_total"); return CGS_ERR_ARG; }
+    for (int f = 0; f < F; ++f)
+      if (p.foff[f + 1] < p.foff[f]) { set_error("frame_offsets must not decrease"); return CGS_ERR_ARG; }
+  }
+  if (Mtot > INT32_MAX / 2) { set_error("too many samples"); return CGS_ERR_UNSUPPORTED; }
+  // tiles and kernel shapes
+  p.T = (d == 3) ? 8 : 8;
+  for (int a = 0; a < 3; ++a) p.nt[a] = (a < d) ? (p.n[a] + p.T - 1) / p.T : 1;
+  p.tiles_per_frame = (int64_t)p.nt[0] * p.nt[1] * p.nt[2];
+  p.grid_cg = 8;
+  p.grid_w = std::max(1, std::min(4, (C + 7) / 8));
+  p.interp_cg = (C <= 8) ? 8 : (C <= 16 ? 16 : 32);
+  p.visits_per_item = (d == 3) ? 64 : 256;
+
+  if (flags & 0x8000u) {   // internal: caller-provided stream set later
+  }
+  HIPCHK(hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking));
+  p.own_stream = true;
+
+  // tables
+  for (int a = 0; a < d; ++a) {
+    std::vector<float2> tw(p.n[a]);
+    for (int q = 0; q < p.n[a]; ++q) {
+      const double ang = -2.0 * M_PI * (double)q / (double)p.n[a];
+      tw[q] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    CHK(dalloc(&p.tw[a], tw.size()));
+    HIPCHK(hipMemcpy(p.tw[a], tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
+    std::vector<float> di(p.N[a]);
+    for (int i = 0; i < p.N[a]; ++i) {
+      const double kk = 2.0 * M_PI * (double)(i - p.N[a] / 2) / (double)p.n[a];
+      const double s2 = p.b[a] * p.b[a] - kk * kk;
+      di[i] = (float)(1.0 / bessel_i0(m * std::sqrt(s2 > 0 ? s2 : 0.0)));   // 1/phi_hat
+    }
+    CHK(dalloc(&p.dinv[a], di.size()));
+    HIPCHK(hipMemcpy(p.dinv[a], di.data(), sizeof(float) * di.size(), hipMemcpyHostToDevice));
+  }
+  CHK(dalloc(&p.d_foff, (size_t)F + 1));
+  HIPCHK(hipMemcpy(p.d_foff, p.foff.data(), sizeof(int64_t) * (F + 1), hipMemcpyHostToDevice));
+
+  // buffers
+  const size_t FC = (size_t)F * C;
+  CHK(dalloc(&p.grid, FC * p.ntot));
+  size_t t1n = (size_t)C * p.n[0] * p.N[1] * (d == 3 ? p.N[2] : 1) * F;
+  CHK(dalloc(&p.t1, t1n));
+  if (d == 3) CHK(dalloc(&p.t2, (size_t)C * p.n[0] * p.n[1] * p.N[2] * F));
+  CHK(dalloc(&p.Xc, FC * p.Ntot));
+  CHK(dalloc(&p.Xc2, FC * p.Ntot));
+  CHK(dalloc(&p.Xc3, FC * p.Ntot));
+  for (auto& b : p.ybuf) CHK(dalloc(&b, (size_t)Mtot * C));
+  for (auto& b : p.img) CHK(dalloc(&b, (size_t)F * p.Ntot));
+  CHK(dalloc(&p.w, (size_t)Mtot));
+  CHK(dalloc(&p.S, (size_t)p.Ntot * C));
+  CHK(dalloc(&p.Imap, (size_t)p.Ntot));
+  p.n_partials = std::max<size_t>((size_t)2 * 512 * F, (size_t)256 * F * C);
+  CHK(dalloc(&p.partials, p.n_partials));
+  CHK(dalloc(&p.cgst, (size_t)F));
+  CHK(dalloc(&p.cnr, (size_t)8 * F * C));
+  CHK(dalloc(&p.fdone, (size_t)F));
+  CHK(dalloc(&p.fmask, (size_t)F));
+  HIPCHK(hipMemset(p.fdone, 0, sizeof(int) * F));
+  HIPCHK(hipMemset(p.fmask, 0, sizeof(int) * F));
+  HIPCHK(hipHostMalloc((void**)&p.h_cgst, sizeof(CgState) * F, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&p.h_done, sizeof(int) * 2 * F, hipHostMallocDefault));
+  if (!(flags & CGS_DEVICE_IO)) {
+    p.ext_a_bytes = sizeof(float2) * (size_t)Mtot * C;
+    p.ext_b_bytes = sizeof(float2) * (size_t)p.Ntot * std::max(F, C);
+    CHK(dalloc((float2**)&p.ext_a, (size_t)Mtot * C));
+    CHK(dalloc((float2**)&p.ext_b, (size_t)p.Ntot * std::max(F, C)));
+  }
+  return CGS_OK;
+}
+
+static int require_ready(const Plan& p, bool need_S) {
+  if (!p.have_k) { set_error("trajectory not set (cgs_set_trajectory)"); return CGS_ERR_STATE; }
+  if (!p.have_w) { set_error("weights not set (cgs_set_weights)"); return CGS_ERR_STATE; }
+  if (need_S && !p.have_S) { set_error("sensitivities not set (cgs_set_sensitivities)"); return CGS_ERR_STATE; }
+  return CGS_OK;
+}
+
+// frame mask with only `frame` active
+static void mask_one_frame(Plan& p, int frame) {
+  launch_fill_int(p.fmask, 1, p.F, p.stream);
+  launch_fill_int(p.fmask + frame, 0, 1, p.stream);
+}
+
+// stage host input into a device buffer (or pass a device pointer through)
+static int stage_in(Plan& p, const void* src, float2* staging, size_t bytes, const float2** out) {
+  if (p.flags & CGS_DEVICE_IO) { *out = (const float2*)src; return CGS_OK; }
+  HIPCHK(hipMemcpyAsync(staging, src, bytes, hipMemcpyHostToDevice, p.stream));
+  *out = staging;
+  return CGS_OK;
+}
+static int stage_out(Plan& p, void* dst, const float2* dev, size_t bytes) {
+  if (p.flags & CGS_DEVICE_IO) return CGS_OK;
+  HIPCHK(hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToHost, p.stream));
+  HIPCHK(hipStreamSynchronize(p.stream));
+  return CGS_OK;
+}
+
+static int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string("kernel launch failed: ") + hipGetErrorString(e));
+    return CGS_ERR_HIP;
+  }
+  return CGS_OK;
+}
+
+// ---- CG solve -------------------------------------------------------------------------
+static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, const cgs_solve_opts& o,
+                 cgs_solve_report* rep) {
+  const int F = p.F;
+  const size_t ib = sizeof(float2) * (size_t)F * p.Ntot;
+  float2 *x = p.img[0], *r = p.img[1], *pv = p.img[2], *q = p.img[3];
+  double* partA = p.partials;
+  double* partB = p.partials + 512 * F;
+  const int hcap = o.maxit + 1;
+  if (hcap > p.hist_cap) {
+    dfree(p.hist);
+    CHK(dalloc(&p.hist, (size_t)F * hcap));
+    p.hist_cap = hcap;
+  }
+  HIPCHK(hipMemsetAsync(p.hist, 0, sizeof(double) * F * p.hist_cap, p.stream));
+  HIPCHK(hipMemsetAsync(p.fdone, 0, sizeof(int) * F, p.stream));
+  {
+    Prof pr(p, CGS_K_OTHER, 2.0 * sizeof(float2) * p.Mtot * p.C);
+    launch_y_ext2int(p, y_ext, p.ybuf[0], p.C, -1, p.stream);
+  }
+  // b = E^H y  -> r
+  CHK(op_EH(p, o.eh_mode, p.ybuf[0], r, o.ftiter, o.fteps, nullptr, 0.f, nullptr, p.fdone));
+  const bool warm = x0 != nullptr;
+  if (warm) {
+    HIPCHK(hipMemcpyAsync(x, x0, ib, hipMemcpyDeviceToDevice, p.stream));
+    CHK(op_E(p, x, p.ybuf[1], p.fdone));
+    CHK(op_EH(p, o.eh_mode, p.ybuf[1], q, o.ftiter, o.fteps, x, (float)o.lambda, nullptr, p.fdone));
+    launch_sqnorm_partials(p, r, partA, p.stream);             // ||b||^2
+    launch_axpby_img(p, r, r, 1.f, q, -1.f, p.stream);          // r = b - (A + lambda) x0
+    launch_sqnorm_partials(p, r, partB, p.stream);
+  } else {
+    HIPCHK(hipMemsetAsync(x, 0, ib, p.stream));
+    launch_sqnorm_partials(p, r, partA, p.stream);
+  }
+  launch_cg_init(p, partA, partB, o.tol_kind, o.tol, o.maxit, warm, p.stream);
+  HIPCHK(hipMemcpyAsync(pv, r, ib, hipMemcpyDeviceToDevice, p.stream));
+  CHK(check_launch());
+
+  hipEvent_t ev[2];
+  HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int status = CGS_OK;
+  for (int it = 0; it < o.maxit; ++it) {
+    CHK(op_E(p, pv, p.ybuf[1], p.fdone));
+    CHK(op_EH(p, o.eh_mode, p.ybuf[1], q, o.ftiter, o.fteps, pv, (float)o.lambda, p.partials, p.fdone));
+    {
+      Prof pr(p, CGS_K_CG, 7.0 * 8.0 * F * p.Ntot);
+      launch_cg_alpha(p, p.partials, p.stream);
+      launch_cg_xr(p, x, r, pv, q, p.partials, p.stream);
+      launch_cg_beta(p, p.partials, o.tol_kind, o.tol, o.maxit, p.stream);
+      launch_cg_p(p, pv, r, p.stream);
+    }
+    CHK(check_launch());
+    // pipelined convergence test: read the flags of the previous iteration
+    const int par = it & 1;
+    HIPCHK(hipMemcpyAsync(p.h_done + par * F, p.fdone, sizeof(int) * F, hipMemcpyDeviceToHost, p.stream));
+    HIPCHK(hipEventRecord(ev[par], p.stream));
+    if (it >= 1) {
+      HIPCHK(hipEventSynchronize(ev[par ^ 1]));
+      bool all = true;
+      for (int f = 0; f < F; ++f) all &= (p.h_done[(par ^ 1) * F + f] != 0);
+      if (all) break;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(xout, x, ib, hipMemcpyDeviceToDevice, p.stream));
+  HIPCHK(hipMemcpyAsync(p.h_cgst, p.cgst, sizeof(CgState) * F, hipMemcpyDeviceToHost, p.stream));
+  HIPCHK(hipStreamSynchronize(p.stream));
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  int maxi = 0;
+  for (int f = 0; f < F; ++f) {
+    maxi = std::max(maxi, p.h_cgst[f].it);
+    if (p.h_cgst[f].nan) status = CGS_ERR_NAN;
+  }
+  if (rep) {
+    rep->nframes = F;
+    rep->max_iters = maxi;
+    rep->status = status;
+    for (int f = 0; f < F && f < 1024; ++f) {
+      rep->iters[f] = p.h_cgst[f].it;
+      rep->final_rel[f] = p.h_cgst[f].rel;
+    }
+  }
+  return CGS_OK;
+}
+
+}  // namespace cgs
+
+// =========================================================================================
+// C ABI
+// =========================================================================================
+using namespace cgs;
+
+#define GUARD_BEGIN try {
+#define GUARD_END                                     \
+  }                                                   \
+  catch (const std::exception& e) {                   \
+    set_error(std::string("exception: ") + e.what()); \
+    return CGS_ERR_ARG;                               \
+  }                                                   \
+  catch (...) {                                       \
+    set_error("unknown exception");                   \
+    return CGS_ERR_ARG;                               \
+  }
+
+extern "C" {
+
+const char* cgs_status_string(int s) {
+  switch (s) {
+    case CGS_OK: return "ok";
+    case CGS_ERR_ARG: return "bad argument";
+    case CGS_ERR_STATE: return "call out of order";
+    case CGS_ERR_HIP: return "HIP runtime error";
+    case CGS_ERR_ALLOC: return "device allocation failed";
+    case CGS_ERR_UNSUPPORTED: return "unsupported parameters";
+    case CGS_ERR_NAN: return "NaN in solver scalars";
+    default: return "unknown status";
+  }
+}
+
+const char* cgs_last_error(void) { return g_err.c_str(); }
+
+int cgs_plan_create(cgs_plan* plan, int d, const int* N, int64_t M_total, double sigma, int m, int ncoils,
+                    int nframes, const int64_t* frame_offsets, unsigned flags) {
+  GUARD_BEGIN
+  if (!plan) { set_error("null plan pointer"); return CGS_ERR_ARG; }
+  *plan = nullptr;
+  cgs_plan h = new cgs_plan_s();
+  int rc = create_plan(h->p, d, N, M_total, sigma, m, ncoils, nframes, frame_offsets, flags);
+  if (rc != CGS_OK) {
+    destroy_plan(h->p);
+    delete h;
+    return rc;
+  }
+  *plan = h;
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_plan_destroy(cgs_plan plan) {
+  GUARD_BEGIN
+  if (!plan) return CGS_OK;
+  destroy_plan(plan->p);
+  delete plan;
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_set_stream(cgs_plan plan, void* stream) {
+  GUARD_BEGIN
+  if (!plan) return CGS_ERR_ARG;
+  Plan& p = plan->p;
+  HIPCHK(hipStreamSynchronize(p.stream));
+  if (p.own_stream) (void)hipStreamDestroy(p.stream);
+  p.stream = (hipStream_t)stream;
+  p.own_stream = false;
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_sync(cgs_plan plan) {
+  GUARD_BEGIN
+  if (!plan) return CGS_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(plan->p.stream));
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_set_trajectory(cgs_plan plan, const float* k) {
+  GUARD_BEGIN
+  if (!plan || !k) { set_error("null argument"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  HIPCHK(hipStreamSynchronize(p.stream));
+  CHK(build_samples(p, k));
+  p.have_k = true;
+  p.have_w = false;   // weights follow the sample order (NFFT.hpp:258-296: Weights after KSpace)
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_set_weights(cgs_plan plan, const float* w) {
+  GUARD_BEGIN
+  if (!plan || !w) { set_error("null argument"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  if (!p.have_k) { set_error("set the trajectory before the weights (NFFT::Weights after KSpace)"); return CGS_ERR_STATE; }
+  std::vector<float> ws((size_t)p.Mtot);
+  for (int64_t s = 0; s < p.Mtot; ++s) ws[s] = w[p.h_perm[s]];
+  HIPCHK(hipStreamSynchronize(p.stream));
+  HIPCHK(hipMemcpy(p.w, ws.data(), sizeof(float) * ws.size(), hipMemcpyHostToDevice));
+  p.have_w = true;
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_set_sensitivities(cgs_plan plan, const void* S) {
+  GUARD_BEGIN
+  if (!plan || !S) { set_error("null argument"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  const float2* s = (const float2*)S;
+  std::vector<float2> si((size_t)p.Ntot * p.C);
+  std::vector<float> I((size_t)p.Ntot);
+  for (int64_t i = 0; i < p.Ntot; ++i) {
+    float acc = 0.f;   // IntensityMap accumulates in the maps' precision (MRI.hpp:21-26)
+    for (int c = 0; c < p.C; ++c) {
+      const float2 v = s[i + p.Ntot * c];
+      si[i * p.C + c] = v;
+      acc += v.x * v.x + v.y * v.y;
+    }
+    I[i] = (float)(1.0 / (std::sqrt((double)acc) + 1.e-9));
+  }
+  HIPCHK(hipStreamSynchronize(p.stream));
+  HIPCHK(hipMemcpy(p.S, si.data(), sizeof(float2) * si.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.Imap, I.data(), sizeof(float) * I.size(), hipMemcpyHostToDevice));
+  p.have_S = true;
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_E(cgs_plan plan, const void* x, void* y) {
+  GUARD_BEGIN
+  if (!plan || !x || !y) { set_error("null argument"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  CHK(require_ready(p, true));
+  const float2* xd;
+  CHK(stage_in(p, x, (float2*)p.ext_b, sizeof(float2) * p.Ntot * p.F, &xd));
+  float2* yd = (p.flags & CGS_DEVICE_IO) ? (float2*)y : (float2*)p.ext_a;
+  CHK(op_E(p, xd, p.ybuf[1], nullptr));
+  {
+    Prof pr(p, CGS_K_OTHER, 2.0 * sizeof(float2) * p.Mtot * p.C);
+    launch_y_int2ext(p, p.ybuf[1], yd, p.C, -1, p.stream);
+  }
+  CHK(check_launch());
+  CHK(stage_out(p, y, yd, sizeof(float2) * p.Mtot * p.C));
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_EH(cgs_plan plan, const void* y, void* x, int mode, int ftiter, double fteps) {
+  GUARD_BEGIN
+  if (!plan || !y || !x) { set_error("null argument"); return CGS_ERR_ARG; }
+  if (mode != CGS_EH_TRUE && mode != CGS_EH_REF) { set_error("bad E^H mode"); return CGS_ERR_ARG; }
+  if (ftiter < 0) { set_error("ftiter must be >= 0"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  CHK(require_ready(p, true));
+  const float2* yd;
+  CHK(stage_in(p, y, (float2*)p.ext_a, sizeof(float2) * p.Mtot * p.C, &yd));
+  float2* xd = (p.flags & CGS_DEVICE_IO) ? (float2*)x : (float2*)p.ext_b;
+  {
+    Prof pr(p, CGS_K_OTHER, 2.0 * sizeof(float2) * p.Mtot * p.C);
+    launch_y_ext2int(p, yd, p.ybuf[0], p.C, -1, p.stream);
+  }
+  CHK(op_EH(p, mode, p.ybuf[0], xd, ftiter, fteps, nullptr, 0.f, nullptr, nullptr));
+  CHK(check_launch());
+  CHK(stage_out(p, x, xd, sizeof(float2) * p.Ntot * p.F));
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_normal(cgs_plan plan, const void* x, void* q, double lambda) {
+  GUARD_BEGIN
+  if (!plan || !x || !q) { set_error("null argument"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  CHK(require_ready(p, true));
+  const float2* xd;
+  CHK(stage_in(p, x, (float2*)p.ext_b, sizeof(float2) * p.Ntot * p.F, &xd));
+  float2* qd = (p.flags & CGS_DEVICE_IO) ? (float2*)q : p.img[5];
+  if (!(p.flags & CGS_DEVICE_IO)) {
+    HIPCHK(hipMemcpyAsync(p.img[4], xd, sizeof(float2) * p.Ntot * p.F, hipMemcpyDeviceToDevice, p.stream));
+    xd = p.img[4];
+  }
+  CHK(op_E(p, xd, p.ybuf[1], nullptr));
+  CHK(op_EH_true(p, p.ybuf[1], qd, xd, (float)lambda, nullptr, nullptr));
+  CHK(check_launch());
+  CHK(stage_out(p, q, qd, sizeof(float2) * p.Ntot * p.F));
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_nfft_forward(cgs_plan plan, int frame, const void* img, void* f, int batch) {
+  GUARD_BEGIN
+  if (!plan || !img || !f) { set_error("null argument"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  if (frame < 0 || frame >= p.F || batch < 1 || batch > p.C) { set_error("bad frame or batch"); return CGS_ERR_ARG; }
+  if (!p.have_k) { set_error("trajectory not set"); return CGS_ERR_STATE; }
+  const int64_t Mf = p.foff[frame + 1] - p.foff[frame];
+  const float2* id;
+  CHK(stage_in(p, img, (float2*)p.ext_b, sizeof(float2) * p.Ntot * batch, &id));
+  float2* fd = (p.flags & CGS_DEVICE_IO) ? (float2*)f : (float2*)p.ext_a;
+  mask_one_frame(p, frame);
+  launch_img_ext2coil(p, id, p.Xc, batch, frame, p.stream);
+  CHK(fft_forward(p, 2, p.Xc, p.fmask));
+  CHK(interp(p, p.ybuf[1], p.fmask));
+  launch_y_int2ext(p, p.ybuf[1], fd, batch, frame, p.stream);
+  CHK(check_launch());
+  CHK(stage_out(p, f, fd, sizeof(float2) * Mf * batch));
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_nfft_adjoint(cgs_plan plan, int frame, const void* f, void* img, int batch) {
+  GUARD_BEGIN
+  if (!plan || !img || !f) { set_error("null argument"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  if (frame < 0 || frame >= p.F || batch < 1 || batch > p.C) { set_error("bad frame or batch"); return CGS_ERR_ARG; }
+  if (!p.have_k) { set_error("trajectory not set"); return CGS_ERR_STATE; }
+  const int64_t Mf = p.foff[frame + 1] - p.foff[frame];
+  const float2* fd;
+  CHK(stage_in(p, f, (float2*)p.ext_a, sizeof(float2) * Mf * batch, &fd));
+  float2* id = (p.flags & CGS_DEVICE_IO) ? (float2*)img : (float2*)p.ext_b;
+  mask_one_frame(p, frame);
+  launch_y_ext2int(p, fd, p.ybuf[1], batch, frame, p.stream);
+  CHK(gridding(p, p.ybuf[1], false, p.fmask));
+  CHK(fft_adjoint(p, p.Xc, p.fmask));
+  launch_img_coil2ext(p, p.Xc, id, batch, frame, p.stream);
+  CHK(check_launch());
+  CHK(stage_out(p, img, id, sizeof(float2) * p.Ntot * batch));
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_nfft_inverse_ref(cgs_plan plan, int frame, const void* f, void* img, int batch, int maxit, double eps) {
+  GUARD_BEGIN
+  if (!plan || !img || !f) { set_error("null argument"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  if (frame < 0 || frame >= p.F || batch < 1 || batch > p.C || maxit < 0) { set_error("bad frame, batch or maxit"); return CGS_ERR_ARG; }
+  CHK(require_ready(p, false));
+  const int64_t Mf = p.foff[frame + 1] - p.foff[frame];
+  const float2* fd;
+  CHK(stage_in(p, f, (float2*)p.ext_a, sizeof(float2) * Mf * batch, &fd));
+  float2* id = (p.flags & CGS_DEVICE_IO) ? (float2*)img : (float2*)p.ext_b;
+  mask_one_frame(p, frame);
+  launch_y_ext2int(p, fd, p.ybuf[0], batch, frame, p.stream);
+  CHK(cgnr(p, p.ybuf[0], maxit, eps, p.fmask));
+  launch_img_coil2ext(p, p.Xc2, id, batch, frame, p.stream);
+  CHK(check_launch());
+  CHK(stage_out(p, img, id, sizeof(float2) * p.Ntot * batch));
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_solve(cgs_plan plan, const void* y, const void* x0, void* x, const cgs_solve_opts* opts,
+              cgs_solve_report* report) {
+  GUARD_BEGIN
+  if (!plan || !y || !x || !opts) { set_error("null argument"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  CHK(require_ready(p, true));
+  cgs_solve_opts o = *opts;
+  if (o.maxit < 0 || o.ftiter < 0 || !(o.tol >= 0)) { set_error("bad solver options"); return CGS_ERR_ARG; }
+  if (o.eh_mode != CGS_EH_TRUE && o.eh_mode != CGS_EH_REF) { set_error("bad E^H mode"); return CGS_ERR_ARG; }
+  const size_t ib = sizeof(float2) * (size_t)p.Ntot * p.F;
+  const float2* yd;
+  CHK(stage_in(p, y, (float2*)p.ext_a, sizeof(float2) * p.Mtot * p.C, &yd));
+  const float2* x0d = nullptr;
+  if (x0) {
+    if (p.flags & CGS_DEVICE_IO) x0d = (const float2*)x0;
+    else {
+      HIPCHK(hipMemcpyAsync(p.img[4], x0, ib, hipMemcpyHostToDevice, p.stream));
+      x0d = p.img[4];
+    }
+  }
+  float2* xd = (p.flags & CGS_DEVICE_IO) ? (float2*)x : (float2*)p.ext_b;
+  if (o.maxit == 0) {   // CGLS.hpp:32-33: return A/y
+    Prof pr(p, CGS_K_OTHER, 2.0 * sizeof(float2) * p.Mtot * p.C);
+    launch_y_ext2int(p, yd, p.ybuf[0], p.C, -1, p.stream);
+    CHK(op_EH(p, o.eh_mode, p.ybuf[0], xd, o.ftiter, o.fteps, nullptr, 0.f, nullptr, nullptr));
+    if (report) { std::memset(report, 0, sizeof(*report)); report->nframes = p.F; }
+  } else {
+    CHK(solve(p, yd, x0d, xd, o, report));
+  }
+  CHK(check_launch());
+  CHK(stage_out(p, x, xd, ib));
+  return (report && report->status == CGS_ERR_NAN) ? CGS_ERR_NAN : CGS_OK;
+  GUARD_END
+}
+
+int cgs_kernel_stats(cgs_plan plan, int kernel, int64_t* launches, double* ms, double* bytes, int reset) {
+  GUARD_BEGIN
+  if (!plan || kernel < 0 || kernel >= CGS_K_COUNT) { set_error("bad kernel id"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  CHK(drain_profile(p));
+  if (launches) *launches = p.stat[kernel].launches;
+  if (ms) *ms = p.stat[kernel].ms;
+  if (bytes) *bytes = p.stat[kernel].bytes;
+  if (reset) p.stat[kernel] = Plan::Stat{};
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_plan_info(cgs_plan plan, int* n, int* tile, int64_t* visits, int* grid_items) {
+  GUARD_BEGIN
+  if (!plan) return CGS_ERR_ARG;
+  Plan& p = plan->p;
+  if (n) for (int a = 0; a < 3; ++a) n[a] = p.n[a];
+  if (tile) *tile = p.T;
+  if (visits) *visits = p.n_visits;
+  if (grid_items) *grid_items = p.n_grid_items;
+  return CGS_OK;
+  GUARD_END
+}
+
+}  // extern "C"

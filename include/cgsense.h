@@ -1,0 +1,156 @@
+/*
+ * cgsense.h -- C ABI of the MI355X (gfx950) non-Cartesian CG-SENSE library
+ * (libcgsense_hip.so).
+ *
+ * One plan holds one trajectory (optionally split into frames of uneven sample
+ * counts), its density weights, the coil sensitivities and every device buffer
+ * the operators need.  All entry points return a status code (CGS_OK == 0) and
+ * never throw.  A plan is not re-entrant (the same rule as codeare's operators,
+ * whose const methods write mutable scratch, NCSENSE.hpp:485-487); distinct
+ * plans are independent.  Each plan enqueues its work on one HIP stream.
+ *
+ * Layouts are codeare's (column-major Matrix, src/matrix/Matrix.hpp:579):
+ *   image   x[i0 + N0*(i1 + N1*i2)]                      complex64 (re,im)
+ *   frames  x[i + Ntot*f]                                (N^d, F)
+ *   data    y[j + M_f*c] inside frame f's block, blocks at C*frame_offsets[f]
+ *           (one frame: the (M, C) matrix of NCSENSE::Trafo, NCSENSE.hpp:173)
+ *   maps    S[i + Ntot*c]                                (N^d, C)
+ *   k-space k[t + d*j], k(t, j) in [-0.5, 0.5) pairs with image axis d-1-t
+ *           (NFFT.hpp:267-268 feeds NFFT3's row-major plan.x)
+ *
+ * By default every data pointer given to an operator is host memory (codeare
+ * Matrix storage) and the call returns when the result is in it.  With
+ * CGS_DEVICE_IO the pointers are device memory, calls are asynchronous on the
+ * plan's stream, and cgs_sync() waits.
+ */
+#ifndef CGSENSE_H
+#define CGSENSE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cgs_plan_s* cgs_plan;
+
+/* status codes (codeare::error_code analogue, src/common.h:30-60) */
+enum {
+  CGS_OK = 0,
+  CGS_ERR_ARG = 1,          /* bad argument (shape, size, null pointer) */
+  CGS_ERR_STATE = 2,        /* call out of order (e.g. no trajectory yet) */
+  CGS_ERR_HIP = 3,          /* HIP runtime error */
+  CGS_ERR_ALLOC = 4,        /* device allocation failed */
+  CGS_ERR_UNSUPPORTED = 5,  /* parameters outside what the kernels implement */
+  CGS_ERR_NAN = 6           /* NaN in a solver scalar (CGLS.hpp:41 break) */
+};
+
+/* plan flags */
+#define CGS_DEVICE_IO      0x1u  /* operator data pointers are device memory */
+#define CGS_PROFILE        0x2u  /* time every kernel launch with HIP events */
+
+/* E^H variants */
+#define CGS_EH_TRUE        0     /* x = sum_c conj(S_c) .* NFFT^H(w .* y_c)   */
+#define CGS_EH_REF         1     /* x = I .* sum_c conj(S_c) .* ITrafo_c(y_c) (NCSENSE.hpp:289-298) */
+
+/* stopping rules of cgs_solve */
+#define CGS_TOL_NORM_RATIO    0  /* ||r|| / ||E^H y|| <= tol                   */
+#define CGS_TOL_SQUARED_RATIO 1  /* rn / xn <= tol, codeare's cgeps (CGLS.hpp:41) */
+
+typedef struct {
+  int maxit;          /* CG iterations at most (codeare cgiter / cgmaxit)   */
+  double tol;         /* stopping threshold, see tol_kind (codeare cgeps)   */
+  int tol_kind;       /* CGS_TOL_NORM_RATIO or CGS_TOL_SQUARED_RATIO        */
+  double lambda;      /* Tikhonov weight (codeare lambda, CGLS.hpp:48-49)   */
+  int eh_mode;        /* CGS_EH_TRUE or CGS_EH_REF                          */
+  int ftiter;         /* CGNR steps of the reference E^H (codeare ftiter)   */
+  double fteps;       /* CGNR absolute break on the weighted residual       */
+} cgs_solve_opts;
+
+typedef struct {
+  int iters[1024];      /* iterations run, per frame (first 1024 frames)       */
+  double final_rel[1024];/* final residual measure per frame (tol_kind units)   */
+  int nframes;
+  int max_iters;
+  int status;           /* CGS_OK, or CGS_ERR_NAN if a frame hit a NaN scalar */
+} cgs_solve_report;
+
+/* --- plan life cycle (NFFT.hpp:66-191, NCSENSE.hpp:69-184) --------------- */
+
+/* d = 2 or 3; N[d] image sides (codeare imsz); M_total samples over all frames;
+ * sigma = oversampling (codeare alpha, n = ceil(sigma*N)); m = KB cut-off;
+ * ncoils = C; nframes = F with frame_offsets[F+1] sample offsets (NULL when
+ * F == 1).  Replaces constructing C NFFT<cxfl> plans plus NCSENSE<cxfl>. */
+int cgs_plan_create(cgs_plan* plan, int d, const int* N, int64_t M_total,
+                    double sigma, int m, int ncoils, int nframes,
+                    const int64_t* frame_offsets, unsigned flags);
+int cgs_plan_destroy(cgs_plan plan);
+
+/* Use this stream (hipStream_t) instead of the plan's own. */
+int cgs_set_stream(cgs_plan plan, void* hip_stream);
+int cgs_sync(cgs_plan plan);
+
+/* NCSENSE::KSpace / NFFT::KSpace (NCSENSE.hpp:204-236, NFFT.hpp:258-273).
+ * k: host (d, M_total) float.  Sorts samples into grid tiles and precomputes
+ * the window (the PRE_PSI analogue of NFFT::Weights, NFFT.hpp:281-296). */
+int cgs_set_trajectory(cgs_plan plan, const float* k);
+/* NCSENSE::Weights / NFFT::Weights (NCSENSE.hpp:244-248, NFFT.hpp:281-296):
+ * host (M_total) float density weights; also fixes the damping mask
+ * (NFFTTraits.hpp:225-260). */
+int cgs_set_weights(cgs_plan plan, const float* w);
+/* NCSENSE::Sensitivities (NCSENSE.hpp:193-197): host (N^d, C) complex64 maps;
+ * also computes IntensityMap (MRI.hpp:6-30). */
+int cgs_set_sensitivities(cgs_plan plan, const void* S);
+
+/* --- NFFT<cxfl> operators (one frame at a time, batch images) -------------- */
+/* nfft_trafo of `batch` images (N^d, batch) -> (M_f, batch) for frame `frame`
+ * (NFFT::Trafo, NFFT.hpp:305-361).  batch <= ncoils. */
+int cgs_nfft_forward(cgs_plan plan, int frame, const void* img, void* f, int batch);
+/* true adjoint nfft_adjoint (NFFTTraits.hpp:201-204), no weights. */
+int cgs_nfft_adjoint(cgs_plan plan, int frame, const void* f, void* img, int batch);
+/* NFFT::Adjoint semantics: `maxit` steps of the weighted, damped CGNR
+ * (NFFT.hpp:370-436, NFFTTraits.hpp:122-138), per image. */
+int cgs_nfft_inverse_ref(cgs_plan plan, int frame, const void* f, void* img,
+                         int batch, int maxit, double eps);
+
+/* --- NCSENSE<cxfl> operators (all frames) ----------------------------------- */
+/* E (NCSENSE::Trafo / operator*, NCSENSE.hpp:310-348, :423-425) */
+int cgs_E(cgs_plan plan, const void* x, void* y);
+/* E^H: CGS_EH_TRUE, or CGS_EH_REF = NCSENSE::operator/ (NCSENSE.hpp:251-301) */
+int cgs_EH(cgs_plan plan, const void* y, void* x, int eh_mode, int ftiter, double fteps);
+/* E^H E x + lambda x with the true adjoint (one CG operator application). */
+int cgs_normal(cgs_plan plan, const void* x, void* q, double lambda);
+
+/* CG solve (NCSENSE::Adjoint / operator->* -> CGLS::Solve, NCSENSE.hpp:408-414,
+ * CGLS.hpp:30-59).  x0 may be NULL (zero start, as codeare) or a warm start.
+ * Every frame has its own scalars and stops on its own. report may be NULL. */
+int cgs_solve(cgs_plan plan, const void* y, const void* x0, void* x,
+              const cgs_solve_opts* opts, cgs_solve_report* report);
+
+/* --- profiling (CGS_PROFILE plans) ------------------------------------------ */
+/* kernel ids for cgs_kernel_stats */
+enum {
+  CGS_K_FFT_FWD = 0,   /* deapodise/pad/coil-multiply + forward FFT passes */
+  CGS_K_INTERP = 1,    /* window gather (forward B)                        */
+  CGS_K_GRID = 2,      /* tile-binned window scatter (adjoint B^T)          */
+  CGS_K_FFT_ADJ = 3,   /* adjoint FFT passes + crop/deapodise               */
+  CGS_K_COIL = 4,      /* conj(S) coil combine                              */
+  CGS_K_CG = 5,        /* CG / CGNR vector and scalar kernels              */
+  CGS_K_OTHER = 6,     /* layout conversions                               */
+  CGS_K_COUNT = 7
+};
+/* Accumulated launches, total milliseconds and algorithmic bytes per kernel
+ * class since the last reset (reset = 1 clears after reading). */
+int cgs_kernel_stats(cgs_plan plan, int kernel, int64_t* launches, double* ms,
+                     double* bytes, int reset);
+/* Sizes the plan resolved: n[3] oversampled sides, tile side, sorted sample
+ * count, gridding work items. */
+int cgs_plan_info(cgs_plan plan, int* n, int* tile, int64_t* visits, int* grid_items);
+
+const char* cgs_status_string(int status);
+const char* cgs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CGSENSE_H */

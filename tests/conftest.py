@@ -1,0 +1,26 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def ora():
+    from oracle import ora as _ora
+    _ora.build()
+    return _ora
+
+
+@pytest.fixture(scope="session")
+def cgs():
+    import codeare_amd
+    codeare_amd.load_library()
+    return codeare_amd

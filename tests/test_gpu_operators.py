@@ -1,0 +1,193 @@
+"""GPU parity of the NFFT and NCSENSE operators against the FP64 CPU oracle.
+
+Tolerances (relative L2, stated per test): single operator applications
+<= 1e-5 against the same algorithm in FP64; the oracle itself is pinned to the
+direct NDFT in tests/test_oracle.py.
+"""
+import numpy as np
+import pytest
+
+from helpers import rel, rand_img, rand_traj
+
+pytestmark = pytest.mark.gpu
+
+OP_TOL = 1e-5
+
+
+@pytest.mark.parametrize("d,N,M,sigma,m", [
+    (2, 32, 3000, 2.0, 3),
+    (2, 48, 2500, 2.0, 3),     # n = 96 = 2^5 * 3
+    (2, 32, 2000, 1.5, 3),     # n = 48
+    (2, 32, 2000, 2.0, 1),     # codeare's default m
+    (3, 16, 4000, 2.0, 3),
+    (3, 16, 3000, 1.5, 3),     # n = 24
+])
+def test_nfft_forward_adjoint(cgs, ora, d, N, M, sigma, m):
+    rng = np.random.default_rng(7 + d + N + m)
+    k = rand_traj(rng, d, M)
+    img = rand_img(rng, (N,) * d)
+    f = rand_img(rng, (M,))
+    P = cgs.Plan((N,) * d, M, sigma, m, ncoils=1)
+    P.kspace(k)
+    P.weights(np.ones(M, np.float32))
+    O = ora.NFFT((N,) * d, M, sigma, m)
+    O.kspace(k)
+    fg = P.nfft_forward(img)
+    fo = O.trafo(img)
+    e1 = rel(fg, fo)
+    ag = P.nfft_adjoint(f)
+    ao = O.adjoint(f)
+    e2 = rel(ag, ao)
+    print(f"d={d} N={N} sigma={sigma} m={m}: forward {e1:.2e} adjoint {e2:.2e}")
+    assert e1 <= OP_TOL and e2 <= OP_TOL
+
+
+def test_nfft_batch_equals_single(cgs):
+    rng = np.random.default_rng(3)
+    N, M, B = 32, 1500, 4
+    k = rand_traj(rng, 2, M)
+    imgs = rand_img(rng, (N, N, B))
+    P = cgs.Plan((N, N), M, 2.0, 3, ncoils=B)
+    P.kspace(k)
+    P.weights(np.ones(M, np.float32))
+    fb = P.nfft_forward(imgs)
+    for b in range(B):
+        P1 = cgs.Plan((N, N), M, 2.0, 3, ncoils=1)
+        P1.kspace(k)
+        P1.weights(np.ones(M, np.float32))
+        assert rel(fb[:, b], P1.nfft_forward(imgs[..., b])) <= 1e-6
+
+
+def _c1_problem(cgs, ora, N=128, coils=8, spokes=64, samples=256):
+    from codeare_amd import data
+    prob = data.make_problem(1, 0, N=N, coils=coils, spokes=spokes, samples=samples)
+    wl = prob.wl
+    O = ora.NFFT((wl.N,) * wl.d, prob.k.shape[1], wl.sigma, wl.m)
+    O.kspace(prob.k)
+    y = data.make_data(prob, lambda x: O.E(prob.maps, x))
+    P = cgs.Plan((wl.N,) * wl.d, prob.k.shape[1], wl.sigma, wl.m, ncoils=wl.coils)
+    P.kspace(prob.k)
+    P.weights(prob.w)
+    P.sensitivities(prob.maps)
+    return prob, y, P, O
+
+
+def test_E_EH_true_c1(cgs, ora):
+    prob, y, P, O = _c1_problem(cgs, ora)
+    yg = P.E(prob.image)
+    yo = O.E(prob.maps, prob.image)
+    e1 = rel(yg, yo)
+    xg = P.EH(y, mode=cgs.CGS_EH_TRUE)
+    xo = O.EH_true(prob.maps, prob.w, y)
+    e2 = rel(xg, xo)
+    print(f"E {e1:.2e}  E^H(true) {e2:.2e}")
+    assert e1 <= OP_TOL and e2 <= OP_TOL
+
+
+def test_EH_ref_c1(cgs, ora):
+    prob, y, P, O = _c1_problem(cgs, ora)
+    for ftiter, fteps in [(0, 1e-3), (1, 1e-3), (3, 1e-3), (3, 1e30)]:
+        xg = P.EH(y, mode=cgs.CGS_EH_REF, ftiter=ftiter, fteps=fteps)
+        xo = O.EH_ref(prob.maps, prob.w, y, ftiter=ftiter, fteps=fteps)
+        e = rel(xg, xo) if np.linalg.norm(xo) > 0 else float(np.linalg.norm(xg))
+        print(f"E^H(ref) ftiter={ftiter} fteps={fteps}: {e:.2e}")
+        assert e <= OP_TOL
+
+
+def test_nfft_inverse_ref(cgs, ora):
+    rng = np.random.default_rng(11)
+    N, M = 32, 3000
+    k = rand_traj(rng, 2, M)
+    w = rng.uniform(0.1, 1.0, M).astype(np.float32)
+    f = rand_img(rng, (M,))
+    P = cgs.Plan((N, N), M, 2.0, 3, ncoils=1)
+    P.kspace(k)
+    P.weights(w)
+    O = ora.NFFT((N, N), M, 2.0, 3)
+    O.kspace(k)
+    for maxit in (1, 3, 5):
+        g = P.nfft_inverse_ref(f, maxit=maxit, eps=1e-12)
+        o, it, _ = O.itrafo(f, w, maxit=maxit, eps=1e-12)
+        e = rel(g, o)
+        print(f"ITrafo maxit={maxit}: {e:.2e}")
+        assert it == maxit
+        assert e <= 1e-5
+
+
+def test_cg_true_c1(cgs, ora):
+    """Workload 1: 10 CG iterations from zero, lambda 0 (images to 1e-4)."""
+    prob, y, P, O = _c1_problem(cgs, ora)
+    xg, rep = P.solve(y, maxit=10, tol=0.0, lam=0.0)
+    xo, it, hist = O.cg_true(prob.maps, prob.w, y, maxit=10, tol=0.0, lam=0.0)
+    e = rel(xg, xo)
+    print(f"CG(10) image {e:.2e}; iters gpu {rep['iters'][0]} oracle {it}; rel resid {rep['final_rel'][0]:.3e} vs {hist[-1]:.3e}")
+    assert rep["iters"][0] == it == 10
+    assert e <= 1e-4
+    assert abs(rep["final_rel"][0] - hist[-1]) <= 1e-3 * hist[-1] + 1e-7
+
+
+def test_cgls_ref_c1(cgs, ora):
+    """codeare CGLS with the reference E^H (ftiter 3), squared-ratio stop."""
+    prob, y, P, O = _c1_problem(cgs, ora, N=64, coils=4, spokes=48, samples=128)
+    xg, rep = P.solve(y, maxit=5, tol=1e-7, tol_kind=cgs.CGS_TOL_SQUARED_RATIO, lam=5e-2,
+                      eh_mode=cgs.CGS_EH_REF, ftiter=3, fteps=1e-3)
+    xo, it, hist = O.cgls_ref(prob.maps, prob.w, y, mode=1, ftiter=3, fteps=1e-3, maxit=5,
+                              cgeps=1e-7, lam=5e-2)
+    e = rel(xg, xo)
+    print(f"CGLS(ref) image {e:.2e}; iters gpu {rep['iters'][0]} oracle {it}")
+    assert rep["iters"][0] == it
+    assert e <= 1e-4
+
+
+def test_frames_match_single_plans(cgs, ora):
+    """Uneven frames in one plan give the per-frame single-plan results."""
+    from codeare_amd import data
+    rng = np.random.default_rng(5)
+    N, C, R = 48, 4, 96
+    counts = [7, 13, 4]
+    ks = []
+    first = 0
+    for c in counts:
+        ks.append(data.radial_2d(c, R, True, first))
+        first += c
+    k = np.asfortranarray(np.concatenate(ks, axis=1))
+    off = np.concatenate([[0], np.cumsum(np.array(counts) * R)]).astype(np.int64)
+    w = data.density(k, "k", R)
+    S = data.coil_maps(N, 2, C, rng)
+    x = rand_img(rng, (N, N, len(counts)))
+    P = cgs.Plan((N, N), k.shape[1], 2.0, 3, ncoils=C, frame_offsets=off)
+    P.kspace(k)
+    P.weights(w)
+    P.sensitivities(S)
+    y = P.E(x)
+    xe = P.EH(y)
+    for f in range(len(counts)):
+        Mf = int(off[f + 1] - off[f])
+        P1 = cgs.Plan((N, N), Mf, 2.0, 3, ncoils=C)
+        P1.kspace(k[:, off[f]:off[f + 1]])
+        P1.weights(w[off[f]:off[f + 1]])
+        P1.sensitivities(S)
+        yf = P1.E(x[..., f])
+        seg = y[C * off[f]: C * off[f + 1]].reshape((Mf, C), order="F")
+        assert rel(seg, yf) <= 1e-6
+        assert rel(xe[..., f], P1.EH(yf)) <= 1e-5
+
+
+def test_adjoint_identity_c2_shape(cgs):
+    """<E x, W y> == <x, E^H W y> at the full 256^2 x 32-coil golden-angle size."""
+    from codeare_amd import data
+    prob = data.make_problem(2, 0)
+    P = cgs.Plan((256, 256), prob.k.shape[1], 2.0, 3, ncoils=32)
+    P.kspace(prob.k)
+    P.weights(prob.w)
+    P.sensitivities(prob.maps)
+    rng = np.random.default_rng(1)
+    x = rand_img(rng, (256, 256))
+    y = rand_img(rng, (prob.k.shape[1], 32))
+    Ex = P.E(x).astype(np.complex128)
+    EHy = P.EH(y).astype(np.complex128)
+    lhs = np.vdot(Ex * prob.w[:, None], y.astype(np.complex128))
+    rhs = np.vdot(x.astype(np.complex128).reshape(-1, order="F"), EHy.reshape(-1, order="F"))
+    e = abs(lhs - rhs) / abs(lhs)
+    print(f"adjoint identity {e:.2e}")
+    assert e <= 1e-5
