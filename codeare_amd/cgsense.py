@@ -31,6 +31,8 @@ CGS_EH_TRUE = 0
 CGS_EH_REF = 1
 CGS_TOL_NORM_RATIO = 0
 CGS_TOL_SQUARED_RATIO = 1
+CGS_NORMAL_NUFFT = 0
+CGS_NORMAL_TOEPLITZ = 1
 KERNELS = ["fft_fwd", "interp", "grid", "fft_adj", "coil", "cg", "other"]
 
 EXPORTED = [
@@ -39,6 +41,7 @@ EXPORTED = [
     "cgs_nfft_forward", "cgs_nfft_adjoint", "cgs_nfft_inverse_ref",
     "cgs_E", "cgs_EH", "cgs_normal", "cgs_solve",
     "cgs_kernel_stats", "cgs_plan_info", "cgs_status_string", "cgs_last_error",
+    "cgs_residual_history",
 ]
 
 
@@ -51,7 +54,7 @@ class CgsError(RuntimeError):
 class SolveOpts(ct.Structure):
     _fields_ = [("maxit", ct.c_int), ("tol", ct.c_double), ("tol_kind", ct.c_int),
                 ("lambda_", ct.c_double), ("eh_mode", ct.c_int), ("ftiter", ct.c_int),
-                ("fteps", ct.c_double)]
+                ("fteps", ct.c_double), ("normal_op", ct.c_int)]
 
 
 class SolveReport(ct.Structure):
@@ -85,10 +88,11 @@ def load_library(path: str | None = None):
     L.cgs_nfft_inverse_ref.argtypes = [P, I, P, P, I, I, D]
     L.cgs_E.argtypes = [P, P, P]
     L.cgs_EH.argtypes = [P, P, P, I, I, D]
-    L.cgs_normal.argtypes = [P, P, P, D]
+    L.cgs_normal.argtypes = [P, P, P, D, I]
     L.cgs_solve.argtypes = [P, P, P, P, ct.POINTER(SolveOpts), ct.POINTER(SolveReport)]
     L.cgs_kernel_stats.argtypes = [P, I, ct.POINTER(I64), ct.POINTER(D), ct.POINTER(D), I]
     L.cgs_plan_info.argtypes = [P, ct.POINTER(I), ct.POINTER(I), ct.POINTER(I64), ct.POINTER(I)]
+    L.cgs_residual_history.argtypes = [P, I, P, I]
     L.cgs_status_string.restype = ct.c_char_p
     L.cgs_status_string.argtypes = [I]
     L.cgs_last_error.restype = ct.c_char_p
@@ -252,22 +256,23 @@ class Plan:
         _check(L.cgs_EH(self._h, _ptr(y), _ptr(x), int(mode), int(ftiter), float(fteps)))
         return x
 
-    def normal(self, x, lam: float = 0.0, out=None):
+    def normal(self, x, lam: float = 0.0, out=None, toeplitz: bool = False):
         L = load_library()
+        op = CGS_NORMAL_TOEPLITZ if toeplitz else CGS_NORMAL_NUFFT
         if self.device_io:
-            _check(L.cgs_normal(self._h, _ptr(x), _ptr(out), float(lam)))
+            _check(L.cgs_normal(self._h, _ptr(x), _ptr(out), float(lam), op))
             return out
         x = _c64(x)
         q = np.zeros(self._img_shape(), dtype=np.complex64, order="F")
-        _check(L.cgs_normal(self._h, _ptr(x), _ptr(q), float(lam)))
+        _check(L.cgs_normal(self._h, _ptr(x), _ptr(q), float(lam), op))
         return q
 
     def solve(self, y, x0=None, maxit: int = 10, tol: float = 1e-5, tol_kind: int = CGS_TOL_NORM_RATIO,
               lam: float = 0.0, eh_mode: int = CGS_EH_TRUE, ftiter: int = 3, fteps: float = 1e-3,
-              out=None, check_nan: bool = True):
+              out=None, check_nan: bool = True, toeplitz: bool = False):
         L = load_library()
         o = SolveOpts(int(maxit), float(tol), int(tol_kind), float(lam), int(eh_mode), int(ftiter),
-                      float(fteps))
+                      float(fteps), CGS_NORMAL_TOEPLITZ if toeplitz else CGS_NORMAL_NUFFT)
         rep = SolveReport()
         if self.device_io:
             rc = L.cgs_solve(self._h, _ptr(y), _ptr(x0) if x0 is not None else None, _ptr(out),
@@ -287,6 +292,13 @@ class Plan:
         report = {"iters": list(rep.iters[:F]), "final_rel": list(rep.final_rel[:F]),
                   "max_iters": rep.max_iters, "status": rep.status}
         return x, report
+
+    def residual_history(self, frame: int = 0, cap: int = 100000):
+        buf = np.zeros(cap, dtype=np.float64)
+        n = load_library().cgs_residual_history(self._h, int(frame), _ptr(buf), int(cap))
+        if n < 0:
+            _check(-n)
+        return buf[:n]
 
     def kernel_stats(self, reset: bool = True):
         L = load_library()

@@ -40,9 +40,16 @@ struct FftArgs {
   const float2* in;
   float2* out;
   int n, nin, nout, lb, ldn, sign;
+  int pad_in, crop_out;   // input in image-index order (zero-pad) / output cropped
+  // Toeplitz convolution pass: forward FFT, multiply by the real kernel
+  // spectrum That[(f*conv_cells + r)*n + pos] (r = inner index / C), inverse FFT
+  const float* That;
+  int64_t conv_cells;
   int64_t inner, outers, outer_per_frame;
   const float2* tw;     // W_n^p = exp(-2 pi i p / n)
   int nst, radix[8];
+  uint32_t ns_magic[8]; // ceil(2^32 / Ns) of each stage (Ns = product of earlier radices)
+  int lb_log2;          // lb = 1 << lb_log2
   // fused operations of the first forward / last adjoint pass (axis 0)
   const float2* x;      // image frames (load mode 1)
   const float2* S;      // maps (load mode 1)
@@ -83,6 +90,13 @@ struct GridArgs {
   int64_t ntot, tilecells;
 };
 
+struct FftTab {          // tables of one FFT length
+  int n = 0;
+  float2* tw = nullptr;   // exp(-2 pi i p / n)
+  int radix[8] = {0};
+  int nst = 0;
+};
+
 struct Plan {
   // geometry
   int d = 0, N[3] = {1, 1, 1}, n[3] = {1, 1, 1}, m = 0, taps = 0;
@@ -90,12 +104,16 @@ struct Plan {
   int64_t Mtot = 0, Ntot = 0, ntot = 0;
   unsigned flags = 0;
   double b[3] = {0, 0, 0};
+  double sigma = 1.0;
   std::vector<int64_t> foff;          // F+1 sample offsets
   int T = 8, nt[3] = {1, 1, 1};       // tile side and tiles per axis
   int64_t tiles_per_frame = 1;
   int grid_cg = 8, grid_w = 4;        // gridding coil group / waves per block
   int interp_cg = 32;
   int visits_per_item = 256;
+  bool use_interp_lds = true;         // 2D: gather from the tile staged in LDS
+  bool use_grid_batched = true;       // batched visit lists (else one visit at a time)
+  int tile_item_max = 64;             // samples per tile-level interp item
 
   // stream / state
   hipStream_t stream = nullptr;
@@ -103,8 +121,8 @@ struct Plan {
   bool have_k = false, have_w = false, have_S = false;
 
   // constant tables
-  float2* tw[3] = {nullptr, nullptr, nullptr};
-  int radix[3][8] = {{0}}, nst[3] = {0, 0, 0};
+  FftTab fa[3];                       // FFT lengths n_a of the NFFT grid
+  FftTab ft[3];                       // FFT lengths 2 N_a of the Toeplitz embedding
   float* dinv[3] = {nullptr, nullptr, nullptr};
   int64_t* d_foff = nullptr;
 
@@ -121,6 +139,8 @@ struct Plan {
   std::vector<int> h_perm;
   int4* interp_items = nullptr;
   int n_interp_items = 0;
+  int4* tile_items = nullptr;         // {frame, tile, sbeg, send}
+  int n_tile_items = 0;
   GridItem* grid_items = nullptr;
   int n_grid_items = 0;
   ReduceItem* reduce_items = nullptr;
@@ -150,6 +170,14 @@ struct Plan {
   int* fmask = nullptr;               // F scratch flags (single-frame NFFT calls)
   double* hist = nullptr;             // residual history [F][maxit+1]
   int hist_cap = 0;
+  // Toeplitz embedding of E^H W E (kernel spectrum per frame)
+  bool have_toep = false;
+  float* That = nullptr;              // F * prod(2 N_a), conv-pass layout
+  float2* tt1 = nullptr;              // (C, 2N0, N1[, N2]) per frame
+  float2* tt2 = nullptr;              // (C, 2N0, 2N1, N2) per frame (3D)
+  bool tt_alias = false;              // tt1/tt2 alias t1/t2 (sigma == 2)
+  std::vector<float> h_k;             // trajectory copy (host)
+  std::vector<float> h_w;             // weights copy (host, original order)
   CgState* h_cgst = nullptr;          // pinned mirror
   int* h_done = nullptr;              // pinned
 
@@ -166,6 +194,11 @@ void launch_fft(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s);
 void launch_interp(const SampArgs& a, int cg, hipStream_t s);
 void launch_grid(const GridArgs& a, int cg, int waves, int nitems, int ncgroups, hipStream_t s);
 void launch_grid_reduce(const Plan& p, const float2* partial, float2* grid, hipStream_t s);
+void launch_grid_batched(const GridArgs& a, int cg, int waves, int nitems, int ncgroups, hipStream_t s);
+void launch_interp_lds(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int cg, int maxs,
+                       int ncgroups, hipStream_t s);
+size_t grid_batched_lds(int d, int T, int taps, int CG, int rowstride);
+size_t interp_lds_bytes(int T, int taps, int CG, int rowstride, int maxs);
 void launch_coil_combine(const Plan& p, const float2* Xc, float2* x, const float2* pvec,
                          float lambda, bool use_imap, double* partials, hipStream_t s);
 // vector / scalar kernels
@@ -198,6 +231,8 @@ void launch_cnr_update_image(const Plan& p, float2* fhat, float2* phat, const fl
                              int step, hipStream_t s);
 void launch_cnr_update_samples(const Plan& p, float2* r, const float2* v, hipStream_t s);
 void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+void launch_toep_layout(const float2* Khat, float* That, int64_t cells, int nl, int F, float scale,
+                        hipStream_t s);
 void launch_fill_int(int* dst, int value, int n, hipStream_t s);
 
 // errors

@@ -2,17 +2,19 @@
 //
 // One pass transforms every line along one axis of an array viewed as
 // [outer][P][inner] (inner contiguous).  A block stages `lb` adjacent lines in
-// LDS ([lb][n + pad]), so global loads and stores move lb*8 contiguous bytes per
-// position; the transform is a mixed-radix Stockham autosort (radix 2,3,4,8)
-// done in place in LDS with a barrier between the read and write halves of
-// every stage.
+// LDS ([lb][n + 1]), so global loads and stores move lb*8 contiguous bytes per
+// position; each line is then transformed by one wave with a mixed-radix
+// Stockham autosort (radix 2,3,4,8) in place in LDS.  Global loads are issued
+// eight at a time before any of them is waited on, and the twiddle table is
+// staged in LDS, so a block pays about one memory latency per phase.
 //
 // Forward passes read image-index order (i in [0, nin), frequency i - nin/2,
 // zero-padded into bin (i - nin/2) mod n) and write grid order; adjoint passes
 // read grid order and write image-index order (crop).  Pass 1 of the forward
 // transform fuses the deapodisation 1/phi_hat, the zero padding and either the
 // coil multiply S_c .* x (E) or the damping mask w_hat (the CGNR's A(w_hat p));
-// the last adjoint pass fuses the crop and the deapodisation.
+// the last adjoint pass fuses the crop and the deapodisation.  The Toeplitz
+// pass runs forward FFT, spectrum multiply and inverse FFT on the same lines.
 #include "cgs_internal.hpp"
 
 namespace cgs {
@@ -81,46 +83,63 @@ __device__ __forceinline__ void dft<8>(float2* v, int sign) {
   v[7] = csub(e[3], o3);
 }
 
-// One Stockham stage of radix R over all lb lines held in LDS.
-template <int R>
-__device__ __forceinline__ void fft_stage(float2* lds, const FftArgs& a, int Ns, int tid) {
-  constexpr int MAXB = (kFftMaxElem / R + kFftThreads - 1) / kFftThreads;
-  const int n = a.n, nb = n / R, total = a.lb * nb;
+// One Stockham stage of radix R on one line held in LDS, done by one wave.
+// Lane j handles butterflies j, j+64, ...; all of its reads are issued before
+// its writes, and a wave's LDS operations execute in order, so the stage runs
+// in place without a block barrier.  MB = butterflies per lane at most.
+// j / Ns is formed as umulhi(j, ceil(2^32 / Ns)) (exact for j < 2^16).
+template <int R, int MB>
+__device__ __forceinline__ void wave_stage(float2* line, int n, int Ns, uint32_t magic, int lane, int sign,
+                                           const float2* __restrict__ tw) {
+  const int nb = n / R;
   const int tstep0 = n / (Ns * R);
-  float2 v[MAXB][R];
-  int dst[MAXB];
+  float2 v[MB][R];
 #pragma unroll
-  for (int k = 0; k < MAXB; ++k) {
-    const int bidx = tid + k * kFftThreads;
-    if (bidx < total) {
-      const int q = bidx / nb, j = bidx - q * nb;
-      const int kk = j % Ns;
-      const float2* src = lds + q * a.ldn + j;
+  for (int k = 0; k < MB; ++k) {
+    const int j = lane + 64 * k;
+    if (j < nb) {
+      const int kk = (Ns == 1) ? 0 : j - (int)__umulhi((uint32_t)j, magic) * Ns;
       const int tstep = tstep0 * kk;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        float2 x = src[r * nb];
-        if (r > 0 && kk > 0) {
-          float2 w = a.tw[r * tstep];
-          if (a.sign > 0) w.y = -w.y;
+        float2 x = line[j + r * nb];
+        if (r > 0) {
+          float2 w = tw[r * tstep];
+          if (sign > 0) w.y = -w.y;
           x = cmul(x, w);
         }
         v[k][r] = x;
       }
-      dft<R>(v[k], a.sign);
-      dst[k] = q * a.ldn + (j / Ns) * Ns * R + kk;
+      dft<R>(v[k], sign);
     }
   }
-  __syncthreads();
+  __builtin_amdgcn_wave_barrier();
 #pragma unroll
-  for (int k = 0; k < MAXB; ++k) {
-    const int bidx = tid + k * kFftThreads;
-    if (bidx < total) {
+  for (int k = 0; k < MB; ++k) {
+    const int j = lane + 64 * k;
+    if (j < nb) {
+      const int jq = (Ns == 1) ? j : (int)__umulhi((uint32_t)j, magic);
+      const int kk = j - jq * Ns;
+      float2* d = line + jq * Ns * R + kk;
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[dst[k] + r * Ns] = v[k][r];
+      for (int r = 0; r < R; ++r) d[r * Ns] = v[k][r];
     }
   }
-  __syncthreads();
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int MAXN>
+__device__ __forceinline__ void line_fft(float2* line, const FftArgs& a, int lane, int sign, const float2* tw) {
+  int Ns = 1;
+  for (int s = 0; s < a.nst; ++s) {
+    const int R = a.radix[s];
+    const uint32_t mg = a.ns_magic[s];
+    if (R == 8) wave_stage<8, (MAXN / 8 + 63) / 64>(line, a.n, Ns, mg, lane, sign, tw);
+    else if (R == 4) wave_stage<4, (MAXN / 4 + 63) / 64>(line, a.n, Ns, mg, lane, sign, tw);
+    else if (R == 2) wave_stage<2, (MAXN / 2 + 63) / 64>(line, a.n, Ns, mg, lane, sign, tw);
+    else wave_stage<3, (MAXN / 3 + 63) / 64>(line, a.n, Ns, mg, lane, sign, tw);
+    Ns *= R;
+  }
 }
 
 __device__ __forceinline__ float deapod(const FftArgs& a, int i0, int64_t rest) {
@@ -142,97 +161,147 @@ __device__ __forceinline__ bool inside_mask(const FftArgs& a, int i0, int64_t re
   return s <= (int64_t)nh * nh;
 }
 
-// LD: 0 plain, 1 E (x .* S_c .* D), 2 plain .* D, 3 plain .* D .* w_hat
+constexpr int kUnroll = 8;   // global loads in flight per thread in the staging loops
+
+// LD: 0 plain, 1 E (x .* S_c .* D), 2 plain .* D, 3 plain .* D .* w_hat, 4 x .* S_c
 // ST: 0 plain, 1 .* D
-template <int LD, int ST>
+// CONV: forward FFT, multiply by the Toeplitz kernel spectrum, inverse FFT
+template <int LD, int ST, int MAXN, int CONV>
 __global__ void __launch_bounds__(kFftThreads) k_fft(FftArgs a) {
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x;
-  const int64_t nib = (a.inner + a.lb - 1) / a.lb;
+  const int lbl = a.lb_log2, lb = 1 << lbl, n = a.n;
+  const int64_t nib = (a.inner + lb - 1) >> lbl;
   const int64_t o = blockIdx.x / nib;
-  const int64_t q0 = (blockIdx.x - o * nib) * a.lb;
+  const int64_t q0 = (blockIdx.x - o * nib) << lbl;
   const int64_t f = o / a.outer_per_frame;
   if (a.fdone && a.fdone[f]) return;
   const int64_t rest = o - f * a.outer_per_frame;
-  const int lb = a.lb, n = a.n;
   const int nq = (int)min((int64_t)lb, a.inner - q0);
+  float2* tw = lds + (size_t)lb * a.ldn;
+
+  for (int i = tid; i < n; i += kFftThreads) tw[i] = a.tw[i];
 
   // ---- load (with zero padding for forward passes) ----
-  const bool pad = a.sign < 0;   // forward passes read image-index order
-  for (int idx = tid; idx < lb * n; idx += kFftThreads) {
-    const int q = idx % lb, pos = idx / lb;
-    float2 v = make_float2(0.f, 0.f);
-    int p = pos;
-    if (pad) {   // bin pos holds image index p = pos + nin/2 (mod n) when p < nin
-      p = pos + a.nin / 2;
-      if (p >= n) p -= n;
-    }
-    if (q < nq && p < a.nin) {
-      const int64_t qq = q0 + q;
-      if (LD == 1) {
-        const int64_t i = p + (int64_t)a.N0 * rest;
-        const float2 xv = a.x[f * a.Ntot + i];
-        const float2 sv = a.S[i * a.C + qq];
-        const float dv = deapod(a, p, rest);
-        float2 t = cmul(xv, sv);
-        v = make_float2(t.x * dv, t.y * dv);
-      } else {
-        v = a.in[(o * a.nin + p) * a.inner + qq];
-        if (LD >= 2) {
-          float dv = deapod(a, p, rest);
-          if (LD == 3 && !inside_mask(a, p, rest)) dv = 0.f;
-          v = make_float2(v.x * dv, v.y * dv);
+  const bool pad = a.pad_in != 0;   // forward passes read image-index order
+  const int total = lb * n;
+  for (int b0 = tid; b0 < total; b0 += kFftThreads * kUnroll) {
+    float2 v[kUnroll];
+    int dst[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int idx = b0 + u * kFftThreads;
+      v[u] = make_float2(0.f, 0.f);
+      dst[u] = -1;
+      if (idx < total) {
+        const int q = idx & (lb - 1), pos = idx >> lbl;
+        dst[u] = q * a.ldn + pos;
+        int p = pos;
+        if (pad) {   // bin pos holds image index p = pos + nin/2 (mod n) when p < nin
+          p = pos + a.nin / 2;
+          if (p >= n) p -= n;
+        }
+        if (q < nq && p < a.nin) {
+          const int64_t qq = q0 + q;
+          if (LD == 1 || LD == 4) {
+            const int64_t i = p + (int64_t)a.N0 * rest;
+            const float2 xv = a.x[f * a.Ntot + i];
+            const float2 sv = a.S[i * a.C + qq];
+            float2 t = cmul(xv, sv);
+            if (LD == 1) {
+              const float dv = deapod(a, p, rest);
+              t = make_float2(t.x * dv, t.y * dv);
+            }
+            v[u] = t;
+          } else {
+            float2 t = a.in[(o * a.nin + p) * a.inner + qq];
+            if (LD >= 2) {
+              float dv = deapod(a, p, rest);
+              if (LD == 3 && !inside_mask(a, p, rest)) dv = 0.f;
+              t = make_float2(t.x * dv, t.y * dv);
+            }
+            v[u] = t;
+          }
         }
       }
     }
-    lds[q * a.ldn + pos] = v;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      if (dst[u] >= 0) lds[dst[u]] = v[u];
   }
   __syncthreads();
 
-  // ---- Stockham stages ----
-  int Ns = 1;
-  for (int s = 0; s < a.nst; ++s) {
-    const int R = a.radix[s];
-    if (R == 8) fft_stage<8>(lds, a, Ns, tid);
-    else if (R == 4) fft_stage<4>(lds, a, Ns, tid);
-    else if (R == 2) fft_stage<2>(lds, a, Ns, tid);
-    else fft_stage<3>(lds, a, Ns, tid);
-    Ns *= R;
+  // ---- Stockham stages: wave w transforms lines w, w+4, ... ----
+  {
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int q = wave; q < nq; q += kFftThreads / 64) {
+      float2* line = lds + q * a.ldn;
+      if (CONV) {
+        line_fft<MAXN>(line, a, lane, -1, tw);
+        const float* th = a.That + (f * a.conv_cells + (q0 + q) / a.C) * n;
+        for (int pos = lane; pos < n; pos += 64) {
+          const float t = th[pos];
+          float2 v = line[pos];
+          line[pos] = make_float2(v.x * t, v.y * t);
+        }
+        __builtin_amdgcn_wave_barrier();
+        line_fft<MAXN>(line, a, lane, +1, tw);
+      } else {
+        line_fft<MAXN>(line, a, lane, a.sign, tw);
+      }
+    }
   }
+  __syncthreads();
 
   // ---- store (with crop for adjoint passes) ----
-  const bool crop = a.sign > 0;
-  for (int idx = tid; idx < lb * a.nout; idx += kFftThreads) {
-    const int q = idx % lb, p = idx / lb;
-    if (q >= nq) continue;
-    int pos = p;
-    if (crop) {
-      pos = p - a.nout / 2;
-      if (pos < 0) pos += n;
+  const bool crop = a.crop_out != 0;
+  const int totout = lb * a.nout;
+  for (int b0 = tid; b0 < totout; b0 += kFftThreads * kUnroll) {
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int idx = b0 + u * kFftThreads;
+      if (idx < totout) {
+        const int q = idx & (lb - 1), p = idx >> lbl;
+        if (q < nq) {
+          int pos = p;
+          if (crop) {
+            pos = p - a.nout / 2;
+            if (pos < 0) pos += n;
+          }
+          float2 v = lds[q * a.ldn + pos];
+          if (ST == 1) {
+            const float dv = deapod(a, p, rest);
+            v = make_float2(v.x * dv, v.y * dv);
+          }
+          a.out[(o * a.nout + p) * a.inner + q0 + q] = v;
+        }
+      }
     }
-    float2 v = lds[q * a.ldn + pos];
-    if (ST == 1) {
-      const float dv = deapod(a, p, rest);
-      v = make_float2(v.x * dv, v.y * dv);
-    }
-    a.out[(o * a.nout + p) * a.inner + q0 + q] = v;
   }
 }
+
+size_t fft_lds_bytes(const FftArgs& a) { return sizeof(float2) * ((size_t)a.lb * a.ldn + a.n); }
 
 void launch_fft(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s) {
   const int64_t nib = (a.inner + a.lb - 1) / a.lb;
   const dim3 grid((unsigned)(nib * a.outers));
-  const size_t shm = sizeof(float2) * (size_t)a.lb * a.ldn;
-#define CGS_FFT_CASE(L, S_)                                                   \
-  if (ld_mode == L && st_mode == S_) {                                        \
-    hipLaunchKernelGGL((k_fft<L, S_>), grid, dim3(kFftThreads), shm, s, a);  \
-    return;                                                                   \
+  const size_t shm = fft_lds_bytes(a);
+  const int conv = a.That != nullptr;
+#define CGS_FFT_CASE(L, S_, CV)                                                          \
+  if (ld_mode == L && st_mode == S_ && conv == CV) {                                     \
+    if (a.n <= 512)                                                                      \
+      hipLaunchKernelGGL((k_fft<L, S_, 512, CV>), grid, dim3(kFftThreads), shm, s, a);   \
+    else                                                                                 \
+      hipLaunchKernelGGL((k_fft<L, S_, 4096, CV>), grid, dim3(kFftThreads), shm, s, a);  \
+    return;                                                                              \
   }
-  CGS_FFT_CASE(0, 0)
-  CGS_FFT_CASE(1, 0)
-  CGS_FFT_CASE(2, 0)
-  CGS_FFT_CASE(3, 0)
-  CGS_FFT_CASE(0, 1)
+  CGS_FFT_CASE(0, 0, 0)
+  CGS_FFT_CASE(1, 0, 0)
+  CGS_FFT_CASE(2, 0, 0)
+  CGS_FFT_CASE(3, 0, 0)
+  CGS_FFT_CASE(4, 0, 0)
+  CGS_FFT_CASE(0, 1, 0)
+  CGS_FFT_CASE(0, 0, 1)
 #undef CGS_FFT_CASE
 }
 

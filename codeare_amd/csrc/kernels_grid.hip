@@ -198,6 +198,272 @@ __global__ void k_grid_reduce(const ReduceItem* items, const float2* partial, fl
 }
 
 // ---------------------------------------------------------------------------
+// forward B with the grid staged in LDS (2D).  Work item = (frame, tile,
+// samples [sbeg, send) whose base cell lies in the tile); block = 4 waves; one
+// coil group of CG coils per blockIdx.y.  The extended region of the tile,
+// (T + taps - 1)^2 cells, and the items' window values are staged once; every
+// tap is then an LDS read.
+template <int CG, int TAPS>
+__global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* items, int nitems, int T,
+                                                    int rowstride, int maxs, int nt0) {
+  constexpr int RS = 64 / CG;
+  extern __shared__ float2 sh[];
+  const int item = blockIdx.x;
+  const int4 it = items[item];             // frame, tile, sbeg, send
+  if (a.fdone && a.fdone[it.x]) return;
+  const int E = T + TAPS - 1;
+  const int m = TAPS / 2 - 1;
+  float2* ext = sh;                                          // [E][rowstride]
+  float* wpsi = (float*)(sh + (size_t)E * rowstride);        // [maxs][2*TAPS]
+  int* wrel = (int*)(wpsi + (size_t)maxs * 2 * TAPS);        // [maxs][2]
+  const int cbase = blockIdx.y * CG;
+  const int t0 = it.y % nt0, t1 = it.y / nt0;
+  const int org0 = t0 * T - m, org1 = t1 * T - m;
+  const int64_t gbase = (int64_t)it.x * a.ntot;
+  {
+    const int c = threadIdx.x % CG;
+    const bool cok = cbase + c < a.C;
+    for (int e = threadIdx.x / CG; e < E * E; e += 256 / CG) {
+      const int e1 = e / E, e0 = e - e1 * E;
+      int l0 = org0 + e0, l1 = org1 + e1;   // in [-m, n + T + m): one wrap suffices
+      if (l0 < 0) l0 += a.n0; else if (l0 >= a.n0) l0 -= a.n0;
+      if (l1 < 0) l1 += a.n1; else if (l1 >= a.n1) l1 -= a.n1;
+      float2 v = make_float2(0.f, 0.f);
+      if (cok) v = a.grid[(gbase + l0 + (int64_t)a.n0 * l1) * a.C + cbase + c];
+      ext[e1 * rowstride + e0 * CG + c] = v;
+    }
+  }
+  const int ns = it.w - it.z;
+  for (int idx = threadIdx.x; idx < ns * 2 * TAPS; idx += blockDim.x)
+    wpsi[idx] = a.psi[(int64_t)it.z * 2 * TAPS + idx];
+  for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+    const int s = it.z + k;
+    int r0 = a.base[2 * s] - org0, r1 = a.base[2 * s + 1] - org1;
+    if (r0 < 0) r0 += a.n0; else if (r0 >= a.n0) r0 -= a.n0;
+    if (r1 < 0) r1 += a.n1; else if (r1 >= a.n1) r1 -= a.n1;
+    wrel[2 * k] = r0;
+    wrel[2 * k + 1] = r1;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cl = lane % CG, rs = lane / CG;
+  const int c = cbase + cl;
+  for (int k = wave; k < ns; k += 4) {
+    const float* ps = wpsi + k * 2 * TAPS;
+    const int r0 = wrel[2 * k], r1 = wrel[2 * k + 1];
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int s1 = rs; s1 < TAPS; s1 += RS) {
+      const float2* row = ext + (r1 + s1) * rowstride + r0 * CG + cl;
+      float2 racc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int s0 = 0; s0 < TAPS; ++s0) {
+        const float2 v = row[s0 * CG];
+        const float p0 = ps[s0];
+        racc.x += p0 * v.x;
+        racc.y += p0 * v.y;
+      }
+      const float wr = ps[TAPS + s1];
+      acc.x += wr * racc.x;
+      acc.y += wr * racc.y;
+    }
+#pragma unroll
+    for (int msk = CG; msk < 64; msk <<= 1) {
+      const float2 o = shfl_xor2(acc, msk);
+      acc.x += o.x;
+      acc.y += o.y;
+    }
+    if (rs == 0 && c < a.C) a.y[(int64_t)(it.z + k) * a.C + c] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint B^T, owner computes, visits processed in batches of 64: the batch's
+// sample ids, in-tile tap ranges, window values and (weighted) data are first
+// loaded cooperatively into LDS, then every wave runs through the batch.
+// Tile row x1 belongs to row slot x1 mod (W*RS) (slot = wave*RS + rs), so
+// each (cell, coil) has one writer and visits are applied in list
+// order: race-free, deterministic, no atomics.  Slots own absolute tile rows
+// (row x1 == slot mod W*RS), never footprint-relative rows, so two waves at
+// different visits can not reach the same cell.  W*RS must divide T.
+template <int CG, int TAPS, int D>
+__global__ void __launch_bounds__(256) k_grid_batched(GridArgs a) {
+  constexpr int RS = 64 / CG;
+  constexpr int B = 64;
+  constexpr int NPSI = D * TAPS;
+  extern __shared__ float2 sh[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int W = blockDim.x >> 6;
+  const int cl = lane % CG, rs = lane / CG;
+  const int slot = wave * RS + rs, nslots = W * RS;
+  const GridItem it = a.items[blockIdx.x];
+  if (a.fdone && a.fdone[it.frame]) return;
+  const int T = a.T, rowstride = a.rowstride;
+  const int cbase = blockIdx.y * CG;
+  const int c = cbase + cl;
+  const int nrows_tile = (D == 3) ? T * T : T;
+  float2* tile = sh;                                               // [rows][rowstride]
+  float2* ybat = tile + (size_t)nrows_tile * rowstride;            // [B][CG]
+  float* bpsi = (float*)(ybat + B * CG);                           // [B][NPSI]
+  int* bmeta = (int*)(bpsi + B * NPSI);                            // [B][8]: s, rel0..2, lo0,hi0 packed
+
+  int t0 = it.tile % a.nt0, t1 = (it.tile / a.nt0) % a.nt1, t2 = (D == 3) ? it.tile / (a.nt0 * a.nt1) : 0;
+  const int ts0 = t0 * T, ts1 = t1 * T, ts2 = t2 * T;
+  const int te0 = min(T, a.n0 - ts0), te1 = min(T, a.n1 - ts1), te2 = (D == 3) ? min(T, a.n2 - ts2) : 1;
+
+  for (int idx = threadIdx.x; idx < nrows_tile * rowstride; idx += blockDim.x) tile[idx] = make_float2(0.f, 0.f);
+
+  for (int vb = it.vbeg; vb < it.vend; vb += B) {
+    const int nb = min(B, it.vend - vb);
+    __syncthreads();
+    if (threadIdx.x < nb) {
+      const int s = a.visits[vb + threadIdx.x];
+      const int* bs = a.base + (int64_t)s * D;
+      int rel0 = bs[0] - ts0; if (rel0 < 0) rel0 += a.n0; if (rel0 >= te0) rel0 -= a.n0;
+      int rel1 = bs[1] - ts1; if (rel1 < 0) rel1 += a.n1; if (rel1 >= te1) rel1 -= a.n1;
+      int rel2 = 0;
+      if (D == 3) { rel2 = bs[2] - ts2; if (rel2 < 0) rel2 += a.n2; if (rel2 >= te2) rel2 -= a.n2; }
+      int* mt = bmeta + threadIdx.x * 8;
+      mt[0] = s;
+      mt[1] = rel0; mt[2] = rel1; mt[3] = rel2;
+      mt[4] = max(0, -rel0) | (min(TAPS, te0 - rel0) << 16);
+      mt[5] = max(0, -rel1) | (min(TAPS, te1 - rel1) << 16);
+      mt[6] = (D == 3) ? (max(0, -rel2) | (min(TAPS, te2 - rel2) << 16)) : (0 | (1 << 16));
+      const float* ps = a.psi + (int64_t)s * NPSI;
+#pragma unroll
+      for (int q = 0; q < NPSI; ++q) bpsi[threadIdx.x * NPSI + q] = ps[q];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nb * CG; idx += blockDim.x) {
+      const int v = idx / CG, cc = idx % CG;
+      const int s = bmeta[v * 8];
+      float2 yv = make_float2(0.f, 0.f);
+      if (cbase + cc < a.C) {
+        yv = a.y[(int64_t)s * a.C + cbase + cc];
+        if (a.w) { const float ww = a.w[s]; yv.x *= ww; yv.y *= ww; }
+      }
+      ybat[idx] = yv;
+    }
+    __syncthreads();
+    for (int v = 0; v < nb; ++v) {
+      const int* mt = bmeta + v * 8;
+      const int rel0 = mt[1], rel1 = mt[2], rel2 = mt[3];
+      const int lo0 = mt[4] & 0xffff, hi0 = mt[4] >> 16;
+      const int lo1 = mt[5] & 0xffff, hi1 = mt[5] >> 16;
+      const int lo2 = mt[6] & 0xffff, hi2 = mt[6] >> 16;
+      const float* ps = bpsi + v * NPSI;
+      const float2 yv = ybat[v * CG + cl];
+      float p0v[TAPS];
+#pragma unroll
+      for (int s0 = 0; s0 < TAPS; ++s0) p0v[s0] = ps[s0];
+      // this slot owns the tile rows x1 == slot (mod nslots) of every plane
+      const int r1a = rel1 + lo1, r1b = rel1 + hi1;
+      int first = r1a + ((slot - r1a) % nslots + nslots) % nslots;
+      for (int s2 = lo2; s2 < hi2; ++s2) {
+        const float w2 = (D == 3) ? ps[2 * TAPS + s2] : 1.f;
+        const int plane = (D == 3) ? (rel2 + s2) * T : 0;
+        for (int x1 = first; x1 < r1b; x1 += nslots) {
+          const int s1 = x1 - rel1;
+          const float wr = ps[TAPS + s1] * w2;
+          const float vx = yv.x * wr, vy = yv.y * wr;
+          float2* dst = tile + (x1 + plane) * rowstride + rel0 * CG + cl;
+          // all reads of the row first (distinct cells), then the updates
+          float2 cur[TAPS];
+#pragma unroll
+          for (int s0 = 0; s0 < TAPS; ++s0)
+            if (s0 >= lo0 && s0 < hi0) cur[s0] = dst[s0 * CG];
+#pragma unroll
+          for (int s0 = 0; s0 < TAPS; ++s0)
+            if (s0 >= lo0 && s0 < hi0) {
+              cur[s0].x += p0v[s0] * vx;
+              cur[s0].y += p0v[s0] * vy;
+              dst[s0 * CG] = cur[s0];
+            }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  const int ncell = te0 * te1 * te2;
+  for (int idx = threadIdx.x; idx < ncell * CG; idx += blockDim.x) {
+    const int cc = idx % CG, cell = idx / CG;
+    const int cg = cbase + cc;
+    if (cg >= a.C) continue;
+    const int x0 = cell % te0, x1 = (cell / te0) % te1, x2 = cell / (te0 * te1);
+    const float2 v = tile[(x1 + x2 * T) * rowstride + x0 * CG + cc];
+    if (it.slot < 0) {
+      const int64_t l = (ts0 + x0) + (int64_t)a.n0 * ((ts1 + x1) + (int64_t)a.n1 * (ts2 + x2));
+      a.grid[((int64_t)it.frame * a.ntot + l) * a.C + cg] = v;
+    } else {
+      const int64_t lc = x0 + (int64_t)T * (x1 + (int64_t)T * x2);
+      a.partial[((int64_t)it.slot * a.tilecells + lc) * a.C + cg] = v;
+    }
+  }
+}
+
+size_t grid_batched_lds(int d, int T, int taps, int CG, int rowstride) {
+  const int nrows_tile = (d == 3) ? T * T : T;
+  return sizeof(float2) * ((size_t)nrows_tile * rowstride + 64 * CG) + sizeof(float) * 64 * d * taps +
+         sizeof(int) * 64 * 8;
+}
+
+template <int CG, int D>
+static void grid_batched_taps(const GridArgs& a, dim3 g, dim3 b, size_t shm, hipStream_t s) {
+  switch (a.taps) {
+    case 4: hipLaunchKernelGGL((k_grid_batched<CG, 4, D>), g, b, shm, s, a); break;
+    case 6: hipLaunchKernelGGL((k_grid_batched<CG, 6, D>), g, b, shm, s, a); break;
+    case 8: hipLaunchKernelGGL((k_grid_batched<CG, 8, D>), g, b, shm, s, a); break;
+    case 10: hipLaunchKernelGGL((k_grid_batched<CG, 10, D>), g, b, shm, s, a); break;
+    default: break;
+  }
+}
+
+void launch_grid_batched(const GridArgs& a, int cg, int waves, int nitems, int ncgroups, hipStream_t s) {
+  if (nitems == 0) return;
+  const dim3 g((unsigned)nitems, (unsigned)ncgroups), b((unsigned)(64 * waves));
+  const size_t shm = grid_batched_lds(a.d, a.T, a.taps, cg, a.rowstride);
+  if (a.d == 2) {
+    if (cg == 8) grid_batched_taps<8, 2>(a, g, b, shm, s);
+    else if (cg == 16) grid_batched_taps<16, 2>(a, g, b, shm, s);
+    else grid_batched_taps<32, 2>(a, g, b, shm, s);
+  } else {
+    if (cg == 8) grid_batched_taps<8, 3>(a, g, b, shm, s);
+    else if (cg == 16) grid_batched_taps<16, 3>(a, g, b, shm, s);
+    else grid_batched_taps<32, 3>(a, g, b, shm, s);
+  }
+}
+
+size_t interp_lds_bytes(int T, int taps, int CG, int rowstride, int maxs) {
+  const int E = T + taps - 1;
+  return sizeof(float2) * (size_t)E * rowstride + sizeof(float) * (size_t)maxs * 2 * taps +
+         sizeof(int) * (size_t)maxs * 2;
+}
+
+template <int CG>
+static void interp_lds_taps(const SampArgs& a, const int4* items, int nitems, int T, int rowstride, int maxs,
+                            int nt0, dim3 g, size_t shm, hipStream_t s) {
+  switch (a.taps) {
+    case 4: hipLaunchKernelGGL((k_interp_lds<CG, 4>), g, dim3(256), shm, s, a, items, nitems, T, rowstride, maxs, nt0); break;
+    case 6: hipLaunchKernelGGL((k_interp_lds<CG, 6>), g, dim3(256), shm, s, a, items, nitems, T, rowstride, maxs, nt0); break;
+    case 8: hipLaunchKernelGGL((k_interp_lds<CG, 8>), g, dim3(256), shm, s, a, items, nitems, T, rowstride, maxs, nt0); break;
+    case 10: hipLaunchKernelGGL((k_interp_lds<CG, 10>), g, dim3(256), shm, s, a, items, nitems, T, rowstride, maxs, nt0); break;
+    default: break;
+  }
+}
+
+void launch_interp_lds(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int cg, int maxs,
+                       int ncgroups, hipStream_t s) {
+  if (nitems == 0) return;
+  const int rowstride = (T + a.taps - 1) * cg + ((cg == 8) ? 8 : 0);
+  const size_t shm = interp_lds_bytes(T, a.taps, cg, rowstride, maxs);
+  const dim3 g((unsigned)nitems, (unsigned)ncgroups);
+  if (cg == 8) interp_lds_taps<8>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, s);
+  else if (cg == 16) interp_lds_taps<16>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, s);
+  else interp_lds_taps<32>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, s);
+}
+
+// ---------------------------------------------------------------------------
 template <int CG, int D>
 static void interp_taps(const SampArgs& a, dim3 g, hipStream_t s) {
   switch (a.taps) {

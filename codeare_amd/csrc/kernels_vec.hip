@@ -501,6 +501,22 @@ void launch_cnr_update_samples(const Plan& p, float2* r, const float2* v, hipStr
                      p.fmask);
 }
 
+// Toeplitz spectrum: grid order Khat[f][r + cells*pos] -> conv-pass layout
+// That[(f*cells + r)*nl + pos] = Re(Khat) * scale
+__global__ void k_toep_layout(const float2* Khat, float* That, int64_t cells, int nl, int F, float scale) {
+  const int64_t tot = (int64_t)F * cells * nl;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int pos = (int)(idx % nl);
+    const int64_t fr = idx / nl;
+    const int64_t r = fr % cells, f = fr / cells;
+    That[idx] = Khat[f * cells * nl + r + cells * pos].x * scale;
+  }
+}
+
+void launch_toep_layout(const float2* Khat, float* That, int64_t cells, int nl, int F, float scale, hipStream_t s) {
+  hipLaunchKernelGGL(k_toep_layout, dim3(2048), dim3(256), 0, s, Khat, That, cells, nl, F, scale);
+}
+
 void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
   (void)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
 }

@@ -16,6 +16,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <cstdlib>
 #include <vector>
 
 #include "cgs_internal.hpp"
@@ -65,6 +66,10 @@ static double kb_phi(double t, int m, double b) {
   return b / M_PI;
 }
 
+static bool factorize(int n, int* radix, int* nst);
+
+static int make_tab(FftTab& t, int n);
+
 static bool factorize(int n, int* radix, int* nst) {
   int c2 = 0, r = n;
   while (r % 2 == 0) { r /= 2; ++c2; }
@@ -98,6 +103,22 @@ template <typename T>
 static void dfree(T*& ptr) {
   if (ptr) (void)hipFree((void*)ptr);
   ptr = nullptr;
+}
+
+static int make_tab(FftTab& t, int n) {
+  t.n = n;
+  if (!factorize(n, t.radix, &t.nst)) {
+    set_error("FFT length " + std::to_string(n) + " is not 2^a 3^b");
+    return CGS_ERR_UNSUPPORTED;
+  }
+  std::vector<float2> tw(n);
+  for (int q = 0; q < n; ++q) {
+    const double ang = -2.0 * M_PI * (double)q / (double)n;
+    tw[q] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+  CHK(dalloc(&t.tw, tw.size()));
+  HIPCHK(hipMemcpy(t.tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
+  return CGS_OK;
 }
 
 // ---- profiling -----------------------------------------------------------------
@@ -152,13 +173,15 @@ static int pick_lb(int64_t inner, int n) {
   return lb;
 }
 
-static FftArgs fft_base(const Plan& p, int axis, int sign) {
+static FftArgs fft_base(const Plan& p, const FftTab& tab, int sign) {
   FftArgs a{};
-  a.n = p.n[axis];
+  a.n = tab.n;
   a.sign = sign;
-  a.tw = p.tw[axis];
-  a.nst = p.nst[axis];
-  for (int k = 0; k < 8; ++k) a.radix[k] = p.radix[axis][k];
+  a.pad_in = sign < 0;    // forward passes read image-index order
+  a.crop_out = sign > 0;  // adjoint passes write image-index order
+  a.tw = tab.tw;
+  a.nst = tab.nst;
+  for (int k = 0; k < 8; ++k) a.radix[k] = tab.radix[k];
   a.dinv0 = p.dinv[0];
   a.dinv1 = p.dinv[1];
   a.dinv2 = p.d == 3 ? p.dinv[2] : p.dinv[1];
@@ -174,7 +197,14 @@ static FftArgs fft_base(const Plan& p, int axis, int sign) {
 
 static void fft_finish(FftArgs& a) {
   a.lb = pick_lb(a.inner, a.n);
+  a.lb_log2 = 0;
+  while ((1 << a.lb_log2) < a.lb) ++a.lb_log2;
   a.ldn = a.n + 1;
+  uint64_t Ns = 1;
+  for (int s = 0; s < a.nst; ++s) {
+    a.ns_magic[s] = (Ns == 1) ? 0u : (uint32_t)(((1ull << 32) + Ns - 1) / Ns);
+    Ns *= (uint64_t)a.radix[s];
+  }
 }
 
 // image (mode 1: x .* S_c; 2: X; 3: X .* w_hat) -> grid, forward transform
@@ -184,7 +214,7 @@ static int fft_forward(Plan& p, int mode, const float2* src, const int* fl) {
   double bytes = (mode == 1 ? (double)(1 + C) * p.Ntot * 8 : 2.0 * C * p.Ntot * 8) * F;
   // pass 1, axis 0
   {
-    FftArgs a = fft_base(p, 0, -1);
+    FftArgs a = fft_base(p, p.fa[0], -1);
     a.in = mode == 1 ? nullptr : src;
     a.x = mode == 1 ? src : nullptr;
     a.out = (d == 2) ? p.t1 : p.t1;
@@ -200,7 +230,7 @@ static int fft_forward(Plan& p, int mode, const float2* src, const int* fl) {
   }
   // pass 2, axis 1
   {
-    FftArgs a = fft_base(p, 1, -1);
+    FftArgs a = fft_base(p, p.fa[1], -1);
     a.in = p.t1;
     a.out = (d == 2) ? p.grid : p.t2;
     a.nin = (int)N1; a.nout = (int)n1;
@@ -214,7 +244,7 @@ static int fft_forward(Plan& p, int mode, const float2* src, const int* fl) {
     launch_fft(a, 0, 0, p.stream);
   }
   if (d == 3) {
-    FftArgs a = fft_base(p, 2, -1);
+    FftArgs a = fft_base(p, p.fa[2], -1);
     a.in = p.t2;
     a.out = p.grid;
     a.nin = (int)N2; a.nout = (int)n2;
@@ -236,7 +266,7 @@ static int fft_adjoint(Plan& p, float2* X, const int* fl) {
   const int64_t N0 = p.N[0], N1 = p.N[1], N2 = p.N[2], n0 = p.n[0], n1 = p.n[1], n2 = p.n[2];
   if (d == 2) {
     {
-      FftArgs a = fft_base(p, 1, +1);
+      FftArgs a = fft_base(p, p.fa[1], +1);
       a.in = p.grid; a.out = p.t1;
       a.nin = (int)n1; a.nout = (int)N1;
       a.inner = C * n0; a.outer_per_frame = 1; a.outers = F; a.fdone = fl;
@@ -245,7 +275,7 @@ static int fft_adjoint(Plan& p, float2* X, const int* fl) {
       launch_fft(a, 0, 0, p.stream);
     }
     {
-      FftArgs a = fft_base(p, 0, +1);
+      FftArgs a = fft_base(p, p.fa[0], +1);
       a.in = p.t1; a.out = X;
       a.nin = (int)n0; a.nout = (int)N0;
       a.inner = C; a.outer_per_frame = N1; a.outers = N1 * F; a.fdone = fl;
@@ -255,7 +285,7 @@ static int fft_adjoint(Plan& p, float2* X, const int* fl) {
     }
   } else {
     {
-      FftArgs a = fft_base(p, 2, +1);
+      FftArgs a = fft_base(p, p.fa[2], +1);
       a.in = p.grid; a.out = p.t2;
       a.nin = (int)n2; a.nout = (int)N2;
       a.inner = C * n0 * n1; a.outer_per_frame = 1; a.outers = F; a.fdone = fl;
@@ -264,7 +294,7 @@ static int fft_adjoint(Plan& p, float2* X, const int* fl) {
       launch_fft(a, 0, 0, p.stream);
     }
     {
-      FftArgs a = fft_base(p, 1, +1);
+      FftArgs a = fft_base(p, p.fa[1], +1);
       a.in = p.t2; a.out = p.t1;
       a.nin = (int)n1; a.nout = (int)N1;
       a.inner = C * n0; a.outer_per_frame = N2; a.outers = N2 * F; a.fdone = fl;
@@ -273,7 +303,7 @@ static int fft_adjoint(Plan& p, float2* X, const int* fl) {
       launch_fft(a, 0, 0, p.stream);
     }
     {
-      FftArgs a = fft_base(p, 0, +1);
+      FftArgs a = fft_base(p, p.fa[0], +1);
       a.in = p.t1; a.out = X;
       a.nin = (int)n0; a.nout = (int)N0;
       a.inner = C; a.outer_per_frame = N1 * N2; a.outers = N1 * N2 * F; a.fdone = fl;
@@ -297,7 +327,12 @@ static int interp(Plan& p, float2* y, const int* fl) {
   a.fdone = fl;
   const double bytes = (double)p.Mtot * p.d * 4 + (double)p.C * p.ntot * p.F * 8 + (double)p.C * p.Mtot * 8;
   Prof pr(p, CGS_K_INTERP, bytes);
-  launch_interp(a, p.interp_cg, p.stream);
+  if (p.d == 2 && p.use_interp_lds) {
+    const int ng = (p.C + p.interp_cg - 1) / p.interp_cg;
+    launch_interp_lds(a, p.tile_items, p.n_tile_items, p.T, p.nt[0], p.interp_cg, p.tile_item_max, ng, p.stream);
+  } else {
+    launch_interp(a, p.interp_cg, p.stream);
+  }
   return CGS_OK;
 }
 
@@ -307,8 +342,9 @@ static int gridding(Plan& p, const float2* y, bool weighted, const int* fl) {
   a.y = y; a.grid = p.grid; a.partial = p.partial;
   a.items = p.grid_items; a.visits = p.visits; a.fdone = fl;
   a.d = p.d; a.taps = p.taps; a.C = p.C; a.T = p.T;
-  a.CGW = p.grid_cg * p.grid_w;
-  a.rowstride = p.T * a.CGW + ((p.grid_cg == 8) ? 8 : 0);
+  const bool batched = p.use_grid_batched;
+  a.CGW = batched ? p.grid_cg : p.grid_cg * p.grid_w;
+  a.rowstride = p.T * a.CGW + ((a.CGW == 8) ? 8 : 0);
   a.n0 = p.n[0]; a.n1 = p.n[1]; a.n2 = p.n[2];
   a.nt0 = p.nt[0]; a.nt1 = p.nt[1]; a.nt2 = p.nt[2];
   a.ntot = p.ntot;
@@ -317,7 +353,10 @@ static int gridding(Plan& p, const float2* y, bool weighted, const int* fl) {
   const double bytes = (double)p.Mtot * p.d * 4 + (double)p.Mtot * 4 + (double)p.C * p.Mtot * 8 +
                        (double)p.C * p.ntot * p.F * 8;
   Prof pr(p, CGS_K_GRID, bytes);
-  launch_grid(a, p.grid_cg, p.grid_w, p.n_grid_items, ngroups, p.stream);
+  if (batched)
+    launch_grid_batched(a, p.grid_cg, p.grid_w, p.n_grid_items, ngroups, p.stream);
+  else
+    launch_grid(a, p.grid_cg, p.grid_w, p.n_grid_items, ngroups, p.stream);
   launch_grid_reduce(p, p.partial, p.grid, p.stream);
   return CGS_OK;
 }
@@ -409,10 +448,16 @@ static int op_EH(Plan& p, int mode, const float2* y, float2* x, int ftiter, doub
   return op_EH_true(p, y, x, pv, lambda, partials, fl);
 }
 
+static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma, int m, int C, int F,
+                       const int64_t* foff, unsigned flags);
+static void destroy_plan(Plan& p);
+static void toep_drop(Plan& p);
+
 // ---- trajectory presort -----------------------------------------------------------------
 static void free_samples(Plan& p) {
   dfree(p.base); dfree(p.psi); dfree(p.perm); dfree(p.sframe);
-  dfree(p.interp_items); dfree(p.grid_items); dfree(p.reduce_items); dfree(p.visits); dfree(p.partial);
+  dfree(p.interp_items); dfree(p.tile_items); dfree(p.grid_items);
+  p.n_tile_items = 0; dfree(p.reduce_items); dfree(p.visits); dfree(p.partial);
   p.n_interp_items = p.n_grid_items = p.n_reduce_items = 0;
   p.n_visits = 0;
   p.n_slots = 0;
@@ -490,6 +535,15 @@ static int build_samples(Plan& p, const float* k) {
   for (int f = 0; f < p.F; ++f)
     for (int64_t s = p.foff[f]; s < p.foff[f + 1]; s += ipw)
       items.push_back(make_int4(f, (int)s, (int)std::min<int64_t>(ipw, p.foff[f + 1] - s), 0));
+  // tile-level interp items {frame, tile, sbeg, send}: the samples whose base
+  // cell lies in one tile, at most tile_item_max per item (heaviest first)
+  std::vector<int4> titems;
+  for (int64_t kk = 0; kk < nkeys; ++kk) {
+    const int f = (int)(kk / tpf), t = (int)(kk % tpf);
+    for (int64_t s = cnt[kk]; s < cnt[kk + 1]; s += p.tile_item_max)
+      titems.push_back(make_int4(f, t, (int)s, (int)std::min<int64_t>(s + p.tile_item_max, cnt[kk + 1])));
+  }
+  std::stable_sort(titems.begin(), titems.end(), [](const int4& x, const int4& y) { return (x.w - x.z) > (y.w - y.z); });
   // gridding visit lists: every (tile, sample) pair whose footprint meets the tile
   std::vector<int64_t> vcnt((size_t)nkeys + 1, 0);
   auto tiles_of = [&](int64_t s, int a, int* out) {
@@ -559,6 +613,10 @@ static int build_samples(Plan& p, const float* k) {
   CHK(dalloc(&p.perm, perm.size()));
   CHK(dalloc(&p.sframe, hframe.size()));
   CHK(dalloc(&p.interp_items, items.size()));
+  CHK(dalloc(&p.tile_items, titems.size()));
+  if (!titems.empty())
+    HIPCHK(hipMemcpy(p.tile_items, titems.data(), sizeof(int4) * titems.size(), hipMemcpyHostToDevice));
+  p.n_tile_items = (int)titems.size();
   CHK(dalloc(&p.grid_items, gitems.size()));
   CHK(dalloc(&p.reduce_items, ritems.size()));
   CHK(dalloc(&p.visits, vis.size()));
@@ -588,7 +646,8 @@ static int build_samples(Plan& p, const float* k) {
 static void destroy_plan(Plan& p) {
   if (p.stream) (void)hipStreamSynchronize(p.stream);
   free_samples(p);
-  for (int a = 0; a < 3; ++a) { dfree(p.tw[a]); dfree(p.dinv[a]); }
+  for (int a = 0; a < 3; ++a) { dfree(p.fa[a].tw); dfree(p.ft[a].tw); dfree(p.dinv[a]); }
+  toep_drop(p);
   dfree(p.d_foff); dfree(p.S); dfree(p.Imap); dfree(p.w);
   dfree(p.grid); dfree(p.t1); dfree(p.t2); dfree(p.Xc); dfree(p.Xc2); dfree(p.Xc3);
   for (auto& b : p.ybuf) dfree(b);
@@ -618,9 +677,12 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
     if (N[a] < 2 || (N[a] & 1)) { set_error("image sides must be even and >= 2"); return CGS_ERR_ARG; }
     p.N[a] = N[a];
     p.n[a] = (int)std::ceil(sigma * (double)N[a]);   // NFFT.hpp:108-109
-    if (!factorize(p.n[a], p.radix[a], &p.nst[a])) {
-      set_error("oversampled size " + std::to_string(p.n[a]) + " is not 2^a 3^b");
-      return CGS_ERR_UNSUPPORTED;
+    {
+      int rtmp[8], ntmp = 0;
+      if (!factorize(p.n[a], rtmp, &ntmp)) {
+        set_error("oversampled size " + std::to_string(p.n[a]) + " is not 2^a 3^b");
+        return CGS_ERR_UNSUPPORTED;
+      }
     }
     if (p.n[a] < 2 * p.taps) { set_error("oversampled grid smaller than two window widths"); return CGS_ERR_UNSUPPORTED; }
     if (p.n[a] > 4096) { set_error("oversampled side above 4096"); return CGS_ERR_UNSUPPORTED; }
@@ -643,10 +705,35 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   p.T = (d == 3) ? 8 : 8;
   for (int a = 0; a < 3; ++a) p.nt[a] = (a < d) ? (p.n[a] + p.T - 1) / p.T : 1;
   p.tiles_per_frame = (int64_t)p.nt[0] * p.nt[1] * p.nt[2];
-  p.grid_cg = 8;
-  p.grid_w = std::max(1, std::min(4, (C + 7) / 8));
+  // batched gridding: CG coils per block, W = CG/8 waves so that the W*RS = 8
+  // row slots divide the tile side
+  p.use_grid_batched = true;
+  p.grid_cg = (C <= 8) ? 8 : (C <= 16 ? 16 : 32);
+  p.grid_w = p.grid_cg / 8;
   p.interp_cg = (C <= 8) ? 8 : (C <= 16 ? 16 : 32);
-  p.visits_per_item = (d == 3) ? 64 : 256;
+  p.use_interp_lds = (d == 2);
+  p.tile_item_max = 64;
+  p.visits_per_item = (d == 3) ? 128 : 512;
+  // shape overrides for tuning sweeps (tile side must be a multiple of 8)
+  if (const char* e = std::getenv("CGS_TILE")) {
+    const int t = std::atoi(e);
+    if (t >= 8 && t % 8 == 0) {
+      p.T = t;
+      for (int a = 0; a < 3; ++a) p.nt[a] = (a < d) ? (p.n[a] + p.T - 1) / p.T : 1;
+      p.tiles_per_frame = (int64_t)p.nt[0] * p.nt[1] * p.nt[2];
+    }
+  }
+  if (const char* e = std::getenv("CGS_GRID_CG")) {
+    const int g = std::atoi(e);
+    if (g == 8 || g == 16 || g == 32) { p.grid_cg = g; p.grid_w = g / 8; }
+  }
+  if (const char* e = std::getenv("CGS_INTERP_CG")) {
+    const int g = std::atoi(e);
+    if (g == 8 || g == 16 || g == 32) p.interp_cg = g;
+  }
+  if (const char* e = std::getenv("CGS_VISITS_PER_ITEM")) p.visits_per_item = std::max(16, std::atoi(e));
+  if (const char* e = std::getenv("CGS_TILE_ITEM_MAX")) p.tile_item_max = std::max(8, std::atoi(e));
+  if (const char* e = std::getenv("CGS_INTERP_LDS")) p.use_interp_lds = (d == 2) && std::atoi(e) != 0;
 
   if (flags & 0x8000u) {   // internal: caller-provided stream set later
   }
@@ -654,14 +741,9 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   p.own_stream = true;
 
   // tables
+  p.sigma = sigma;
   for (int a = 0; a < d; ++a) {
-    std::vector<float2> tw(p.n[a]);
-    for (int q = 0; q < p.n[a]; ++q) {
-      const double ang = -2.0 * M_PI * (double)q / (double)p.n[a];
-      tw[q] = make_float2((float)std::cos(ang), (float)std::sin(ang));
-    }
-    CHK(dalloc(&p.tw[a], tw.size()));
-    HIPCHK(hipMemcpy(p.tw[a], tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
+    CHK(make_tab(p.fa[a], p.n[a]));
     std::vector<float> di(p.N[a]);
     for (int i = 0; i < p.N[a]; ++i) {
       const double kk = 2.0 * M_PI * (double)(i - p.N[a] / 2) / (double)p.n[a];
@@ -743,6 +825,137 @@ static int check_launch() {
   return CGS_OK;
 }
 
+// ---- Toeplitz embedding of E^H W E ------------------------------------------------------
+// (E^H W E x)(i) = sum_c conj(S_c(i)) sum_i' K(i - i') S_c(i') x(i'),
+// K(D) = sum_j w_j exp(+2 pi i k_j . D), D in [-N, N)^d.
+// K is the adjoint NFFT of the weights onto a (2N)^d image (a helper plan with
+// image side 2N); its (2N)^d FFT divided by (2N)^d is the real spectrum That,
+// so the coil convolution becomes zero-pad -> FFT_2N -> .* That -> IFFT_2N -> crop.
+// The gridding and the window gather then leave the CG loop entirely.
+static int toeplitz_prepare(Plan& p) {
+  if (p.have_toep) return CGS_OK;
+  if (p.h_k.empty() || p.h_w.empty()) { set_error("Toeplitz operator needs trajectory and weights"); return CGS_ERR_STATE; }
+  const int d = p.d, F = p.F;
+  int N2[3] = {1, 1, 1};
+  int64_t tot2 = 1;
+  for (int a = 0; a < d; ++a) {
+    N2[a] = 2 * p.N[a];
+    tot2 *= N2[a];
+    if (!p.ft[a].tw) CHK(make_tab(p.ft[a], N2[a]));
+  }
+  HIPCHK(hipStreamSynchronize(p.stream));
+  Plan q;
+  int rc = create_plan(q, d, N2, p.Mtot, 2.0, p.m, 1, F, p.foff.data(), CGS_DEVICE_IO);
+  if (rc == CGS_OK) rc = build_samples(q, p.h_k.data());
+  if (rc != CGS_OK) { destroy_plan(q); return rc; }
+  {
+    std::vector<float2> wy((size_t)q.Mtot);
+    for (int64_t s = 0; s < q.Mtot; ++s) wy[s] = make_float2(p.h_w[q.h_perm[s]], 0.f);
+    hipError_t e = hipMemcpy(q.ybuf[0], wy.data(), sizeof(float2) * wy.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { destroy_plan(q); set_error("copy of weights failed"); return CGS_ERR_HIP; }
+  }
+  gridding(q, q.ybuf[0], false, nullptr);   // K(D) per frame -> q.Xc [f][(2N)^d]
+  fft_adjoint(q, q.Xc, nullptr);
+  // forward (2N)^d FFT of K, with K(D) at bin D mod 2N (the pad mapping with nin = n)
+  const float2* src = q.Xc;
+  float2* bufs[2] = {q.t1, q.grid};
+  int64_t lower = 1, upper = tot2;
+  for (int a = 0; a < d; ++a) {
+    upper /= N2[a];
+    FftArgs fa = fft_base(q, p.ft[a], -1);
+    fa.in = src;
+    fa.out = bufs[a & 1];
+    fa.nin = fa.nout = N2[a];
+    fa.inner = lower;
+    fa.outer_per_frame = upper;
+    fa.outers = upper * F;
+    fft_finish(fa);
+    launch_fft(fa, 0, 0, q.stream);
+    src = fa.out;
+    lower *= N2[a];
+  }
+  rc = CGS_OK;
+  {
+    const int64_t cells = tot2 / N2[d - 1];
+    hipError_t e = hipMalloc((void**)&p.That, sizeof(float) * (size_t)tot2 * F);
+    if (e != hipSuccess) { destroy_plan(q); set_error("allocation of the Toeplitz spectrum failed"); return CGS_ERR_ALLOC; }
+    launch_toep_layout(src, p.That, cells, N2[d - 1], F, 1.0f / (float)tot2, q.stream);
+  }
+  hipError_t e = hipStreamSynchronize(q.stream);
+  destroy_plan(q);
+  if (e != hipSuccess) { set_error(std::string("Toeplitz preparation failed: ") + hipGetErrorString(e)); return CGS_ERR_HIP; }
+  // intermediates: alias the NFFT ones when n == 2N
+  bool alias = true;
+  for (int a = 0; a < d; ++a) alias &= (p.n[a] == N2[a]);
+  if (alias) {
+    p.tt1 = p.t1;
+    p.tt2 = p.t2;
+  } else {
+    CHK(dalloc(&p.tt1, (size_t)p.C * N2[0] * p.N[1] * (d == 3 ? p.N[2] : 1) * F));
+    if (d == 3) CHK(dalloc(&p.tt2, (size_t)p.C * N2[0] * N2[1] * p.N[2] * F));
+  }
+  p.tt_alias = alias;
+  p.have_toep = true;
+  return CGS_OK;
+}
+
+static void toep_drop(Plan& p) {
+  dfree(p.That);
+  if (!p.tt_alias) { dfree(p.tt1); dfree(p.tt2); }
+  p.tt1 = p.tt2 = nullptr;
+  p.have_toep = false;
+}
+
+// q = E^H W E x + lambda x with the Toeplitz embedding (and partial <x, q>)
+static int op_normal_toeplitz(Plan& p, const float2* x, float2* qo, float lambda, double* partials,
+                              const int* fl) {
+  const int d = p.d, C = p.C, F = p.F;
+  const int64_t N0 = p.N[0], N1 = p.N[1], N2 = p.N[2];
+  const int64_t M0 = 2 * N0, M1 = 2 * N1;
+  auto pass = [&](const FftTab& tab, int sign, const float2* in, const float2* xin, float2* out, int nin,
+                  int nout, int64_t inner, int64_t opf, int ld, const float* That, int64_t cells, double bytes) {
+    FftArgs a = fft_base(p, tab, sign);
+    a.in = in; a.x = xin; a.out = out;
+    a.nin = nin; a.nout = nout;
+    a.inner = inner; a.outer_per_frame = opf; a.outers = opf * F;
+    a.fdone = fl;
+    if (That) { a.That = That; a.conv_cells = cells; a.pad_in = 1; a.crop_out = 1; }
+    fft_finish(a);
+    Prof pr(p, sign < 0 ? CGS_K_FFT_FWD : CGS_K_FFT_ADJ, bytes * F);
+    launch_fft(a, ld, 0, p.stream);
+  };
+  if (d == 2) {
+    pass(p.ft[0], -1, nullptr, x, p.tt1, (int)N0, (int)M0, C, N1, 4, nullptr, 0,
+         ((double)(1 + C) * N0 * N1 + (double)C * M0 * N1) * 8);
+    pass(p.ft[1], -1, p.tt1, nullptr, p.tt1, (int)N1, (int)N1, C * M0, 1, 0, p.That, M0,
+         2.0 * C * M0 * N1 * 8 + 4.0 * M0 * M1);
+    pass(p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1, 0, nullptr, 0,
+         (double)C * (M0 + N0) * N1 * 8);
+  } else {
+    pass(p.ft[0], -1, nullptr, x, p.tt1, (int)N0, (int)M0, C, N1 * N2, 4, nullptr, 0,
+         ((double)(1 + C) * N0 * N1 * N2 + (double)C * M0 * N1 * N2) * 8);
+    pass(p.ft[1], -1, p.tt1, nullptr, p.tt2, (int)N1, (int)M1, C * M0, N2, 0, nullptr, 0,
+         (double)C * M0 * (N1 + M1) * N2 * 8);
+    pass(p.ft[2], -1, p.tt2, nullptr, p.tt2, (int)N2, (int)N2, C * M0 * M1, 1, 0, p.That, M0 * M1,
+         2.0 * C * M0 * M1 * N2 * 8 + 4.0 * M0 * M1 * 2 * N2);
+    pass(p.ft[1], +1, p.tt2, nullptr, p.tt1, (int)M1, (int)N1, C * M0, N2, 0, nullptr, 0,
+         (double)C * M0 * (M1 + N1) * N2 * 8);
+    pass(p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1 * N2, 0, nullptr, 0,
+         (double)C * (M0 + N0) * N1 * N2 * 8);
+  }
+  Prof pr(p, CGS_K_COIL, coil_bytes(p, true));
+  launch_coil_combine(p, p.Xc, qo, x, lambda, false, partials, p.stream);
+  return CGS_OK;
+}
+
+// one application of the CG operator (E^H W E + lambda) for p, with <p, q> partials
+static int op_normal(Plan& p, int normal_op, const float2* x, float2* qo, float lambda, double* partials,
+                     const int* fl) {
+  if (normal_op == CGS_NORMAL_TOEPLITZ) return op_normal_toeplitz(p, x, qo, lambda, partials, fl);
+  CHK(op_E(p, x, p.ybuf[1], fl));
+  return op_EH_true(p, p.ybuf[1], qo, x, lambda, partials, fl);
+}
+
 // ---- CG solve -------------------------------------------------------------------------
 static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, const cgs_solve_opts& o,
                  cgs_solve_report* rep) {
@@ -766,10 +979,15 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
   // b = E^H y  -> r
   CHK(op_EH(p, o.eh_mode, p.ybuf[0], r, o.ftiter, o.fteps, nullptr, 0.f, nullptr, p.fdone));
   const bool warm = x0 != nullptr;
+  if (o.eh_mode == CGS_EH_TRUE && o.normal_op == CGS_NORMAL_TOEPLITZ) CHK(toeplitz_prepare(p));
   if (warm) {
     HIPCHK(hipMemcpyAsync(x, x0, ib, hipMemcpyDeviceToDevice, p.stream));
-    CHK(op_E(p, x, p.ybuf[1], p.fdone));
-    CHK(op_EH(p, o.eh_mode, p.ybuf[1], q, o.ftiter, o.fteps, x, (float)o.lambda, nullptr, p.fdone));
+    if (o.eh_mode == CGS_EH_TRUE) {
+      CHK(op_normal(p, o.normal_op, x, q, (float)o.lambda, nullptr, p.fdone));
+    } else {
+      CHK(op_E(p, x, p.ybuf[1], p.fdone));
+      CHK(op_EH(p, o.eh_mode, p.ybuf[1], q, o.ftiter, o.fteps, x, (float)o.lambda, nullptr, p.fdone));
+    }
     launch_sqnorm_partials(p, r, partA, p.stream);             // ||b||^2
     launch_axpby_img(p, r, r, 1.f, q, -1.f, p.stream);          // r = b - (A + lambda) x0
     launch_sqnorm_partials(p, r, partB, p.stream);
@@ -786,8 +1004,12 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
   HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
   int status = CGS_OK;
   for (int it = 0; it < o.maxit; ++it) {
-    CHK(op_E(p, pv, p.ybuf[1], p.fdone));
-    CHK(op_EH(p, o.eh_mode, p.ybuf[1], q, o.ftiter, o.fteps, pv, (float)o.lambda, p.partials, p.fdone));
+    if (o.eh_mode == CGS_EH_TRUE) {
+      CHK(op_normal(p, o.normal_op, pv, q, (float)o.lambda, p.partials, p.fdone));
+    } else {
+      CHK(op_E(p, pv, p.ybuf[1], p.fdone));
+      CHK(op_EH(p, o.eh_mode, p.ybuf[1], q, o.ftiter, o.fteps, pv, (float)o.lambda, p.partials, p.fdone));
+    }
     {
       Prof pr(p, CGS_K_CG, 7.0 * 8.0 * F * p.Ntot);
       launch_cg_alpha(p, p.partials, p.stream);
@@ -917,6 +1139,8 @@ int cgs_set_trajectory(cgs_plan plan, const float* k) {
   Plan& p = plan->p;
   HIPCHK(hipStreamSynchronize(p.stream));
   CHK(build_samples(p, k));
+  p.h_k.assign(k, k + (size_t)p.d * p.Mtot);
+  toep_drop(p);
   p.have_k = true;
   p.have_w = false;   // weights follow the sample order (NFFT.hpp:258-296: Weights after KSpace)
   return CGS_OK;
@@ -932,6 +1156,8 @@ int cgs_set_weights(cgs_plan plan, const float* w) {
   for (int64_t s = 0; s < p.Mtot; ++s) ws[s] = w[p.h_perm[s]];
   HIPCHK(hipStreamSynchronize(p.stream));
   HIPCHK(hipMemcpy(p.w, ws.data(), sizeof(float) * ws.size(), hipMemcpyHostToDevice));
+  p.h_w.assign(w, w + (size_t)p.Mtot);
+  toep_drop(p);
   p.have_w = true;
   return CGS_OK;
   GUARD_END
@@ -1001,7 +1227,7 @@ int cgs_EH(cgs_plan plan, const void* y, void* x, int mode, int ftiter, double f
   GUARD_END
 }
 
-int cgs_normal(cgs_plan plan, const void* x, void* q, double lambda) {
+int cgs_normal(cgs_plan plan, const void* x, void* q, double lambda, int normal_op) {
   GUARD_BEGIN
   if (!plan || !x || !q) { set_error("null argument"); return CGS_ERR_ARG; }
   Plan& p = plan->p;
@@ -1013,8 +1239,9 @@ int cgs_normal(cgs_plan plan, const void* x, void* q, double lambda) {
     HIPCHK(hipMemcpyAsync(p.img[4], xd, sizeof(float2) * p.Ntot * p.F, hipMemcpyDeviceToDevice, p.stream));
     xd = p.img[4];
   }
-  CHK(op_E(p, xd, p.ybuf[1], nullptr));
-  CHK(op_EH_true(p, p.ybuf[1], qd, xd, (float)lambda, nullptr, nullptr));
+  if (normal_op != CGS_NORMAL_NUFFT && normal_op != CGS_NORMAL_TOEPLITZ) { set_error("bad normal operator"); return CGS_ERR_ARG; }
+  if (normal_op == CGS_NORMAL_TOEPLITZ) CHK(toeplitz_prepare(p));
+  CHK(op_normal(p, normal_op, xd, qd, (float)lambda, nullptr, nullptr));
   CHK(check_launch());
   CHK(stage_out(p, q, qd, sizeof(float2) * p.Ntot * p.F));
   return CGS_OK;
@@ -1092,6 +1319,11 @@ int cgs_solve(cgs_plan plan, const void* y, const void* x0, void* x, const cgs_s
   cgs_solve_opts o = *opts;
   if (o.maxit < 0 || o.ftiter < 0 || !(o.tol >= 0)) { set_error("bad solver options"); return CGS_ERR_ARG; }
   if (o.eh_mode != CGS_EH_TRUE && o.eh_mode != CGS_EH_REF) { set_error("bad E^H mode"); return CGS_ERR_ARG; }
+  if (o.normal_op != CGS_NORMAL_NUFFT && o.normal_op != CGS_NORMAL_TOEPLITZ) { set_error("bad normal operator"); return CGS_ERR_ARG; }
+  if (o.normal_op == CGS_NORMAL_TOEPLITZ && o.eh_mode != CGS_EH_TRUE) {
+    set_error("the Toeplitz normal operator exists for the true adjoint only");
+    return CGS_ERR_ARG;
+  }
   const size_t ib = sizeof(float2) * (size_t)p.Ntot * p.F;
   const float2* yd;
   CHK(stage_in(p, y, (float2*)p.ext_a, sizeof(float2) * p.Mtot * p.C, &yd));
@@ -1128,6 +1360,20 @@ int cgs_kernel_stats(cgs_plan plan, int kernel, int64_t* launches, double* ms, d
   if (bytes) *bytes = p.stat[kernel].bytes;
   if (reset) p.stat[kernel] = Plan::Stat{};
   return CGS_OK;
+  GUARD_END
+}
+
+int cgs_residual_history(cgs_plan plan, int frame, double* out, int cap) {
+  GUARD_BEGIN
+  if (!plan || !out || cap < 0) { set_error("null argument"); return -CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  if (frame < 0 || frame >= p.F || !p.hist) { set_error("no history for this frame"); return -CGS_ERR_STATE; }
+  HIPCHK(hipStreamSynchronize(p.stream));
+  CgState st;
+  HIPCHK(hipMemcpy(&st, p.cgst + frame, sizeof(CgState), hipMemcpyDeviceToHost));
+  const int n = std::min(cap, std::min(st.it + 1, p.hist_cap));
+  if (n > 0) HIPCHK(hipMemcpy(out, p.hist + (size_t)frame * p.hist_cap, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return n;
   GUARD_END
 }
 

@@ -53,6 +53,10 @@ enum {
 #define CGS_EH_TRUE        0     /* x = sum_c conj(S_c) .* NFFT^H(w .* y_c)   */
 #define CGS_EH_REF         1     /* x = I .* sum_c conj(S_c) .* ITrafo_c(y_c) (NCSENSE.hpp:289-298) */
 
+/* CG operator E^H W E of the true-adjoint solve */
+#define CGS_NORMAL_NUFFT      0  /* E then E^H: window gather, FFTs, gridding scatter */
+#define CGS_NORMAL_TOEPLITZ   1  /* Toeplitz embedding: pad, FFT_2N, .* kernel, IFFT_2N, crop */
+
 /* stopping rules of cgs_solve */
 #define CGS_TOL_NORM_RATIO    0  /* ||r|| / ||E^H y|| <= tol                   */
 #define CGS_TOL_SQUARED_RATIO 1  /* rn / xn <= tol, codeare's cgeps (CGLS.hpp:41) */
@@ -65,6 +69,7 @@ typedef struct {
   int eh_mode;        /* CGS_EH_TRUE or CGS_EH_REF                          */
   int ftiter;         /* CGNR steps of the reference E^H (codeare ftiter)   */
   double fteps;       /* CGNR absolute break on the weighted residual       */
+  int normal_op;      /* CGS_NORMAL_NUFFT or CGS_NORMAL_TOEPLITZ (true E^H) */
 } cgs_solve_opts;
 
 typedef struct {
@@ -118,14 +123,20 @@ int cgs_nfft_inverse_ref(cgs_plan plan, int frame, const void* f, void* img,
 int cgs_E(cgs_plan plan, const void* x, void* y);
 /* E^H: CGS_EH_TRUE, or CGS_EH_REF = NCSENSE::operator/ (NCSENSE.hpp:251-301) */
 int cgs_EH(cgs_plan plan, const void* y, void* x, int eh_mode, int ftiter, double fteps);
-/* E^H E x + lambda x with the true adjoint (one CG operator application). */
-int cgs_normal(cgs_plan plan, const void* x, void* q, double lambda);
+/* E^H W E x + lambda x with the true adjoint (one CG operator application),
+ * through E and E^H or through the Toeplitz embedding. */
+int cgs_normal(cgs_plan plan, const void* x, void* q, double lambda, int normal_op);
 
 /* CG solve (NCSENSE::Adjoint / operator->* -> CGLS::Solve, NCSENSE.hpp:408-414,
  * CGLS.hpp:30-59).  x0 may be NULL (zero start, as codeare) or a warm start.
  * Every frame has its own scalars and stops on its own. report may be NULL. */
 int cgs_solve(cgs_plan plan, const void* y, const void* x0, void* x,
               const cgs_solve_opts* opts, cgs_solve_report* report);
+
+/* Residual measure (tol_kind units) of the last cgs_solve for one frame, one
+ * entry per iteration starting with the initial one (CGLS's _res history,
+ * CGLS.hpp:38, here per solve).  Returns the number of entries written. */
+int cgs_residual_history(cgs_plan plan, int frame, double* out, int cap);
 
 /* --- profiling (CGS_PROFILE plans) ------------------------------------------ */
 /* kernel ids for cgs_kernel_stats */

@@ -191,3 +191,39 @@ def test_adjoint_identity_c2_shape(cgs):
     e = abs(lhs - rhs) / abs(lhs)
     print(f"adjoint identity {e:.2e}")
     assert e <= 1e-5
+
+
+@pytest.mark.parametrize("d,N,sigma,C", [(2, 64, 2.0, 4), (2, 48, 1.5, 3), (3, 16, 2.0, 2), (3, 16, 1.5, 2)])
+def test_toeplitz_normal_matches_nufft(cgs, ora, d, N, sigma, C):
+    """The Toeplitz-embedded E^H W E equals E^H W E through the NFFT (and the
+    oracle's FP64 E^H W E) to NFFT accuracy."""
+    from codeare_amd import data
+    rng = np.random.default_rng(60 + d + N)
+    M = 3000 if d == 2 else 6000
+    k = rand_traj(rng, d, M)
+    w = rng.uniform(0.1, 1.0, M).astype(np.float32)
+    S = data.coil_maps(N, d, C, rng)
+    x = rand_img(rng, (N,) * d)
+    P = cgs.Plan((N,) * d, M, sigma, 3, ncoils=C)
+    P.kspace(k)
+    P.weights(w)
+    P.sensitivities(S)
+    qn = P.normal(x, lam=0.5)
+    qt = P.normal(x, lam=0.5, toeplitz=True)
+    O = ora.NFFT((N,) * d, M, sigma, 3)
+    O.kspace(k)
+    qo = O.EH_true(S, w, O.E(S, x)) + 0.5 * x
+    e1, e2 = rel(qt, qn), rel(qt, qo)
+    print(f"d={d} N={N} sigma={sigma}: toeplitz vs nufft {e1:.2e}, vs oracle {e2:.2e}")
+    tol = 2e-5 if sigma == 2.0 else 2e-4
+    assert e1 <= tol and e2 <= tol
+
+
+def test_toeplitz_cg_matches_nufft_cg_c1(cgs, ora):
+    prob, y, P, O = _c1_problem(cgs, ora)
+    xn, rn = P.solve(y, maxit=10, tol=0.0)
+    xt, rt = P.solve(y, maxit=10, tol=0.0, toeplitz=True)
+    e = rel(xt, xn)
+    print(f"CG(10) toeplitz vs nufft image {e:.2e}")
+    assert rn["iters"][0] == rt["iters"][0] == 10
+    assert e <= 1e-4
