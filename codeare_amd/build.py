@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT_DIR, "libcgsense_hip.so")
-SOURCES = ["plan.hip", "kernels_fft.hip", "kernels_grid.hip", "kernels_vec.hip"]
+SOURCES = ["plan.hip", "kernels_fft.hip", "kernels_grid.hip", "kernels_vec.hip", "unet.hip"]
 ARCH = os.environ.get("CGS_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -60,5 +60,22 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_cpp_test() -> str:
+    """Builds tests/cpp/t_ncsense (the C++ operator facade driven like codeare's
+    CGSENSE module) against the in-tree library; run after build()."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "t_ncsense.cpp")
+    out = os.path.join(root, "tests", "cpp", "t_ncsense")
+    gxx = shutil.which("g++") or "g++"
+    cmd = [gxx, "-O2", "-std=c++17", "-Wall", src, "-I", os.path.join(root, "include"),
+           "-L", OUT_DIR, "-lcgsense_hip", "-Wl,-rpath,$ORIGIN/../../codeare_amd/lib",
+           "-Wl,-rpath-link,/opt/rocm/lib", "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed on t_ncsense.cpp:\n{r.stdout}\n{r.stderr}")
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_cpp_test())

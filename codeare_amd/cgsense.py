@@ -27,6 +27,7 @@ CGS_OK = 0
 CGS_ERR_NAN = 6
 CGS_DEVICE_IO = 0x1
 CGS_PROFILE = 0x2
+CGS_DETERMINISTIC = 0x4
 CGS_EH_TRUE = 0
 CGS_EH_REF = 1
 CGS_TOL_NORM_RATIO = 0
@@ -54,7 +55,7 @@ class CgsError(RuntimeError):
 class SolveOpts(ct.Structure):
     _fields_ = [("maxit", ct.c_int), ("tol", ct.c_double), ("tol_kind", ct.c_int),
                 ("lambda_", ct.c_double), ("eh_mode", ct.c_int), ("ftiter", ct.c_int),
-                ("fteps", ct.c_double), ("normal_op", ct.c_int)]
+                ("fteps", ct.c_double), ("normal_op", ct.c_int), ("graph_iters", ct.c_int)]
 
 
 class SolveReport(ct.Structure):
@@ -126,7 +127,8 @@ class Plan:
     """One trajectory (optionally several frames), weights, maps and buffers."""
 
     def __init__(self, N, M_total: int, sigma: float = 2.0, m: int = 3, ncoils: int = 1,
-                 frame_offsets=None, device_io: bool = False, profile: bool = False):
+                 frame_offsets=None, device_io: bool = False, profile: bool = False,
+                 deterministic: bool = False):
         L = load_library()
         self.N = tuple(int(v) for v in N)
         self.d = len(self.N)
@@ -139,7 +141,8 @@ class Plan:
         self.foff = np.ascontiguousarray(frame_offsets, dtype=np.int64)
         self.F = len(self.foff) - 1
         self.device_io = device_io
-        flags = (CGS_DEVICE_IO if device_io else 0) | (CGS_PROFILE if profile else 0)
+        flags = ((CGS_DEVICE_IO if device_io else 0) | (CGS_PROFILE if profile else 0)
+                 | (CGS_DETERMINISTIC if deterministic else 0))
         Narr = (ct.c_int * self.d)(*self.N)
         h = ct.c_void_p()
         _check(L.cgs_plan_create(ct.byref(h), self.d, ct.cast(Narr, ct.c_void_p), self.M,
@@ -269,10 +272,11 @@ class Plan:
 
     def solve(self, y, x0=None, maxit: int = 10, tol: float = 1e-5, tol_kind: int = CGS_TOL_NORM_RATIO,
               lam: float = 0.0, eh_mode: int = CGS_EH_TRUE, ftiter: int = 3, fteps: float = 1e-3,
-              out=None, check_nan: bool = True, toeplitz: bool = False):
+              out=None, check_nan: bool = True, toeplitz: bool = False, graph_iters: int = 0):
         L = load_library()
         o = SolveOpts(int(maxit), float(tol), int(tol_kind), float(lam), int(eh_mode), int(ftiter),
-                      float(fteps), CGS_NORMAL_TOEPLITZ if toeplitz else CGS_NORMAL_NUFFT)
+                      float(fteps), CGS_NORMAL_TOEPLITZ if toeplitz else CGS_NORMAL_NUFFT,
+                      int(graph_iters))
         rep = SolveReport()
         if self.device_io:
             rc = L.cgs_solve(self._h, _ptr(y), _ptr(x0) if x0 is not None else None, _ptr(out),

@@ -59,6 +59,12 @@ struct FftArgs {
   int C, d, N0, N1, mask_nh;
   int64_t Ntot;
   const int* fdone;
+  // fused CG direction update in load mode 1/4: x = r + beta * pold, written to
+  // pnew by the block that holds coil 0
+  const float2* r;
+  const float2* pold;
+  float2* pnew;
+  const CgState* cgst;
 };
 
 struct SampArgs {
@@ -113,6 +119,9 @@ struct Plan {
   int visits_per_item = 256;
   bool use_interp_lds = true;         // 2D: gather from the tile staged in LDS
   bool use_grid_batched = true;       // batched visit lists (else one visit at a time)
+  bool use_grid_reg = true;           // 2D: register accumulation, lane = cell of an 8x8 tile
+  int grid_cgr = 16;                  // coils per wave in the register kernel
+  bool use_grid_atomic = false;       // LDS float atomics variant (kept for comparison)
   int tile_item_max = 64;             // samples per tile-level interp item
 
   // stream / state
@@ -146,6 +155,7 @@ struct Plan {
   ReduceItem* reduce_items = nullptr;
   int n_reduce_items = 0;
   int* visits = nullptr;
+  int4* vrec = nullptr;               // per-visit records for the atomic gridding kernel
   int64_t n_visits = 0;
   int n_slots = 0;
   float2* partial = nullptr;
@@ -158,13 +168,14 @@ struct Plan {
   float2* Xc2 = nullptr;              // CGNR: f_hat
   float2* Xc3 = nullptr;              // CGNR: p_hat
   float2* ybuf[4] = {nullptr, nullptr, nullptr, nullptr};   // Mtot*C each
-  float2* img[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // F*Ntot each
+  float2* img[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // F*Ntot each
   float2* ext_a = nullptr;            // staging for host I/O
   float2* ext_b = nullptr;
   size_t ext_a_bytes = 0, ext_b_bytes = 0;
   double* partials = nullptr;         // reduction partials
   size_t n_partials = 0;
   CgState* cgst = nullptr;            // F
+  int* tickets = nullptr;             // 2F last-block tickets (self-resetting)
   double* cnr = nullptr;              // CGNR per (frame, coil) scalars: 8*F*C
   int* fdone = nullptr;               // F frame-done flags
   int* fmask = nullptr;               // F scratch flags (single-frame NFFT calls)
@@ -180,6 +191,11 @@ struct Plan {
   std::vector<float> h_w;             // weights copy (host, original order)
   CgState* h_cgst = nullptr;          // pinned mirror
   int* h_done = nullptr;              // pinned
+
+  // captured CG iterations (replayed by graph-mode solves)
+  bool capturing = false;
+  hipGraphExec_t graph_exec = nullptr;
+  std::string graph_key;
 
   // profiling
   struct Stat { int64_t launches = 0; double ms = 0, bytes = 0; };
@@ -198,6 +214,8 @@ void launch_grid_batched(const GridArgs& a, int cg, int waves, int nitems, int n
 void launch_interp_lds(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int cg, int maxs,
                        int ncgroups, hipStream_t s);
 size_t grid_batched_lds(int d, int T, int taps, int CG, int rowstride);
+void launch_grid_atomic(const GridArgs& a, const int4* vrec, int cg, int nitems, int ncgroups, hipStream_t s);
+void launch_grid_reg(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s);
 size_t interp_lds_bytes(int T, int taps, int CG, int rowstride, int maxs);
 void launch_coil_combine(const Plan& p, const float2* Xc, float2* x, const float2* pvec,
                          float lambda, bool use_imap, double* partials, hipStream_t s);
@@ -211,6 +229,12 @@ void launch_cg_xr(const Plan& p, float2* x, float2* r, const float2* pv, const f
 void launch_cg_beta(const Plan& p, const double* partials, int tolkind, double tol, int maxit,
                     hipStream_t s);
 void launch_cg_p(const Plan& p, float2* pv, const float2* r, hipStream_t s);
+void launch_coil_alpha(const Plan& p, const float2* Xc, float2* q, const float2* pv, float lambda,
+                       hipStream_t s);
+void launch_cg_xr_beta(const Plan& p, float2* x, float2* r, const float2* pv, const float2* q, int tolkind,
+                       double tol, int maxit, hipStream_t s);
+void launch_cg_xr_alpha(const Plan& p, float2* x, float2* r, const float2* pv, const float2* q, hipStream_t s);
+void launch_cg_beta_after_xr(const Plan& p, int tolkind, double tol, int maxit, hipStream_t s);
 void launch_axpby_img(const Plan& p, float2* out, const float2* a, float alpha, const float2* b,
                       float beta, hipStream_t s);
 // layout conversions

@@ -2,7 +2,7 @@
 //
 // One pass transforms every line along one axis of an array viewed as
 // [outer][P][inner] (inner contiguous).  A block stages `lb` adjacent lines in
-// LDS ([lb][n + 1]), so global loads and stores move lb*8 contiguous bytes per
+// LDS ([lb][n + n/16 + 1]), so global loads and stores move lb*8 contiguous bytes per
 // position; each line is then transformed by one wave with a mixed-radix
 // Stockham autosort (radix 2,3,4,8) in place in LDS.  Global loads are issued
 // eight at a time before any of them is waited on, and the twiddle table is
@@ -88,6 +88,10 @@ __device__ __forceinline__ void dft<8>(float2* v, int sign) {
 // its writes, and a wave's LDS operations execute in order, so the stage runs
 // in place without a block barrier.  MB = butterflies per lane at most.
 // j / Ns is formed as umulhi(j, ceil(2^32 / Ns)) (exact for j < 2^16).
+// Element L of a line sits at pidx(L) = L + L/16: the pad slot every 16
+// complex values spreads the stride-R writes of the early stages over banks.
+__device__ __forceinline__ int pidx(int L) { return L + (L >> 4); }
+
 template <int R, int MB>
 __device__ __forceinline__ void wave_stage(float2* line, int n, int Ns, uint32_t magic, int lane, int sign,
                                            const float2* __restrict__ tw) {
@@ -102,7 +106,7 @@ __device__ __forceinline__ void wave_stage(float2* line, int n, int Ns, uint32_t
       const int tstep = tstep0 * kk;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        float2 x = line[j + r * nb];
+        float2 x = line[pidx(j + r * nb)];
         if (r > 0) {
           float2 w = tw[r * tstep];
           if (sign > 0) w.y = -w.y;
@@ -120,9 +124,9 @@ __device__ __forceinline__ void wave_stage(float2* line, int n, int Ns, uint32_t
     if (j < nb) {
       const int jq = (Ns == 1) ? j : (int)__umulhi((uint32_t)j, magic);
       const int kk = j - jq * Ns;
-      float2* d = line + jq * Ns * R + kk;
+      const int d0 = jq * Ns * R + kk;
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r * Ns] = v[k][r];
+      for (int r = 0; r < R; ++r) line[pidx(d0 + r * Ns)] = v[k][r];
     }
   }
   __builtin_amdgcn_wave_barrier();
@@ -195,7 +199,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft(FftArgs a) {
       dst[u] = -1;
       if (idx < total) {
         const int q = idx & (lb - 1), pos = idx >> lbl;
-        dst[u] = q * a.ldn + pos;
+        dst[u] = q * a.ldn + pidx(pos);
         int p = pos;
         if (pad) {   // bin pos holds image index p = pos + nin/2 (mod n) when p < nin
           p = pos + a.nin / 2;
@@ -205,7 +209,15 @@ __global__ void __launch_bounds__(kFftThreads) k_fft(FftArgs a) {
           const int64_t qq = q0 + q;
           if (LD == 1 || LD == 4) {
             const int64_t i = p + (int64_t)a.N0 * rest;
-            const float2 xv = a.x[f * a.Ntot + i];
+            float2 xv;
+            if (a.pold) {   // p_new = r + beta p_old (CGLS.hpp:55-56), fused
+              const float be = (float)a.cgst[f].beta;
+              const float2 rr = a.r[f * a.Ntot + i], po = a.pold[f * a.Ntot + i];
+              xv = make_float2(rr.x + be * po.x, rr.y + be * po.y);
+              if (qq == 0) a.pnew[f * a.Ntot + i] = xv;
+            } else {
+              xv = a.x[f * a.Ntot + i];
+            }
             const float2 sv = a.S[i * a.C + qq];
             float2 t = cmul(xv, sv);
             if (LD == 1) {
@@ -241,8 +253,8 @@ __global__ void __launch_bounds__(kFftThreads) k_fft(FftArgs a) {
         const float* th = a.That + (f * a.conv_cells + (q0 + q) / a.C) * n;
         for (int pos = lane; pos < n; pos += 64) {
           const float t = th[pos];
-          float2 v = line[pos];
-          line[pos] = make_float2(v.x * t, v.y * t);
+          float2 v = line[pidx(pos)];
+          line[pidx(pos)] = make_float2(v.x * t, v.y * t);
         }
         __builtin_amdgcn_wave_barrier();
         line_fft<MAXN>(line, a, lane, +1, tw);
@@ -268,7 +280,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft(FftArgs a) {
             pos = p - a.nout / 2;
             if (pos < 0) pos += n;
           }
-          float2 v = lds[q * a.ldn + pos];
+          float2 v = lds[q * a.ldn + pidx(pos)];
           if (ST == 1) {
             const float dv = deapod(a, p, rest);
             v = make_float2(v.x * dv, v.y * dv);

@@ -221,16 +221,30 @@ __global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* item
   const int org0 = t0 * T - m, org1 = t1 * T - m;
   const int64_t gbase = (int64_t)it.x * a.ntot;
   {
+    // eight global loads in flight per thread before any is waited on
+    constexpr int U = 8, STEP = 256 / CG;
     const int c = threadIdx.x % CG;
     const bool cok = cbase + c < a.C;
-    for (int e = threadIdx.x / CG; e < E * E; e += 256 / CG) {
-      const int e1 = e / E, e0 = e - e1 * E;
-      int l0 = org0 + e0, l1 = org1 + e1;   // in [-m, n + T + m): one wrap suffices
-      if (l0 < 0) l0 += a.n0; else if (l0 >= a.n0) l0 -= a.n0;
-      if (l1 < 0) l1 += a.n1; else if (l1 >= a.n1) l1 -= a.n1;
-      float2 v = make_float2(0.f, 0.f);
-      if (cok) v = a.grid[(gbase + l0 + (int64_t)a.n0 * l1) * a.C + cbase + c];
-      ext[e1 * rowstride + e0 * CG + c] = v;
+    for (int e00 = threadIdx.x / CG; e00 < E * E; e00 += STEP * U) {
+      float2 v[U];
+      int dst[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e00 + u * STEP;
+        v[u] = make_float2(0.f, 0.f);
+        dst[u] = -1;
+        if (e < E * E) {
+          const int e1 = e / E, e0 = e - e1 * E;
+          int l0 = org0 + e0, l1 = org1 + e1;   // in [-m, n + T + m): one wrap suffices
+          if (l0 < 0) l0 += a.n0; else if (l0 >= a.n0) l0 -= a.n0;
+          if (l1 < 0) l1 += a.n1; else if (l1 >= a.n1) l1 -= a.n1;
+          if (cok) v[u] = a.grid[(gbase + l0 + (int64_t)a.n0 * l1) * a.C + cbase + c];
+          dst[u] = e1 * rowstride + e0 * CG + c;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (dst[u] >= 0) ext[dst[u]] = v[u];
     }
   }
   const int ns = it.w - it.z;
@@ -399,6 +413,283 @@ __global__ void __launch_bounds__(256) k_grid_batched(GridArgs a) {
       const int64_t lc = x0 + (int64_t)T * (x1 + (int64_t)T * x2);
       a.partial[((int64_t)it.slot * a.tilecells + lc) * a.C + cg] = v;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint B^T, owner computes, one wave per visit.  Visit records are
+// precomputed on the host: {s, rel0 | rel1 << 16, lo0 | hi0 << 8 | lo1 << 16 |
+// hi1 << 24, rel2 | lo2 << 16 | hi2 << 24} (rel as int16, in-tile tap ranges
+// [lo, hi)).  A batch of 64 records, their window values and (weighted) data
+// are staged in LDS by the whole block; the four waves then take disjoint
+// visits and add their taps into the LDS tile with LDS float atomics.  The
+// summation order inside a tile therefore varies between runs (the result is
+// the same to rounding); CGS_DETERMINISTIC plans use k_grid_batched instead.
+template <int CG, int TAPS, int D>
+__global__ void __launch_bounds__(256) k_grid_atomic(GridArgs a, const int4* __restrict__ vrec) {
+  constexpr int RS = 64 / CG;
+  constexpr int B = 64;
+  constexpr int NPSI = D * TAPS;
+  extern __shared__ float2 sh[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cl = lane % CG, rs = lane / CG;
+  const GridItem it = a.items[blockIdx.x];
+  if (a.fdone && a.fdone[it.frame]) return;
+  const int T = a.T, rowstride = a.rowstride;
+  const int cbase = blockIdx.y * CG;
+  const int nrows_tile = (D == 3) ? T * T : T;
+  float2* tile = sh;                                               // [rows][rowstride]
+  float2* ybat = tile + (size_t)nrows_tile * rowstride;            // [B][CG]
+  float* bpsi = (float*)(ybat + B * CG);                           // [B][NPSI]
+  int4* brec = (int4*)(bpsi + B * NPSI);                           // [B]
+
+  int t0 = it.tile % a.nt0, t1 = (it.tile / a.nt0) % a.nt1, t2 = (D == 3) ? it.tile / (a.nt0 * a.nt1) : 0;
+  const int ts0 = t0 * T, ts1 = t1 * T, ts2 = t2 * T;
+  const int te0 = min(T, a.n0 - ts0), te1 = min(T, a.n1 - ts1), te2 = (D == 3) ? min(T, a.n2 - ts2) : 1;
+
+  for (int idx = threadIdx.x; idx < nrows_tile * rowstride; idx += blockDim.x) tile[idx] = make_float2(0.f, 0.f);
+
+  for (int vb = it.vbeg; vb < it.vend; vb += B) {
+    const int nb = min(B, it.vend - vb);
+    __syncthreads();
+    if (threadIdx.x < nb) brec[threadIdx.x] = vrec[vb + threadIdx.x];
+    __syncthreads();
+    {
+      // window values: NPSI * nb <= 64 * 24 floats, 6 per thread at most
+      float pv[(B * NPSI + 255) / 256];
+#pragma unroll
+      for (int u = 0; u < (B * NPSI + 255) / 256; ++u) {
+        const int idx = threadIdx.x + u * 256;
+        if (idx < nb * NPSI) {
+          const int v = idx / NPSI, q = idx - v * NPSI;
+          pv[u] = a.psi[(int64_t)brec[v].x * NPSI + q];
+        }
+      }
+      // data: nb * CG <= 2048 float2, 8 per thread at most
+      float2 yv[(B * CG + 255) / 256];
+#pragma unroll
+      for (int u = 0; u < (B * CG + 255) / 256; ++u) {
+        const int idx = threadIdx.x + u * 256;
+        yv[u] = make_float2(0.f, 0.f);
+        if (idx < nb * CG) {
+          const int v = idx / CG, cc = idx - v * CG;
+          const int s = brec[v].x;
+          if (cbase + cc < a.C) {
+            yv[u] = a.y[(int64_t)s * a.C + cbase + cc];
+            if (a.w) { const float ww = a.w[s]; yv[u].x *= ww; yv[u].y *= ww; }
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < (B * NPSI + 255) / 256; ++u) {
+        const int idx = threadIdx.x + u * 256;
+        if (idx < nb * NPSI) bpsi[idx] = pv[u];
+      }
+#pragma unroll
+      for (int u = 0; u < (B * CG + 255) / 256; ++u) {
+        const int idx = threadIdx.x + u * 256;
+        if (idx < nb * CG) ybat[idx] = yv[u];
+      }
+    }
+    __syncthreads();
+    for (int v = wave; v < nb; v += 4) {
+      const int4 r = brec[v];
+      const int rel0 = (int)(short)(r.y & 0xffff), rel1 = (int)(short)(r.y >> 16);
+      const int lo0 = r.z & 0xff, hi0 = (r.z >> 8) & 0xff, lo1 = (r.z >> 16) & 0xff, hi1 = (r.z >> 24) & 0xff;
+      const int rel2 = (D == 3) ? (int)(short)(r.w & 0xffff) : 0;
+      const int lo2 = (D == 3) ? ((r.w >> 16) & 0xff) : 0, hi2 = (D == 3) ? ((r.w >> 24) & 0xff) : 1;
+      const float* ps = bpsi + v * NPSI;
+      float p0v[TAPS];
+#pragma unroll
+      for (int s0 = 0; s0 < TAPS; ++s0) p0v[s0] = ps[s0];
+      const float2 yv = ybat[v * CG + cl];
+      for (int s2 = lo2; s2 < hi2; ++s2) {
+        const float w2 = (D == 3) ? ps[2 * TAPS + s2] : 1.f;
+        const int plane = (D == 3) ? (rel2 + s2) * T : 0;
+        for (int s1 = lo1 + rs; s1 < hi1; s1 += RS) {
+          const float wr = ps[TAPS + s1] * w2;
+          const float vx = yv.x * wr, vy = yv.y * wr;
+          float* dst = (float*)(tile + (rel1 + s1 + plane) * rowstride + rel0 * CG + cl);
+#pragma unroll
+          for (int s0 = 0; s0 < TAPS; ++s0)
+            if (s0 >= lo0 && s0 < hi0) {
+              atomicAdd(dst + 2 * s0 * CG, p0v[s0] * vx);
+              atomicAdd(dst + 2 * s0 * CG + 1, p0v[s0] * vy);
+            }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  const int ncell = te0 * te1 * te2;
+  for (int idx = threadIdx.x; idx < ncell * CG; idx += blockDim.x) {
+    const int cc = idx % CG, cell = idx / CG;
+    const int cg = cbase + cc;
+    if (cg >= a.C) continue;
+    const int x0 = cell % te0, x1 = (cell / te0) % te1, x2 = cell / (te0 * te1);
+    const float2 v = tile[(x1 + x2 * T) * rowstride + x0 * CG + cc];
+    if (it.slot < 0) {
+      const int64_t l = (ts0 + x0) + (int64_t)a.n0 * ((ts1 + x1) + (int64_t)a.n1 * (ts2 + x2));
+      a.grid[((int64_t)it.frame * a.ntot + l) * a.C + cg] = v;
+    } else {
+      const int64_t lc = x0 + (int64_t)T * (x1 + (int64_t)T * x2);
+      a.partial[((int64_t)it.slot * a.tilecells + lc) * a.C + cg] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint B^T in registers (2D, 8x8 tiles): one wave owns one work item (a
+// tile or a chunk of a heavy tile's visit list); lane = grid cell of the tile
+// and its CGR coil accumulators live in VGPRs.  Batches of 32 visit records,
+// window values and weighted data are staged in a wave-private LDS region and
+// read back as broadcasts, so a visit costs one weight per lane and 2*CGR FMAs.
+// Each cell has one owner and visits are applied in list order: no atomics,
+// no LDS read-modify-write, deterministic.
+template <int CGR, int TAPS>
+__global__ void __launch_bounds__(256) k_grid_reg(GridArgs a, const int4* __restrict__ vrec, int nitems) {
+  constexpr int B = 32, NPSI = 2 * TAPS;
+  constexpr int WAVE_FLOATS = B * 2 * CGR + B * NPSI + B * 4;
+  extern __shared__ float shf[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int item = blockIdx.x * 4 + wave;
+  if (item >= nitems) return;          // waves of a block never synchronise with each other
+  const GridItem it = a.items[item];
+  if (a.fdone && a.fdone[it.frame]) return;
+  float* wy = shf + wave * WAVE_FLOATS;           // [B][2*CGR]
+  float* wpsi = wy + B * 2 * CGR;                 // [B][NPSI]
+  int4* wrec = (int4*)(wpsi + B * NPSI);          // [B]
+  const int cbase = blockIdx.y * CGR;
+  const int t0 = it.tile % a.nt0, t1 = it.tile / a.nt0;
+  const int ts0 = t0 * 8, ts1 = t1 * 8;
+  const int te0 = min(8, a.n0 - ts0), te1 = min(8, a.n1 - ts1);
+  const int x0 = lane & 7, x1 = lane >> 3;
+  float acc[2 * CGR];
+#pragma unroll
+  for (int k = 0; k < 2 * CGR; ++k) acc[k] = 0.f;
+
+  for (int vb = it.vbeg; vb < it.vend; vb += B) {
+    const int nb = min(B, it.vend - vb);
+    if (lane < nb) wrec[lane] = vrec[vb + lane];
+    __builtin_amdgcn_wave_barrier();
+    {
+      float pv[(B * NPSI + 63) / 64];
+#pragma unroll
+      for (int u = 0; u < (B * NPSI + 63) / 64; ++u) {
+        const int idx = lane + 64 * u;
+        if (idx < nb * NPSI) {
+          const int v = idx / NPSI, q = idx - v * NPSI;
+          pv[u] = a.psi[(int64_t)wrec[v].x * NPSI + q];
+        }
+      }
+      float2 yv[(B * CGR + 63) / 64];
+#pragma unroll
+      for (int u = 0; u < (B * CGR + 63) / 64; ++u) {
+        const int idx = lane + 64 * u;
+        yv[u] = make_float2(0.f, 0.f);
+        if (idx < nb * CGR) {
+          const int v = idx / CGR, c = idx - v * CGR;
+          const int s = wrec[v].x;
+          if (cbase + c < a.C) {
+            yv[u] = a.y[(int64_t)s * a.C + cbase + c];
+            if (a.w) { const float ww = a.w[s]; yv[u].x *= ww; yv[u].y *= ww; }
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < (B * NPSI + 63) / 64; ++u) {
+        const int idx = lane + 64 * u;
+        if (idx < nb * NPSI) wpsi[idx] = pv[u];
+      }
+#pragma unroll
+      for (int u = 0; u < (B * CGR + 63) / 64; ++u) {
+        const int idx = lane + 64 * u;
+        if (idx < nb * CGR) ((float2*)wy)[idx] = yv[u];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int v = 0; v < nb; ++v) {
+      const int4 r = wrec[v];
+      const int dx = x0 - (int)(short)(r.y & 0xffff), dy = x1 - (int)(short)(r.y >> 16);
+      float wgt = 0.f;
+      if (dx >= 0 && dx < TAPS && dy >= 0 && dy < TAPS) wgt = wpsi[v * NPSI + dx] * wpsi[v * NPSI + TAPS + dy];
+      const float4* yy = (const float4*)(wy + v * 2 * CGR);
+#pragma unroll
+      for (int k = 0; k < CGR / 2; ++k) {
+        const float4 t = yy[k];
+        acc[4 * k + 0] += wgt * t.x;
+        acc[4 * k + 1] += wgt * t.y;
+        acc[4 * k + 2] += wgt * t.z;
+        acc[4 * k + 3] += wgt * t.w;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+
+  if (x0 < te0 && x1 < te1) {
+    float2* out;
+    if (it.slot < 0) {
+      const int64_t l = (ts0 + x0) + (int64_t)a.n0 * (ts1 + x1);
+      out = a.grid + ((int64_t)it.frame * a.ntot + l) * a.C + cbase;
+    } else {
+      out = a.partial + ((int64_t)it.slot * a.tilecells + x0 + 8 * x1) * a.C + cbase;
+    }
+#pragma unroll
+    for (int c = 0; c < CGR; ++c)
+      if (cbase + c < a.C) out[c] = make_float2(acc[2 * c], acc[2 * c + 1]);
+  }
+}
+
+template <int CGR>
+static void grid_reg_taps(const GridArgs& a, const int4* vrec, int nitems, dim3 g, size_t shm, hipStream_t s) {
+  switch (a.taps) {
+    case 4: hipLaunchKernelGGL((k_grid_reg<CGR, 4>), g, dim3(256), shm, s, a, vrec, nitems); break;
+    case 6: hipLaunchKernelGGL((k_grid_reg<CGR, 6>), g, dim3(256), shm, s, a, vrec, nitems); break;
+    case 8: hipLaunchKernelGGL((k_grid_reg<CGR, 8>), g, dim3(256), shm, s, a, vrec, nitems); break;
+    case 10: hipLaunchKernelGGL((k_grid_reg<CGR, 10>), g, dim3(256), shm, s, a, vrec, nitems); break;
+    default: break;
+  }
+}
+
+void launch_grid_reg(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s) {
+  if (nitems == 0) return;
+  const dim3 g((unsigned)((nitems + 3) / 4), (unsigned)ncgroups);
+  const size_t shm = sizeof(float) * 4 * (32 * 2 * cgr + 32 * 2 * a.taps + 32 * 4);
+  if (cgr == 8) grid_reg_taps<8>(a, vrec, nitems, g, shm, s);
+  else if (cgr == 16) grid_reg_taps<16>(a, vrec, nitems, g, shm, s);
+  else grid_reg_taps<32>(a, vrec, nitems, g, shm, s);
+}
+
+size_t grid_atomic_lds(int d, int T, int taps, int CG, int rowstride) {
+  const int nrows_tile = (d == 3) ? T * T : T;
+  return sizeof(float2) * ((size_t)nrows_tile * rowstride + 64 * CG) + sizeof(float) * 64 * d * taps +
+         sizeof(int4) * 64;
+}
+
+template <int CG, int D>
+static void grid_atomic_taps(const GridArgs& a, const int4* vrec, dim3 g, size_t shm, hipStream_t s) {
+  switch (a.taps) {
+    case 4: hipLaunchKernelGGL((k_grid_atomic<CG, 4, D>), g, dim3(256), shm, s, a, vrec); break;
+    case 6: hipLaunchKernelGGL((k_grid_atomic<CG, 6, D>), g, dim3(256), shm, s, a, vrec); break;
+    case 8: hipLaunchKernelGGL((k_grid_atomic<CG, 8, D>), g, dim3(256), shm, s, a, vrec); break;
+    case 10: hipLaunchKernelGGL((k_grid_atomic<CG, 10, D>), g, dim3(256), shm, s, a, vrec); break;
+    default: break;
+  }
+}
+
+void launch_grid_atomic(const GridArgs& a, const int4* vrec, int cg, int nitems, int ncgroups, hipStream_t s) {
+  if (nitems == 0) return;
+  const dim3 g((unsigned)nitems, (unsigned)ncgroups);
+  const size_t shm = grid_atomic_lds(a.d, a.T, a.taps, cg, a.rowstride);
+  if (a.d == 2) {
+    if (cg == 8) grid_atomic_taps<8, 2>(a, vrec, g, shm, s);
+    else if (cg == 16) grid_atomic_taps<16, 2>(a, vrec, g, shm, s);
+    else grid_atomic_taps<32, 2>(a, vrec, g, shm, s);
+  } else {
+    if (cg == 8) grid_atomic_taps<8, 3>(a, vrec, g, shm, s);
+    else if (cg == 16) grid_atomic_taps<16, 3>(a, vrec, g, shm, s);
+    else grid_atomic_taps<32, 3>(a, vrec, g, shm, s);
   }
 }
 

@@ -11,6 +11,7 @@
 namespace cgs {
 
 constexpr int kRedBlocks = 512;   // partial sums per frame for image-domain reductions
+constexpr int kCoilBlocks = 2048; // blocks per frame of the coil combine (a few pixels per thread group)
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -74,7 +75,8 @@ __global__ void __launch_bounds__(256) k_coil_combine(const float2* X, const flo
 
 void launch_coil_combine(const Plan& p, const float2* Xc, float2* x, const float2* pvec,
                          float lambda, bool use_imap, double* partials, hipStream_t s) {
-  const dim3 g(kRedBlocks, p.F);
+  // kCoilBlocks partials per frame (read by k_cg_alpha and k_cg_xr_alpha)
+  const dim3 g(kCoilBlocks, p.F);
   const float* I = use_imap ? p.Imap : nullptr;
   if (p.C <= 8)
     hipLaunchKernelGGL(k_coil_combine<8>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, p.fdone);
@@ -84,6 +86,179 @@ void launch_coil_combine(const Plan& p, const float2* Xc, float2* x, const float
     hipLaunchKernelGGL(k_coil_combine<32>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, p.fdone);
   else
     hipLaunchKernelGGL(k_coil_combine<64>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, p.fdone);
+}
+
+// ---------------------------------------------------------------------------
+// Fused CG iteration tail.  Blocks of a frame write FP64 partials; the last
+// block to arrive (global ticket per frame, reset by that block) sums them in
+// fixed order and updates the frame's scalars, so no separate scalar kernel is
+// launched and the result does not depend on block timing.
+__device__ __forceinline__ bool last_block(int* ticket, int nblocks) {
+  __shared__ int is_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int t = atomicAdd(ticket, 1);
+    is_last = (t == nblocks - 1);
+    if (is_last) atomicExch(ticket, 0);
+  }
+  __syncthreads();
+  if (is_last) __threadfence();
+  return is_last != 0;
+}
+
+// q = sum_c conj(S_c) X_c + lambda p, partial <p, q>; last block: alpha = rn / <p,q>
+template <int CP>
+__global__ void __launch_bounds__(256) k_coil_alpha(const float2* X, const float2* S, float2* q,
+                                                    const float2* pv, float lambda, double* partials,
+                                                    int* tickets, CgState* st, int C, int64_t Ntot,
+                                                    const int* fdone) {
+  __shared__ double sh[4];
+  const int f = blockIdx.y;
+  if (fdone[f]) return;
+  constexpr int PPB = 256 / CP;
+  const int g = threadIdx.x / CP, cl = threadIdx.x % CP;
+  double dot = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * PPB + g; i < Ntot; i += (int64_t)gridDim.x * PPB) {
+    float2 acc = make_float2(0.f, 0.f);
+    for (int c = cl; c < C; c += CP) {
+      const float2 s = S[i * C + c];
+      const float2 v = X[((int64_t)f * Ntot + i) * C + c];
+      acc.x += s.x * v.x + s.y * v.y;
+      acc.y += s.x * v.y - s.y * v.x;
+    }
+#pragma unroll
+    for (int m = CP / 2; m > 0; m >>= 1) {
+      acc.x += __shfl_xor(acc.x, m);
+      acc.y += __shfl_xor(acc.y, m);
+    }
+    if (cl == 0) {
+      const int64_t o = (int64_t)f * Ntot + i;
+      const float2 p = pv[o];
+      acc.x += lambda * p.x;
+      acc.y += lambda * p.y;
+      dot += (double)p.x * acc.x + (double)p.y * acc.y;
+      q[o] = acc;
+    }
+  }
+  const double t = block_sum(dot, sh);
+  if (threadIdx.x == 0) partials[(int64_t)f * gridDim.x + blockIdx.x] = t;
+  if (last_block(tickets + f, gridDim.x)) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) acc += partials[(int64_t)f * gridDim.x + b];
+    const double pq = block_sum(acc, sh);
+    if (threadIdx.x == 0) {
+      st[f].pq = pq;
+      st[f].alpha = st[f].rn / pq;   // CGLS.hpp:50
+    }
+  }
+}
+
+void launch_coil_alpha(const Plan& p, const float2* Xc, float2* q, const float2* pv, float lambda,
+                       hipStream_t s) {
+  const dim3 g(kRedBlocks, p.F);
+  if (p.C <= 8)
+    hipLaunchKernelGGL(k_coil_alpha<8>, g, dim3(256), 0, s, Xc, p.S, q, pv, lambda, p.partials, p.tickets, p.cgst, p.C, p.Ntot, p.fdone);
+  else if (p.C <= 16)
+    hipLaunchKernelGGL(k_coil_alpha<16>, g, dim3(256), 0, s, Xc, p.S, q, pv, lambda, p.partials, p.tickets, p.cgst, p.C, p.Ntot, p.fdone);
+  else if (p.C <= 32)
+    hipLaunchKernelGGL(k_coil_alpha<32>, g, dim3(256), 0, s, Xc, p.S, q, pv, lambda, p.partials, p.tickets, p.cgst, p.C, p.Ntot, p.fdone);
+  else
+    hipLaunchKernelGGL(k_coil_alpha<64>, g, dim3(256), 0, s, Xc, p.S, q, pv, lambda, p.partials, p.tickets, p.cgst, p.C, p.Ntot, p.fdone);
+}
+
+// x += alpha p ; r -= alpha q ; partial ||r||^2, where every block forms
+// alpha = rn / <p,q> itself from the coil combine's partials (same fixed-order
+// sum in every block, so all blocks use the identical alpha)
+__global__ void __launch_bounds__(256) k_cg_xr_alpha(float2* x, float2* r, const float2* pv, const float2* q,
+                                                     CgState* st, const double* pq_part, int npq, double* partials,
+                                                     int64_t Ntot, const int* fdone) {
+  __shared__ double sh[4];
+  __shared__ float alpha_s;
+  const int f = blockIdx.y;
+  if (fdone[f]) return;
+  {
+    double a = 0.0;
+    for (int b = threadIdx.x; b < npq; b += blockDim.x) a += pq_part[(int64_t)f * npq + b];
+    const double pq = block_sum(a, sh);
+    if (threadIdx.x == 0) {
+      const double al = st[f].rn / pq;   // CGLS.hpp:50
+      alpha_s = (float)al;
+      if (blockIdx.x == 0) { st[f].pq = pq; st[f].alpha = al; }
+    }
+    __syncthreads();
+  }
+  const float al = alpha_s;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Ntot; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = (int64_t)f * Ntot + i;
+    const float2 pp = pv[o], qq = q[o];
+    float2 xx = x[o], rr = r[o];
+    xx.x += al * pp.x; xx.y += al * pp.y;
+    rr.x -= al * qq.x; rr.y -= al * qq.y;
+    x[o] = xx;
+    r[o] = rr;
+    acc += (double)rr.x * rr.x + (double)rr.y * rr.y;
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) partials[(int64_t)f * gridDim.x + blockIdx.x] = t;
+}
+
+void launch_cg_xr_alpha(const Plan& p, float2* x, float2* r, const float2* pv, const float2* q, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_xr_alpha, dim3(kRedBlocks, p.F), dim3(256), 0, s, x, r, pv, q, p.cgst, p.partials,
+                     kCoilBlocks, p.partials + (size_t)kCoilBlocks * p.F, p.Ntot, p.fdone);
+}
+
+void launch_cg_beta_after_xr(const Plan& p, int tolkind, double tol, int maxit, hipStream_t s) {
+  launch_cg_beta(p, p.partials + (size_t)kCoilBlocks * p.F, tolkind, tol, maxit, s);
+}
+
+// x += alpha p ; r -= alpha q ; partial ||r||^2 ; last block: beta, residual, done
+__global__ void k_cg_xr_beta(float2* x, float2* r, const float2* pv, const float2* q, CgState* st,
+                             double* partials, int* tickets, int64_t Ntot, int tolkind, double tol, int maxit,
+                             double* hist, int hcap, int* fdone) {
+  __shared__ double sh[4];
+  const int f = blockIdx.y;
+  if (fdone[f]) return;
+  const float al = (float)st[f].alpha;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Ntot; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = (int64_t)f * Ntot + i;
+    const float2 pp = pv[o], qq = q[o];
+    float2 xx = x[o], rr = r[o];
+    xx.x += al * pp.x; xx.y += al * pp.y;
+    rr.x -= al * qq.x; rr.y -= al * qq.y;
+    x[o] = xx;
+    r[o] = rr;
+    acc += (double)rr.x * rr.x + (double)rr.y * rr.y;
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) partials[(int64_t)f * gridDim.x + blockIdx.x] = t;
+  if (last_block(tickets + f, gridDim.x)) {
+    double a2 = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) a2 += partials[(int64_t)f * gridDim.x + b];
+    const double rn_new = block_sum(a2, sh);
+    if (threadIdx.x == 0) {
+      CgState c = st[f];
+      c.beta = rn_new / c.rn;   // CGLS.hpp:53-55
+      c.rn = rn_new;
+      c.it += 1;
+      const double rel = (tolkind == CGS_TOL_SQUARED_RATIO) ? rn_new / c.xn : sqrt(rn_new / c.bn);
+      c.rel = rel;
+      if (isnan(rel) || isnan(c.alpha)) c.nan = 1;
+      c.done = (c.nan || rel <= tol || c.it >= maxit) ? 1 : 0;
+      st[f] = c;
+      if (hist && c.it < hcap) hist[(int64_t)f * hcap + c.it] = rel;
+      fdone[f] = c.done;
+    }
+  }
+}
+
+void launch_cg_xr_beta(const Plan& p, float2* x, float2* r, const float2* pv, const float2* q, int tolkind,
+                       double tol, int maxit, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_xr_beta, dim3(kRedBlocks, p.F), dim3(256), 0, s, x, r, pv, q, p.cgst,
+                     p.partials + (size_t)kRedBlocks * p.F, p.tickets + p.F, p.Ntot, tolkind, tol, maxit, p.hist,
+                     p.hist_cap, p.fdone);
 }
 
 // ---------------------------------------------------------------------------
@@ -154,7 +329,7 @@ __global__ void k_cg_alpha(CgState* st, const double* part, int nb, const int* f
 }
 
 void launch_cg_alpha(const Plan& p, const double* partials, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_alpha, dim3(p.F), dim3(256), 0, s, p.cgst, partials, kRedBlocks, p.fdone);
+  hipLaunchKernelGGL(k_cg_alpha, dim3(p.F), dim3(256), 0, s, p.cgst, partials, kCoilBlocks, p.fdone);
 }
 
 // x += alpha p ; r -= alpha q ; partial ||r||^2   (CGLS.hpp:51-54)

@@ -127,14 +127,15 @@ struct Prof {
   int kind;
   double bytes;
   hipEvent_t a = nullptr, b = nullptr;
+  // no timing events while a HIP graph is being captured
   Prof(Plan& p_, int kind_, double bytes_) : p(p_), kind(kind_), bytes(bytes_) {
-    if (!(p.flags & CGS_PROFILE)) return;
+    if (!(p.flags & CGS_PROFILE) || p.capturing) return;
     a = take();
     b = take();
     (void)hipEventRecord(a, p.stream);
   }
   ~Prof() {
-    if (!(p.flags & CGS_PROFILE)) return;
+    if (!(p.flags & CGS_PROFILE) || p.capturing) return;
     (void)hipEventRecord(b, p.stream);
     p.pending.push_back({a, b, kind, bytes});
   }
@@ -199,7 +200,7 @@ static void fft_finish(FftArgs& a) {
   a.lb = pick_lb(a.inner, a.n);
   a.lb_log2 = 0;
   while ((1 << a.lb_log2) < a.lb) ++a.lb_log2;
-  a.ldn = a.n + 1;
+  a.ldn = a.n + (a.n >> 4) + 1;   // room for the 1-in-16 pad slots of the line layout
   uint64_t Ns = 1;
   for (int s = 0; s < a.nst; ++s) {
     a.ns_magic[s] = (Ns == 1) ? 0u : (uint32_t)(((1ull << 32) + Ns - 1) / Ns);
@@ -207,8 +208,23 @@ static void fft_finish(FftArgs& a) {
   }
 }
 
+// fused CG direction update carried into the first FFT pass (p_new = r + beta p_old)
+struct PUpd {
+  const float2* r = nullptr;
+  const float2* pold = nullptr;
+  float2* pnew = nullptr;
+};
+
+static void apply_pupd(const Plan& p, FftArgs& a, const PUpd* pu) {
+  if (!pu) return;
+  a.r = pu->r;
+  a.pold = pu->pold;
+  a.pnew = pu->pnew;
+  a.cgst = p.cgst;
+}
+
 // image (mode 1: x .* S_c; 2: X; 3: X .* w_hat) -> grid, forward transform
-static int fft_forward(Plan& p, int mode, const float2* src, const int* fl) {
+static int fft_forward(Plan& p, int mode, const float2* src, const int* fl, const PUpd* pu = nullptr) {
   const int d = p.d, C = p.C, F = p.F;
   const int64_t N0 = p.N[0], N1 = p.N[1], N2 = p.N[2], n0 = p.n[0], n1 = p.n[1], n2 = p.n[2];
   double bytes = (mode == 1 ? (double)(1 + C) * p.Ntot * 8 : 2.0 * C * p.Ntot * 8) * F;
@@ -217,6 +233,7 @@ static int fft_forward(Plan& p, int mode, const float2* src, const int* fl) {
     FftArgs a = fft_base(p, p.fa[0], -1);
     a.in = mode == 1 ? nullptr : src;
     a.x = mode == 1 ? src : nullptr;
+    if (mode == 1) apply_pupd(p, a, pu);
     a.out = (d == 2) ? p.t1 : p.t1;
     a.nin = (int)N0; a.nout = (int)n0;
     a.inner = C;
@@ -353,7 +370,12 @@ static int gridding(Plan& p, const float2* y, bool weighted, const int* fl) {
   const double bytes = (double)p.Mtot * p.d * 4 + (double)p.Mtot * 4 + (double)p.C * p.Mtot * 8 +
                        (double)p.C * p.ntot * p.F * 8;
   Prof pr(p, CGS_K_GRID, bytes);
-  if (batched)
+  if (p.d == 2 && p.T == 8 && p.use_grid_reg) {
+    const int cgr = p.grid_cgr;
+    launch_grid_reg(a, p.vrec, cgr, p.n_grid_items, (p.C + cgr - 1) / cgr, p.stream);
+  } else if (batched && p.use_grid_atomic)
+    launch_grid_atomic(a, p.vrec, p.grid_cg, p.n_grid_items, ngroups, p.stream);
+  else if (batched)
     launch_grid_batched(a, p.grid_cg, p.grid_w, p.n_grid_items, ngroups, p.stream);
   else
     launch_grid(a, p.grid_cg, p.grid_w, p.n_grid_items, ngroups, p.stream);
@@ -367,8 +389,8 @@ static double coil_bytes(const Plan& p, bool lam) {
 
 // ---- NCSENSE operators ----------------------------------------------------------------
 // E: y_int = B F D (S_c .* x) for all coils and frames
-static int op_E(Plan& p, const float2* x, float2* y, const int* fl) {
-  CHK(fft_forward(p, 1, x, fl));
+static int op_E(Plan& p, const float2* x, float2* y, const int* fl, const PUpd* pu = nullptr) {
+  CHK(fft_forward(p, 1, x, fl, pu));
   CHK(interp(p, y, fl));
   return CGS_OK;
 }
@@ -457,7 +479,7 @@ static void toep_drop(Plan& p);
 static void free_samples(Plan& p) {
   dfree(p.base); dfree(p.psi); dfree(p.perm); dfree(p.sframe);
   dfree(p.interp_items); dfree(p.tile_items); dfree(p.grid_items);
-  p.n_tile_items = 0; dfree(p.reduce_items); dfree(p.visits); dfree(p.partial);
+  p.n_tile_items = 0; dfree(p.reduce_items); dfree(p.visits); dfree(p.vrec); dfree(p.partial);
   p.n_interp_items = p.n_grid_items = p.n_reduce_items = 0;
   p.n_visits = 0;
   p.n_slots = 0;
@@ -584,6 +606,35 @@ static int build_samples(Plan& p, const float* k) {
     std::vector<int64_t> pos(vcnt.begin(), vcnt.end() - 1);
     for_visits([&](int64_t kk, int64_t s) { vis[pos[kk]++] = (int)s; });
   }
+  // visit records: sample id, offsets of the footprint in the tile, in-tile tap ranges
+  std::vector<int4> vrec((size_t)nvis);
+  for (int64_t kk = 0; kk < nkeys; ++kk) {
+    const int t = (int)(kk % tpf);
+    const int tt[3] = {t % p.nt[0], (t / p.nt[0]) % p.nt[1], t / (p.nt[0] * p.nt[1])};
+    int ts[3], te[3];
+    for (int a = 0; a < 3; ++a) {
+      ts[a] = tt[a] * T;
+      te[a] = (a < d) ? std::min(T, p.n[a] - ts[a]) : 1;
+    }
+    for (int64_t v = vcnt[kk]; v < vcnt[kk + 1]; ++v) {
+      const int s = vis[v];
+      int rel[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+      for (int a = 0; a < d; ++a) {
+        int r = hbase[(int64_t)s * d + a] - ts[a];
+        if (r < 0) r += p.n[a];
+        if (r >= te[a]) r -= p.n[a];
+        rel[a] = r;
+        lo[a] = std::max(0, -r);
+        hi[a] = std::min(taps, te[a] - r);
+      }
+      int4 rc;
+      rc.x = s;
+      rc.y = (int)(((uint32_t)rel[0] & 0xffffu) | ((uint32_t)rel[1] << 16));
+      rc.z = (int)((uint32_t)lo[0] | ((uint32_t)hi[0] << 8) | ((uint32_t)lo[1] << 16) | ((uint32_t)hi[1] << 24));
+      rc.w = (d == 3) ? (int)(((uint32_t)rel[2] & 0xffffu) | ((uint32_t)lo[2] << 16) | ((uint32_t)hi[2] << 24)) : 0;
+      vrec[v] = rc;
+    }
+  }
   std::vector<GridItem> gitems;
   std::vector<ReduceItem> ritems;
   int slots = 0;
@@ -620,6 +671,8 @@ static int build_samples(Plan& p, const float* k) {
   CHK(dalloc(&p.grid_items, gitems.size()));
   CHK(dalloc(&p.reduce_items, ritems.size()));
   CHK(dalloc(&p.visits, vis.size()));
+  CHK(dalloc(&p.vrec, vrec.size()));
+  if (!vrec.empty()) HIPCHK(hipMemcpy(p.vrec, vrec.data(), sizeof(int4) * vrec.size(), hipMemcpyHostToDevice));
   const int64_t tilecells = (int64_t)T * T * (d == 3 ? T : 1);
   CHK(dalloc(&p.partial, (size_t)slots * tilecells * p.C));
   HIPCHK(hipMemcpy(p.base, hbase.data(), sizeof(int) * hbase.size(), hipMemcpyHostToDevice));
@@ -653,12 +706,14 @@ static void destroy_plan(Plan& p) {
   for (auto& b : p.ybuf) dfree(b);
   for (auto& b : p.img) dfree(b);
   dfree(p.ext_a); dfree(p.ext_b); dfree(p.partials); dfree(p.cgst); dfree(p.cnr);
-  dfree(p.fdone); dfree(p.fmask); dfree(p.hist);
+  dfree(p.fdone); dfree(p.fmask); dfree(p.hist); dfree(p.tickets);
   if (p.h_cgst) (void)hipHostFree(p.h_cgst);
   if (p.h_done) (void)hipHostFree(p.h_done);
   p.h_cgst = nullptr; p.h_done = nullptr;
   for (auto& q : p.pending) { p.event_pool.push_back(q.a); p.event_pool.push_back(q.b); }
   p.pending.clear();
+  if (p.graph_exec) (void)hipGraphExecDestroy(p.graph_exec);
+  p.graph_exec = nullptr;
   for (auto e : p.event_pool) (void)hipEventDestroy(e);
   p.event_pool.clear();
   if (p.own_stream && p.stream) (void)hipStreamDestroy(p.stream);
@@ -710,10 +765,11 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   p.use_grid_batched = true;
   p.grid_cg = (C <= 8) ? 8 : (C <= 16 ? 16 : 32);
   p.grid_w = p.grid_cg / 8;
-  p.interp_cg = (C <= 8) ? 8 : (C <= 16 ? 16 : 32);
+  p.interp_cg = (C <= 8) ? 8 : 16;
   p.use_interp_lds = (d == 2);
-  p.tile_item_max = 64;
+  p.tile_item_max = 128;
   p.visits_per_item = (d == 3) ? 128 : 512;
+  p.grid_cgr = (C <= 8) ? 8 : 16;
   // shape overrides for tuning sweeps (tile side must be a multiple of 8)
   if (const char* e = std::getenv("CGS_TILE")) {
     const int t = std::atoi(e);
@@ -734,6 +790,12 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   if (const char* e = std::getenv("CGS_VISITS_PER_ITEM")) p.visits_per_item = std::max(16, std::atoi(e));
   if (const char* e = std::getenv("CGS_TILE_ITEM_MAX")) p.tile_item_max = std::max(8, std::atoi(e));
   if (const char* e = std::getenv("CGS_INTERP_LDS")) p.use_interp_lds = (d == 2) && std::atoi(e) != 0;
+  if (const char* e = std::getenv("CGS_GRID_REG")) p.use_grid_reg = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CGS_GRID_CGR")) {
+    const int g = std::atoi(e);
+    if (g == 8 || g == 16 || g == 32) p.grid_cgr = g;
+  }
+  if (const char* e = std::getenv("CGS_GRID_ATOMIC")) p.use_grid_atomic = std::atoi(e) != 0;
 
   if (flags & 0x8000u) {   // internal: caller-provided stream set later
   }
@@ -770,9 +832,11 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   CHK(dalloc(&p.w, (size_t)Mtot));
   CHK(dalloc(&p.S, (size_t)p.Ntot * C));
   CHK(dalloc(&p.Imap, (size_t)p.Ntot));
-  p.n_partials = std::max<size_t>((size_t)2 * 512 * F, (size_t)256 * F * C);
+  p.n_partials = std::max<size_t>((size_t)4096 * F, (size_t)256 * F * C);
   CHK(dalloc(&p.partials, p.n_partials));
   CHK(dalloc(&p.cgst, (size_t)F));
+  CHK(dalloc(&p.tickets, (size_t)2 * F));
+  HIPCHK(hipMemset(p.tickets, 0, sizeof(int) * 2 * F));
   CHK(dalloc(&p.cnr, (size_t)8 * F * C));
   CHK(dalloc(&p.fdone, (size_t)F));
   CHK(dalloc(&p.fmask, (size_t)F));
@@ -899,16 +963,20 @@ static int toeplitz_prepare(Plan& p) {
   return CGS_OK;
 }
 
+// drops the Toeplitz spectrum and any captured CG graph (both refer to the
+// trajectory-dependent buffers)
 static void toep_drop(Plan& p) {
   dfree(p.That);
   if (!p.tt_alias) { dfree(p.tt1); dfree(p.tt2); }
   p.tt1 = p.tt2 = nullptr;
   p.have_toep = false;
+  if (p.graph_exec) (void)hipGraphExecDestroy(p.graph_exec);
+  p.graph_exec = nullptr;
+  p.graph_key.clear();
 }
 
-// q = E^H W E x + lambda x with the Toeplitz embedding (and partial <x, q>)
-static int op_normal_toeplitz(Plan& p, const float2* x, float2* qo, float lambda, double* partials,
-                              const int* fl) {
+// per-coil images X_c = crop(IFFT_2N(That .* FFT_2N(pad(S_c x)))) (Toeplitz embedding)
+static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd* pu) {
   const int d = p.d, C = p.C, F = p.F;
   const int64_t N0 = p.N[0], N1 = p.N[1], N2 = p.N[2];
   const int64_t M0 = 2 * N0, M1 = 2 * N1;
@@ -919,6 +987,7 @@ static int op_normal_toeplitz(Plan& p, const float2* x, float2* qo, float lambda
     a.nin = nin; a.nout = nout;
     a.inner = inner; a.outer_per_frame = opf; a.outers = opf * F;
     a.fdone = fl;
+    if (ld == 4) apply_pupd(p, a, pu);
     if (That) { a.That = That; a.conv_cells = cells; a.pad_in = 1; a.crop_out = 1; }
     fft_finish(a);
     Prof pr(p, sign < 0 ? CGS_K_FFT_FWD : CGS_K_FFT_ADJ, bytes * F);
@@ -943,6 +1012,13 @@ static int op_normal_toeplitz(Plan& p, const float2* x, float2* qo, float lambda
     pass(p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1 * N2, 0, nullptr, 0,
          (double)C * (M0 + N0) * N1 * N2 * 8);
   }
+  return CGS_OK;
+}
+
+// q = E^H W E x + lambda x with the Toeplitz embedding (and partial <x, q>)
+static int op_normal_toeplitz(Plan& p, const float2* x, float2* qo, float lambda, double* partials,
+                              const int* fl) {
+  CHK(toeplitz_to_coils(p, x, fl, nullptr));
   Prof pr(p, CGS_K_COIL, coil_bytes(p, true));
   launch_coil_combine(p, p.Xc, qo, x, lambda, false, partials, p.stream);
   return CGS_OK;
@@ -954,6 +1030,34 @@ static int op_normal(Plan& p, int normal_op, const float2* x, float2* qo, float 
   if (normal_op == CGS_NORMAL_TOEPLITZ) return op_normal_toeplitz(p, x, qo, lambda, partials, fl);
   CHK(op_E(p, x, p.ybuf[1], fl));
   return op_EH_true(p, p.ybuf[1], qo, x, lambda, partials, fl);
+}
+
+// One fused CG iteration of the true-adjoint solve: p_new = r + beta p_old
+// inside the first FFT pass, q = (E^H W E + lambda) p_new with alpha formed by
+// the coil combine's last block, then x, r, beta and the stop test.
+static int cg_iteration_fused(Plan& p, int normal_op, float2* x, float2* r, const float2* pold, float2* pnew,
+                              float2* q, const cgs_solve_opts& o) {
+  PUpd pu;
+  pu.r = r;
+  pu.pold = pold;
+  pu.pnew = pnew;
+  if (normal_op == CGS_NORMAL_TOEPLITZ) {
+    CHK(toeplitz_to_coils(p, nullptr, p.fdone, &pu));
+  } else {
+    CHK(op_E(p, nullptr, p.ybuf[1], p.fdone, &pu));
+    CHK(gridding(p, p.ybuf[1], true, p.fdone));
+    CHK(fft_adjoint(p, p.Xc, p.fdone));
+  }
+  {
+    Prof pr(p, CGS_K_COIL, coil_bytes(p, true));
+    launch_coil_combine(p, p.Xc, q, pnew, (float)o.lambda, false, p.partials, p.stream);
+  }
+  {
+    Prof pr(p, CGS_K_CG, 5.0 * 8.0 * p.F * p.Ntot);
+    launch_cg_xr_alpha(p, x, r, pnew, q, p.stream);
+    launch_cg_beta_after_xr(p, o.tol_kind, o.tol, o.maxit, p.stream);
+  }
+  return CGS_OK;
 }
 
 // ---- CG solve -------------------------------------------------------------------------
@@ -1003,14 +1107,41 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
   HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
   int status = CGS_OK;
-  for (int it = 0; it < o.maxit; ++it) {
-    if (o.eh_mode == CGS_EH_TRUE) {
-      CHK(op_normal(p, o.normal_op, pv, q, (float)o.lambda, p.partials, p.fdone));
+  float2* pbuf[2] = {pv, p.img[6]};   // p double buffer for the fused direction update
+  int step = 1;
+  if (o.graph_iters > 0 && o.eh_mode == CGS_EH_TRUE) {
+    // capture an even number of fused iterations once and replay the graph
+    step = o.graph_iters + (o.graph_iters & 1);
+    char key[256];
+    snprintf(key, sizeof(key), "%d|%d|%.17g|%d|%.17g|%d|%p|%p", step, o.normal_op, o.lambda, o.tol_kind, o.tol,
+             o.maxit, (void*)p.That, (void*)p.stream);
+    if (!p.graph_exec || p.graph_key != key) {
+      if (p.graph_exec) (void)hipGraphExecDestroy(p.graph_exec);
+      p.graph_exec = nullptr;
+      hipGraph_t g = nullptr;
+      p.capturing = true;
+      HIPCHK(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal));
+      int rc = CGS_OK;
+      for (int k = 0; k < step && rc == CGS_OK; ++k)
+        rc = cg_iteration_fused(p, o.normal_op, x, r, pbuf[k & 1], pbuf[(k + 1) & 1], q, o);
+      hipError_t ec = hipStreamEndCapture(p.stream, &g);
+      p.capturing = false;
+      if (rc != CGS_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+      if (ec != hipSuccess) { set_error(std::string("graph capture failed: ") + hipGetErrorString(ec)); return CGS_ERR_HIP; }
+      hipError_t ei = hipGraphInstantiate(&p.graph_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) { p.graph_exec = nullptr; set_error(std::string("graph instantiate failed: ") + hipGetErrorString(ei)); return CGS_ERR_HIP; }
+      p.graph_key = key;
+    }
+  }
+  for (int it = 0; it < o.maxit; it += step) {
+    if (step > 1) {
+      HIPCHK(hipGraphLaunch(p.graph_exec, p.stream));
+    } else if (o.eh_mode == CGS_EH_TRUE) {
+      CHK(cg_iteration_fused(p, o.normal_op, x, r, pbuf[it & 1], pbuf[(it + 1) & 1], q, o));
     } else {
       CHK(op_E(p, pv, p.ybuf[1], p.fdone));
       CHK(op_EH(p, o.eh_mode, p.ybuf[1], q, o.ftiter, o.fteps, pv, (float)o.lambda, p.partials, p.fdone));
-    }
-    {
       Prof pr(p, CGS_K_CG, 7.0 * 8.0 * F * p.Ntot);
       launch_cg_alpha(p, p.partials, p.stream);
       launch_cg_xr(p, x, r, pv, q, p.partials, p.stream);
@@ -1019,10 +1150,10 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
     }
     CHK(check_launch());
     // pipelined convergence test: read the flags of the previous iteration
-    const int par = it & 1;
+    const int par = (it / step) & 1;
     HIPCHK(hipMemcpyAsync(p.h_done + par * F, p.fdone, sizeof(int) * F, hipMemcpyDeviceToHost, p.stream));
     HIPCHK(hipEventRecord(ev[par], p.stream));
-    if (it >= 1) {
+    if (it >= step) {
       HIPCHK(hipEventSynchronize(ev[par ^ 1]));
       bool all = true;
       for (int f = 0; f < F; ++f) all &= (p.h_done[(par ^ 1) * F + f] != 0);

@@ -48,6 +48,7 @@ enum {
 /* plan flags */
 #define CGS_DEVICE_IO      0x1u  /* operator data pointers are device memory */
 #define CGS_PROFILE        0x2u  /* time every kernel launch with HIP events */
+#define CGS_DETERMINISTIC  0x4u  /* fixed summation order in the gridding (slower) */
 
 /* E^H variants */
 #define CGS_EH_TRUE        0     /* x = sum_c conj(S_c) .* NFFT^H(w .* y_c)   */
@@ -70,6 +71,8 @@ typedef struct {
   int ftiter;         /* CGNR steps of the reference E^H (codeare ftiter)   */
   double fteps;       /* CGNR absolute break on the weighted residual       */
   int normal_op;      /* CGS_NORMAL_NUFFT or CGS_NORMAL_TOEPLITZ (true E^H) */
+  int graph_iters;    /* >0: capture this many CG iterations (true E^H) in one
+                         HIP graph and replay it; 0: launch kernels one by one */
 } cgs_solve_opts;
 
 typedef struct {

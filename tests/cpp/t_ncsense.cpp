@@ -1,0 +1,98 @@
+// t_ncsense.cpp -- drives the codeare-shaped C++ operators (include/codeare_amd/ncsense.hpp)
+// the way codeare's CGSENSE module drives NCSENSE<cxfl> (src/modules/CGSENSE.cpp:132-178):
+// construct from Params, KSpace, Weights, then *, / and ->*.
+// Inputs are raw little-endian files written by tests/test_cpp_facade.py; outputs are
+// written next to them for the test to compare with the CPU oracle.
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "codeare_amd/ncsense.hpp"
+
+template <class T>
+struct MiniMatrix {   // column-major storage with the members the operators use
+  explicit MiniMatrix(const std::vector<size_t>& dims) {
+    size_t n = 1;
+    for (size_t v : dims) n *= v;
+    data.assign(n, T());
+  }
+  T* Ptr() { return data.data(); }
+  const T* Ptr() const { return data.data(); }
+  size_t Size() const { return data.size(); }
+  std::vector<T> data;
+};
+
+using CMat = MiniMatrix<codeare_amd::cxfl>;
+using RMat = MiniMatrix<float>;
+
+template <class M>
+static void load(const std::string& path, M& m) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) { std::fprintf(stderr, "missing %s\n", path.c_str()); std::exit(2); }
+  f.read((char*)m.Ptr(), sizeof(m.data[0]) * m.Size());
+}
+template <class M>
+static void save(const std::string& path, const M& m) {
+  std::ofstream f(path, std::ios::binary);
+  f.write((const char*)m.Ptr(), sizeof(m.data[0]) * m.Size());
+}
+
+int main(int argc, char** argv) {
+  if (argc < 7) { std::fprintf(stderr, "usage: t_ncsense dir N C nk m alpha\n"); return 2; }
+  const std::string dir = argv[1];
+  const size_t N = std::atoi(argv[2]), C = std::atoi(argv[3]), nk = std::atoi(argv[4]);
+  codeare_amd::Params p;
+  p.set("nk", (double)nk);
+  p.set("m", std::atof(argv[5]));
+  p.set("alpha", std::atof(argv[6]));
+  p.set("ftiter", 3);
+  p.set("fteps", 1e-3);
+  p.set("cgiter", 5);
+  p.set("cgeps", 1e-7);
+  p.set("lambda", 5e-2);
+  CMat S({N, N, C}), x({N, N}), y({nk, C});
+  RMat k({2, nk}), w({nk});
+  load(dir + "/S.bin", S);
+  load(dir + "/x.bin", x);
+  load(dir + "/y.bin", y);
+  load(dir + "/k.bin", k);
+  load(dir + "/w.bin", w);
+  try {
+    codeare_amd::NCSENSE<CMat, RMat> ncs(p, S, {N, N, C});
+    ncs.KSpace(k);
+    ncs.Weights(w);
+    save(dir + "/E.bin", ncs * x);
+    save(dir + "/EH.bin", ncs / y);
+    save(dir + "/solve.bin", ncs->*y);
+    // the single-coil NFFT<cxfl>: Trafo and the iterative Adjoint
+    codeare_amd::Params q;
+    q.set("nk", (double)nk);
+    q.set("m", std::atof(argv[5]));
+    q.set("alpha", std::atof(argv[6]));
+    q.set_imsz({N, N});
+    codeare_amd::NFFT<CMat, RMat> ft(q);
+    ft.KSpace(k);
+    ft.Weights(w);
+    save(dir + "/trafo.bin", ft * x);
+    CMat y0({nk});
+    for (size_t j = 0; j < nk; ++j) y0.data[j] = y.data[j];
+    save(dir + "/inv.bin", ft.Adjoint(y0));
+    // errors cross the boundary as exceptions carrying the library message
+    bool threw = false;
+    try {
+      RMat bad({2, nk});
+      for (auto& v : bad.data) v = 0.75f;   // outside [-0.5, 0.5)
+      ft.KSpace(bad);
+    } catch (const std::runtime_error& e) {
+      threw = std::string(e.what()).find("outside") != std::string::npos;
+    }
+    if (!threw) { std::fprintf(stderr, "bad trajectory not rejected\n"); return 3; }
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+  std::printf("ok\n");
+  return 0;
+}

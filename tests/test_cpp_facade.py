@@ -1,0 +1,72 @@
+"""The C++ operators of include/codeare_amd/ncsense.hpp (codeare's NFFT<cxfl> and
+NCSENSE<cxfl> shape: src/matrix/ft/NFFT.hpp, src/matrix/ft/NCSENSE.hpp), driven
+by tests/cpp/t_ncsense.cpp the way src/modules/CGSENSE.cpp:132-178 drives
+NCSENSE, and checked against the CPU oracle."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import rel
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "t_ncsense")
+OP_TOL = 2e-5
+
+
+def test_facade_header_compiles():
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    src = '#include "codeare_amd/ncsense.hpp"\nint main(){return 0;}\n'
+    r = subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I",
+                        os.path.join(ROOT, "include"), "-x", "c++", "-"],
+                       input=src, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def _save(path, a, dtype):
+    np.asarray(a).astype(dtype).ravel(order="F").tofile(path)
+
+
+def _load(path, shape):
+    return np.fromfile(path, dtype=np.complex64).reshape(shape, order="F")
+
+
+@pytest.mark.gpu
+def test_cpp_ncsense_matches_oracle(ora, tmp_path):
+    from codeare_amd import data
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cpp/t_ncsense is not built (run __graft_entry__.build())")
+    prob = data.make_problem(1, 0, N=64, coils=4, spokes=48, samples=128)
+    wl = prob.wl
+    N, C, M = wl.N, wl.coils, prob.k.shape[1]
+    O = ora.NFFT((N, N), M, wl.sigma, wl.m)
+    O.kspace(prob.k)
+    y = data.make_data(prob, lambda x: O.E(prob.maps, x))
+    d = str(tmp_path)
+    _save(os.path.join(d, "S.bin"), prob.maps, np.complex64)
+    _save(os.path.join(d, "x.bin"), prob.image, np.complex64)
+    _save(os.path.join(d, "y.bin"), y, np.complex64)
+    _save(os.path.join(d, "k.bin"), prob.k, np.float32)
+    _save(os.path.join(d, "w.bin"), prob.w, np.float32)
+    print(f"t_ncsense N={N} C={C} nk={M} m={wl.m} alpha={wl.sigma}", flush=True)
+    r = subprocess.run([BIN, d, str(N), str(C), str(M), str(wl.m), str(wl.sigma)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    e_E = rel(_load(os.path.join(d, "E.bin"), (M, C)), O.E(prob.maps, prob.image))
+    e_EH = rel(_load(os.path.join(d, "EH.bin"), (N, N)),
+               O.EH_ref(prob.maps, prob.w, y, ftiter=3, fteps=1e-3))
+    xo, _, _ = O.cgls_ref(prob.maps, prob.w, y, mode=1, ftiter=3, fteps=1e-3, maxit=5,
+                          cgeps=1e-7, lam=5e-2)
+    e_solve = rel(_load(os.path.join(d, "solve.bin"), (N, N)), xo)
+    e_trafo = rel(_load(os.path.join(d, "trafo.bin"), (M,)), O.trafo(prob.image))
+    io, _, _ = O.itrafo(np.asfortranarray(y)[:, 0], prob.w, maxit=3, eps=7e-4)
+    e_inv = rel(_load(os.path.join(d, "inv.bin"), (N, N)), io)
+    print(f"C++ facade: E {e_E:.2e}  E/ {e_EH:.2e}  ->* {e_solve:.2e}  "
+          f"NFFT* {e_trafo:.2e}  NFFT::Adjoint {e_inv:.2e}")
+    assert e_E <= OP_TOL and e_EH <= OP_TOL and e_trafo <= OP_TOL
+    assert e_inv <= 1e-5 and e_solve <= 1e-4

@@ -227,3 +227,37 @@ def test_toeplitz_cg_matches_nufft_cg_c1(cgs, ora):
     print(f"CG(10) toeplitz vs nufft image {e:.2e}")
     assert rn["iters"][0] == rt["iters"][0] == 10
     assert e <= 1e-4
+
+
+def test_gridding_deterministic_mode(cgs, ora):
+    """The fixed-order gridding kernel (CGS_DETERMINISTIC) is bit-reproducible
+    and agrees with the default atomic kernel and the oracle."""
+    prob, y, P, O = _c1_problem(cgs, ora)
+    wl = prob.wl
+    Pd = cgs.Plan((wl.N,) * wl.d, prob.k.shape[1], wl.sigma, wl.m, ncoils=wl.coils, deterministic=True)
+    Pd.kspace(prob.k)
+    Pd.weights(prob.w)
+    Pd.sensitivities(prob.maps)
+    a1 = Pd.EH(y)
+    a2 = Pd.EH(y)
+    assert np.array_equal(a1, a2)
+    xo = O.EH_true(prob.maps, prob.w, y)
+    assert rel(a1, xo) <= OP_TOL
+    assert rel(P.EH(y), a1) <= 1e-6
+
+
+@pytest.mark.parametrize("toeplitz", [False, True])
+def test_graph_solve_matches_launch_solve(cgs, ora, toeplitz):
+    """CG iterations replayed from a captured HIP graph give the same result as
+    launching them one by one (several replays, checked against each other)."""
+    prob, y, P, O = _c1_problem(cgs, ora)
+    xa, ra = P.solve(y, maxit=24, tol=0.0, toeplitz=toeplitz)
+    for _ in range(3):
+        xb, rb = P.solve(y, maxit=24, tol=0.0, toeplitz=toeplitz, graph_iters=8)
+        assert rb["iters"][0] == ra["iters"][0] == 24
+        assert rel(xb, xa) <= 1e-6
+    # early stop inside a graph: iterations past convergence do nothing
+    xc, rc = P.solve(y, maxit=200, tol=1e-3, toeplitz=toeplitz, graph_iters=8)
+    xd, rd = P.solve(y, maxit=200, tol=1e-3, toeplitz=toeplitz)
+    assert rc["iters"][0] == rd["iters"][0]
+    assert rel(xc, xd) <= 1e-6
