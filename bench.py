@@ -34,20 +34,41 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", type=int, default=2)
     ap.add_argument("--maxit", type=int, default=3000)
+    ap.add_argument("--graph-iters", type=int, default=8, help="CG iterations per HIP graph launch (0: none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary measurements")
     ap.add_argument("--cpu-iters", type=int, default=2, help="CG iterations in the CPU sample")
+    ap.add_argument("--no-dynamic", action="store_true", help="skip the 64-frame batched series")
+    ap.add_argument("--volume", action="store_true", help="also time the 3D koosh-ball workload")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with 3 device copies of 256 MiB (counter calibration under rocprofv3 --pmc)")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "codeare_amd", "weights", "unet_golden2d_256_c32.npz"),
+                    help="trained warm-start network (skipped if absent)")
     return ap.parse_args()
 
 
 def per_kernel(stats):
     per = {}
     for k, s in stats.items():
-        if s["launches"] > 0:
+        if s["launches"] > 0 and s["ms"] > 0:
             per[k] = {"launches": s["launches"], "ms_total": round(s["ms"], 3),
                       "us_per_launch": round(1e3 * s["ms"] / s["launches"], 2),
-                      "GBps": round((s["bytes"] / (s["ms"] * 1e-3)) / 1e9, 1) if s["ms"] > 0 else None}
+                      "bytes_per_launch": s["bytes"] / s["launches"],
+                      "GBps": round((s["bytes"] / (s["ms"] * 1e-3)) / 1e9, 1)}
     return per
+
+
+def traffic_for(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None if that workload/kernel has not been counted."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t[workload][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main():
@@ -88,11 +109,13 @@ def main():
     tol = wl.tol if wl.tol > 0 else 1e-5
     lam = wl.lam
 
-    def solve(toeplitz, lam_=lam):
-        _, rep = plan.solve(y, maxit=args.maxit, tol=tol, lam=lam_, out=x, toeplitz=toeplitz)
+    def solve(toeplitz, lam_=lam, graph=args.graph_iters):
+        _, rep = plan.solve(y, maxit=args.maxit, tol=tol, lam=lam_, out=x, toeplitz=toeplitz,
+                            graph_iters=graph)
         return rep
 
-    # ---- timed region: plain (zero-start) CG-SENSE solves ----
+    # ---- timed region: plain (zero-start) CG-SENSE solves, K iterations per HIP graph ----
+    plan.profile_mask([])          # launches and bytes counted, no timing events
     for _ in range(args.warmup):
         solve(True)
     plan.kernel_stats(reset=True)
@@ -102,7 +125,7 @@ def main():
     plan.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    per = per_kernel(plan.kernel_stats(reset=True))
+    plan.kernel_stats(reset=True)
     ms_per_step = 1e3 * elapsed / args.steps
     solves_per_s = args.steps / elapsed
     iters = [r["max_iters"] for r in reps]
@@ -111,11 +134,22 @@ def main():
     xt = x_true.cpu().numpy()
     err_phantom = float(np.linalg.norm(xr - xt) / np.linalg.norm(xt))
 
+    # ---- kernel durations with HIP events on the plan's stream: one solve with
+    # every class timed (the breakdown), then one with only the dominant class
+    # timed (its average launch duration, least perturbed) ----
+    plan.profile_mask(None)
+    solve(True, graph=0)
+    per = per_kernel(plan.kernel_stats(reset=True))
     dom = max(per, key=lambda k: per[k]["ms_total"])
-    ach = per[dom]["GBps"]
+    plan.profile_mask([dom])
+    solve(True, graph=0)
+    pd = per_kernel(plan.kernel_stats(reset=True))[dom]
+    plan.profile_mask(None)
+    ach = pd["GBps"]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "us_per_launch": per[dom]["us_per_launch"]}
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_for(wl.name, dom),
+                "us_per_launch": pd["us_per_launch"], "launches": pd["launches"],
+                "bytes_per_launch": round(pd["bytes_per_launch"])}
 
     out = {
         "metric": METRIC,
@@ -140,16 +174,20 @@ def main():
         "rel_err_vs_phantom": round(err_phantom, 5),
         "plan_setup_s": round(t_plan, 3),
         "roofline": roofline,
+        "graph_iters": args.graph_iters,
         "kernels": per,
     }
 
     if not args.no_extra:
         # same solve with E then E^H in every iteration (gather, FFTs, scatter)
+        plan.profile_mask([])
         solve(False)
-        plan.kernel_stats(reset=True)
         t0 = time.perf_counter()
         rep = solve(False)
         t_nufft = time.perf_counter() - t0
+        plan.profile_mask(None)
+        plan.kernel_stats(reset=True)
+        solve(False, graph=0)
         pn = per_kernel(plan.kernel_stats(reset=True))
         out["nufft_operator_solve"] = {"solves_per_s": round(1.0 / t_nufft, 4), "iters": rep["max_iters"],
                                        "final_rel_resid": rep["final_rel"][0],
@@ -167,20 +205,173 @@ def main():
             lmax = float(torch.linalg.vector_norm(qv) / torch.linalg.vector_norm(xs))
             xs = qv / torch.linalg.vector_norm(qv)
         lam_s = 1e-3 * lmax
+        plan.profile_mask([])
         solve(True, lam_s)
         t0 = time.perf_counter()
         rep = solve(True, lam_s)
         t_s = time.perf_counter() - t0
+        plan.profile_mask(None)
         xr = x.cpu().numpy()
         out["scaled_lambda_solve"] = {"lambda": round(lam_s, 3), "lambda_max": round(lmax, 1),
                                       "solves_per_s": round(1.0 / t_s, 3), "iters": rep["max_iters"],
                                       "final_rel_resid": rep["final_rel"][0],
                                       "rel_err_vs_phantom": round(float(np.linalg.norm(xr - xt) / np.linalg.norm(xt)), 5)}
 
+        out["warm_start_solve"] = warm_start_leg(args, plan, y, x, tol, lam, iters[-1], ms_per_step, xt)
+        if not args.no_dynamic:
+            out["dynamic_batch"] = dynamic_leg(args)
+    if args.volume:
+        out["volume_3d"] = volume_leg(args)
+
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prob, y_noisy, iters[-1], lam, args.cpu_iters)
         out["speedup_vs_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 1)
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = 2 * src.numel() * 4
     print(json.dumps(out))
+
+
+def warm_start_leg(args, plan, y, x, tol, lam, plain_iters, plain_ms, xt):
+    """CG from the U-Net's output vs from zero, same y, same stopping test.
+    The warm path's time includes E^H W y, the first steepest-descent scaling
+    and the network inference."""
+    import numpy as np
+    import torch
+    if not os.path.exists(args.weights):
+        return {"skipped": f"no trained network at {os.path.relpath(args.weights, ROOT)}"}
+    from codeare_amd.warmstart import UNet
+    N0, N1 = plan.N
+    net = UNet(N1, N0, levels=4, max_batch=1)
+    net.load(args.weights)
+    dev = x.device
+    b = torch.empty_like(x)
+    Ab = torch.empty_like(x)
+    xin = torch.empty((1, N1, N0), dtype=torch.complex64, device=dev)
+    x0 = torch.empty((1, N1, N0), dtype=torch.complex64, device=dev)
+
+    def warm():
+        plan.EH(y, out=b)
+        plan.normal(b, lam=lam, out=Ab, toeplitz=True)
+        plan.sync()
+        bb = torch.vdot(b, b).real
+        bAb = torch.vdot(b, Ab).real
+        xa = (bb / bAb) * b
+        s = torch.sqrt(torch.mean(xa.abs() ** 2))
+        xin.copy_((xa / s).view(1, N1, N0))
+        torch.cuda.synchronize()
+        net.forward(xin, out=x0)
+        x0.mul_(s)
+        torch.cuda.synchronize()
+        _, rep = plan.solve(y, x0=x0, maxit=args.maxit, tol=tol, lam=lam, out=x, toeplitz=True)
+        return rep
+
+    warm()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = [warm() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    xr = x.cpu().numpy()
+    wi = reps[-1]["max_iters"]
+    return {"solves_per_s": round(args.steps / el, 4), "ms_per_solve": round(1e3 * el / args.steps, 3),
+            "iters": wi, "plain_iters": plain_iters, "iteration_ratio": round(plain_iters / max(1, wi), 3),
+            "speedup_vs_plain": round(plain_ms / (1e3 * el / args.steps), 3),
+            "final_rel_resid": reps[-1]["final_rel"][0],
+            "rel_err_vs_phantom": round(float(np.linalg.norm(xr - xt) / np.linalg.norm(xt)), 5),
+            "network": os.path.relpath(args.weights, ROOT)}
+
+
+def dynamic_leg(args):
+    """64 frames of 192^2 x 16 coils, 13..89 golden-angle spokes x 384 per frame,
+    solved together (per-frame scalars and stopping) to 1e-5 with K-iteration
+    HIP graphs; per-frame latency = host-observed time to the check that saw
+    the frame stop."""
+    import numpy as np
+    import torch
+    from codeare_amd import cgsense, data
+    prob = data.make_problem(4, 0)
+    wl = prob.wl
+    M = prob.k.shape[1]
+    t0 = time.perf_counter()
+    plan = cgsense.Plan((wl.N, wl.N), M, wl.sigma, wl.m, ncoils=wl.coils, frame_offsets=prob.frame_offsets,
+                        device_io=True, autotune=True)
+    plan.kspace(prob.k)
+    plan.weights(prob.w)
+    plan.sensitivities(prob.maps)
+    t_plan = time.perf_counter() - t0
+    dev = torch.device("cuda:0")
+    F = plan.F
+    xt = torch.from_numpy(np.ascontiguousarray(prob.image.reshape(-1, order="F"))).to(dev)
+    y = torch.empty(M * wl.coils, dtype=torch.complex64, device=dev)
+    plan.E(xt, out=y)
+    plan.sync()
+    yh = data.add_noise(y.cpu().numpy(), wl.snr_db, prob.rng_state)
+    y = torch.from_numpy(yh).to(dev)
+    x = torch.empty(plan.Ntot * F, dtype=torch.complex64, device=dev)
+    torch.cuda.synchronize()
+    tol = wl.tol
+
+    def run():
+        _, rep = plan.solve(y, maxit=args.maxit, tol=tol, lam=wl.lam, out=x, toeplitz=True, graph_iters=8)
+        return rep
+
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep = run()
+    el = time.perf_counter() - t0
+    lat = np.array(rep["done_ms"][:F])
+    its = np.array(rep["iters"][:F])
+    counts = np.diff(prob.frame_offsets) // wl.samples
+    return {"workload": wl.name, "frames": F, "spokes_per_frame": [int(counts.min()), int(counts.max())],
+            "lambda": wl.lam, "tol": tol, "graph_iters": 8,
+            "solves_per_s": round(F / el, 3), "batch_ms": round(1e3 * el, 2),
+            "frame_latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
+            "frame_latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+            "iters_min_median_max": [int(its.min()), int(np.median(its)), int(its.max())],
+            "final_rel_resid_max": float(np.max(rep["final_rel"][:F])),
+            "frames_converged": int(np.sum(np.array(rep["final_rel"][:F]) <= tol)),
+            "plan_setup_s": round(t_plan, 2)}
+
+
+def volume_leg(args):
+    """3D koosh ball: 128^3, 16 coils, 20000 x 256 samples, sigma 1.5, m 3,
+    15 CG iterations from zero (lambda 0) through the Toeplitz operator."""
+    import numpy as np
+    import torch
+    from codeare_amd import cgsense, data
+    prob = data.make_problem(3, 0)
+    wl = prob.wl
+    M = prob.k.shape[1]
+    t0 = time.perf_counter()
+    plan = cgsense.Plan((wl.N,) * 3, M, wl.sigma, wl.m, ncoils=wl.coils, device_io=True, profile=True)
+    plan.kspace(prob.k)
+    plan.weights(prob.w)
+    plan.sensitivities(prob.maps)
+    t_plan = time.perf_counter() - t0
+    dev = torch.device("cuda:0")
+    xt = torch.from_numpy(np.ascontiguousarray(prob.image.reshape(-1, order="F"))).to(dev)
+    y = torch.empty(M * wl.coils, dtype=torch.complex64, device=dev)
+    plan.E(xt, out=y)
+    x = torch.empty(plan.Ntot, dtype=torch.complex64, device=dev)
+    res = {"workload": wl.name, "plan_setup_s": round(t_plan, 2)}
+    for toep in (True, False):
+        plan.solve(y, maxit=2, tol=0.0, lam=wl.lam, out=x, toeplitz=toep)
+        plan.kernel_stats(reset=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, rep = plan.solve(y, maxit=wl.maxit, tol=0.0, lam=wl.lam, out=x, toeplitz=toep)
+        el = time.perf_counter() - t0
+        per = per_kernel(plan.kernel_stats(reset=True))
+        res["toeplitz" if toep else "nufft"] = {
+            "solve_s": round(el, 3), "iters": rep["max_iters"], "ms_per_iter": round(1e3 * el / wl.maxit, 2),
+            "kernels": per}
+    return res
 
 
 def cpu_baseline(prob, y, gpu_iters, lam, cpu_iters):

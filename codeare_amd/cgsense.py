@@ -28,13 +28,14 @@ CGS_ERR_NAN = 6
 CGS_DEVICE_IO = 0x1
 CGS_PROFILE = 0x2
 CGS_DETERMINISTIC = 0x4
+CGS_AUTOTUNE = 0x8
 CGS_EH_TRUE = 0
 CGS_EH_REF = 1
 CGS_TOL_NORM_RATIO = 0
 CGS_TOL_SQUARED_RATIO = 1
 CGS_NORMAL_NUFFT = 0
 CGS_NORMAL_TOEPLITZ = 1
-KERNELS = ["fft_fwd", "interp", "grid", "fft_adj", "coil", "cg", "other"]
+KERNELS = ["fft_fwd", "interp", "grid", "fft_adj", "coil", "cg", "other", "toep_pad", "toep_conv", "toep_crop"]
 
 EXPORTED = [
     "cgs_plan_create", "cgs_plan_destroy", "cgs_set_stream", "cgs_sync",
@@ -42,7 +43,7 @@ EXPORTED = [
     "cgs_nfft_forward", "cgs_nfft_adjoint", "cgs_nfft_inverse_ref",
     "cgs_E", "cgs_EH", "cgs_normal", "cgs_solve",
     "cgs_kernel_stats", "cgs_plan_info", "cgs_status_string", "cgs_last_error",
-    "cgs_residual_history",
+    "cgs_residual_history", "cgs_autotune", "cgs_profile_mask",
 ]
 
 
@@ -58,9 +59,16 @@ class SolveOpts(ct.Structure):
                 ("fteps", ct.c_double), ("normal_op", ct.c_int), ("graph_iters", ct.c_int)]
 
 
+class TuneReport(ct.Structure):
+    _fields_ = [("visits_per_item", ct.c_int), ("grid_coils", ct.c_int), ("interp_coils", ct.c_int),
+                ("from_cache", ct.c_int), ("grid_ms", ct.c_double), ("interp_ms", ct.c_double),
+                ("default_grid_ms", ct.c_double), ("default_interp_ms", ct.c_double)]
+
+
 class SolveReport(ct.Structure):
     _fields_ = [("iters", ct.c_int * 1024), ("final_rel", ct.c_double * 1024),
-                ("nframes", ct.c_int), ("max_iters", ct.c_int), ("status", ct.c_int)]
+                ("nframes", ct.c_int), ("max_iters", ct.c_int), ("status", ct.c_int),
+                ("done_ms", ct.c_double * 1024), ("total_ms", ct.c_double)]
 
 
 _lib = None
@@ -94,6 +102,8 @@ def load_library(path: str | None = None):
     L.cgs_kernel_stats.argtypes = [P, I, ct.POINTER(I64), ct.POINTER(D), ct.POINTER(D), I]
     L.cgs_plan_info.argtypes = [P, ct.POINTER(I), ct.POINTER(I), ct.POINTER(I64), ct.POINTER(I)]
     L.cgs_residual_history.argtypes = [P, I, P, I]
+    L.cgs_autotune.argtypes = [P, I, I, ct.POINTER(TuneReport)]
+    L.cgs_profile_mask.argtypes = [P, ct.c_uint]
     L.cgs_status_string.restype = ct.c_char_p
     L.cgs_status_string.argtypes = [I]
     L.cgs_last_error.restype = ct.c_char_p
@@ -128,7 +138,7 @@ class Plan:
 
     def __init__(self, N, M_total: int, sigma: float = 2.0, m: int = 3, ncoils: int = 1,
                  frame_offsets=None, device_io: bool = False, profile: bool = False,
-                 deterministic: bool = False):
+                 deterministic: bool = False, autotune: bool = False):
         L = load_library()
         self.N = tuple(int(v) for v in N)
         self.d = len(self.N)
@@ -142,7 +152,7 @@ class Plan:
         self.F = len(self.foff) - 1
         self.device_io = device_io
         flags = ((CGS_DEVICE_IO if device_io else 0) | (CGS_PROFILE if profile else 0)
-                 | (CGS_DETERMINISTIC if deterministic else 0))
+                 | (CGS_DETERMINISTIC if deterministic else 0) | (CGS_AUTOTUNE if autotune else 0))
         Narr = (ct.c_int * self.d)(*self.N)
         h = ct.c_void_p()
         _check(L.cgs_plan_create(ct.byref(h), self.d, ct.cast(Narr, ct.c_void_p), self.M,
@@ -294,8 +304,15 @@ class Plan:
             _check(rc)
         F = min(self.F, 1024)
         report = {"iters": list(rep.iters[:F]), "final_rel": list(rep.final_rel[:F]),
-                  "max_iters": rep.max_iters, "status": rep.status}
+                  "max_iters": rep.max_iters, "status": rep.status,
+                  "done_ms": list(rep.done_ms[:F]), "total_ms": rep.total_ms}
         return x, report
+
+    def autotune(self, reps: int = 5, use_cache: bool = False) -> dict:
+        """Time the gridding/gather launch shapes on this trajectory, keep the fastest."""
+        r = TuneReport()
+        _check(load_library().cgs_autotune(self._h, int(reps), int(use_cache), ct.byref(r)))
+        return {k: getattr(r, k) for k, _ in TuneReport._fields_}
 
     def residual_history(self, frame: int = 0, cap: int = 100000):
         buf = np.zeros(cap, dtype=np.float64)
@@ -303,6 +320,11 @@ class Plan:
         if n < 0:
             _check(-n)
         return buf[:n]
+
+    def profile_mask(self, kernels=None):
+        """Time only these kernel classes (names from KERNELS); None = all."""
+        mask = 0xFFFFFFFF if kernels is None else sum(1 << KERNELS.index(k) for k in kernels)
+        _check(load_library().cgs_profile_mask(self._h, ct.c_uint(mask)))
 
     def kernel_stats(self, reset: bool = True):
         L = load_library()

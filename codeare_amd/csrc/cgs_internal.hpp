@@ -200,6 +200,7 @@ struct Plan {
   // profiling
   struct Stat { int64_t launches = 0; double ms = 0, bytes = 0; };
   Stat stat[CGS_K_COUNT];
+  unsigned prof_mask = ~0u;           // kernel classes timed with events
   struct Pending { hipEvent_t a, b; int kind; double bytes; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;

@@ -17,6 +17,7 @@
 #include <string>
 #include <thread>
 #include <cstdlib>
+#include <chrono>
 #include <vector>
 
 #include "cgs_internal.hpp"
@@ -127,15 +128,23 @@ struct Prof {
   int kind;
   double bytes;
   hipEvent_t a = nullptr, b = nullptr;
-  // no timing events while a HIP graph is being captured
+  bool timed = false;
+  // no timing events while a HIP graph is being captured; classes outside
+  // prof_mask are counted (launches, bytes) but not timed
   Prof(Plan& p_, int kind_, double bytes_) : p(p_), kind(kind_), bytes(bytes_) {
     if (!(p.flags & CGS_PROFILE) || p.capturing) return;
+    if (!((p.prof_mask >> kind) & 1u)) {
+      p.stat[kind].launches += 1;
+      p.stat[kind].bytes += bytes;
+      return;
+    }
+    timed = true;
     a = take();
     b = take();
     (void)hipEventRecord(a, p.stream);
   }
   ~Prof() {
-    if (!(p.flags & CGS_PROFILE) || p.capturing) return;
+    if (!timed) return;
     (void)hipEventRecord(b, p.stream);
     p.pending.push_back({a, b, kind, bytes});
   }
@@ -474,6 +483,7 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
                        const int64_t* foff, unsigned flags);
 static void destroy_plan(Plan& p);
 static void toep_drop(Plan& p);
+static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep);
 
 // ---- trajectory presort -----------------------------------------------------------------
 static void free_samples(Plan& p) {
@@ -980,7 +990,7 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
   const int d = p.d, C = p.C, F = p.F;
   const int64_t N0 = p.N[0], N1 = p.N[1], N2 = p.N[2];
   const int64_t M0 = 2 * N0, M1 = 2 * N1;
-  auto pass = [&](const FftTab& tab, int sign, const float2* in, const float2* xin, float2* out, int nin,
+  auto pass = [&](int kind, const FftTab& tab, int sign, const float2* in, const float2* xin, float2* out, int nin,
                   int nout, int64_t inner, int64_t opf, int ld, const float* That, int64_t cells, double bytes) {
     FftArgs a = fft_base(p, tab, sign);
     a.in = in; a.x = xin; a.out = out;
@@ -990,26 +1000,26 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
     if (ld == 4) apply_pupd(p, a, pu);
     if (That) { a.That = That; a.conv_cells = cells; a.pad_in = 1; a.crop_out = 1; }
     fft_finish(a);
-    Prof pr(p, sign < 0 ? CGS_K_FFT_FWD : CGS_K_FFT_ADJ, bytes * F);
+    Prof pr(p, kind, bytes * F);
     launch_fft(a, ld, 0, p.stream);
   };
   if (d == 2) {
-    pass(p.ft[0], -1, nullptr, x, p.tt1, (int)N0, (int)M0, C, N1, 4, nullptr, 0,
+    pass(CGS_K_TOEP_PAD, p.ft[0], -1, nullptr, x, p.tt1, (int)N0, (int)M0, C, N1, 4, nullptr, 0,
          ((double)(1 + C) * N0 * N1 + (double)C * M0 * N1) * 8);
-    pass(p.ft[1], -1, p.tt1, nullptr, p.tt1, (int)N1, (int)N1, C * M0, 1, 0, p.That, M0,
+    pass(CGS_K_TOEP_CONV, p.ft[1], -1, p.tt1, nullptr, p.tt1, (int)N1, (int)N1, C * M0, 1, 0, p.That, M0,
          2.0 * C * M0 * N1 * 8 + 4.0 * M0 * M1);
-    pass(p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1, 0, nullptr, 0,
+    pass(CGS_K_TOEP_CROP, p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1, 0, nullptr, 0,
          (double)C * (M0 + N0) * N1 * 8);
   } else {
-    pass(p.ft[0], -1, nullptr, x, p.tt1, (int)N0, (int)M0, C, N1 * N2, 4, nullptr, 0,
+    pass(CGS_K_TOEP_PAD, p.ft[0], -1, nullptr, x, p.tt1, (int)N0, (int)M0, C, N1 * N2, 4, nullptr, 0,
          ((double)(1 + C) * N0 * N1 * N2 + (double)C * M0 * N1 * N2) * 8);
-    pass(p.ft[1], -1, p.tt1, nullptr, p.tt2, (int)N1, (int)M1, C * M0, N2, 0, nullptr, 0,
+    pass(CGS_K_TOEP_PAD, p.ft[1], -1, p.tt1, nullptr, p.tt2, (int)N1, (int)M1, C * M0, N2, 0, nullptr, 0,
          (double)C * M0 * (N1 + M1) * N2 * 8);
-    pass(p.ft[2], -1, p.tt2, nullptr, p.tt2, (int)N2, (int)N2, C * M0 * M1, 1, 0, p.That, M0 * M1,
+    pass(CGS_K_TOEP_CONV, p.ft[2], -1, p.tt2, nullptr, p.tt2, (int)N2, (int)N2, C * M0 * M1, 1, 0, p.That, M0 * M1,
          2.0 * C * M0 * M1 * N2 * 8 + 4.0 * M0 * M1 * 2 * N2);
-    pass(p.ft[1], +1, p.tt2, nullptr, p.tt1, (int)M1, (int)N1, C * M0, N2, 0, nullptr, 0,
+    pass(CGS_K_TOEP_CROP, p.ft[1], +1, p.tt2, nullptr, p.tt1, (int)M1, (int)N1, C * M0, N2, 0, nullptr, 0,
          (double)C * M0 * (M1 + N1) * N2 * 8);
-    pass(p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1 * N2, 0, nullptr, 0,
+    pass(CGS_K_TOEP_CROP, p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1 * N2, 0, nullptr, 0,
          (double)C * (M0 + N0) * N1 * N2 * 8);
   }
   return CGS_OK;
@@ -1061,9 +1071,14 @@ static int cg_iteration_fused(Plan& p, int normal_op, float2* x, float2* r, cons
 }
 
 // ---- CG solve -------------------------------------------------------------------------
+static double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, const cgs_solve_opts& o,
-                 cgs_solve_report* rep) {
+                 cgs_solve_report* rep, std::chrono::steady_clock::time_point t_start) {
   const int F = p.F;
+  std::vector<double> done_ms((size_t)F, -1.0);
   const size_t ib = sizeof(float2) * (size_t)F * p.Ntot;
   float2 *x = p.img[0], *r = p.img[1], *pv = p.img[2], *q = p.img[3];
   double* partA = p.partials;
@@ -1156,7 +1171,12 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
     if (it >= step) {
       HIPCHK(hipEventSynchronize(ev[par ^ 1]));
       bool all = true;
-      for (int f = 0; f < F; ++f) all &= (p.h_done[(par ^ 1) * F + f] != 0);
+      const double now = ms_since(t_start);
+      for (int f = 0; f < F; ++f) {
+        const bool d = p.h_done[(par ^ 1) * F + f] != 0;
+        if (d && done_ms[f] < 0) done_ms[f] = now;
+        all &= d;
+      }
       if (all) break;
     }
   }
@@ -1165,6 +1185,9 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
   HIPCHK(hipStreamSynchronize(p.stream));
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
+  const double t_end = ms_since(t_start);
+  for (int f = 0; f < F; ++f)
+    if (done_ms[f] < 0) done_ms[f] = t_end;
   int maxi = 0;
   for (int f = 0; f < F; ++f) {
     maxi = std::max(maxi, p.h_cgst[f].it);
@@ -1177,6 +1200,7 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
     for (int f = 0; f < F && f < 1024; ++f) {
       rep->iters[f] = p.h_cgst[f].it;
       rep->final_rel[f] = p.h_cgst[f].rel;
+      rep->done_ms[f] = done_ms[f];
     }
   }
   return CGS_OK;
@@ -1274,7 +1298,128 @@ int cgs_set_trajectory(cgs_plan plan, const float* k) {
   toep_drop(p);
   p.have_k = true;
   p.have_w = false;   // weights follow the sample order (NFFT.hpp:258-296: Weights after KSpace)
+  if (p.flags & CGS_AUTOTUNE) CHK(autotune(p, 5, true, nullptr));
   return CGS_OK;
+  GUARD_END
+}
+
+// ---- launch-shape autotuner ------------------------------------------------------------
+namespace cgs {
+struct TuneChoice {
+  int vpi, cgr, icg;
+  double grid_ms, interp_ms, def_grid_ms, def_interp_ms;
+};
+static std::mutex g_tune_mu;
+static std::vector<std::pair<std::string, TuneChoice>> g_tune_cache;
+
+static std::string tune_key(const Plan& p) {
+  int64_t per = std::max<int64_t>(1, p.Mtot / p.F), b = 1;
+  while (b < per) b <<= 1;
+  char key[160];
+  snprintf(key, sizeof(key), "%d|%d|%d|%d|%d|%d|%lld|%d", p.d, p.n[0], p.n[1], p.n[2], p.C, p.F, (long long)b, p.m);
+  return key;
+}
+
+static int time_kernel(Plan& p, int reps, bool grid, double* ms) {
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  // one untimed run, then `reps` timed ones
+  for (int r = 0; r <= reps; ++r) {
+    if (r == 1) HIPCHK(hipEventRecord(e0, p.stream));
+    if (grid) CHK(gridding(p, p.ybuf[0], false, nullptr));
+    else CHK(interp(p, p.ybuf[1], nullptr));
+  }
+  HIPCHK(hipEventRecord(e1, p.stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  CHK(check_launch());
+  *ms = t / std::max(1, reps);
+  return CGS_OK;
+}
+
+static int apply_vpi(Plan& p, int vpi) {
+  if (p.visits_per_item == vpi) return CGS_OK;
+  p.visits_per_item = vpi;
+  HIPCHK(hipStreamSynchronize(p.stream));
+  CHK(build_samples(p, p.h_k.data()));
+  if (p.have_w) {   // weights are stored in the sorted sample order
+    std::vector<float> ws((size_t)p.Mtot);
+    for (int64_t s = 0; s < p.Mtot; ++s) ws[s] = p.h_w[p.h_perm[s]];
+    HIPCHK(hipMemcpy(p.w, ws.data(), sizeof(float) * ws.size(), hipMemcpyHostToDevice));
+  }
+  return CGS_OK;
+}
+
+static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
+  if (!p.have_k || p.h_k.empty()) { set_error("autotune needs the trajectory"); return CGS_ERR_STATE; }
+  const std::string key = tune_key(p);
+  TuneChoice best{};
+  bool cached = false;
+  if (use_cache) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto& kv : g_tune_cache)
+      if (kv.first == key) { best = kv.second; cached = true; }
+  }
+  if (!cached) {
+    reps = std::max(1, std::min(reps, 50));
+    HIPCHK(hipMemsetAsync(p.ybuf[0], 0, sizeof(float2) * (size_t)p.Mtot * p.C, p.stream));
+    HIPCHK(hipMemsetAsync(p.grid, 0, sizeof(float2) * (size_t)p.ntot * p.C * p.F, p.stream));
+    const bool reg = p.d == 2 && p.T == 8 && p.use_grid_reg;
+    std::vector<int> vpis = (p.d == 2) ? std::vector<int>{256, 512, 1024} : std::vector<int>{64, 128, 256};
+    std::vector<int> cgrs = (reg && p.C > 8) ? std::vector<int>{8, 16} : std::vector<int>{p.grid_cgr};
+    std::vector<int> icgs = (p.C > 8) ? std::vector<int>{8, 16} : std::vector<int>{p.interp_cg};
+    const int vpi0 = p.visits_per_item, cgr0 = p.grid_cgr, icg0 = p.interp_cg;
+    double g0 = 0, i0 = 0;
+    CHK(time_kernel(p, reps, true, &g0));
+    CHK(time_kernel(p, reps, false, &i0));
+    best = TuneChoice{vpi0, cgr0, icg0, g0, i0, g0, i0};
+    for (int vpi : vpis) {
+      CHK(apply_vpi(p, vpi));
+      for (int cgr : cgrs) {
+        p.grid_cgr = cgr;
+        double t = 0;
+        CHK(time_kernel(p, reps, true, &t));
+        if (t < best.grid_ms) { best.grid_ms = t; best.vpi = vpi; best.cgr = cgr; }
+      }
+    }
+    for (int icg : icgs) {   // the gather does not depend on the visit lists
+      p.interp_cg = icg;
+      double t = 0;
+      CHK(time_kernel(p, reps, false, &t));
+      if (t < best.interp_ms) { best.interp_ms = t; best.icg = icg; }
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache.emplace_back(key, best);
+  }
+  CHK(apply_vpi(p, best.vpi));
+  p.grid_cgr = best.cgr;
+  p.interp_cg = best.icg;
+  // captured graphs hold the old launch shapes
+  if (p.graph_exec) (void)hipGraphExecDestroy(p.graph_exec);
+  p.graph_exec = nullptr;
+  p.graph_key.clear();
+  if (rep) {
+    rep->visits_per_item = best.vpi;
+    rep->grid_coils = best.cgr;
+    rep->interp_coils = best.icg;
+    rep->from_cache = cached ? 1 : 0;
+    rep->grid_ms = best.grid_ms;
+    rep->interp_ms = best.interp_ms;
+    rep->default_grid_ms = best.def_grid_ms;
+    rep->default_interp_ms = best.def_interp_ms;
+  }
+  return CGS_OK;
+}
+}  // namespace cgs
+
+int cgs_autotune(cgs_plan plan, int reps, int use_cache, cgs_tune_report* report) {
+  GUARD_BEGIN
+  if (!plan) { set_error("null argument"); return CGS_ERR_ARG; }
+  return autotune(plan->p, reps, use_cache != 0, report);
   GUARD_END
 }
 
@@ -1444,6 +1589,7 @@ int cgs_nfft_inverse_ref(cgs_plan plan, int frame, const void* f, void* img, int
 int cgs_solve(cgs_plan plan, const void* y, const void* x0, void* x, const cgs_solve_opts* opts,
               cgs_solve_report* report) {
   GUARD_BEGIN
+  const auto t_start = std::chrono::steady_clock::now();
   if (!plan || !y || !x || !opts) { set_error("null argument"); return CGS_ERR_ARG; }
   Plan& p = plan->p;
   CHK(require_ready(p, true));
@@ -1473,11 +1619,20 @@ int cgs_solve(cgs_plan plan, const void* y, const void* x0, void* x, const cgs_s
     CHK(op_EH(p, o.eh_mode, p.ybuf[0], xd, o.ftiter, o.fteps, nullptr, 0.f, nullptr, nullptr));
     if (report) { std::memset(report, 0, sizeof(*report)); report->nframes = p.F; }
   } else {
-    CHK(solve(p, yd, x0d, xd, o, report));
+    CHK(solve(p, yd, x0d, xd, o, report, t_start));
   }
   CHK(check_launch());
   CHK(stage_out(p, x, xd, ib));
+  if (report) report->total_ms = ms_since(t_start);
   return (report && report->status == CGS_ERR_NAN) ? CGS_ERR_NAN : CGS_OK;
+  GUARD_END
+}
+
+int cgs_profile_mask(cgs_plan plan, unsigned mask) {
+  GUARD_BEGIN
+  if (!plan) { set_error("null argument"); return CGS_ERR_ARG; }
+  plan->p.prof_mask = mask;
+  return CGS_OK;
   GUARD_END
 }
 

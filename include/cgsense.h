@@ -49,6 +49,7 @@ enum {
 #define CGS_DEVICE_IO      0x1u  /* operator data pointers are device memory */
 #define CGS_PROFILE        0x2u  /* time every kernel launch with HIP events */
 #define CGS_DETERMINISTIC  0x4u  /* fixed summation order in the gridding (slower) */
+#define CGS_AUTOTUNE       0x8u  /* cgs_set_trajectory runs (or reuses) cgs_autotune */
 
 /* E^H variants */
 #define CGS_EH_TRUE        0     /* x = sum_c conj(S_c) .* NFFT^H(w .* y_c)   */
@@ -81,6 +82,10 @@ typedef struct {
   int nframes;
   int max_iters;
   int status;           /* CGS_OK, or CGS_ERR_NAN if a frame hit a NaN scalar */
+  double done_ms[1024]; /* per frame: host-observed time from the call's start
+                           to the convergence check that saw the frame stop
+                           (one launch step late at most; for latency figures) */
+  double total_ms;      /* whole call, upload and download included          */
 } cgs_solve_report;
 
 /* --- plan life cycle (NFFT.hpp:66-191, NCSENSE.hpp:69-184) --------------- */
@@ -141,6 +146,23 @@ int cgs_solve(cgs_plan plan, const void* y, const void* x0, void* x,
  * CGLS.hpp:38, here per solve).  Returns the number of entries written. */
 int cgs_residual_history(cgs_plan plan, int frame, double* out, int cap);
 
+/* --- launch-shape autotuner (no counterpart in codeare) --------------------- */
+typedef struct {
+  int visits_per_item;   /* gridding: tile visits per work item (chunking)      */
+  int grid_coils;        /* gridding: coils per register-accumulator pass       */
+  int interp_coils;      /* gather: coils per LDS-staged pass                   */
+  int from_cache;        /* 1: taken from an earlier plan of the same bucket     */
+  double grid_ms, interp_ms;                  /* chosen configuration             */
+  double default_grid_ms, default_interp_ms;  /* the plan's defaults              */
+} cgs_tune_report;
+/* Times the gridding scatter and the window gather of this plan's trajectory
+ * over a small grid of launch shapes (`reps` timed runs each) and keeps the
+ * fastest.  Results are cached per process under (d, n, C, frames, samples
+ * per frame rounded to a power of two); with use_cache != 0 a cached bucket is
+ * applied without timing.  Needs the trajectory; clobbers the plan's
+ * intermediate buffers (not the trajectory, weights, maps or Toeplitz data). */
+int cgs_autotune(cgs_plan plan, int reps, int use_cache, cgs_tune_report* report);
+
 /* --- profiling (CGS_PROFILE plans) ------------------------------------------ */
 /* kernel ids for cgs_kernel_stats */
 enum {
@@ -151,12 +173,19 @@ enum {
   CGS_K_COIL = 4,      /* conj(S) coil combine                              */
   CGS_K_CG = 5,        /* CG / CGNR vector and scalar kernels              */
   CGS_K_OTHER = 6,     /* layout conversions                               */
-  CGS_K_COUNT = 7
+  CGS_K_TOEP_PAD = 7,  /* Toeplitz: x.*S_c, zero-pad to 2N, FFT along axis 0  */
+  CGS_K_TOEP_CONV = 8, /* Toeplitz: FFT, .* kernel spectrum, IFFT along the
+                          last axis (one fused pass)                        */
+  CGS_K_TOEP_CROP = 9, /* Toeplitz: IFFT along axis 0 (and 1 in 3D), crop    */
+  CGS_K_COUNT = 10
 };
 /* Accumulated launches, total milliseconds and algorithmic bytes per kernel
  * class since the last reset (reset = 1 clears after reading). */
 int cgs_kernel_stats(cgs_plan plan, int kernel, int64_t* launches, double* ms,
                      double* bytes, int reset);
+/* Time only the kernel classes whose bit (1 << id) is set (default: all);
+ * the others are still counted.  Fewer events, less perturbation. */
+int cgs_profile_mask(cgs_plan plan, unsigned mask);
 /* Sizes the plan resolved: n[3] oversampled sides, tile side, sorted sample
  * count, gridding work items. */
 int cgs_plan_info(cgs_plan plan, int* n, int* tile, int64_t* visits, int* grid_items);

@@ -1,4 +1,4 @@
-"""The C ABI library loads, exports every entry point include/cgsense.h declares,
+"""The C ABI library loads, exports every entry point include/*.h declares,
 and validates arguments on the host before touching the GPU."""
 import ctypes as ct
 import os
@@ -10,10 +10,13 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    src = open(os.path.join(ROOT, "include", "cgsense.h")).read()
+def declared_functions(header="cgsense.h"):
+    src = open(os.path.join(ROOT, "include", header)).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(cgs_[A-Za-z_0-9]+)\s*\(", src)))
+
+
+HEADERS = sorted(h for h in os.listdir(os.path.join(ROOT, "include")) if h.endswith(".h"))
 
 
 def test_header_declares_expected():
@@ -23,15 +26,33 @@ def test_header_declares_expected():
         assert n in names
 
 
-def test_library_exports_all_symbols(cgs):
+@pytest.mark.parametrize("header", HEADERS)
+def test_library_exports_all_symbols(cgs, header):
     L = cgs.load_library()
-    for name in declared_functions():
+    names = declared_functions(header)
+    assert names
+    for name in names:
         assert hasattr(L, name), name
 
 
 def test_python_mirror_lists_all_symbols():
     from codeare_amd.cgsense import EXPORTED
-    assert sorted(EXPORTED) == declared_functions()
+    from codeare_amd.warmstart import UNET_EXPORTED
+    assert sorted(EXPORTED) == declared_functions("cgsense.h")
+    assert sorted(UNET_EXPORTED) == declared_functions("cgs_unet.h")
+
+
+def test_unet_argument_validation_without_gpu(cgs):
+    from codeare_amd.warmstart import _lib
+    L = _lib()
+    h = ct.c_void_p()
+    # 4 levels need sides that are multiples of 128
+    assert L.cgs_unet_create(ct.byref(h), 96, 96, 4, 32, 2, 1) == 5
+    # only 32 base channels are built
+    assert L.cgs_unet_create(ct.byref(h), 128, 128, 4, 16, 2, 1) == 5
+    assert L.cgs_unet_create(ct.byref(h), 128, 128, 1, 32, 2, 1) == 5
+    assert L.cgs_unet_create(None, 128, 128, 4, 32, 2, 1) == 1
+    assert L.cgs_unet_destroy(None) == 0
 
 
 def test_argument_validation_without_gpu(cgs):

@@ -1,0 +1,87 @@
+"""Error of the GPU path against the FP64 direct-NUDFT oracle (oracle/nudft_torch.py)
+at the full 256^2 x 32-coil golden-angle size.  The torch oracle is itself
+pinned to the dense NDFT matrices of oracle/ndft.py on small problems (CPU)."""
+import numpy as np
+import pytest
+
+from helpers import rel
+
+
+def _small(seed=3, N=16, M=300, C=3):
+    from helpers import rand_img, rand_traj
+    rng = np.random.default_rng(seed)
+    k = rand_traj(rng, 2, M)
+    w = rng.uniform(0.1, 1.0, M)
+    S = rand_img(rng, (N, N, C)).astype(np.complex128)
+    x = rand_img(rng, (N, N)).astype(np.complex128)
+    y = rand_img(rng, (M, C)).astype(np.complex128)
+    return k, w, S, x, y
+
+
+def test_torch_nudft_matches_dense_ndft():
+    torch = pytest.importorskip("torch")
+    from oracle import ndft, nudft_torch as T
+    k, w, S, x, y = _small()
+    N = S.shape[0]
+    E = ndft.dense_E(k, (N, N), S)                       # rows (j fastest, c slowest)
+    W = np.tile(w, S.shape[-1])
+    b_dense = (E.conj().T @ (W * y.reshape(-1, order="F"))).reshape((N, N), order="F")
+    St = torch.from_numpy(S)
+    b = T.eh(k, w, y, St).numpy()
+    assert rel(b, b_dense) <= 1e-12
+    q_dense = (E.conj().T @ (W * (E @ x.reshape(-1, order="F")))).reshape((N, N), order="F") + 0.3 * x
+    Kh = T.kernel_spectrum(k, w, N, N, St.device)
+    q = T.normal(torch.from_numpy(x), St, Kh, 0.3).numpy()
+    assert rel(q, q_dense) <= 1e-12
+    # CG: equals the dense normal-equation CG iterate for iterate
+    AHA = E.conj().T @ (W[:, None] * E)
+    for it in (1, 4, 9):
+        xd, _, _ = ndft.cg_normal_dense(AHA, b_dense.reshape(-1, order="F"), 0.3, it, 0.0)
+        xt, n, _ = T.cg(St, Kh, torch.from_numpy(b_dense), 0.3, maxit=it, tol=0.0)
+        assert n == it
+        assert rel(xt.numpy(), xd.reshape((N, N), order="F")) <= 1e-10
+
+
+@pytest.mark.gpu
+def test_c2_against_fp64_nudft():
+    """256^2, 32 coils, 402 x 512 golden-angle, sigma 2, m 3, lambda 1e-3:
+    E^H W y, E^H W E x and CG iterates (equal iteration counts) of the GPU path
+    against the FP64 direct NUDFT."""
+    import torch
+    from codeare_amd import cgsense, data
+    from oracle import nudft_torch as T
+    prob = data.make_problem(2, 0)
+    wl = prob.wl
+    M, C, N = prob.k.shape[1], wl.coils, wl.N
+    P = cgsense.Plan((N, N), M, wl.sigma, wl.m, ncoils=C)
+    P.kspace(prob.k)
+    P.weights(prob.w)
+    P.sensitivities(prob.maps)
+    y = data.make_data(prob, P.E)
+    dev = torch.device("cuda:0")
+    S = torch.from_numpy(np.ascontiguousarray(prob.maps.astype(np.complex128))).to(dev)
+    b = T.eh(prob.k, prob.w, y, S)
+    e_b = rel(P.EH(y), b.cpu().numpy())
+    Kh = T.kernel_spectrum(prob.k, prob.w, N, N, dev)
+    x = torch.from_numpy(np.ascontiguousarray(prob.image.astype(np.complex128))).to(dev)
+    q = T.normal(x, S, Kh, wl.lam).cpu().numpy()
+    e_q = rel(P.normal(prob.image, lam=wl.lam, toeplitz=True), q)
+    e_qn = rel(P.normal(prob.image, lam=wl.lam), q)
+    print(f"C2 vs FP64 NUDFT: E^H W y {e_b:.2e}, normal (Toeplitz) {e_q:.2e}, normal (NFFT) {e_qn:.2e}")
+    assert e_b <= 2e-5 and e_q <= 2e-5 and e_qn <= 2e-5
+    for it in (10, 50):
+        xg, rep = P.solve(y, maxit=it, tol=0.0, lam=wl.lam, toeplitz=True)
+        xo, n, _ = T.cg(S, Kh, b, wl.lam, maxit=it, tol=0.0)
+        e = rel(xg, xo.cpu().numpy())
+        print(f"  CG({it}) image vs FP64 NUDFT CG({n}): {e:.2e}")
+        assert rep["iters"][0] == n == it
+        assert e <= 1e-4
+    # to the stated tolerance: both converge; report the difference
+    xg, rep = P.solve(y, maxit=3000, tol=1e-5, lam=wl.lam, toeplitz=True, graph_iters=8)
+    xo, n, _ = T.cg(S, Kh, b, wl.lam, maxit=3000, tol=1e-5)
+    e = rel(xg, xo.cpu().numpy())
+    ep = rel(xg, prob.image)
+    eo = rel(xo.cpu().numpy(), prob.image)
+    print(f"  to 1e-5: GPU {rep['iters'][0]} its, FP64 NUDFT {n} its; image difference {e:.2e}; "
+          f"vs phantom {ep:.3e} (GPU) {eo:.3e} (FP64)")
+    assert e <= 1e-2

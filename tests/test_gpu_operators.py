@@ -173,6 +173,55 @@ def test_frames_match_single_plans(cgs, ora):
         assert rel(xe[..., f], P1.EH(yf)) <= 1e-5
 
 
+def _frames_problem(cgs, N=48, C=4, R=96, counts=(7, 13, 4, 21), seed=6):
+    from codeare_amd import data
+    rng = np.random.default_rng(seed)
+    ks, first = [], 0
+    for c in counts:
+        ks.append(data.radial_2d(c, R, True, first))
+        first += c
+    k = np.asfortranarray(np.concatenate(ks, axis=1))
+    off = np.concatenate([[0], np.cumsum(np.array(counts) * R)]).astype(np.int64)
+    w = data.density(k, "k", R)
+    S = data.coil_maps(N, 2, C, rng)
+    x = rand_img(rng, (N, N, len(counts)))
+    P = cgs.Plan((N, N), k.shape[1], 2.0, 3, ncoils=C, frame_offsets=off)
+    P.kspace(k)
+    P.weights(w)
+    P.sensitivities(S)
+    return P, k, w, S, x, off
+
+
+def test_frames_toeplitz_and_per_frame_cg(cgs):
+    """Batched frames: the Toeplitz normal operator per frame, and a joint CG
+    whose frames stop at their own iteration, each equal to its single-plan
+    solve (iteration count and image)."""
+    P, k, w, S, x, off = _frames_problem(cgs)
+    C = S.shape[-1]
+    qt = P.normal(x, lam=0.1, toeplitz=True)
+    qn = P.normal(x, lam=0.1)
+    e = rel(qt, qn)
+    print(f"frames: toeplitz vs nufft normal {e:.2e}")
+    assert e <= 2e-5
+    y = P.E(x)
+    for toep in (False, True):
+        xb, rb = P.solve(y, maxit=300, tol=1e-4, lam=0.05, toeplitz=toep, graph_iters=4 if toep else 0)
+        for f in range(len(off) - 1):
+            Mf = int(off[f + 1] - off[f])
+            P1 = cgs.Plan(P.N, Mf, 2.0, 3, ncoils=C)
+            P1.kspace(k[:, off[f]:off[f + 1]])
+            P1.weights(w[off[f]:off[f + 1]])
+            P1.sensitivities(S)
+            yf = y[C * off[f]: C * off[f + 1]].reshape((Mf, C), order="F")
+            x1, r1 = P1.solve(yf, maxit=300, tol=1e-4, lam=0.05, toeplitz=toep)
+            ef = rel(xb[..., f], x1)
+            print(f"  toeplitz={toep} frame {f}: iters {rb['iters'][f]} vs {r1['iters'][0]}, image {ef:.2e}")
+            # same kernels and per-frame scalars: equal counts (a 1-iteration
+            # flip at the tolerance boundary is allowed, with a looser image bound)
+            assert abs(rb["iters"][f] - r1["iters"][0]) <= 1
+            assert ef <= (1e-5 if rb["iters"][f] == r1["iters"][0] else 1e-3)
+
+
 def test_adjoint_identity_c2_shape(cgs):
     """<E x, W y> == <x, E^H W y> at the full 256^2 x 32-coil golden-angle size."""
     from codeare_amd import data
@@ -261,3 +310,28 @@ def test_graph_solve_matches_launch_solve(cgs, ora, toeplitz):
     xd, rd = P.solve(y, maxit=200, tol=1e-3, toeplitz=toeplitz)
     assert rc["iters"][0] == rd["iters"][0]
     assert rel(xc, xd) <= 1e-6
+
+
+def test_autotune_keeps_results(cgs, ora):
+    """The autotuner changes launch shapes only: E^H and a short CG give the
+    same results before and after it; a second plan of the same bucket takes
+    the cached choice."""
+    prob, y, P, O = _c1_problem(cgs, ora, N=128, coils=16, spokes=96, samples=256)
+    x1 = P.EH(y)
+    s1, r1 = P.solve(y, maxit=6, tol=0.0, toeplitz=True)
+    rep = P.autotune(reps=3)
+    print(f"autotune: {rep}")
+    assert rep["visits_per_item"] in (256, 512, 1024) and rep["grid_coils"] in (8, 16)
+    assert rep["grid_ms"] <= rep["default_grid_ms"] + 1e-9
+    x2 = P.EH(y)
+    s2, r2 = P.solve(y, maxit=6, tol=0.0, toeplitz=True)
+    assert rel(x2, x1) <= 1e-6 and rel(s2, s1) <= 1e-5
+    assert rel(x2, O.EH_true(prob.maps, prob.w, y)) <= OP_TOL
+    wl = prob.wl
+    Q = cgs.Plan((wl.N,) * wl.d, prob.k.shape[1], wl.sigma, wl.m, ncoils=wl.coils, autotune=True)
+    Q.kspace(prob.k)
+    Q.weights(prob.w)
+    Q.sensitivities(prob.maps)
+    rq = Q.autotune(use_cache=True)
+    assert rq["from_cache"] == 1 and rq["visits_per_item"] == rep["visits_per_item"]
+    assert rel(Q.EH(y), x1) <= 1e-6

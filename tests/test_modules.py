@@ -1,0 +1,98 @@
+"""codeare's .cod files and the CGSENSE / NuFFT module harnesses
+(src/modules/CGSENSE.cpp, src/modules/NuFFT.cpp) over the GPU operators."""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from codeare_amd import codfile
+from helpers import rel
+
+
+def test_cod_round_trip(tmp_path):
+    rng = np.random.default_rng(1)
+    for a in [rng.normal(size=(3, 4)).astype(np.float32),
+              (rng.normal(size=(5, 2, 3)) + 1j * rng.normal(size=(5, 2, 3))).astype(np.complex64),
+              rng.normal(size=7), np.arange(6, dtype=np.int16).reshape(2, 3)]:
+        p = str(tmp_path / "m.cod")
+        codfile.write_cod(p, a, "signals", res=[0.5] * a.ndim)
+        b, name, res = codfile.read_cod(p)
+        assert name == "signals" and b.dtype == a.dtype and b.shape == a.shape
+        assert np.array_equal(a, b) and res == (0.5,) * a.ndim
+
+
+def test_cod_byte_layout(tmp_path):
+    """A file assembled field by field as CODFile::Write emits it (CODFile.hpp:150-186)."""
+    a = np.asfortranarray(np.array([[1 + 2j, 3 + 4j, 5 + 6j], [7 + 8j, 9 + 10j, 11 + 12j]], np.complex64))
+    raw = struct.pack("<i", 2) + struct.pack("<Q", 2) + struct.pack("<2Q", 2, 3) + struct.pack("<2f", 1, 1)
+    raw += struct.pack("<Q", 6) + b"kspace"
+    raw += np.array([1 + 2j, 7 + 8j, 3 + 4j, 9 + 10j, 5 + 6j, 11 + 12j], np.complex64).tobytes()
+    raw += b"543f562189f1e82beb9c177f89f67822"
+    p = str(tmp_path / "k.cod")
+    with open(p, "wb") as f:
+        f.write(raw)
+    b, name, _ = codfile.read_cod(p)
+    assert name == "kspace" and np.array_equal(a, b)
+    q = str(tmp_path / "w.cod")
+    codfile.write_cod(q, a, "kspace")
+    assert open(q, "rb").read() == raw
+
+
+def test_cod_rejects_truncated(tmp_path):
+    p = str(tmp_path / "t.cod")
+    codfile.write_cod(p, np.ones((4, 4), np.float32), "w")
+    data = open(p, "rb").read()
+    with open(p, "wb") as f:
+        f.write(data[:-40])
+    with pytest.raises(ValueError):
+        codfile.read_cod(p)
+
+
+@pytest.mark.gpu
+def test_cgsense_module_matches_oracle(ora, tmp_path):
+    """The CGSENSE module from .cod files: NCSENSE with sigma 1 (no alpha
+    passed), fteps 1e-3, the reference E^H and CGLS -- against the oracle's
+    CGLS with the same settings."""
+    from codeare_amd import data, modules
+    prob = data.make_problem(1, 0, N=64, coils=4, spokes=48, samples=128)
+    N, C, M = 64, 4, prob.k.shape[1]
+    O = ora.NFFT((N, N), M, 1.0, 3)
+    O.kspace(prob.k)
+    y = data.make_data(prob, lambda x: O.E(prob.maps, x))
+    files = {}
+    for name, arr in [("sensitivities", prob.maps), ("kspace", prob.k), ("weights", prob.w), ("signals", y)]:
+        files[name] = str(tmp_path / f"{name}.cod")
+        codfile.write_cod(files[name], arr, name)
+    out = str(tmp_path / "image.cod")
+    rc = modules.main(["CGSENSE", f"nk={M}", "m=3", "lambda=5e-2", "cgeps=1e-7", "cgmaxit=6", "ftmaxit=3",
+                       *[f"{k}={v}" for k, v in files.items()], f"image={out}"])
+    assert rc == 0
+    img, name, _ = codfile.read_cod(out)
+    xo, it, _ = O.cgls_ref(prob.maps, prob.w, y, mode=1, ftiter=3, fteps=1e-3, maxit=6, cgeps=1e-7, lam=5e-2)
+    e = rel(img, xo)
+    print(f"CGSENSE module (sigma 1): image vs oracle {e:.2e}, oracle iterations {it}")
+    assert name == "image" and img.shape == (N, N)
+    assert e <= 1e-4
+
+
+@pytest.mark.gpu
+def test_nufft_module_matches_oracle(ora):
+    from codeare_amd import modules
+    from helpers import rand_traj, rand_img
+    rng = np.random.default_rng(8)
+    N, M = 32, 2000
+    k = rand_traj(rng, 2, M)
+    w = rng.uniform(0.2, 1.0, M).astype(np.float32)
+    f = rand_img(rng, (M,))
+    s = modules.NuFFT()
+    s.init(dim=2, Nx=N, Ny=N, M=M, shots=1, maxit=50, epsilon=1e-9, m=3, alpha=2.0)
+    ws = {"kspace": k, "weights": w, "data": f}
+    s.prepare(ws)
+    img = s.process(ws)
+    O = ora.NFFT((N, N), M, 2.0, 3)
+    O.kspace(k)
+    io, it, _ = O.itrafo(f, w, maxit=3, eps=7e-4)   # NFFT defaults, not the attributes
+    e = rel(img, io)
+    print(f"NuFFT module: {e:.2e} (oracle ITrafo steps {it})")
+    assert e <= 1e-5

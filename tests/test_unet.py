@@ -1,0 +1,165 @@
+"""Warm-start U-Net: the CPU restatement (oracle/unet_ref.py) pinned by finite
+differences, and the BF16-MFMA HIP kernels checked against it."""
+import numpy as np
+import pytest
+
+from oracle import unet_ref as U
+
+
+def _rand_batch(rng, B, H, W, scale=1.0):
+    return (scale * (rng.normal(size=(B, H, W)) + 1j * rng.normal(size=(B, H, W)))).astype(np.complex64)
+
+
+# ---------------------------------------------------------------------------
+# CPU: the oracle itself
+def test_bf16_round_known_values():
+    a = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, -2.5, 1e-3, 0.0, 1.0 + 2 ** -7 + 2 ** -8])
+    r = U.bf16_round(a)
+    assert r[0] == 1.0
+    assert r[1] == 1.0                      # tie -> even
+    assert r[2] == 1.0 + 2 ** -7            # nearest
+    assert r[3] == -2.5
+    assert abs(r[4] - 1e-3) <= 1e-3 * 2 ** -8
+    assert r[5] == 0.0
+    assert r[6] == 1.0 + 2 ** -6            # tie -> even (upward)
+
+
+def test_param_count_matches_layer_table():
+    # 4 levels, 32 base channels: the 1.95 M parameter warm-start network
+    n = U.num_params(4)
+    assert n == 1950658
+    assert U.num_params(2) == sum(9 * co * (ca + cb) + co for ca, cb, co in U.layer_table(2)) + 66
+
+
+def test_conv_dgrad_is_adjoint_of_conv():
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(2, 8, 8, 5))
+    W = rng.normal(size=(9, 4, 5))
+    G = rng.normal(size=(2, 8, 8, 4))
+    lhs = np.sum(U.conv3x3(X, W) * G)
+    rhs = np.sum(X * U.conv3x3_dgrad(G, W))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    # weight gradient: <conv(X, dW), G> = <dW, wgrad(X, G)>
+    dW = rng.normal(size=W.shape)
+    assert abs(np.sum(U.conv3x3(X, dW) * G) - np.sum(dW * U.conv3x3_wgrad(X, G))) <= 1e-9 * abs(lhs)
+
+
+def test_conv_matches_direct_sum():
+    rng = np.random.default_rng(4)
+    X = rng.normal(size=(1, 5, 6, 3))
+    W = rng.normal(size=(9, 2, 3))
+    out = U.conv3x3(X, W)
+    y, x, co = 2, 0, 1
+    s = 0.0
+    for ky in range(3):
+        for kx in range(3):
+            yy, xx = y + ky - 1, x + kx - 1
+            if 0 <= yy < 5 and 0 <= xx < 6:
+                s += X[0, yy, xx].dot(W[3 * ky + kx, co])
+    assert abs(out[0, y, x, co] - s) <= 1e-12
+
+
+def test_oracle_backward_matches_finite_differences():
+    """FP64 mode: analytic gradient of the loss vs central differences."""
+    rng = np.random.default_rng(5)
+    L = 2
+    theta = U.init_params(L, 11).astype(np.float64)
+    x = _rand_batch(rng, 2, 32, 32).astype(np.complex128)
+    t = _rand_batch(rng, 2, 32, 32, 0.5).astype(np.complex128)
+    net = U.UNetRef(L, theta, bf16=False)
+    net.forward(x, t)
+    g = net.backward()
+    assert g.size == theta.size == U.num_params(L)
+    # indices spread over every layer (weights and biases) and the head
+    idx, off = [], 0
+    for ca, cb, co in U.layer_table(L):
+        nw = 9 * co * (ca + cb)
+        live_ci = 2 if ca == 16 else ca + cb
+        for _ in range(3):
+            tap, o, ci = rng.integers(9), rng.integers(co), rng.integers(live_ci)
+            idx.append(off + (tap * co + o) * (ca + cb) + ci)
+        idx.append(off + nw + int(rng.integers(co)))
+        off += nw + co
+    idx += [off + 3, off + 40, off + 64, off + 65]
+    h = 1e-6
+    worst = 0.0
+    for i in idx:
+        tp = theta.copy(); tp[i] += h
+        tm = theta.copy(); tm[i] -= h
+        lp = U.UNetRef(L, tp, bf16=False).forward(x, t)[1]
+        lm = U.UNetRef(L, tm, bf16=False).forward(x, t)[1]
+        fd = (lp - lm) / (2 * h)
+        err = abs(fd - g[i]) / max(abs(g[i]), 1e-4 * np.max(np.abs(g)))
+        worst = max(worst, err)
+    assert worst <= 1e-4, worst
+
+
+# ---------------------------------------------------------------------------
+# GPU: HIP kernels vs the oracle (BF16 storage points reproduced)
+def _net(levels, H, W, B, seed=7):
+    from codeare_amd.warmstart import UNet
+    net = UNet(H, W, levels=levels, max_batch=B, seed=seed)
+    theta = U.init_params(levels, seed)
+    net.set_params(theta)
+    return net, theta
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("levels,H,W,B", [(2, 32, 32, 3), (4, 128, 128, 2), (3, 64, 128, 1)])
+def test_unet_forward_matches_oracle(levels, H, W, B):
+    rng = np.random.default_rng(20 + levels)
+    net, theta = _net(levels, H, W, B)
+    x = _rand_batch(rng, B, H, W)
+    y = net.forward(x)
+    yo, _ = U.UNetRef(levels, theta, bf16=True).forward(x.astype(np.complex128))
+    # the network part alone (output minus the residual input)
+    e = np.linalg.norm((y - x) - (yo - x)) / np.linalg.norm(yo - x)
+    print(f"U-Net L={levels} {H}x{W} B={B}: forward rel err {e:.2e}")
+    assert e <= 3e-3
+
+
+@pytest.mark.gpu
+def test_unet_gradients_match_oracle():
+    rng = np.random.default_rng(31)
+    L, H, W, B = 2, 32, 32, 2
+    net, theta = _net(L, H, W, B)
+    x = _rand_batch(rng, B, H, W)
+    t = _rand_batch(rng, B, H, W, 0.5)
+    loss = net.gradients(x, t)
+    g = net.grads()
+    ref = U.UNetRef(L, theta, bf16=True)
+    _, lo = ref.forward(x.astype(np.complex128), t.astype(np.complex128))
+    go = ref.backward()
+    assert abs(loss - lo) <= 1e-3 * lo
+    off = 0
+    for k, (ca, cb, co) in enumerate(U.layer_table(L)):
+        nw = 9 * co * (ca + cb)
+        for name, sl in (("W", slice(off, off + nw)), ("b", slice(off + nw, off + nw + co))):
+            e = np.linalg.norm(g[sl] - go[sl]) / max(np.linalg.norm(go[sl]), 1e-30)
+            print(f"layer {k} {name}: rel err {e:.2e}")
+            assert e <= 2e-2
+        off += nw + co
+    e = np.linalg.norm(g[off:] - go[off:]) / np.linalg.norm(go[off:])
+    print(f"head: rel err {e:.2e}")
+    assert e <= 1e-3
+
+
+@pytest.mark.gpu
+def test_unet_training_step_deterministic_and_learns():
+    rng = np.random.default_rng(41)
+    L, H, W, B = 2, 32, 32, 4
+    x = _rand_batch(rng, B, H, W)
+    # target: a smooth (3x3 box-blurred) version of x -- learnable by the net
+    xr = np.pad(x, ((0, 0), (1, 1), (1, 1)))
+    t = sum(xr[:, i:i + H, j:j + W] for i in range(3) for j in range(3)) / 9.0
+    t = t.astype(np.complex64)
+    losses = {}
+    finals = []
+    for rep in range(2):
+        net, _ = _net(L, H, W, B, seed=9)
+        ls = [net.train_step(x, t, lr=1e-3) for _ in range(60)]
+        losses[rep] = ls
+        finals.append(net.params())
+    print(f"loss {losses[0][0]:.4e} -> {losses[0][-1]:.4e}")
+    assert np.array_equal(finals[0], finals[1])          # fixed-order reductions
+    assert losses[0][-1] <= 0.3 * losses[0][0]
