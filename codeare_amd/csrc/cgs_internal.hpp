@@ -210,7 +210,7 @@ struct Plan {
 void launch_fft(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s);
 void launch_interp(const SampArgs& a, int cg, hipStream_t s);
 void launch_grid(const GridArgs& a, int cg, int waves, int nitems, int ncgroups, hipStream_t s);
-void launch_grid_reduce(const Plan& p, const float2* partial, float2* grid, hipStream_t s);
+void launch_grid_reduce(const Plan& p, const float2* partial, float2* grid, const int* fl, hipStream_t s);
 void launch_grid_batched(const GridArgs& a, int cg, int waves, int nitems, int ncgroups, hipStream_t s);
 void launch_interp_lds(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int cg, int maxs,
                        int ncgroups, hipStream_t s);
@@ -218,8 +218,10 @@ size_t grid_batched_lds(int d, int T, int taps, int CG, int rowstride);
 void launch_grid_atomic(const GridArgs& a, const int4* vrec, int cg, int nitems, int ncgroups, hipStream_t s);
 void launch_grid_reg(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s);
 size_t interp_lds_bytes(int T, int taps, int CG, int rowstride, int maxs);
+void launch_partial_combine(const Plan& p, const float2* part, int G, float2* q, const float2* pvec, float lambda,
+                            double* partials, const int* fl, hipStream_t s);
 void launch_coil_combine(const Plan& p, const float2* Xc, float2* x, const float2* pvec,
-                         float lambda, bool use_imap, double* partials, hipStream_t s);
+                         float lambda, bool use_imap, double* partials, const int* fl, hipStream_t s);
 // vector / scalar kernels
 void launch_sqnorm_partials(const Plan& p, const float2* v, double* partials, hipStream_t s);
 void launch_cg_init(const Plan& p, const double* part_bn, const double* part_rn, int tolkind,

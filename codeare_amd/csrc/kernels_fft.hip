@@ -168,7 +168,10 @@ __device__ __forceinline__ bool inside_mask(const FftArgs& a, int i0, int64_t re
 constexpr int kUnroll = 8;   // global loads in flight per thread in the staging loops
 
 // LD: 0 plain, 1 E (x .* S_c .* D), 2 plain .* D, 3 plain .* D .* w_hat, 4 x .* S_c
-// ST: 0 plain, 1 .* D
+// ST: 0 plain, 1 .* D, 2 crop + coil combine: the block's lb lines are lb coils
+//     of one image row (axis-0 pass, inner = C); it writes
+//     sum_{c in block} conj(S_c) X_c to partial image g = q0 / lb:
+//     out[(f*G + g)*Ntot + i], G = ceil(C / lb)
 // CONV: forward FFT, multiply by the Toeplitz kernel spectrum, inverse FFT
 template <int LD, int ST, int MAXN, int CONV>
 __global__ void __launch_bounds__(kFftThreads) k_fft(FftArgs a) {
@@ -268,6 +271,38 @@ __global__ void __launch_bounds__(kFftThreads) k_fft(FftArgs a) {
   // ---- store (with crop for adjoint passes) ----
   const bool crop = a.crop_out != 0;
   const int totout = lb * a.nout;
+  if (ST == 2) {
+    // every thread runs every round (the shuffles need the whole wave); a
+    // group of lb adjacent lanes holds the lb coils of one output pixel
+    const int64_t g = q0 >> lbl;
+    float2* outp = a.out + (f * nib + g) * a.Ntot + (int64_t)a.N0 * rest;
+    for (int b0 = 0; b0 < totout; b0 += kFftThreads * kUnroll) {
+      float2 v[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int idx = b0 + u * kFftThreads + tid;
+        const int q = idx & (lb - 1), p = idx >> lbl;
+        v[u] = make_float2(0.f, 0.f);
+        if (idx < totout && q < nq) {
+          int pos = p - a.nout / 2;
+          if (pos < 0) pos += n;
+          const float2 xv = lds[q * a.ldn + pidx(pos)];
+          const float2 sv = a.S[(p + (int64_t)a.N0 * rest) * a.C + q0 + q];
+          v[u] = make_float2(sv.x * xv.x + sv.y * xv.y, sv.x * xv.y - sv.y * xv.x);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        for (int m = lb >> 1; m > 0; m >>= 1) {
+          v[u].x += __shfl_xor(v[u].x, m);
+          v[u].y += __shfl_xor(v[u].y, m);
+        }
+        const int idx = b0 + u * kFftThreads + tid;
+        if (idx < totout && (idx & (lb - 1)) == 0) outp[idx >> lbl] = v[u];
+      }
+    }
+    return;
+  }
   for (int b0 = tid; b0 < totout; b0 += kFftThreads * kUnroll) {
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
@@ -314,6 +349,7 @@ void launch_fft(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s) {
   CGS_FFT_CASE(4, 0, 0)
   CGS_FFT_CASE(0, 1, 0)
   CGS_FFT_CASE(0, 0, 1)
+  CGS_FFT_CASE(0, 2, 0)
 #undef CGS_FFT_CASE
 }
 

@@ -807,12 +807,12 @@ void launch_grid(const GridArgs& a, int cg, int waves, int nitems, int ncgroups,
   }
 }
 
-void launch_grid_reduce(const Plan& p, const float2* partial, float2* grid, hipStream_t s) {
+void launch_grid_reduce(const Plan& p, const float2* partial, float2* grid, const int* fl, hipStream_t s) {
   if (p.n_reduce_items == 0) return;
   const int64_t tilecells = (int64_t)p.T * p.T * (p.d == 3 ? p.T : 1);
   hipLaunchKernelGGL(k_grid_reduce, dim3((unsigned)p.n_reduce_items), dim3(256), 0, s,
                      p.reduce_items, partial, grid, p.C, p.T, p.d, p.n[0], p.n[1], p.n[2],
-                     p.nt[0], p.nt[1], p.ntot, tilecells, p.fdone);
+                     p.nt[0], p.nt[1], p.ntot, tilecells, fl);
 }
 
 }  // namespace cgs

@@ -74,18 +74,57 @@ __global__ void __launch_bounds__(256) k_coil_combine(const float2* X, const flo
 }
 
 void launch_coil_combine(const Plan& p, const float2* Xc, float2* x, const float2* pvec,
-                         float lambda, bool use_imap, double* partials, hipStream_t s) {
+                         float lambda, bool use_imap, double* partials, const int* fl, hipStream_t s) {
   // kCoilBlocks partials per frame (read by k_cg_alpha and k_cg_xr_alpha)
   const dim3 g(kCoilBlocks, p.F);
   const float* I = use_imap ? p.Imap : nullptr;
   if (p.C <= 8)
-    hipLaunchKernelGGL(k_coil_combine<8>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, p.fdone);
+    hipLaunchKernelGGL(k_coil_combine<8>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, fl);
   else if (p.C <= 16)
-    hipLaunchKernelGGL(k_coil_combine<16>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, p.fdone);
+    hipLaunchKernelGGL(k_coil_combine<16>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, fl);
   else if (p.C <= 32)
-    hipLaunchKernelGGL(k_coil_combine<32>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, p.fdone);
+    hipLaunchKernelGGL(k_coil_combine<32>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, fl);
   else
-    hipLaunchKernelGGL(k_coil_combine<64>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, p.fdone);
+    hipLaunchKernelGGL(k_coil_combine<64>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, fl);
+}
+
+// q = sum_g part_g + lambda p, with <p, q> partials (kCoilBlocks per frame,
+// the layout k_coil_combine writes); part[(f*G + g)*Ntot + i] are the partial
+// coil sums of the fused crop pass
+__global__ void __launch_bounds__(256) k_partial_combine(const float2* __restrict__ part, int G, int64_t Ntot,
+                                                         float2* __restrict__ q, const float2* __restrict__ pv,
+                                                         float lambda, double* __restrict__ partials,
+                                                         const int* __restrict__ fdone) {
+  __shared__ double sh[4];
+  const int f = blockIdx.y;
+  if (fdone && fdone[f]) return;
+  double dot = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < Ntot; i += (int64_t)gridDim.x * 256) {
+    float2 acc = make_float2(0.f, 0.f);
+    for (int g = 0; g < G; ++g) {
+      const float2 v = part[((int64_t)f * G + g) * Ntot + i];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    const int64_t o = (int64_t)f * Ntot + i;
+    if (pv) {
+      const float2 p = pv[o];
+      acc.x += lambda * p.x;
+      acc.y += lambda * p.y;
+      dot += (double)p.x * acc.x + (double)p.y * acc.y;
+    }
+    q[o] = acc;
+  }
+  if (partials) {
+    const double t = block_sum(dot, sh);
+    if (threadIdx.x == 0) partials[(int64_t)f * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+void launch_partial_combine(const Plan& p, const float2* part, int G, float2* q, const float2* pvec, float lambda,
+                            double* partials, const int* fl, hipStream_t s) {
+  const dim3 g(kCoilBlocks, p.F);
+  hipLaunchKernelGGL(k_partial_combine, g, dim3(256), 0, s, part, G, p.Ntot, q, pvec, lambda, partials, fl);
 }
 
 // ---------------------------------------------------------------------------

@@ -388,7 +388,7 @@ static int gridding(Plan& p, const float2* y, bool weighted, const int* fl) {
     launch_grid_batched(a, p.grid_cg, p.grid_w, p.n_grid_items, ngroups, p.stream);
   else
     launch_grid(a, p.grid_cg, p.grid_w, p.n_grid_items, ngroups, p.stream);
-  launch_grid_reduce(p, p.partial, p.grid, p.stream);
+  launch_grid_reduce(p, p.partial, p.grid, fl, p.stream);
   return CGS_OK;
 }
 
@@ -410,7 +410,7 @@ static int op_EH_true(Plan& p, const float2* y, float2* x, const float2* pv, flo
   CHK(gridding(p, y, true, fl));
   CHK(fft_adjoint(p, p.Xc, fl));
   Prof pr(p, CGS_K_COIL, coil_bytes(p, pv != nullptr));
-  launch_coil_combine(p, p.Xc, x, pv, lambda, false, partials, p.stream);
+  launch_coil_combine(p, p.Xc, x, pv, lambda, false, partials, fl, p.stream);
   return CGS_OK;
 }
 
@@ -469,7 +469,7 @@ static int op_EH_ref(Plan& p, const float2* y, float2* x, int ftiter, double fte
                      float lambda, double* partials, const int* fl) {
   CHK(cgnr(p, y, ftiter, fteps, fl));
   Prof pr(p, CGS_K_COIL, coil_bytes(p, pv != nullptr));
-  launch_coil_combine(p, p.Xc2, x, pv, lambda, true, partials, p.stream);
+  launch_coil_combine(p, p.Xc2, x, pv, lambda, true, partials, fl, p.stream);
   return CGS_OK;
 }
 
@@ -986,12 +986,15 @@ static void toep_drop(Plan& p) {
 }
 
 // per-coil images X_c = crop(IFFT_2N(That .* FFT_2N(pad(S_c x)))) (Toeplitz embedding)
-static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd* pu) {
+static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd* pu, int* G_out) {
   const int d = p.d, C = p.C, F = p.F;
   const int64_t N0 = p.N[0], N1 = p.N[1], N2 = p.N[2];
   const int64_t M0 = 2 * N0, M1 = 2 * N1;
+  // the last pass (axis 0, inner = C) crops and sums conj(S_c) X_c over the
+  // block's coils into G partial images in p.Xc (store mode 2)
   auto pass = [&](int kind, const FftTab& tab, int sign, const float2* in, const float2* xin, float2* out, int nin,
-                  int nout, int64_t inner, int64_t opf, int ld, const float* That, int64_t cells, double bytes) {
+                  int nout, int64_t inner, int64_t opf, int ld, const float* That, int64_t cells, double bytes,
+                  int st = 0) {
     FftArgs a = fft_base(p, tab, sign);
     a.in = in; a.x = xin; a.out = out;
     a.nin = nin; a.nout = nout;
@@ -1000,16 +1003,21 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
     if (ld == 4) apply_pupd(p, a, pu);
     if (That) { a.That = That; a.conv_cells = cells; a.pad_in = 1; a.crop_out = 1; }
     fft_finish(a);
+    if (st == 2) {
+      const int G = (int)((inner + a.lb - 1) / a.lb);
+      *G_out = G;
+      bytes = (double)C * (nin * (double)(opf)) * 8 + (double)C * nout * (double)opf * 8 +
+              (double)G * nout * (double)opf * 8;
+    }
     Prof pr(p, kind, bytes * F);
-    launch_fft(a, ld, 0, p.stream);
+    launch_fft(a, ld, st, p.stream);
   };
   if (d == 2) {
     pass(CGS_K_TOEP_PAD, p.ft[0], -1, nullptr, x, p.tt1, (int)N0, (int)M0, C, N1, 4, nullptr, 0,
          ((double)(1 + C) * N0 * N1 + (double)C * M0 * N1) * 8);
     pass(CGS_K_TOEP_CONV, p.ft[1], -1, p.tt1, nullptr, p.tt1, (int)N1, (int)N1, C * M0, 1, 0, p.That, M0,
          2.0 * C * M0 * N1 * 8 + 4.0 * M0 * M1);
-    pass(CGS_K_TOEP_CROP, p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1, 0, nullptr, 0,
-         (double)C * (M0 + N0) * N1 * 8);
+    pass(CGS_K_TOEP_CROP, p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1, 0, nullptr, 0, 0.0, 2);
   } else {
     pass(CGS_K_TOEP_PAD, p.ft[0], -1, nullptr, x, p.tt1, (int)N0, (int)M0, C, N1 * N2, 4, nullptr, 0,
          ((double)(1 + C) * N0 * N1 * N2 + (double)C * M0 * N1 * N2) * 8);
@@ -1019,8 +1027,7 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
          2.0 * C * M0 * M1 * N2 * 8 + 4.0 * M0 * M1 * 2 * N2);
     pass(CGS_K_TOEP_CROP, p.ft[1], +1, p.tt2, nullptr, p.tt1, (int)M1, (int)N1, C * M0, N2, 0, nullptr, 0,
          (double)C * M0 * (M1 + N1) * N2 * 8);
-    pass(CGS_K_TOEP_CROP, p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1 * N2, 0, nullptr, 0,
-         (double)C * (M0 + N0) * N1 * N2 * 8);
+    pass(CGS_K_TOEP_CROP, p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1 * N2, 0, nullptr, 0, 0.0, 2);
   }
   return CGS_OK;
 }
@@ -1028,9 +1035,10 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
 // q = E^H W E x + lambda x with the Toeplitz embedding (and partial <x, q>)
 static int op_normal_toeplitz(Plan& p, const float2* x, float2* qo, float lambda, double* partials,
                               const int* fl) {
-  CHK(toeplitz_to_coils(p, x, fl, nullptr));
-  Prof pr(p, CGS_K_COIL, coil_bytes(p, true));
-  launch_coil_combine(p, p.Xc, qo, x, lambda, false, partials, p.stream);
+  int G = 0;
+  CHK(toeplitz_to_coils(p, x, fl, nullptr, &G));
+  Prof pr(p, CGS_K_COIL, ((double)G + 3.0) * p.Ntot * p.F * 8);
+  launch_partial_combine(p, p.Xc, G, qo, x, lambda, partials, fl, p.stream);
   return CGS_OK;
 }
 
@@ -1052,15 +1060,16 @@ static int cg_iteration_fused(Plan& p, int normal_op, float2* x, float2* r, cons
   pu.pold = pold;
   pu.pnew = pnew;
   if (normal_op == CGS_NORMAL_TOEPLITZ) {
-    CHK(toeplitz_to_coils(p, nullptr, p.fdone, &pu));
+    int G = 0;
+    CHK(toeplitz_to_coils(p, nullptr, p.fdone, &pu, &G));
+    Prof pr(p, CGS_K_COIL, ((double)G + 3.0) * p.Ntot * p.F * 8);
+    launch_partial_combine(p, p.Xc, G, q, pnew, (float)o.lambda, p.partials, p.fdone, p.stream);
   } else {
     CHK(op_E(p, nullptr, p.ybuf[1], p.fdone, &pu));
     CHK(gridding(p, p.ybuf[1], true, p.fdone));
     CHK(fft_adjoint(p, p.Xc, p.fdone));
-  }
-  {
     Prof pr(p, CGS_K_COIL, coil_bytes(p, true));
-    launch_coil_combine(p, p.Xc, q, pnew, (float)o.lambda, false, p.partials, p.stream);
+    launch_coil_combine(p, p.Xc, q, pnew, (float)o.lambda, false, p.partials, p.fdone, p.stream);
   }
   {
     Prof pr(p, CGS_K_CG, 5.0 * 8.0 * p.F * p.Ntot);
@@ -1182,6 +1191,7 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
   }
   HIPCHK(hipMemcpyAsync(xout, x, ib, hipMemcpyDeviceToDevice, p.stream));
   HIPCHK(hipMemcpyAsync(p.h_cgst, p.cgst, sizeof(CgState) * F, hipMemcpyDeviceToHost, p.stream));
+  HIPCHK(hipMemsetAsync(p.fdone, 0, sizeof(int) * F, p.stream));
   HIPCHK(hipStreamSynchronize(p.stream));
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
