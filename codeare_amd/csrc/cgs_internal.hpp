@@ -50,6 +50,7 @@ struct FftArgs {
   int nst, radix[8];
   uint32_t ns_magic[8]; // ceil(2^32 / Ns) of each stage (Ns = product of earlier radices)
   int lb_log2;          // lb = 1 << lb_log2
+  int use_reg;          // register-resident kernel (n = 256 or 512, 16 lines per block)
   // fused operations of the first forward / last adjoint pass (axis 0)
   const float2* x;      // image frames (load mode 1)
   const float2* S;      // maps (load mode 1)
@@ -208,6 +209,8 @@ struct Plan {
 
 // ---- launchers (kernels_*.hip) ----------------------------------------------
 void launch_fft(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s);
+bool fft_reg_supported(const FftArgs& a, int ld_mode, int st_mode);
+void launch_fft_reg(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s);
 void launch_interp(const SampArgs& a, int cg, hipStream_t s);
 void launch_grid(const GridArgs& a, int cg, int waves, int nitems, int ncgroups, hipStream_t s);
 void launch_grid_reduce(const Plan& p, const float2* partial, float2* grid, const int* fl, hipStream_t s);

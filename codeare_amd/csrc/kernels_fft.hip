@@ -330,6 +330,10 @@ __global__ void __launch_bounds__(kFftThreads) k_fft(FftArgs a) {
 size_t fft_lds_bytes(const FftArgs& a) { return sizeof(float2) * ((size_t)a.lb * a.ldn + a.n); }
 
 void launch_fft(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s) {
+  if (a.use_reg) {
+    launch_fft_reg(a, ld_mode, st_mode, s);
+    return;
+  }
   const int64_t nib = (a.inner + a.lb - 1) / a.lb;
   const dim3 grid((unsigned)(nib * a.outers));
   const size_t shm = fft_lds_bytes(a);

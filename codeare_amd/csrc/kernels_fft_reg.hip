@@ -1,0 +1,357 @@
+// kernels_fft_reg.hip -- register-resident FFT passes for line lengths 256 and
+// 512 (the oversampled / Toeplitz grids of the 2D workloads).
+//
+// Same job and arguments as k_fft (kernels_fft.hip): a pass views the array as
+// [outer][n][inner] with inner contiguous and transforms the n-point lines.  A
+// 256-thread block takes 16 adjacent lines (16 consecutive inner indices, so
+// every global access moves 128 contiguous bytes); 16 threads share a line and
+// each holds N1 = n / 16 of its points in registers.  With n = 16 * N1 and
+// positions p = j + 16 k (thread j, register k):
+//   X[k1 + N1 k2] = sum_j W_16^(j k2) W_n^(j k1) [ sum_k x[j + 16 k] W_N1^(k k1) ]
+// so a pass is an N1-point FFT in registers, a twiddle, ONE exchange through
+// LDS, and N1/16 16-point FFTs in registers.  The Toeplitz convolution pass
+// multiplies the spectrum in that layout and runs the same steps backwards
+// (two exchanges in all).  Compared with staging whole lines in LDS and
+// running log_R(n) LDS stages per line, LDS traffic drops from ~4 reads and
+// writes per point to one of each, and all N1 loads per thread are in flight
+// together.
+#include <hip/hip_runtime.h>
+
+#include "cgs_internal.hpp"
+
+namespace cgs {
+
+namespace {
+
+constexpr double kPiD = 3.14159265358979323846264338327950288;
+
+constexpr double taylor_sin(double x) {
+  double t = x, s = x;
+  for (int i = 1; i < 16; ++i) {
+    t *= -x * x / ((2.0 * i) * (2.0 * i + 1.0));
+    s += t;
+  }
+  return s;
+}
+constexpr double taylor_cos(double x) {
+  double t = 1.0, s = 1.0;
+  for (int i = 1; i < 16; ++i) {
+    t *= -x * x / ((2.0 * i - 1.0) * (2.0 * i));
+    s += t;
+  }
+  return s;
+}
+// exp(-2 pi i t / L) for t in [0, L): angle reduced to [-pi, pi]
+template <int L>
+struct TwL {
+  float c[L], s[L];
+  constexpr TwL() : c(), s() {
+    for (int t = 0; t < L; ++t) {
+      double a = -2.0 * kPiD * t / L;
+      if (a < -kPiD) a += 2.0 * kPiD;
+      c[t] = (float)taylor_cos(a);
+      s[t] = (float)taylor_sin(a);
+    }
+  }
+};
+template <int L>
+__device__ constexpr TwL<L> kTwL = TwL<L>();
+
+__device__ __forceinline__ float2 cmul2(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// one Stockham radix-R stage (R = 2 or 4) of an N-point DFT held in registers;
+// span Ns; sign s = -1 forward, +1 inverse; every index is a compile-time
+// constant after unrolling, so the arrays stay in VGPRs
+template <int N, int Ns, int R>
+__device__ __forceinline__ void reg_stage(const float2 (&in)[N], float2 (&out)[N], float s) {
+#pragma unroll
+  for (int jj = 0; jj < N / R; ++jj) {
+    const int k = jj % Ns;
+    float2 a[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) a[m] = in[jj + m * (N / R)];
+    if (Ns > 1) {
+#pragma unroll
+      for (int m = 1; m < R; ++m) {
+        const int t = (m * k) % (Ns * R);
+        const float2 w = make_float2(kTwL<Ns * R>.c[t], s < 0 ? kTwL<Ns * R>.s[t] : -kTwL<Ns * R>.s[t]);
+        a[m] = cmul2(a[m], w);
+      }
+    }
+    if (R == 2) {
+      const float2 b0 = make_float2(a[0].x + a[1].x, a[0].y + a[1].y);
+      const float2 b1 = make_float2(a[0].x - a[1].x, a[0].y - a[1].y);
+      a[0] = b0;
+      a[1] = b1;
+    } else {
+      // radix 4: W_4 = exp(s 2 pi i / 4) = s*i (s = -1: -i)
+      const float2 t0 = make_float2(a[0].x + a[2].x, a[0].y + a[2].y);
+      const float2 t1 = make_float2(a[0].x - a[2].x, a[0].y - a[2].y);
+      const float2 t2 = make_float2(a[1].x + a[3].x, a[1].y + a[3].y);
+      const float2 t3 = make_float2(a[1].x - a[3].x, a[1].y - a[3].y);
+      // t3 * (s i) = (-s t3.y, s t3.x)
+      const float2 u3 = make_float2(-s * t3.y, s * t3.x);
+      a[0] = make_float2(t0.x + t2.x, t0.y + t2.y);
+      a[2] = make_float2(t0.x - t2.x, t0.y - t2.y);
+      a[1] = make_float2(t1.x + u3.x, t1.y + u3.y);
+      a[3] = make_float2(t1.x - u3.x, t1.y - u3.y);
+    }
+    const int base = (jj / Ns) * Ns * R + k;
+#pragma unroll
+    for (int m = 0; m < R; ++m) out[base + m * Ns] = a[m];
+  }
+}
+
+// in-place N-point DFT (N = 16 or 32), natural order in and out
+template <int N>
+__device__ __forceinline__ void reg_fft(float2 (&v)[N], float s) {
+  float2 t[N];
+  if (N == 16) {
+    reg_stage<N, 1, 4>(v, t, s);
+    reg_stage<N, 4, 4>(t, v, s);
+  } else {
+    reg_stage<N, 1, 4>(v, t, s);
+    reg_stage<N, 4, 4>(t, v, s);
+    reg_stage<N, 16, 2>(v, t, s);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = t[i];
+  }
+}
+
+__device__ __forceinline__ float deapod_r(const FftArgs& a, int i0, int64_t rest) {
+  float f = a.dinv0[i0];
+  const int i1 = (int)(rest % a.N1);
+  f *= a.dinv1[i1];
+  if (a.d == 3) f *= a.dinv2[(int)(rest / a.N1)];
+  return f;
+}
+
+__device__ __forceinline__ bool inside_mask_r(const FftArgs& a, int i0, int64_t rest) {
+  const int nh = a.mask_nh;
+  const int i1 = (int)(rest % a.N1);
+  int64_t s = (int64_t)(i0 - nh) * (i0 - nh) + (int64_t)(i1 - nh) * (i1 - nh);
+  if (a.d == 3) {
+    const int i2 = (int)(rest / a.N1);
+    s += (int64_t)(i2 - nh) * (i2 - nh);
+  }
+  return s <= (int64_t)nh * nh;
+}
+
+constexpr int kRegLines = 16;
+
+// exchange layout: element (row k1, column j) of line q at
+// ex[k1 * XS + j * 16 + q], XS = 16*16 + 8 (pad), so a wave's 64 lanes
+// (16 lines x 4 columns) touch 64 consecutive slots on writes
+template <int N1>
+__device__ __forceinline__ int xidx(int k1, int j, int q) {
+  return k1 * (16 * 16 + 8) + j * 16 + q;
+}
+
+}  // namespace
+
+template <int N1, int LD, int ST, int CONV>
+__global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
+  constexpr int n = 16 * N1;
+  constexpr int RPT = N1 / 16;   // 16-point rows per thread after the exchange
+  __shared__ float2 ex[N1 * (16 * 16 + 8)];
+  __shared__ float2 tw[n];
+  const int tid = threadIdx.x;
+  const int q = tid & 15, j = tid >> 4;
+  const int64_t nib = (a.inner + kRegLines - 1) / kRegLines;
+  const int64_t o = blockIdx.x / nib;
+  const int64_t q0 = (blockIdx.x - o * nib) * kRegLines;
+  const int64_t f = o / a.outer_per_frame;
+  if (a.fdone && a.fdone[f]) return;
+  const int64_t rest = o - f * a.outer_per_frame;
+  const int nq = (int)min((int64_t)kRegLines, a.inner - q0);
+  const bool live = q < nq;
+  const int64_t qq = q0 + q;
+  const float s = (float)a.sign;
+
+  for (int i = tid; i < n; i += 256) tw[i] = a.tw[i];
+
+  // ---- load positions pos = j + 16 k (image index p with the zero pad) ----
+  float2 v[N1];
+  const bool pad = a.pad_in != 0;
+#pragma unroll
+  for (int k = 0; k < N1; ++k) {
+    const int pos = j + 16 * k;
+    int p = pos;
+    if (pad) {
+      p = pos + a.nin / 2;
+      if (p >= n) p -= n;
+    }
+    float2 t = make_float2(0.f, 0.f);
+    if (live && p < a.nin) {
+      if (LD == 1 || LD == 4) {
+        const int64_t i = p + (int64_t)a.N0 * rest;
+        float2 xv;
+        if (a.pold) {   // p_new = r + beta p_old (CGLS.hpp:55-56), fused
+          const float be = (float)a.cgst[f].beta;
+          const float2 rr = a.r[f * a.Ntot + i], po = a.pold[f * a.Ntot + i];
+          xv = make_float2(rr.x + be * po.x, rr.y + be * po.y);
+          if (qq == 0) a.pnew[f * a.Ntot + i] = xv;
+        } else {
+          xv = a.x[f * a.Ntot + i];
+        }
+        t = cmul2(xv, a.S[i * a.C + qq]);
+        if (LD == 1) {
+          const float dv = deapod_r(a, p, rest);
+          t = make_float2(t.x * dv, t.y * dv);
+        }
+      } else {
+        t = a.in[(o * a.nin + p) * a.inner + qq];
+        if (LD >= 2) {
+          float dv = deapod_r(a, p, rest);
+          if (LD == 3 && !inside_mask_r(a, p, rest)) dv = 0.f;
+          t = make_float2(t.x * dv, t.y * dv);
+        }
+      }
+    }
+    v[k] = t;
+  }
+  __syncthreads();   // twiddle table
+
+  // ---- N1-point DFT over k, twiddle W_n^(s j k1), exchange ----
+  reg_fft<N1>(v, s);
+#pragma unroll
+  for (int k1 = 1; k1 < N1; ++k1) {
+    const float2 w = tw[j * k1];
+    v[k1] = cmul2(v[k1], make_float2(w.x, s < 0 ? w.y : -w.y));
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) ex[xidx<N1>(k1, j, q)] = v[k1];
+  __syncthreads();
+  // rows k1 = j + 16 r: 16-point DFT over the column index
+  float2 X[RPT][16];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) X[r][jj] = ex[xidx<N1>(j + 16 * r, jj, q)];
+    reg_fft<16>(X[r], s);
+  }
+  // thread holds bins P = k1 + N1 k2, k1 = j + 16 r, k2 = 0..15
+
+  if (CONV) {
+    // multiply by the real kernel spectrum, then the inverse transform
+    const float* th = a.That + (f * a.conv_cells + qq / a.C) * n;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+#pragma unroll
+      for (int k2 = 0; k2 < 16; ++k2) {
+        const float t = live ? th[(j + 16 * r) + N1 * k2] : 0.f;
+        X[r][k2] = make_float2(X[r][k2].x * t, X[r][k2].y * t);
+      }
+    const float si = -s;
+    __syncthreads();   // every thread has read its rows of ex
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int k1 = j + 16 * r;
+      reg_fft<16>(X[r], si);   // over k2 -> index j'
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) {
+        const float2 w = tw[jj * k1];
+        ex[xidx<N1>(k1, jj, q)] = cmul2(X[r][jj], make_float2(w.x, si < 0 ? w.y : -w.y));
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) v[k1] = ex[xidx<N1>(k1, j, q)];
+    reg_fft<N1>(v, si);   // over k1 -> k; point j + 16 k
+    // ---- store positions j + 16 k (crop) ----
+    const bool crop = a.crop_out != 0;
+#pragma unroll
+    for (int k = 0; k < N1; ++k) {
+      const int pos = j + 16 * k;
+      int p = pos;
+      if (crop) {
+        p = pos + a.nout / 2;
+        if (p >= n) p -= n;
+      }
+      if (live && p < a.nout) a.out[(o * a.nout + p) * a.inner + qq] = v[k];
+    }
+    return;
+  }
+
+  // ---- store bins P (crop: output index p = P + nout/2 mod n) ----
+  const bool crop = a.crop_out != 0;
+  if (ST == 2) {
+    const int64_t g = q0 / kRegLines;
+    float2* outp = a.out + (f * nib + g) * a.Ntot + (int64_t)a.N0 * rest;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+#pragma unroll
+      for (int k2 = 0; k2 < 16; ++k2) {
+        const int P = (j + 16 * r) + N1 * k2;
+        int p = P + a.nout / 2;
+        if (p >= n) p -= n;
+        float2 c = make_float2(0.f, 0.f);
+        if (live && p < a.nout) {
+          const float2 sv = a.S[(p + (int64_t)a.N0 * rest) * a.C + qq];
+          const float2 xv = X[r][k2];
+          c = make_float2(sv.x * xv.x + sv.y * xv.y, sv.x * xv.y - sv.y * xv.x);
+        }
+#pragma unroll
+        for (int m = 8; m > 0; m >>= 1) {
+          c.x += __shfl_xor(c.x, m);
+          c.y += __shfl_xor(c.y, m);
+        }
+        if (q == 0 && p < a.nout) outp[p] = c;
+      }
+    return;
+  }
+#pragma unroll
+  for (int r = 0; r < RPT; ++r)
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+      const int P = (j + 16 * r) + N1 * k2;
+      int p = P;
+      if (crop) {
+        p = P + a.nout / 2;
+        if (p >= n) p -= n;
+      }
+      if (live && p < a.nout) {
+        float2 val = X[r][k2];
+        if (ST == 1) {
+          const float dv = deapod_r(a, p, rest);
+          val = make_float2(val.x * dv, val.y * dv);
+        }
+        a.out[(o * a.nout + p) * a.inner + qq] = val;
+      }
+    }
+}
+
+bool fft_reg_supported(const FftArgs& a, int ld_mode, int st_mode) {
+  if (!(a.n == 256 || a.n == 512)) return false;
+  if (a.inner < kRegLines) return false;
+  if (st_mode == 2 && a.That) return false;
+  (void)ld_mode;
+  return true;
+}
+
+void launch_fft_reg(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s) {
+  const int64_t nib = (a.inner + kRegLines - 1) / kRegLines;
+  const dim3 grid((unsigned)(nib * a.outers));
+  const int conv = a.That != nullptr;
+#define CGS_FFTR_CASE(L, S_, CV)                                                      \
+  if (ld_mode == L && st_mode == S_ && conv == CV) {                                  \
+    if (a.n == 512)                                                                   \
+      hipLaunchKernelGGL((k_fft_reg<32, L, S_, CV>), grid, dim3(256), 0, s, a);      \
+    else                                                                              \
+      hipLaunchKernelGGL((k_fft_reg<16, L, S_, CV>), grid, dim3(256), 0, s, a);      \
+    return;                                                                           \
+  }
+  CGS_FFTR_CASE(0, 0, 0)
+  CGS_FFTR_CASE(1, 0, 0)
+  CGS_FFTR_CASE(2, 0, 0)
+  CGS_FFTR_CASE(3, 0, 0)
+  CGS_FFTR_CASE(4, 0, 0)
+  CGS_FFTR_CASE(0, 1, 0)
+  CGS_FFTR_CASE(0, 0, 1)
+  CGS_FFTR_CASE(0, 2, 0)
+#undef CGS_FFTR_CASE
+}
+
+}  // namespace cgs

@@ -177,9 +177,20 @@ static int drain_profile(Plan& p) {
 }
 
 // ---- FFT passes ------------------------------------------------------------------
+// lines staged per FFT block; CGS_FFT_LB caps it (read once per process)
+static int fft_lb_cap() {
+  static const int cap = [] {
+    const char* e = std::getenv("CGS_FFT_LB");
+    const int v = e ? std::atoi(e) : 64;
+    return (v >= 1 && v <= 64) ? v : 64;
+  }();
+  return cap;
+}
+
 static int pick_lb(int64_t inner, int n) {
   int lb = 1;
-  while (lb * 2 <= kFftMaxElem / n && lb * 2 <= inner && lb * 2 <= 64) lb *= 2;
+  const int cap = fft_lb_cap();
+  while (lb * 2 <= kFftMaxElem / n && lb * 2 <= inner && lb * 2 <= cap) lb *= 2;
   return lb;
 }
 
@@ -205,8 +216,17 @@ static FftArgs fft_base(const Plan& p, const FftTab& tab, int sign) {
   return a;
 }
 
+static bool fft_reg_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("CGS_FFT_REG");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return on;
+}
+
 static void fft_finish(FftArgs& a) {
-  a.lb = pick_lb(a.inner, a.n);
+  a.use_reg = fft_reg_enabled() && fft_reg_supported(a, 0, 0);
+  a.lb = a.use_reg ? 16 : pick_lb(a.inner, a.n);
   a.lb_log2 = 0;
   while ((1 << a.lb_log2) < a.lb) ++a.lb_log2;
   a.ldn = a.n + (a.n >> 4) + 1;   // room for the 1-in-16 pad slots of the line layout

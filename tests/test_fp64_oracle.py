@@ -76,12 +76,31 @@ def test_c2_against_fp64_nudft():
         print(f"  CG({it}) image vs FP64 NUDFT CG({n}): {e:.2e}")
         assert rep["iters"][0] == n == it
         assert e <= 1e-4
-    # to the stated tolerance: both converge; report the difference
+    # to the stated tolerance.  At the literal lambda = 1e-3 (lambda_max ~ 1e5)
+    # the unsampled k-space corners are unregularised: the converged images
+    # depend on the discretisation (NFFT vs exact NUDFT) there and on the
+    # iteration at which 1e-5 is reached, so only convergence is asserted.
     xg, rep = P.solve(y, maxit=3000, tol=1e-5, lam=wl.lam, toeplitz=True, graph_iters=8)
     xo, n, _ = T.cg(S, Kh, b, wl.lam, maxit=3000, tol=1e-5)
     e = rel(xg, xo.cpu().numpy())
-    ep = rel(xg, prob.image)
-    eo = rel(xo.cpu().numpy(), prob.image)
-    print(f"  to 1e-5: GPU {rep['iters'][0]} its, FP64 NUDFT {n} its; image difference {e:.2e}; "
-          f"vs phantom {ep:.3e} (GPU) {eo:.3e} (FP64)")
-    assert e <= 1e-2
+    print(f"  lambda {wl.lam:g} to 1e-5: GPU {rep['iters'][0]} its, FP64 NUDFT {n} its; image difference "
+          f"{e:.2e}; vs phantom {rel(xg, prob.image):.3e} (GPU) {rel(xo.cpu().numpy(), prob.image):.3e} (FP64)")
+    assert rep["final_rel"][0] <= 1e-5 and n < 3000
+    # lambda = 1e-3 lambda_max (FP64 power iteration): a well-posed problem,
+    # whose converged images must agree
+    v = torch.randn(N, N, dtype=torch.complex128, device=dev)
+    for _ in range(30):
+        w_ = T.normal(v, S, Kh, 0.0)
+        lmax = float(torch.linalg.vector_norm(w_) / torch.linalg.vector_norm(v))
+        v = w_ / torch.linalg.vector_norm(w_)
+    lam_s = 1e-3 * lmax
+    xg, rep = P.solve(y, maxit=3000, tol=1e-5, lam=lam_s, toeplitz=True, graph_iters=8)
+    ng = rep["iters"][0]
+    xo, n, _ = T.cg(S, Kh, b, lam_s, maxit=3000, tol=1e-5)
+    xe, _, _ = T.cg(S, Kh, b, lam_s, maxit=ng, tol=0.0)        # same iteration count
+    e_same = rel(xg, xe.cpu().numpy())
+    e_tol = rel(xg, xo.cpu().numpy())
+    print(f"  lambda {lam_s:.3g} (1e-3 lambda_max) to 1e-5: GPU {ng} its, FP64 NUDFT {n} its; image vs FP64 "
+          f"at {ng} its {e_same:.2e}, vs FP64 at its own stop {e_tol:.2e}; vs phantom {rel(xg, prob.image):.3e}")
+    assert abs(ng - n) <= 2
+    assert e_same <= 1e-4 and e_tol <= 1e-3

@@ -335,3 +335,66 @@ def test_autotune_keeps_results(cgs, ora):
     rq = Q.autotune(use_cache=True)
     assert rq["from_cache"] == 1 and rq["visits_per_item"] == rep["visits_per_item"]
     assert rel(Q.EH(y), x1) <= 1e-6
+
+
+@pytest.mark.parametrize("coils", [16, 32, 8])
+def test_E_EH_oversampled_256_lines(cgs, ora, coils):
+    """n = 256 lines (128^2 at sigma 2): the register-resident FFT passes
+    (16 lines of >= 16 coils per block) and, for 8 coils, the LDS-staged ones
+    on axis 0 -- E, E^H and the Toeplitz normal operator against the oracle."""
+    prob, y, P, O = _c1_problem(cgs, ora, N=128, coils=coils, spokes=96, samples=256)
+    e1 = rel(P.E(prob.image), O.E(prob.maps, prob.image))
+    e2 = rel(P.EH(y), O.EH_true(prob.maps, prob.w, y))
+    q = O.EH_true(prob.maps, prob.w, O.E(prob.maps, prob.image)) + 0.25 * prob.image
+    e3 = rel(P.normal(prob.image, lam=0.25, toeplitz=True), q)
+    e4 = rel(P.normal(prob.image, lam=0.25), q)
+    print(f"C={coils}: E {e1:.2e} E^H {e2:.2e} toeplitz {e3:.2e} nufft normal {e4:.2e}")
+    assert e1 <= OP_TOL and e2 <= OP_TOL and e3 <= 2e-5 and e4 <= 2e-5
+
+
+def test_c2_operators_against_oracle(cgs, ora):
+    """n = 512 lines at the full 256^2 x 32-coil golden-angle size: E, E^H and
+    the Toeplitz normal operator (2N = 512) against the FP64 oracle."""
+    from codeare_amd import data
+    prob = data.make_problem(2, 0)
+    M = prob.k.shape[1]
+    P = cgs.Plan((256, 256), M, 2.0, 3, ncoils=32)
+    P.kspace(prob.k)
+    P.weights(prob.w)
+    P.sensitivities(prob.maps)
+    O = ora.NFFT((256, 256), M, 2.0, 3)
+    O.kspace(prob.k)
+    rng = np.random.default_rng(12)
+    x = rand_img(rng, (256, 256))
+    yo = O.E(prob.maps, x)
+    e1 = rel(P.E(x), yo)
+    e2 = rel(P.EH(yo), O.EH_true(prob.maps, prob.w, yo))
+    q = O.EH_true(prob.maps, prob.w, yo) + 1e-3 * x
+    e3 = rel(P.normal(x, lam=1e-3, toeplitz=True), q)
+    print(f"C2: E {e1:.2e} E^H {e2:.2e} toeplitz normal {e3:.2e}")
+    assert e1 <= OP_TOL and e2 <= OP_TOL and e3 <= 2e-5
+
+
+def test_koosh_3d_medium(cgs, ora):
+    """3D koosh ball at 64^3 (sigma 1.5, m 3, 4 coils): E, E^H, the Toeplitz
+    normal operator and 5 CG iterations against the oracle."""
+    from codeare_amd import data
+    prob = data.make_problem(3, 0, N=64, coils=4, spokes=1500, samples=128)
+    wl = prob.wl
+    M = prob.k.shape[1]
+    P = cgs.Plan((64, 64, 64), M, wl.sigma, wl.m, ncoils=4)
+    P.kspace(prob.k)
+    P.weights(prob.w)
+    P.sensitivities(prob.maps)
+    O = ora.NFFT((64, 64, 64), M, wl.sigma, wl.m)
+    O.kspace(prob.k)
+    y = data.make_data(prob, lambda x: O.E(prob.maps, x))
+    e1 = rel(P.E(prob.image), O.E(prob.maps, prob.image))
+    e2 = rel(P.EH(y), O.EH_true(prob.maps, prob.w, y))
+    q = O.EH_true(prob.maps, prob.w, O.E(prob.maps, prob.image))
+    e3 = rel(P.normal(prob.image, toeplitz=True), q)
+    xg, rep = P.solve(y, maxit=5, tol=0.0, toeplitz=True)
+    xo, it, _ = O.cg_true(prob.maps, prob.w, y, maxit=5, tol=0.0)
+    e4 = rel(xg, xo)
+    print(f"3D 64^3: E {e1:.2e} E^H {e2:.2e} toeplitz normal {e3:.2e} CG(5) {e4:.2e}")
+    assert e1 <= OP_TOL and e2 <= OP_TOL and e3 <= 5e-5 and e4 <= 1e-4

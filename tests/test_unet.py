@@ -111,11 +111,23 @@ def test_unet_forward_matches_oracle(levels, H, W, B):
     net, theta = _net(levels, H, W, B)
     x = _rand_batch(rng, B, H, W)
     y = net.forward(x)
-    yo, _ = U.UNetRef(levels, theta, bf16=True).forward(x.astype(np.complex128))
-    # the network part alone (output minus the residual input)
-    e = np.linalg.norm((y - x) - (yo - x)) / np.linalg.norm(yo - x)
-    print(f"U-Net L={levels} {H}x{W} B={B}: forward rel err {e:.2e}")
-    assert e <= 3e-3
+    xd = x.astype(np.complex128)
+    yo, _ = U.UNetRef(levels, theta, bf16=True).forward(xd)
+    y64, _ = U.UNetRef(levels, theta, bf16=False).forward(xd)
+
+    def err(a, b):   # the network part alone (output minus the residual input)
+        return np.linalg.norm((a - x) - (b - x)) / np.linalg.norm(b - x)
+
+    e = err(y, yo)
+    e_gpu64, e_emu64 = err(y, y64), err(yo, y64)
+    print(f"U-Net L={levels} {H}x{W} B={B}: vs BF16 restatement {e:.2e}; vs FP64: GPU {e_gpu64:.2e}, "
+          f"restatement {e_emu64:.2e}")
+    # GPU and restatement round to BF16 at the same points; their sums differ
+    # in order (FP32 vs FP64), which flips an occasional BF16 rounding and the
+    # flips compound with depth.  A kernel error would also make the GPU much
+    # further from the FP64 network than the restatement is.
+    assert e <= (1e-3 if levels == 2 else 2e-2)
+    assert e_gpu64 <= 1.5 * e_emu64 + 1e-3
 
 
 @pytest.mark.gpu
