@@ -66,6 +66,12 @@ struct FftArgs {
   const float2* pold;
   float2* pnew;
   const CgState* cgst;
+  // fused CG tail (register kernel only): the pad pass writes ||p_new||^2 per
+  // (frame, row) from the block holding coil 0; the crop+combine pass writes
+  // Re<pdot, partial coil sum> per (frame, coil group, row)
+  double* pp_part;
+  double* pq_part;
+  const float2* pdot;
 };
 
 struct SampArgs {
@@ -202,6 +208,8 @@ struct Plan {
   struct Stat { int64_t launches = 0; double ms = 0, bytes = 0; };
   Stat stat[CGS_K_COUNT];
   unsigned prof_mask = ~0u;           // kernel classes timed with events
+  double* tail_pp = nullptr;          // fused CG tail partials (Toeplitz path)
+  double* tail_pq = nullptr;
   struct Pending { hipEvent_t a, b; int kind; double bytes; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
@@ -221,6 +229,8 @@ size_t grid_batched_lds(int d, int T, int taps, int CG, int rowstride);
 void launch_grid_atomic(const GridArgs& a, const int4* vrec, int cg, int nitems, int ncgroups, hipStream_t s);
 void launch_grid_reg(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s);
 size_t interp_lds_bytes(int T, int taps, int CG, int rowstride, int maxs);
+void launch_cg_tail(const Plan& p, float2* x, float2* r, const float2* pv, const float2* part, int G,
+                    float lambda, const double* pq_part, int npq, const double* pp_part, int npp, hipStream_t s);
 void launch_partial_combine(const Plan& p, const float2* part, int G, float2* q, const float2* pvec, float lambda,
                             double* partials, const int* fl, hipStream_t s);
 void launch_coil_combine(const Plan& p, const float2* Xc, float2* x, const float2* pvec,

@@ -17,6 +17,8 @@
 // together.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "cgs_internal.hpp"
 
 namespace cgs {
@@ -141,6 +143,16 @@ __device__ __forceinline__ bool inside_mask_r(const FftArgs& a, int i0, int64_t 
 
 constexpr int kRegLines = 16;
 
+// sum over the 256-thread block (every thread calls it)
+__device__ __forceinline__ double block_sum_d(double v, double* red) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
 // exchange layout: element (row k1, column j) of line q at
 // ex[k1 * XS + j * 16 + q], XS = 16*16 + 8 (pad), so a wave's 64 lanes
 // (16 lines x 4 columns) touch 64 consecutive slots on writes
@@ -151,12 +163,17 @@ __device__ __forceinline__ int xidx(int k1, int j, int q) {
 
 }  // namespace
 
-template <int N1, int LD, int ST, int CONV>
+// XR: rows of the exchange buffer.  XR = N1 exchanges everything at once;
+// XR = 16 exchanges in N1/16 rounds of 16 rows (half the LDS at n = 512, so
+// more blocks fit on a CU, at the price of extra barriers)
+template <int N1, int LD, int ST, int CONV, int XR>
 __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
   constexpr int n = 16 * N1;
   constexpr int RPT = N1 / 16;   // 16-point rows per thread after the exchange
-  __shared__ float2 ex[N1 * (16 * 16 + 8)];
+  static_assert(XR == N1 || XR == 16, "exchange rows");
+  __shared__ float2 ex[XR * (16 * 16 + 8)];
   __shared__ float2 tw[n];
+  __shared__ double red[4];
   const int tid = threadIdx.x;
   const int q = tid & 15, j = tid >> 4;
   const int64_t nib = (a.inner + kRegLines - 1) / kRegLines;
@@ -174,6 +191,7 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
 
   // ---- load positions pos = j + 16 k (image index p with the zero pad) ----
   float2 v[N1];
+  float ppacc = 0.f;   // ||p_new||^2 over this row (coil-0 lanes of the fused CG update)
   const bool pad = a.pad_in != 0;
 #pragma unroll
   for (int k = 0; k < N1; ++k) {
@@ -192,7 +210,10 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
           const float be = (float)a.cgst[f].beta;
           const float2 rr = a.r[f * a.Ntot + i], po = a.pold[f * a.Ntot + i];
           xv = make_float2(rr.x + be * po.x, rr.y + be * po.y);
-          if (qq == 0) a.pnew[f * a.Ntot + i] = xv;
+          if (qq == 0) {
+            a.pnew[f * a.Ntot + i] = xv;
+            ppacc += xv.x * xv.x + xv.y * xv.y;
+          }
         } else {
           xv = a.x[f * a.Ntot + i];
         }
@@ -212,6 +233,10 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
     }
     v[k] = t;
   }
+  if ((LD == 1 || LD == 4) && a.pp_part && q0 == 0) {   // block-uniform condition
+    const double t = block_sum_d(ppacc, red);
+    if (tid == 0) a.pp_part[f * a.outer_per_frame + rest] = t;
+  }
   __syncthreads();   // twiddle table
 
   // ---- N1-point DFT over k, twiddle W_n^(s j k1), exchange ----
@@ -221,16 +246,29 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
     const float2 w = tw[j * k1];
     v[k1] = cmul2(v[k1], make_float2(w.x, s < 0 ? w.y : -w.y));
   }
-#pragma unroll
-  for (int k1 = 0; k1 < N1; ++k1) ex[xidx<N1>(k1, j, q)] = v[k1];
-  __syncthreads();
   // rows k1 = j + 16 r: 16-point DFT over the column index
   float2 X[RPT][16];
+  if (XR == N1) {
 #pragma unroll
-  for (int r = 0; r < RPT; ++r) {
+    for (int k1 = 0; k1 < N1; ++k1) ex[xidx<N1>(k1, j, q)] = v[k1];
+    __syncthreads();
 #pragma unroll
-    for (int jj = 0; jj < 16; ++jj) X[r][jj] = ex[xidx<N1>(j + 16 * r, jj, q)];
-    reg_fft<16>(X[r], s);
+    for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) X[r][jj] = ex[xidx<N1>(j + 16 * r, jj, q)];
+      reg_fft<16>(X[r], s);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {   // round r holds rows 16 r .. 16 r + 15
+      if (r) __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) ex[xidx<N1>(kk, j, q)] = v[16 * r + kk];
+      __syncthreads();
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) X[r][jj] = ex[xidx<N1>(j, jj, q)];
+      reg_fft<16>(X[r], s);
+    }
   }
   // thread holds bins P = k1 + N1 k2, k1 = j + 16 r, k2 = 0..15
 
@@ -245,20 +283,37 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
         X[r][k2] = make_float2(X[r][k2].x * t, X[r][k2].y * t);
       }
     const float si = -s;
-    __syncthreads();   // every thread has read its rows of ex
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-      const int k1 = j + 16 * r;
-      reg_fft<16>(X[r], si);   // over k2 -> index j'
+    for (int r = 0; r < RPT; ++r) reg_fft<16>(X[r], si);   // over k2 -> index j'
+    if (XR == N1) {
+      __syncthreads();   // every thread has read its rows of ex
 #pragma unroll
-      for (int jj = 0; jj < 16; ++jj) {
-        const float2 w = tw[jj * k1];
-        ex[xidx<N1>(k1, jj, q)] = cmul2(X[r][jj], make_float2(w.x, si < 0 ? w.y : -w.y));
+      for (int r = 0; r < RPT; ++r) {
+        const int k1 = j + 16 * r;
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+          const float2 w = tw[jj * k1];
+          ex[xidx<N1>(k1, jj, q)] = cmul2(X[r][jj], make_float2(w.x, si < 0 ? w.y : -w.y));
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) v[k1] = ex[xidx<N1>(k1, j, q)];
+    } else {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const int k1 = j + 16 * r;
+        __syncthreads();   // the previous round's reads are done
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+          const float2 w = tw[jj * k1];
+          ex[xidx<N1>(j, jj, q)] = cmul2(X[r][jj], make_float2(w.x, si < 0 ? w.y : -w.y));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) v[16 * r + kk] = ex[xidx<N1>(kk, j, q)];
       }
     }
-    __syncthreads();
-#pragma unroll
-    for (int k1 = 0; k1 < N1; ++k1) v[k1] = ex[xidx<N1>(k1, j, q)];
     reg_fft<N1>(v, si);   // over k1 -> k; point j + 16 k
     // ---- store positions j + 16 k (crop) ----
     const bool crop = a.crop_out != 0;
@@ -280,6 +335,8 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
   if (ST == 2) {
     const int64_t g = q0 / kRegLines;
     float2* outp = a.out + (f * nib + g) * a.Ntot + (int64_t)a.N0 * rest;
+    const float2* pd = a.pdot ? a.pdot + f * a.Ntot + (int64_t)a.N0 * rest : nullptr;
+    double pqacc = 0.0;
 #pragma unroll
     for (int r = 0; r < RPT; ++r)
 #pragma unroll
@@ -298,8 +355,18 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
           c.x += __shfl_xor(c.x, m);
           c.y += __shfl_xor(c.y, m);
         }
-        if (q == 0 && p < a.nout) outp[p] = c;
+        if (q == 0 && p < a.nout) {
+          outp[p] = c;
+          if (pd) {
+            const float2 pv = pd[p];
+            pqacc += (double)pv.x * c.x + (double)pv.y * c.y;
+          }
+        }
       }
+    if (a.pq_part) {   // block-uniform
+      const double t = block_sum_d(pqacc, red);
+      if (tid == 0) a.pq_part[(f * nib + g) * a.outer_per_frame + rest] = t;
+    }
     return;
   }
 #pragma unroll
@@ -331,16 +398,28 @@ bool fft_reg_supported(const FftArgs& a, int ld_mode, int st_mode) {
   return true;
 }
 
+// exchange in rounds of 16 rows at n = 512 unless CGS_FFT_XFULL=1
+static bool fft_reg_full_exchange() {
+  static const bool full = [] {
+    const char* e = std::getenv("CGS_FFT_XFULL");
+    return e && std::atoi(e) == 1;
+  }();
+  return full;
+}
+
 void launch_fft_reg(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s) {
   const int64_t nib = (a.inner + kRegLines - 1) / kRegLines;
   const dim3 grid((unsigned)(nib * a.outers));
   const int conv = a.That != nullptr;
+  const bool full = fft_reg_full_exchange();
 #define CGS_FFTR_CASE(L, S_, CV)                                                      \
   if (ld_mode == L && st_mode == S_ && conv == CV) {                                  \
-    if (a.n == 512)                                                                   \
-      hipLaunchKernelGGL((k_fft_reg<32, L, S_, CV>), grid, dim3(256), 0, s, a);      \
+    if (a.n == 512 && full)                                                           \
+      hipLaunchKernelGGL((k_fft_reg<32, L, S_, CV, 32>), grid, dim3(256), 0, s, a);  \
+    else if (a.n == 512)                                                              \
+      hipLaunchKernelGGL((k_fft_reg<32, L, S_, CV, 16>), grid, dim3(256), 0, s, a);  \
     else                                                                              \
-      hipLaunchKernelGGL((k_fft_reg<16, L, S_, CV>), grid, dim3(256), 0, s, a);      \
+      hipLaunchKernelGGL((k_fft_reg<16, L, S_, CV, 16>), grid, dim3(256), 0, s, a);  \
     return;                                                                           \
   }
   CGS_FFTR_CASE(0, 0, 0)

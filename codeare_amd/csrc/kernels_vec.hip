@@ -121,6 +121,58 @@ __global__ void __launch_bounds__(256) k_partial_combine(const float2* __restric
   }
 }
 
+// Fused CG tail of the Toeplitz path: alpha = rn / (sum pq_part + lambda sum
+// pp_part) (every block sums the same partials in the same order), then
+// q = sum_g part_g + lambda p computed in place, x += alpha p, r -= alpha q and
+// ||r||^2 partials
+__global__ void __launch_bounds__(256) k_cg_tail(float2* x, float2* r, const float2* pv, const float2* part, int G,
+                                                 float lambda, CgState* st, const double* pq_part, int npq,
+                                                 const double* pp_part, int npp, double* partials, int64_t Ntot,
+                                                 const int* fdone) {
+  __shared__ double sh[4];
+  __shared__ float alpha_s;
+  const int f = blockIdx.y;
+  if (fdone[f]) return;
+  {
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < npq; i += blockDim.x) a += pq_part[(int64_t)f * npq + i];
+    for (int i = threadIdx.x; i < npp; i += blockDim.x) b += pp_part[(int64_t)f * npp + i];
+    const double pq = block_sum(a, sh) + (double)lambda * block_sum(b, sh);
+    if (threadIdx.x == 0) {
+      const double al = st[f].rn / pq;   // CGLS.hpp:50
+      alpha_s = (float)al;
+      if (blockIdx.x == 0) { st[f].pq = pq; st[f].alpha = al; }
+    }
+    __syncthreads();
+  }
+  const float al = alpha_s;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Ntot; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = (int64_t)f * Ntot + i;
+    const float2 pp = pv[o];
+    float2 qq = make_float2(lambda * pp.x, lambda * pp.y);
+    for (int g = 0; g < G; ++g) {
+      const float2 v = part[((int64_t)f * G + g) * Ntot + i];
+      qq.x += v.x;
+      qq.y += v.y;
+    }
+    float2 xx = x[o], rr = r[o];
+    xx.x += al * pp.x; xx.y += al * pp.y;
+    rr.x -= al * qq.x; rr.y -= al * qq.y;
+    x[o] = xx;
+    r[o] = rr;
+    acc += (double)rr.x * rr.x + (double)rr.y * rr.y;
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) partials[(int64_t)f * gridDim.x + blockIdx.x] = t;
+}
+
+void launch_cg_tail(const Plan& p, float2* x, float2* r, const float2* pv, const float2* part, int G,
+                    float lambda, const double* pq_part, int npq, const double* pp_part, int npp, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_tail, dim3(kRedBlocks, p.F), dim3(256), 0, s, x, r, pv, part, G, lambda, p.cgst, pq_part,
+                     npq, pp_part, npp, p.partials + (size_t)kCoilBlocks * p.F, p.Ntot, p.fdone);
+}
+
 void launch_partial_combine(const Plan& p, const float2* part, int G, float2* q, const float2* pvec, float lambda,
                             double* partials, const int* fl, hipStream_t s) {
   const dim3 g(kCoilBlocks, p.F);

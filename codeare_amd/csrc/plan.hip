@@ -975,6 +975,16 @@ static int toeplitz_prepare(Plan& p) {
     if (e != hipSuccess) { destroy_plan(q); set_error("allocation of the Toeplitz spectrum failed"); return CGS_ERR_ALLOC; }
     launch_toep_layout(src, p.That, cells, N2[d - 1], F, 1.0f / (float)tot2, q.stream);
   }
+  {
+    const size_t rows = (size_t)(p.Ntot / p.N[0]) * F;
+    const size_t groups = (size_t)(p.C + 15) / 16;
+    if (hipMalloc((void**)&p.tail_pp, sizeof(double) * rows) != hipSuccess ||
+        hipMalloc((void**)&p.tail_pq, sizeof(double) * rows * groups) != hipSuccess) {
+      destroy_plan(q);
+      set_error("allocation of the CG tail partials failed");
+      return CGS_ERR_ALLOC;
+    }
+  }
   hipError_t e = hipStreamSynchronize(q.stream);
   destroy_plan(q);
   if (e != hipSuccess) { set_error(std::string("Toeplitz preparation failed: ") + hipGetErrorString(e)); return CGS_ERR_HIP; }
@@ -997,6 +1007,8 @@ static int toeplitz_prepare(Plan& p) {
 // trajectory-dependent buffers)
 static void toep_drop(Plan& p) {
   dfree(p.That);
+  dfree(p.tail_pp);
+  dfree(p.tail_pq);
   if (!p.tt_alias) { dfree(p.tt1); dfree(p.tt2); }
   p.tt1 = p.tt2 = nullptr;
   p.have_toep = false;
@@ -1006,7 +1018,12 @@ static void toep_drop(Plan& p) {
 }
 
 // per-coil images X_c = crop(IFFT_2N(That .* FFT_2N(pad(S_c x)))) (Toeplitz embedding)
-static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd* pu, int* G_out) {
+static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd* pu, int* G_out,
+                             bool* tail_out = nullptr) {
+  // inside the CG (pu set) the register FFT passes also produce the partials
+  // of the fused CG tail: ||p||^2 per row (pad pass) and <p, q_partial> per
+  // (coil group, row) (crop pass)
+  int tail_parts = 0;
   const int d = p.d, C = p.C, F = p.F;
   const int64_t N0 = p.N[0], N1 = p.N[1], N2 = p.N[2];
   const int64_t M0 = 2 * N0, M1 = 2 * N1;
@@ -1023,6 +1040,10 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
     if (ld == 4) apply_pupd(p, a, pu);
     if (That) { a.That = That; a.conv_cells = cells; a.pad_in = 1; a.crop_out = 1; }
     fft_finish(a);
+    if (pu && tail_out && p.tail_pp && a.use_reg) {
+      if (ld == 4) { a.pp_part = p.tail_pp; ++tail_parts; }
+      if (st == 2) { a.pq_part = p.tail_pq; a.pdot = pu->pnew; ++tail_parts; }
+    }
     if (st == 2) {
       const int G = (int)((inner + a.lb - 1) / a.lb);
       *G_out = G;
@@ -1049,6 +1070,7 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
          (double)C * M0 * (M1 + N1) * N2 * 8);
     pass(CGS_K_TOEP_CROP, p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1 * N2, 0, nullptr, 0, 0.0, 2);
   }
+  if (tail_out) *tail_out = (tail_parts == 2);
   return CGS_OK;
 }
 
@@ -1081,7 +1103,16 @@ static int cg_iteration_fused(Plan& p, int normal_op, float2* x, float2* r, cons
   pu.pnew = pnew;
   if (normal_op == CGS_NORMAL_TOEPLITZ) {
     int G = 0;
-    CHK(toeplitz_to_coils(p, nullptr, p.fdone, &pu, &G));
+    bool tail = false;
+    CHK(toeplitz_to_coils(p, nullptr, p.fdone, &pu, &G, &tail));
+    if (tail) {
+      // alpha from the FFT passes' partials, q formed in place: no combine kernel
+      const int opf = (int)(p.Ntot / p.N[0]);
+      Prof pr(p, CGS_K_CG, ((double)G + 6.0) * 8.0 * p.F * p.Ntot);
+      launch_cg_tail(p, x, r, pnew, p.Xc, G, (float)o.lambda, p.tail_pq, G * opf, p.tail_pp, opf, p.stream);
+      launch_cg_beta_after_xr(p, o.tol_kind, o.tol, o.maxit, p.stream);
+      return CGS_OK;
+    }
     Prof pr(p, CGS_K_COIL, ((double)G + 3.0) * p.Ntot * p.F * 8);
     launch_partial_combine(p, p.Xc, G, q, pnew, (float)o.lambda, p.partials, p.fdone, p.stream);
   } else {

@@ -348,8 +348,15 @@ def test_E_EH_oversampled_256_lines(cgs, ora, coils):
     q = O.EH_true(prob.maps, prob.w, O.E(prob.maps, prob.image)) + 0.25 * prob.image
     e3 = rel(P.normal(prob.image, lam=0.25, toeplitz=True), q)
     e4 = rel(P.normal(prob.image, lam=0.25), q)
-    print(f"C={coils}: E {e1:.2e} E^H {e2:.2e} toeplitz {e3:.2e} nufft normal {e4:.2e}")
+    # CG through the Toeplitz operator (fused CG tail when C >= 16), launched
+    # one by one and as HIP graphs, against the oracle's FP64-scalar CG
+    xo, it, _ = O.cg_true(prob.maps, prob.w, y, maxit=10, tol=0.0, lam=0.25)
+    xa, ra = P.solve(y, maxit=10, tol=0.0, lam=0.25, toeplitz=True)
+    xb, rb = P.solve(y, maxit=10, tol=0.0, lam=0.25, toeplitz=True, graph_iters=4)
+    e5, e6 = rel(xa, xo), rel(xb, xa)
+    print(f"C={coils}: E {e1:.2e} E^H {e2:.2e} toeplitz {e3:.2e} nufft normal {e4:.2e} CG(10) {e5:.2e} graph {e6:.1e}")
     assert e1 <= OP_TOL and e2 <= OP_TOL and e3 <= 2e-5 and e4 <= 2e-5
+    assert ra["iters"][0] == rb["iters"][0] == it == 10 and e5 <= 1e-4 and e6 <= 1e-6
 
 
 def test_c2_operators_against_oracle(cgs, ora):
