@@ -87,6 +87,17 @@ __device__ __forceinline__ void reg_stage(const float2 (&in)[N], float2 (&out)[N
       const float2 b1 = make_float2(a[0].x - a[1].x, a[0].y - a[1].y);
       a[0] = b0;
       a[1] = b1;
+    } else if (R == 3) {
+      // w = exp(s 2 pi i / 3) = -1/2 + s i sqrt(3)/2
+      const float h = 0.86602540378443864676f * s;
+      const float2 t1 = make_float2(a[1].x + a[2].x, a[1].y + a[2].y);
+      const float2 t2 = make_float2(a[1].x - a[2].x, a[1].y - a[2].y);
+      const float2 b0 = make_float2(a[0].x + t1.x, a[0].y + t1.y);
+      const float2 m = make_float2(a[0].x - 0.5f * t1.x, a[0].y - 0.5f * t1.y);
+      // (i h) t2 = (-h t2.y, h t2.x)
+      a[0] = b0;
+      a[1] = make_float2(m.x - h * t2.y, m.y + h * t2.x);
+      a[2] = make_float2(m.x + h * t2.y, m.y - h * t2.x);
     } else {
       // radix 4: W_4 = exp(s 2 pi i / 4) = s*i (s = -1: -i)
       const float2 t0 = make_float2(a[0].x + a[2].x, a[0].y + a[2].y);
@@ -390,8 +401,284 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
     }
 }
 
+// ----------------------------------------------------------------------------
+// 512-point lines as 8 x 8 x 8: a 1024-thread block takes 16 lines, 64 threads
+// per line, 8 points per thread (p = t + 64 k).  Three in-register 8-point
+// DFTs with two LDS exchanges between them (four-step applied twice:
+// 512 = 8 x 64, 64 = 8 x 8).  Twice the LDS traffic of the 16-thread layout,
+// but 8 points per thread keep VGPRs low, so a CU runs 32 waves instead of
+// 8-12 and every pass has four times as many threads in flight.
+namespace {
+
+constexpr int kR8Row = 144;   // floats2 per exchange row: 8 columns x 16 lines + pad
+
+// 8-point DFT in registers, natural order (radix 2 x 4 Stockham)
+__device__ __forceinline__ void dft8(float2 (&v)[8], float s) {
+  float2 t[8];
+  reg_stage<8, 1, 2>(v, t, s);
+  reg_stage<8, 2, 4>(t, v, s);
+}
+
+__device__ __forceinline__ float2 twid(const float2* tw, int idx, float s) {
+  const float2 w = tw[idx];
+  return make_float2(w.x, s < 0 ? w.y : -w.y);
+}
+
+__device__ __forceinline__ double block_sum_1024(double v, double* red) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < 16; ++w) t += red[w];
+  return t;
+}
+
+}  // namespace
+
+// small DFT of N1 = 1..8 points in registers (natural order)
+template <int N1>
+__device__ __forceinline__ void dftN(float2 (&v)[N1], float s) {
+  if constexpr (N1 == 2) {
+    float2 t[2];
+    reg_stage<2, 1, 2>(v, t, s);
+    v[0] = t[0]; v[1] = t[1];
+  } else if constexpr (N1 == 3) {
+    float2 t[3];
+    reg_stage<3, 1, 3>(v, t, s);
+    v[0] = t[0]; v[1] = t[1]; v[2] = t[2];
+  } else if constexpr (N1 == 4) {
+    float2 t[4];
+    reg_stage<4, 1, 4>(v, t, s);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = t[i];
+  } else if constexpr (N1 == 6) {
+    float2 t[6];
+    reg_stage<6, 1, 2>(v, t, s);
+    reg_stage<6, 2, 3>(t, v, s);
+  } else if constexpr (N1 == 8) {
+    dft8(v, s);
+  }
+}
+
+// n = 64 N1 (N1 in {2, 3, 4, 6, 8}): 64 threads per line hold N1 points each
+// (p = t + 64 k) for an N1-point DFT, then the 64-point DFT over t runs as
+// 8 x 8 on the first 8 N1 threads of the line (8 points each) with two
+// exchanges.  Same fusions as the other FFT kernels.
+template <int N1, int LD, int ST, int CONV>
+__global__ void __launch_bounds__(1024) k_fft_r8(FftArgs a) {
+  constexpr int n = 64 * N1;
+  constexpr int ACT = 8 * N1;   // threads per line busy in the 8 x 8 stages
+  __shared__ float2 ex[ACT * kR8Row];
+  __shared__ float2 tw[n];
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  const int q = tid & 15, t = tid >> 4;   // line in the block, thread in the line
+  const int lo = t & 7, hi = t >> 3;
+  const int64_t nib = (a.inner + kRegLines - 1) / kRegLines;
+  const int64_t o = blockIdx.x / nib;
+  const int64_t q0 = (blockIdx.x - o * nib) * kRegLines;
+  const int64_t f = o / a.outer_per_frame;
+  if (a.fdone && a.fdone[f]) return;
+  const int64_t rest = o - f * a.outer_per_frame;
+  const int nq = (int)min((int64_t)kRegLines, a.inner - q0);
+  const bool live = q < nq;
+  const int64_t qq = q0 + q;
+  const float s = (float)a.sign;
+
+  for (int i = tid; i < n; i += 1024) tw[i] = a.tw[i];
+
+  // ---- load positions t + 64 k ----
+  float2 u[N1];
+  float ppacc = 0.f;
+  const bool pad = a.pad_in != 0;
+#pragma unroll
+  for (int k = 0; k < N1; ++k) {
+    int p = t + 64 * k;
+    if (pad) {
+      p += a.nin / 2;
+      if (p >= n) p -= n;
+    }
+    float2 x = make_float2(0.f, 0.f);
+    if (live && p < a.nin) {
+      if (LD == 1 || LD == 4) {
+        const int64_t i = p + (int64_t)a.N0 * rest;
+        float2 xv;
+        if (a.pold) {   // p_new = r + beta p_old (CGLS.hpp:55-56), fused
+          const float be = (float)a.cgst[f].beta;
+          const float2 rr = a.r[f * a.Ntot + i], po = a.pold[f * a.Ntot + i];
+          xv = make_float2(rr.x + be * po.x, rr.y + be * po.y);
+          if (qq == 0) {
+            a.pnew[f * a.Ntot + i] = xv;
+            ppacc += xv.x * xv.x + xv.y * xv.y;
+          }
+        } else {
+          xv = a.x[f * a.Ntot + i];
+        }
+        x = cmul2(xv, a.S[i * a.C + qq]);
+        if (LD == 1) {
+          const float dv = deapod_r(a, p, rest);
+          x = make_float2(x.x * dv, x.y * dv);
+        }
+      } else {
+        x = a.in[(o * a.nin + p) * a.inner + qq];
+        if (LD >= 2) {
+          float dv = deapod_r(a, p, rest);
+          if (LD == 3 && !inside_mask_r(a, p, rest)) dv = 0.f;
+          x = make_float2(x.x * dv, x.y * dv);
+        }
+      }
+    }
+    u[k] = x;
+  }
+  if ((LD == 1 || LD == 4) && a.pp_part && q0 == 0) {   // block-uniform
+    const double r = block_sum_1024(ppacc, red);
+    if (tid == 0) a.pp_part[f * a.outer_per_frame + rest] = r;
+  }
+  __syncthreads();   // twiddle table
+
+  // stage 1: DFT over k (t = lo + 8 hi), twiddle W_n^(t k1)
+  if (N1 > 1) dftN<N1>(u, s);
+#pragma unroll
+  for (int k1 = 1; k1 < N1; ++k1) u[k1] = cmul2(u[k1], twid(tw, t * k1, s));
+  // exchange 1: element (k1, lo, hi) -> row 8 k1 + lo, column hi
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) ex[(8 * k1 + lo) * kR8Row + hi * 16 + q] = u[k1];
+  __syncthreads();
+  const bool act = t < ACT;   // thread t = 8 k1 + j1 takes row t: values over j2
+  float2 v[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) v[c] = act ? ex[t * kR8Row + c * 16 + q] : make_float2(0.f, 0.f);
+  // stage 2: DFT over j2 -> m1, twiddle W_64^(j1 m1) = W_n^(N1 j1 m1)
+  dft8(v, s);
+#pragma unroll
+  for (int m1 = 1; m1 < 8; ++m1) v[m1] = cmul2(v[m1], twid(tw, N1 * lo * m1, s));
+  __syncthreads();
+  // exchange 2: element (k1 = hi, m1, j1 = lo) -> row 8 k1 + m1, column j1
+  if (act) {
+#pragma unroll
+    for (int m1 = 0; m1 < 8; ++m1) ex[(8 * hi + m1) * kR8Row + lo * 16 + q] = v[m1];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 8; ++c) v[c] = act ? ex[t * kR8Row + c * 16 + q] : make_float2(0.f, 0.f);
+  // stage 3: DFT over j1 -> m2; thread t = 8 k1 + m1 holds bins k1 + N1 m1 + 8 N1 m2
+  dft8(v, s);
+  const int binbase = hi + N1 * lo;
+
+  if (CONV) {
+    const float* th = a.That + (f * a.conv_cells + qq / a.C) * n;
+#pragma unroll
+    for (int m2 = 0; m2 < 8; ++m2) {
+      const float w = (live && act) ? th[binbase + 8 * N1 * m2] : 0.f;
+      v[m2] = make_float2(v[m2].x * w, v[m2].y * w);
+    }
+    const float si = -s;
+    // the inverse runs the stages backwards with conjugate twiddles
+    dft8(v, si);                                            // over m2 -> j1
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int j1 = 0; j1 < 8; ++j1) ex[t * kR8Row + j1 * 16 + q] = v[j1];
+    }
+    __syncthreads();
+    // thread t = 8 k1 + j1 gathers over m1
+#pragma unroll
+    for (int m1 = 0; m1 < 8; ++m1) v[m1] = act ? ex[(8 * hi + m1) * kR8Row + lo * 16 + q] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int m1 = 1; m1 < 8; ++m1) v[m1] = cmul2(v[m1], twid(tw, N1 * lo * m1, si));
+    dft8(v, si);                                            // over m1 -> j2
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int j2 = 0; j2 < 8; ++j2) ex[t * kR8Row + j2 * 16 + q] = v[j2];
+    }
+    __syncthreads();
+    // thread t = j1 + 8 j2 (lo = j1, hi = j2) gathers over k1
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) u[k1] = ex[(8 * k1 + lo) * kR8Row + hi * 16 + q];
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) u[k1] = cmul2(u[k1], twid(tw, t * k1, si));
+    if (N1 > 1) dftN<N1>(u, si);                            // over k1 -> k: point t + 64 k
+    const bool crop = a.crop_out != 0;
+#pragma unroll
+    for (int k = 0; k < N1; ++k) {
+      int p = t + 64 * k;
+      if (crop) {
+        p += a.nout / 2;
+        if (p >= n) p -= n;
+      }
+      if (live && p < a.nout) a.out[(o * a.nout + p) * a.inner + qq] = u[k];
+    }
+    return;
+  }
+
+  const bool crop = a.crop_out != 0;
+  if (ST == 2) {
+    const int64_t g = q0 / kRegLines;
+    float2* outp = a.out + (f * nib + g) * a.Ntot + (int64_t)a.N0 * rest;
+    const float2* pd = a.pdot ? a.pdot + f * a.Ntot + (int64_t)a.N0 * rest : nullptr;
+    double pqacc = 0.0;
+#pragma unroll
+    for (int m2 = 0; m2 < 8; ++m2) {
+      int p = binbase + 8 * N1 * m2 + a.nout / 2;
+      if (p >= n) p -= n;
+      float2 c = make_float2(0.f, 0.f);
+      if (live && act && p < a.nout) {
+        const float2 sv = a.S[(p + (int64_t)a.N0 * rest) * a.C + qq];
+        c = make_float2(sv.x * v[m2].x + sv.y * v[m2].y, sv.x * v[m2].y - sv.y * v[m2].x);
+      }
+#pragma unroll
+      for (int m = 8; m > 0; m >>= 1) {
+        c.x += __shfl_xor(c.x, m);
+        c.y += __shfl_xor(c.y, m);
+      }
+      if (q == 0 && act && p < a.nout) {
+        outp[p] = c;
+        if (pd) {
+          const float2 pv = pd[p];
+          pqacc += (double)pv.x * c.x + (double)pv.y * c.y;
+        }
+      }
+    }
+    if (a.pq_part) {   // block-uniform
+      const double r = block_sum_1024(pqacc, red);
+      if (tid == 0) a.pq_part[(f * nib + g) * a.outer_per_frame + rest] = r;
+    }
+    return;
+  }
+#pragma unroll
+  for (int m2 = 0; m2 < 8; ++m2) {
+    int p = binbase + 8 * N1 * m2;
+    if (crop) {
+      p += a.nout / 2;
+      if (p >= n) p -= n;
+    }
+    if (live && act && p < a.nout) {
+      float2 val = v[m2];
+      if (ST == 1) {
+        const float dv = deapod_r(a, p, rest);
+        val = make_float2(val.x * dv, val.y * dv);
+      }
+      a.out[(o * a.nout + p) * a.inner + qq] = val;
+    }
+  }
+}
+
+// lines of 64 N1 points on the 64-threads-per-line kernel unless CGS_FFT_R8=0
+static bool fft_r8_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("CGS_FFT_R8");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return on;
+}
+
 bool fft_reg_supported(const FftArgs& a, int ld_mode, int st_mode) {
-  if (!(a.n == 256 || a.n == 512)) return false;
+  const bool r8 = fft_r8_enabled() && (a.n == 128 || a.n == 192 || a.n == 384);
+  if (!(a.n == 256 || a.n == 512 || r8)) return false;
   if (a.inner < kRegLines) return false;
   if (st_mode == 2 && a.That) return false;
   (void)ld_mode;
@@ -412,6 +699,32 @@ void launch_fft_reg(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s) {
   const dim3 grid((unsigned)(nib * a.outers));
   const int conv = a.That != nullptr;
   const bool full = fft_reg_full_exchange();
+  // the Toeplitz convolution pass (forward, multiply, inverse) stays on the
+  // 16-threads-per-line kernel at n = 256 / 512, where it measured faster
+  const bool r8 = fft_r8_enabled() && (a.n == 512 || a.n == 384 || a.n == 256 || a.n == 192 || a.n == 128) &&
+                  !(conv && (a.n == 256 || a.n == 512));
+  if (r8) {
+#define CGS_FFT8_CASE(L, S_, CV)                                                        \
+    if (ld_mode == L && st_mode == S_ && conv == CV) {                                  \
+      switch (a.n) {                                                                    \
+        case 512: hipLaunchKernelGGL((k_fft_r8<8, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
+        case 384: hipLaunchKernelGGL((k_fft_r8<6, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
+        case 256: hipLaunchKernelGGL((k_fft_r8<4, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
+        case 192: hipLaunchKernelGGL((k_fft_r8<3, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
+        default:  hipLaunchKernelGGL((k_fft_r8<2, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
+      }                                                                                 \
+      return;                                                                           \
+    }
+    CGS_FFT8_CASE(0, 0, 0)
+    CGS_FFT8_CASE(1, 0, 0)
+    CGS_FFT8_CASE(2, 0, 0)
+    CGS_FFT8_CASE(3, 0, 0)
+    CGS_FFT8_CASE(4, 0, 0)
+    CGS_FFT8_CASE(0, 1, 0)
+    CGS_FFT8_CASE(0, 0, 1)
+    CGS_FFT8_CASE(0, 2, 0)
+#undef CGS_FFT8_CASE
+  }
 #define CGS_FFTR_CASE(L, S_, CV)                                                      \
   if (ld_mode == L && st_mode == S_ && conv == CV) {                                  \
     if (a.n == 512 && full)                                                           \

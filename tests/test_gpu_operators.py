@@ -337,12 +337,13 @@ def test_autotune_keeps_results(cgs, ora):
     assert rel(Q.EH(y), x1) <= 1e-6
 
 
-@pytest.mark.parametrize("coils", [16, 32, 8])
-def test_E_EH_oversampled_256_lines(cgs, ora, coils):
-    """n = 256 lines (128^2 at sigma 2): the register-resident FFT passes
-    (16 lines of >= 16 coils per block) and, for 8 coils, the LDS-staged ones
-    on axis 0 -- E, E^H and the Toeplitz normal operator against the oracle."""
-    prob, y, P, O = _c1_problem(cgs, ora, N=128, coils=coils, spokes=96, samples=256)
+@pytest.mark.parametrize("N,coils", [(128, 16), (128, 32), (128, 8), (64, 16), (96, 16), (192, 16)])
+def test_E_EH_oversampled_lines(cgs, ora, N, coils):
+    """Lines of n = 2N = 128, 192, 256, 384 points (sigma 2; the Toeplitz grid
+    is 2N too): the register-resident FFT passes (16 lines of >= 16 coils per
+    block) and, for 8 coils, the LDS-staged ones on axis 0 -- E, E^H, the
+    Toeplitz normal operator and Toeplitz CG against the oracle."""
+    prob, y, P, O = _c1_problem(cgs, ora, N=N, coils=coils, spokes=max(48, 3 * N // 4), samples=2 * N)
     e1 = rel(P.E(prob.image), O.E(prob.maps, prob.image))
     e2 = rel(P.EH(y), O.EH_true(prob.maps, prob.w, y))
     q = O.EH_true(prob.maps, prob.w, O.E(prob.maps, prob.image)) + 0.25 * prob.image
@@ -354,7 +355,8 @@ def test_E_EH_oversampled_256_lines(cgs, ora, coils):
     xa, ra = P.solve(y, maxit=10, tol=0.0, lam=0.25, toeplitz=True)
     xb, rb = P.solve(y, maxit=10, tol=0.0, lam=0.25, toeplitz=True, graph_iters=4)
     e5, e6 = rel(xa, xo), rel(xb, xa)
-    print(f"C={coils}: E {e1:.2e} E^H {e2:.2e} toeplitz {e3:.2e} nufft normal {e4:.2e} CG(10) {e5:.2e} graph {e6:.1e}")
+    print(f"N={N} C={coils}: E {e1:.2e} E^H {e2:.2e} toeplitz {e3:.2e} nufft normal {e4:.2e} CG(10) {e5:.2e} "
+          f"graph {e6:.1e}")
     assert e1 <= OP_TOL and e2 <= OP_TOL and e3 <= 2e-5 and e4 <= 2e-5
     assert ra["iters"][0] == rb["iters"][0] == it == 10 and e5 <= 1e-4 and e6 <= 1e-6
 
