@@ -11,6 +11,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 #include <string>
@@ -216,6 +217,27 @@ struct Plan {
 };
 
 // ---- launchers (kernels_*.hip) ----------------------------------------------
+// Kernel-timing hand-off: when set, the next FFT launch is issued with
+// hipExtLaunchKernelGGL and these start/stop events (timestamps of the dispatch
+// itself, no marker packets around it), and `consumed` is set.
+struct LaunchEvents {
+  hipEvent_t a = nullptr, b = nullptr;
+  bool consumed = false;
+};
+LaunchEvents& launch_events();
+
+#define CGS_LAUNCH(kernel, grid, block, shm, stream, ...)                                      \
+  do {                                                                                          \
+    ::cgs::LaunchEvents& le_ = ::cgs::launch_events();                                          \
+    if (le_.a) {                                                                                \
+      hipExtLaunchKernelGGL(kernel, grid, block, shm, stream, le_.a, le_.b, 0, __VA_ARGS__);    \
+      le_.a = le_.b = nullptr;                                                                  \
+      le_.consumed = true;                                                                      \
+    } else {                                                                                    \
+      hipLaunchKernelGGL(kernel, grid, block, shm, stream, __VA_ARGS__);                        \
+    }                                                                                           \
+  } while (0)
+
 void launch_fft(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s);
 bool fft_reg_supported(const FftArgs& a, int ld_mode, int st_mode);
 void launch_fft_reg(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s);
@@ -228,6 +250,7 @@ void launch_interp_lds(const SampArgs& a, const int4* items, int nitems, int T, 
 size_t grid_batched_lds(int d, int T, int taps, int CG, int rowstride);
 void launch_grid_atomic(const GridArgs& a, const int4* vrec, int cg, int nitems, int ncgroups, hipStream_t s);
 void launch_grid_reg(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s);
+void launch_grid_reg3(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s);
 size_t interp_lds_bytes(int T, int taps, int CG, int rowstride, int maxs);
 void launch_cg_tail(const Plan& p, float2* x, float2* r, const float2* pv, const float2* part, int G,
                     float lambda, const double* pq_part, int npq, const double* pp_part, int npp, hipStream_t s);

@@ -341,9 +341,9 @@ void launch_fft(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s) {
 #define CGS_FFT_CASE(L, S_, CV)                                                          \
   if (ld_mode == L && st_mode == S_ && conv == CV) {                                     \
     if (a.n <= 512)                                                                      \
-      hipLaunchKernelGGL((k_fft<L, S_, 512, CV>), grid, dim3(kFftThreads), shm, s, a);   \
+      CGS_LAUNCH((k_fft<L, S_, 512, CV>), grid, dim3(kFftThreads), shm, s, a);           \
     else                                                                                 \
-      hipLaunchKernelGGL((k_fft<L, S_, 4096, CV>), grid, dim3(kFftThreads), shm, s, a);  \
+      CGS_LAUNCH((k_fft<L, S_, 4096, CV>), grid, dim3(kFftThreads), shm, s, a);          \
     return;                                                                              \
   }
   CGS_FFT_CASE(0, 0, 0)

@@ -707,11 +707,11 @@ void launch_fft_reg(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s) {
 #define CGS_FFT8_CASE(L, S_, CV)                                                        \
     if (ld_mode == L && st_mode == S_ && conv == CV) {                                  \
       switch (a.n) {                                                                    \
-        case 512: hipLaunchKernelGGL((k_fft_r8<8, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
-        case 384: hipLaunchKernelGGL((k_fft_r8<6, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
-        case 256: hipLaunchKernelGGL((k_fft_r8<4, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
-        case 192: hipLaunchKernelGGL((k_fft_r8<3, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
-        default:  hipLaunchKernelGGL((k_fft_r8<2, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
+        case 512: CGS_LAUNCH((k_fft_r8<8, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
+        case 384: CGS_LAUNCH((k_fft_r8<6, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
+        case 256: CGS_LAUNCH((k_fft_r8<4, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
+        case 192: CGS_LAUNCH((k_fft_r8<3, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
+        default:  CGS_LAUNCH((k_fft_r8<2, L, S_, CV>), grid, dim3(1024), 0, s, a); break; \
       }                                                                                 \
       return;                                                                           \
     }
@@ -728,11 +728,11 @@ void launch_fft_reg(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s) {
 #define CGS_FFTR_CASE(L, S_, CV)                                                      \
   if (ld_mode == L && st_mode == S_ && conv == CV) {                                  \
     if (a.n == 512 && full)                                                           \
-      hipLaunchKernelGGL((k_fft_reg<32, L, S_, CV, 32>), grid, dim3(256), 0, s, a);  \
+      CGS_LAUNCH((k_fft_reg<32, L, S_, CV, 32>), grid, dim3(256), 0, s, a);  \
     else if (a.n == 512)                                                              \
-      hipLaunchKernelGGL((k_fft_reg<32, L, S_, CV, 16>), grid, dim3(256), 0, s, a);  \
+      CGS_LAUNCH((k_fft_reg<32, L, S_, CV, 16>), grid, dim3(256), 0, s, a);  \
     else                                                                              \
-      hipLaunchKernelGGL((k_fft_reg<16, L, S_, CV, 16>), grid, dim3(256), 0, s, a);  \
+      CGS_LAUNCH((k_fft_reg<16, L, S_, CV, 16>), grid, dim3(256), 0, s, a);  \
     return;                                                                           \
   }
   CGS_FFTR_CASE(0, 0, 0)

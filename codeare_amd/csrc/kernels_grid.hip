@@ -641,6 +641,97 @@ __global__ void __launch_bounds__(256) k_grid_reg(GridArgs a, const int4* __rest
   }
 }
 
+// 3D: one block of 8 waves per work item, wave w owning plane w of the
+// 8 x 8 x 8 tile (lane = cell of the plane, CGR coil accumulators in VGPRs).
+// Visit batches (records, window values, weighted data) are staged once per
+// block in LDS; a wave skips the visits whose footprint misses its plane.
+template <int CGR, int TAPS>
+__global__ void __launch_bounds__(512) k_grid_reg3(GridArgs a, const int4* __restrict__ vrec, int nitems) {
+  constexpr int B = 64, NPSI = 3 * TAPS;
+  __shared__ float2 sy[B * CGR];
+  __shared__ float spsi[B * NPSI];
+  __shared__ int4 srec[B];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int item = blockIdx.x;
+  const GridItem it = a.items[item];
+  if (a.fdone && a.fdone[it.frame]) return;   // block-uniform
+  const int cbase = blockIdx.y * CGR;
+  const int t0 = it.tile % a.nt0, t1 = (it.tile / a.nt0) % a.nt1, t2 = it.tile / (a.nt0 * a.nt1);
+  const int ts0 = t0 * 8, ts1 = t1 * 8, ts2 = t2 * 8;
+  const int te0 = min(8, a.n0 - ts0), te1 = min(8, a.n1 - ts1), te2 = min(8, a.n2 - ts2);
+  const int x0 = lane & 7, x1 = lane >> 3;
+  float acc[2 * CGR];
+#pragma unroll
+  for (int k = 0; k < 2 * CGR; ++k) acc[k] = 0.f;
+
+  for (int vb = it.vbeg; vb < it.vend; vb += B) {
+    const int nb = min(B, it.vend - vb);
+    __syncthreads();   // the previous batch is consumed
+    if (tid < nb) srec[tid] = vrec[vb + tid];
+    __syncthreads();
+    for (int idx = tid; idx < nb * NPSI; idx += 512) {
+      const int v = idx / NPSI, q = idx - v * NPSI;
+      spsi[idx] = a.psi[(int64_t)srec[v].x * NPSI + q];
+    }
+    for (int idx = tid; idx < nb * CGR; idx += 512) {
+      const int v = idx / CGR, c = idx - v * CGR;
+      const int smp = srec[v].x;
+      float2 yv = make_float2(0.f, 0.f);
+      if (cbase + c < a.C) {
+        yv = a.y[(int64_t)smp * a.C + cbase + c];
+        if (a.w) { const float ww = a.w[smp]; yv.x *= ww; yv.y *= ww; }
+      }
+      sy[idx] = yv;
+    }
+    __syncthreads();
+    for (int v = 0; v < nb; ++v) {
+      const int4 r = srec[v];
+      const int dz = w - (int)(short)(r.w & 0xffff);
+      if (dz < 0 || dz >= TAPS) continue;          // uniform per wave
+      const int dx = x0 - (int)(short)(r.y & 0xffff), dy = x1 - (int)(short)(r.y >> 16);
+      float wgt = 0.f;
+      if (dx >= 0 && dx < TAPS && dy >= 0 && dy < TAPS)
+        wgt = spsi[v * NPSI + dx] * spsi[v * NPSI + TAPS + dy] * spsi[v * NPSI + 2 * TAPS + dz];
+      const float4* yy = (const float4*)(sy + v * CGR);
+#pragma unroll
+      for (int k = 0; k < CGR / 2; ++k) {
+        const float4 t = yy[k];
+        acc[4 * k + 0] += wgt * t.x;
+        acc[4 * k + 1] += wgt * t.y;
+        acc[4 * k + 2] += wgt * t.z;
+        acc[4 * k + 3] += wgt * t.w;
+      }
+    }
+  }
+
+  if (x0 < te0 && x1 < te1 && w < te2) {
+    float2* out;
+    if (it.slot < 0) {
+      const int64_t l = (ts0 + x0) + (int64_t)a.n0 * ((ts1 + x1) + (int64_t)a.n1 * (ts2 + w));
+      out = a.grid + ((int64_t)it.frame * a.ntot + l) * a.C + cbase;
+    } else {
+      out = a.partial + ((int64_t)it.slot * a.tilecells + x0 + 8 * (x1 + 8 * w)) * a.C + cbase;
+    }
+#pragma unroll
+    for (int c = 0; c < CGR; ++c)
+      if (cbase + c < a.C) out[c] = make_float2(acc[2 * c], acc[2 * c + 1]);
+  }
+}
+
+void launch_grid_reg3(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s) {
+  if (nitems == 0) return;
+  const dim3 g((unsigned)nitems, (unsigned)ncgroups);
+#define CGS_GR3(CG, TP) hipLaunchKernelGGL((k_grid_reg3<CG, TP>), g, dim3(512), 0, s, a, vrec, nitems)
+  if (cgr == 8) {
+    switch (a.taps) { case 4: CGS_GR3(8, 4); break; case 6: CGS_GR3(8, 6); break;
+                      case 8: CGS_GR3(8, 8); break; case 10: CGS_GR3(8, 10); break; default: break; }
+  } else {
+    switch (a.taps) { case 4: CGS_GR3(16, 4); break; case 6: CGS_GR3(16, 6); break;
+                      case 8: CGS_GR3(16, 8); break; case 10: CGS_GR3(16, 10); break; default: break; }
+  }
+#undef CGS_GR3
+}
+
 template <int CGR>
 static void grid_reg_taps(const GridArgs& a, const int4* vrec, int nitems, dim3 g, size_t shm, hipStream_t s) {
   switch (a.taps) {

@@ -31,6 +31,11 @@ namespace cgs {
 static thread_local std::string g_err;
 void set_error(const std::string& m) { g_err = m; }
 
+LaunchEvents& launch_events() {
+  static thread_local LaunchEvents le;
+  return le;
+}
+
 #define HIPCHK(x)                                                                    \
   do {                                                                               \
     hipError_t e_ = (x);                                                             \
@@ -128,9 +133,16 @@ struct Prof {
   int kind;
   double bytes;
   hipEvent_t a = nullptr, b = nullptr;
-  bool timed = false;
+  bool timed = false, ext = false;
   // no timing events while a HIP graph is being captured; classes outside
-  // prof_mask are counted (launches, bytes) but not timed
+  // prof_mask are counted (launches, bytes) but not timed.  FFT passes (one
+  // launch per scope) hand the events to the launch itself
+  // (hipExtLaunchKernelGGL), which timestamps the dispatch; other scopes
+  // bracket their launches with event records.
+  static bool fft_kind(int k) {
+    return k == CGS_K_FFT_FWD || k == CGS_K_FFT_ADJ || k == CGS_K_TOEP_PAD || k == CGS_K_TOEP_CONV ||
+           k == CGS_K_TOEP_CROP;
+  }
   Prof(Plan& p_, int kind_, double bytes_) : p(p_), kind(kind_), bytes(bytes_) {
     if (!(p.flags & CGS_PROFILE) || p.capturing) return;
     if (!((p.prof_mask >> kind) & 1u)) {
@@ -141,11 +153,28 @@ struct Prof {
     timed = true;
     a = take();
     b = take();
-    (void)hipEventRecord(a, p.stream);
+    if (fft_kind(kind)) {
+      ext = true;
+      LaunchEvents& le = launch_events();
+      le.a = a;
+      le.b = b;
+      le.consumed = false;
+    } else {
+      (void)hipEventRecord(a, p.stream);
+    }
   }
   ~Prof() {
     if (!timed) return;
-    (void)hipEventRecord(b, p.stream);
+    if (ext) {
+      LaunchEvents& le = launch_events();
+      if (!le.consumed) {   // no FFT launch happened in the scope
+        (void)hipEventRecord(a, p.stream);
+        (void)hipEventRecord(b, p.stream);
+      }
+      le = LaunchEvents{};
+    } else {
+      (void)hipEventRecord(b, p.stream);
+    }
     p.pending.push_back({a, b, kind, bytes});
   }
   hipEvent_t take() {
@@ -402,6 +431,9 @@ static int gridding(Plan& p, const float2* y, bool weighted, const int* fl) {
   if (p.d == 2 && p.T == 8 && p.use_grid_reg) {
     const int cgr = p.grid_cgr;
     launch_grid_reg(a, p.vrec, cgr, p.n_grid_items, (p.C + cgr - 1) / cgr, p.stream);
+  } else if (p.d == 3 && p.T == 8 && p.use_grid_reg) {
+    const int cgr = std::min(p.grid_cgr, 16);
+    launch_grid_reg3(a, p.vrec, cgr, p.n_grid_items, (p.C + cgr - 1) / cgr, p.stream);
   } else if (batched && p.use_grid_atomic)
     launch_grid_atomic(a, p.vrec, p.grid_cg, p.n_grid_items, ngroups, p.stream);
   else if (batched)
