@@ -125,7 +125,7 @@ struct Plan {
   int grid_cg = 8, grid_w = 4;        // gridding coil group / waves per block
   int interp_cg = 32;
   int visits_per_item = 256;
-  bool use_interp_lds = true;         // 2D: gather from the tile staged in LDS
+  bool use_interp_lds = true;         // gather from the tile staged in LDS
   bool use_grid_batched = true;       // batched visit lists (else one visit at a time)
   bool use_grid_reg = true;           // 2D: register accumulation, lane = cell of an 8x8 tile
   int grid_cgr = 16;                  // coils per wave in the register kernel
@@ -250,6 +250,9 @@ void launch_interp_lds(const SampArgs& a, const int4* items, int nitems, int T, 
 size_t grid_batched_lds(int d, int T, int taps, int CG, int rowstride);
 void launch_grid_atomic(const GridArgs& a, const int4* vrec, int cg, int nitems, int ncgroups, hipStream_t s);
 void launch_grid_reg(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s);
+size_t interp_lds3_bytes(int T, int taps, int cg, int maxs);
+void launch_interp_lds3(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int nt1, int cg, int maxs,
+                        hipStream_t s);
 void launch_grid_reg3(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s);
 size_t interp_lds_bytes(int T, int taps, int CG, int rowstride, int maxs);
 void launch_cg_tail(const Plan& p, float2* x, float2* r, const float2* pv, const float2* part, int G,

@@ -176,19 +176,32 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs a) {
 __global__ void k_grid_reduce(const ReduceItem* items, const float2* partial, float2* grid,
                               int C, int T, int d, int n0, int n1, int n2, int nt0, int nt1,
                               int64_t ntot, int64_t tilecells, const int* fdone) {
+  // blockIdx.y splits the tile's (cell, coil) outputs so heavy tiles (the
+  // k-space centre: thousands of slots in 3D) spread over many blocks
   const ReduceItem it = items[blockIdx.x];
   if (fdone && fdone[it.frame]) return;
   const int t0 = it.tile % nt0, t1 = (it.tile / nt0) % nt1, t2 = (d == 3) ? it.tile / (nt0 * nt1) : 0;
   const int ts0 = t0 * T, ts1 = t1 * T, ts2 = t2 * T;
   const int te0 = min(T, n0 - ts0), te1 = min(T, n1 - ts1), te2 = (d == 3) ? min(T, n2 - ts2) : 1;
   const int ncell = te0 * te1 * te2;
-  for (int idx = threadIdx.x; idx < ncell * C; idx += blockDim.x) {
+  for (int idx = blockIdx.y * blockDim.x + threadIdx.x; idx < ncell * C; idx += gridDim.y * blockDim.x) {
     const int c = idx % C, cell = idx / C;
     const int x0 = cell % te0, x1 = (cell / te0) % te1, x2 = cell / (te0 * te1);
     const int64_t lc = x0 + (int64_t)T * (x1 + (int64_t)T * x2);
+    const float2* src = partial + ((int64_t)it.slot0 * tilecells + lc) * C + c;
+    const int64_t stride = tilecells * C;
     float2 acc = make_float2(0.f, 0.f);
-    for (int k = 0; k < it.nslots; ++k) {
-      const float2 v = partial[((int64_t)(it.slot0 + k) * tilecells + lc) * C + c];
+    int k = 0;
+    for (; k + 4 <= it.nslots; k += 4) {   // four loads in flight, summed in slot order
+      const float2 v0 = src[(k + 0) * stride], v1 = src[(k + 1) * stride];
+      const float2 v2 = src[(k + 2) * stride], v3 = src[(k + 3) * stride];
+      acc.x += v0.x; acc.y += v0.y;
+      acc.x += v1.x; acc.y += v1.y;
+      acc.x += v2.x; acc.y += v2.y;
+      acc.x += v3.x; acc.y += v3.y;
+    }
+    for (; k < it.nslots; ++k) {
+      const float2 v = src[k * stride];
       acc.x += v.x;
       acc.y += v.y;
     }
@@ -846,6 +859,122 @@ void launch_interp_lds(const SampArgs& a, const int4* items, int nitems, int T, 
 }
 
 // ---------------------------------------------------------------------------
+// forward B in 3D with the tile's extended region ((T + taps - 1)^3 cells x CG
+// coils) staged in LDS.  Work item = (frame, tile, samples whose floor(n x)
+// lies in the tile); lanes = (coil, tap row (s1, s2)), a shuffle reduction
+// over the rows.
+template <int CG, int TAPS>
+__global__ void __launch_bounds__(256) k_interp_lds3(SampArgs a, const int4* items, int T, int maxs, int nt0,
+                                                     int nt1) {
+  constexpr int RS = 64 / CG;
+  extern __shared__ float2 sh[];
+  const int4 it = items[blockIdx.x];            // frame, tile, sbeg, send
+  if (a.fdone && a.fdone[it.x]) return;
+  const int E = T + TAPS - 1;
+  const int m = TAPS / 2 - 1;
+  const int E3 = E * E * E;
+  float2* ext = sh;                                     // [E][E][E][CG]
+  float* wpsi = (float*)(sh + (size_t)E3 * CG);         // [maxs][3*TAPS]
+  int* wrel = (int*)(wpsi + (size_t)maxs * 3 * TAPS);   // [maxs][3]
+  const int cbase = blockIdx.y * CG;
+  const int t0 = it.y % nt0, t1 = (it.y / nt0) % nt1, t2 = it.y / (nt0 * nt1);
+  const int org0 = t0 * T - m, org1 = t1 * T - m, org2 = t2 * T - m;
+  const int64_t gbase = (int64_t)it.x * a.ntot;
+  {
+    constexpr int U = 8, STEP = 256 / CG;
+    const int c = threadIdx.x % CG;
+    const bool cok = cbase + c < a.C;
+    for (int e00 = threadIdx.x / CG; e00 < E3; e00 += STEP * U) {
+      float2 v[U];
+      int dst[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e00 + u * STEP;
+        v[u] = make_float2(0.f, 0.f);
+        dst[u] = -1;
+        if (e < E3) {
+          const int e2 = e / (E * E), r = e - e2 * E * E, e1 = r / E, e0 = r - e1 * E;
+          int l0 = org0 + e0, l1 = org1 + e1, l2 = org2 + e2;   // one wrap suffices
+          if (l0 < 0) l0 += a.n0; else if (l0 >= a.n0) l0 -= a.n0;
+          if (l1 < 0) l1 += a.n1; else if (l1 >= a.n1) l1 -= a.n1;
+          if (l2 < 0) l2 += a.n2; else if (l2 >= a.n2) l2 -= a.n2;
+          if (cok) v[u] = a.grid[(gbase + l0 + (int64_t)a.n0 * (l1 + (int64_t)a.n1 * l2)) * a.C + cbase + c];
+          dst[u] = e * CG + c;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (dst[u] >= 0) ext[dst[u]] = v[u];
+    }
+  }
+  const int ns = it.w - it.z;
+  for (int idx = threadIdx.x; idx < ns * 3 * TAPS; idx += blockDim.x)
+    wpsi[idx] = a.psi[(int64_t)it.z * 3 * TAPS + idx];
+  for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+    const int smp = it.z + k;
+    int r0 = a.base[3 * smp] - org0, r1 = a.base[3 * smp + 1] - org1, r2 = a.base[3 * smp + 2] - org2;
+    if (r0 < 0) r0 += a.n0; else if (r0 >= a.n0) r0 -= a.n0;
+    if (r1 < 0) r1 += a.n1; else if (r1 >= a.n1) r1 -= a.n1;
+    if (r2 < 0) r2 += a.n2; else if (r2 >= a.n2) r2 -= a.n2;
+    wrel[3 * k] = r0;
+    wrel[3 * k + 1] = r1;
+    wrel[3 * k + 2] = r2;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cl = lane % CG, rs = lane / CG;
+  const int c = cbase + cl;
+  for (int k = wave; k < ns; k += 4) {
+    const float* ps = wpsi + k * 3 * TAPS;
+    const int r0 = wrel[3 * k], r1 = wrel[3 * k + 1], r2 = wrel[3 * k + 2];
+    float2 acc = make_float2(0.f, 0.f);
+    for (int row = rs; row < TAPS * TAPS; row += RS) {
+      const int s1 = row % TAPS, s2 = row / TAPS;
+      const float2* rp = ext + ((size_t)((r2 + s2) * E + (r1 + s1)) * E + r0) * CG + cl;
+      float2 racc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int s0 = 0; s0 < TAPS; ++s0) {
+        const float2 v = rp[s0 * CG];
+        const float p0 = ps[s0];
+        racc.x += p0 * v.x;
+        racc.y += p0 * v.y;
+      }
+      const float wr = ps[TAPS + s1] * ps[2 * TAPS + s2];
+      acc.x += wr * racc.x;
+      acc.y += wr * racc.y;
+    }
+#pragma unroll
+    for (int msk = CG; msk < 64; msk <<= 1) {
+      const float2 o = shfl_xor2(acc, msk);
+      acc.x += o.x;
+      acc.y += o.y;
+    }
+    if (rs == 0 && c < a.C) a.y[(int64_t)(it.z + k) * a.C + c] = acc;
+  }
+}
+
+size_t interp_lds3_bytes(int T, int taps, int cg, int maxs) {
+  const size_t E = (size_t)T + taps - 1;
+  return sizeof(float2) * E * E * E * cg + sizeof(float) * (size_t)maxs * 3 * taps + sizeof(int) * (size_t)maxs * 3;
+}
+
+void launch_interp_lds3(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int nt1, int cg, int maxs,
+                        hipStream_t s) {
+  if (nitems == 0) return;
+  const size_t shm = interp_lds3_bytes(T, a.taps, cg, maxs);
+  const dim3 g((unsigned)nitems, (unsigned)((a.C + cg - 1) / cg));
+#define CGS_IL3(CG_, TP) hipLaunchKernelGGL((k_interp_lds3<CG_, TP>), g, dim3(256), shm, s, a, items, T, maxs, nt0, nt1)
+  if (cg == 2) {
+    switch (a.taps) { case 4: CGS_IL3(2, 4); break; case 6: CGS_IL3(2, 6); break;
+                      case 8: CGS_IL3(2, 8); break; case 10: CGS_IL3(2, 10); break; default: break; }
+  } else {
+    switch (a.taps) { case 4: CGS_IL3(4, 4); break; case 6: CGS_IL3(4, 6); break;
+                      case 8: CGS_IL3(4, 8); break; case 10: CGS_IL3(4, 10); break; default: break; }
+  }
+#undef CGS_IL3
+}
+
+// ---------------------------------------------------------------------------
 template <int CG, int D>
 static void interp_taps(const SampArgs& a, dim3 g, hipStream_t s) {
   switch (a.taps) {
@@ -901,7 +1030,9 @@ void launch_grid(const GridArgs& a, int cg, int waves, int nitems, int ncgroups,
 void launch_grid_reduce(const Plan& p, const float2* partial, float2* grid, const int* fl, hipStream_t s) {
   if (p.n_reduce_items == 0) return;
   const int64_t tilecells = (int64_t)p.T * p.T * (p.d == 3 ? p.T : 1);
-  hipLaunchKernelGGL(k_grid_reduce, dim3((unsigned)p.n_reduce_items), dim3(256), 0, s,
+  const int64_t outs = tilecells * p.C;
+  const unsigned gy = (unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (outs + 255) / 256));
+  hipLaunchKernelGGL(k_grid_reduce, dim3((unsigned)p.n_reduce_items, gy), dim3(256), 0, s,
                      p.reduce_items, partial, grid, p.C, p.T, p.d, p.n[0], p.n[1], p.n[2],
                      p.nt[0], p.nt[1], p.ntot, tilecells, fl);
 }

@@ -405,6 +405,10 @@ static int interp(Plan& p, float2* y, const int* fl) {
   if (p.d == 2 && p.use_interp_lds) {
     const int ng = (p.C + p.interp_cg - 1) / p.interp_cg;
     launch_interp_lds(a, p.tile_items, p.n_tile_items, p.T, p.nt[0], p.interp_cg, p.tile_item_max, ng, p.stream);
+  } else if (p.d == 3 && p.use_interp_lds && p.T == 8 && p.taps <= 8 &&
+             interp_lds3_bytes(p.T, p.taps, 2, p.tile_item_max) <= 65536) {
+    // (8 + taps - 1)^3 cells x 2 coils staged per item (54 KB at taps 8)
+    launch_interp_lds3(a, p.tile_items, p.n_tile_items, p.T, p.nt[0], p.nt[1], 2, p.tile_item_max, p.stream);
   } else {
     launch_interp(a, p.interp_cg, p.stream);
   }
@@ -829,7 +833,9 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   p.grid_w = p.grid_cg / 8;
   p.interp_cg = (C <= 8) ? 8 : 16;
   p.use_interp_lds = (d == 2);
-  p.tile_item_max = 128;
+  // 3D: the staged (8 + taps - 1)^3 x 2-coil region plus 64 samples' window
+  // values stay under 64 KB of LDS at taps <= 8
+  p.tile_item_max = (d == 3) ? 64 : 128;
   p.visits_per_item = (d == 3) ? 128 : 512;
   p.grid_cgr = (C <= 8) ? 8 : 16;
   // shape overrides for tuning sweeps (tile side must be a multiple of 8)
