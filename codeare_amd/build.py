@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT_DIR, "libcgsense_hip.so")
-SOURCES = ["plan.hip", "kernels_fft.hip", "kernels_fft_reg.hip", "kernels_grid.hip", "kernels_vec.hip", "unet.hip"]
+SOURCES = ["plan.hip", "kernels_fft.hip", "kernels_fft_reg.hip", "kernels_toep.hip", "kernels_grid.hip", "kernels_vec.hip",
+           "unet.hip"]
 ARCH = os.environ.get("CGS_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -75,6 +75,30 @@ struct FftArgs {
   const float2* pdot;
 };
 
+// the three FFT passes of the 2D Toeplitz operator on the lean kernels
+// (kernels_toep.hip): pad (x or p = r + beta pold, times S_c, FFT along axis 0
+// of the 2N0 x N1 grid), conv (FFT along axis 1, times That, IFFT, in place),
+// crop (IFFT along axis 0, crop, sum of conj(S_c) X_c over 16 coils).  Grid
+// layout G[((f*N1 + i1)*2N0 + l0)*C + c]; image positions are not shifted.
+struct ToepArgs {
+  const float2* in;       // crop: grid
+  float2* out;            // pad: grid; conv: grid (in place); crop: partial images P[(f*G + g)*Ntot + i]
+  const float2* x;        // pad input image (when pold is null)
+  const float2* r;        // pad: p_new = r + beta pold, written to pnew by coil 0
+  const float2* pold;
+  float2* pnew;
+  const CgState* cgst;
+  const float2* S;
+  const float* That;      // conv: That[(f*2N0 + l0)*2N1 + bin]
+  const float2* tw;       // exp(-2 pi i p / n), n = 2 N of the pass axis
+  const int* fdone;
+  double* pp_part;        // pad: ||p_new||^2 per (f, i1)
+  double* pq_part;        // crop: Re<pdot, P_g> per (f, g, i1)
+  const float2* pdot;
+  int C, N0, N1, F;
+  int64_t Ntot;
+};
+
 struct SampArgs {
   const int* base;
   const float* psi;
@@ -241,6 +265,11 @@ LaunchEvents& launch_events();
 void launch_fft(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s);
 bool fft_reg_supported(const FftArgs& a, int ld_mode, int st_mode);
 void launch_fft_reg(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s);
+bool toep_pad_crop_supported(int N0, int N1, int C);
+bool toep_conv_supported(int N0, int N1, int C);
+void launch_toep_pad(const ToepArgs& a, hipStream_t s);
+void launch_toep_conv(const ToepArgs& a, hipStream_t s);
+void launch_toep_crop(const ToepArgs& a, hipStream_t s);
 void launch_interp(const SampArgs& a, int cg, hipStream_t s);
 void launch_grid(const GridArgs& a, int cg, int waves, int nitems, int ncgroups, hipStream_t s);
 void launch_grid_reduce(const Plan& p, const float2* partial, float2* grid, const int* fl, hipStream_t s);

@@ -20,44 +20,11 @@
 #include <cstdlib>
 
 #include "cgs_internal.hpp"
+#include "fft_roots.hpp"
 
 namespace cgs {
 
 namespace {
-
-constexpr double kPiD = 3.14159265358979323846264338327950288;
-
-constexpr double taylor_sin(double x) {
-  double t = x, s = x;
-  for (int i = 1; i < 16; ++i) {
-    t *= -x * x / ((2.0 * i) * (2.0 * i + 1.0));
-    s += t;
-  }
-  return s;
-}
-constexpr double taylor_cos(double x) {
-  double t = 1.0, s = 1.0;
-  for (int i = 1; i < 16; ++i) {
-    t *= -x * x / ((2.0 * i - 1.0) * (2.0 * i));
-    s += t;
-  }
-  return s;
-}
-// exp(-2 pi i t / L) for t in [0, L): angle reduced to [-pi, pi]
-template <int L>
-struct TwL {
-  float c[L], s[L];
-  constexpr TwL() : c(), s() {
-    for (int t = 0; t < L; ++t) {
-      double a = -2.0 * kPiD * t / L;
-      if (a < -kPiD) a += 2.0 * kPiD;
-      c[t] = (float)taylor_cos(a);
-      s[t] = (float)taylor_sin(a);
-    }
-  }
-};
-template <int L>
-__device__ constexpr TwL<L> kTwL = TwL<L>();
 
 __device__ __forceinline__ float2 cmul2(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);

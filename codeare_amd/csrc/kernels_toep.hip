@@ -1,0 +1,474 @@
+// kernels_toep.hip -- lean FFT passes of the 2D Toeplitz CG operator.
+//
+// Per CG iteration the Toeplitz normal operator runs three FFT passes over the
+// C x 2N0 x N1 / C x 2N0 x 2N1 grid:
+//   pad   p = r + beta p_old, S_c p, zero-pad, FFT along axis 0
+//   conv  FFT along axis 1, times the real kernel spectrum, IFFT, crop (in place)
+//   crop  IFFT along axis 0, crop, sum over 16 coils of conj(S_c) X_c
+// With 32 coils and 512-point lines the generic FFT kernels issue more vector
+// instructions (64-bit address arithmetic per point, run-time pad/crop tests,
+// per-point exec masks) than their HBM traffic takes time.  These kernels are
+// written for instruction count:
+//  * image positions are not shifted by N/2: a circular convolution commutes
+//    with a common shift of its input and output, so the operator is
+//    unchanged, and the zero half of every padded line and the discarded half
+//    of every cropped line are compile-time facts.  The first (last) DFT stage
+//    is then two half-length DFTs of the even and odd bins;
+//  * global accesses are raw buffer loads/stores whose per-point offsets are
+//    uniform (scalar registers): one VGPR offset per thread;
+//  * twiddles W_n^(a b) come from transposed LDS tables at immediate offsets,
+//    constant twiddles 1, -1 and +-i are not multiplied, and the transform
+//    sign is a template parameter.
+// Every block takes 16 lines = 16 coils of one grid row (C % 16 == 0), so each
+// global access moves 128 contiguous bytes.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdlib>
+
+#include "cgs_internal.hpp"
+#include "fft_roots.hpp"
+
+namespace cgs {
+
+namespace {
+
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+typedef unsigned int u32x2 __attribute__((__vector_size__(2 * sizeof(unsigned int))));
+
+// raw buffer descriptor over [base, base + bytes); word 3 = 0x00020000 selects
+// the 32-bit data format of the gfx9 buffer unit
+__device__ __forceinline__ Rsrc rsrc(const void* base, int64_t bytes) {
+  const int nr = bytes > 0x7fffffff ? 0x7fffffff : (int)bytes;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, nr, 0x00020000);
+}
+__device__ __forceinline__ float2 bld2(Rsrc r, int voff, int soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ float bld1(Rsrc r, int voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bst2(Rsrc r, float2 v, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {   // a * conj(b)
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// a * exp(SG 2 pi i t / L); t is a compile-time constant after unrolling, so
+// the tests fold and 1, -1, +-i cost no multiplies
+template <int L, int SG>
+__device__ __forceinline__ float2 rot(float2 a, int t) {
+  t %= L;
+  if (t == 0) return a;
+  if (2 * t == L) return make_float2(-a.x, -a.y);
+  if (4 * t == L) return SG > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+  if (4 * t == 3 * L) return SG > 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+  const float c = kTwL<L>.c[t];
+  const float s = SG < 0 ? kTwL<L>.s[t] : -kTwL<L>.s[t];
+  return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+}
+
+// one Stockham radix-R stage (R = 2 or 4), span Ns, of an N-point DFT with
+// kernel exp(SG 2 pi i / N); natural order after the last stage
+template <int N, int Ns, int R, int SG>
+__device__ __forceinline__ void stg(const float2 (&in)[N], float2 (&out)[N]) {
+#pragma unroll
+  for (int jj = 0; jj < N / R; ++jj) {
+    const int k = jj % Ns;
+    float2 a[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) a[m] = in[jj + m * (N / R)];
+#pragma unroll
+    for (int m = 1; m < R; ++m) a[m] = rot<Ns * R, SG>(a[m], m * k);
+    if constexpr (R == 2) {
+      const float2 b0 = make_float2(a[0].x + a[1].x, a[0].y + a[1].y);
+      const float2 b1 = make_float2(a[0].x - a[1].x, a[0].y - a[1].y);
+      a[0] = b0;
+      a[1] = b1;
+    } else {
+      const float2 t0 = make_float2(a[0].x + a[2].x, a[0].y + a[2].y);
+      const float2 t1 = make_float2(a[0].x - a[2].x, a[0].y - a[2].y);
+      const float2 t2 = make_float2(a[1].x + a[3].x, a[1].y + a[3].y);
+      const float2 t3 = make_float2(a[1].x - a[3].x, a[1].y - a[3].y);
+      const float2 u3 = SG > 0 ? make_float2(-t3.y, t3.x) : make_float2(t3.y, -t3.x);   // (SG i) t3
+      a[0] = make_float2(t0.x + t2.x, t0.y + t2.y);
+      a[2] = make_float2(t0.x - t2.x, t0.y - t2.y);
+      a[1] = make_float2(t1.x + u3.x, t1.y + u3.y);
+      a[3] = make_float2(t1.x - u3.x, t1.y - u3.y);
+    }
+    const int base = (jj / Ns) * Ns * R + k;
+#pragma unroll
+    for (int m = 0; m < R; ++m) out[base + m * Ns] = a[m];
+  }
+}
+
+// in-place N-point DFT in registers, natural order in and out
+template <int N, int SG>
+__device__ __forceinline__ void dft(float2 (&v)[N]) {
+  float2 t[N];
+  if constexpr (N == 2) {
+    stg<2, 1, 2, SG>(v, t);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = t[i];
+  } else if constexpr (N == 4) {
+    stg<4, 1, 4, SG>(v, t);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = t[i];
+  } else if constexpr (N == 8) {
+    stg<8, 1, 2, SG>(v, t);
+    stg<8, 2, 4, SG>(t, v);
+  } else if constexpr (N == 16) {
+    stg<16, 1, 4, SG>(v, t);
+    stg<16, 4, 4, SG>(t, v);
+  } else {
+    static_assert(N == 32, "DFT length");
+    stg<32, 1, 4, SG>(v, t);
+    stg<32, 4, 4, SG>(t, v);
+    stg<32, 16, 2, SG>(v, t);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = t[i];
+  }
+}
+
+// P-point DFT of v[0 .. P/2) (upper half zero): X[2r] = DFT_{P/2}(v)[r],
+// X[2r+1] = DFT_{P/2}(v_k w^k)[r], w = exp(SG 2 pi i / P); all P bins on exit
+template <int P, int SG>
+__device__ __forceinline__ void dft_zero_top(float2 (&v)[P]) {
+  constexpr int H = P / 2;
+  float2 e[H], o[H];
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    e[k] = v[k];
+    o[k] = rot<P, SG>(v[k], k);
+  }
+  dft<H, SG>(e);
+  dft<H, SG>(o);
+#pragma unroll
+  for (int r = 0; r < H; ++r) {
+    v[2 * r] = e[r];
+    v[2 * r + 1] = o[r];
+  }
+}
+
+// the first P/2 outputs of a P-point DFT: x[k] = E[k] + w^k O[k] with E, O
+// the P/2-point DFTs of the even and odd inputs; v[0 .. P/2) on exit
+template <int P, int SG>
+__device__ __forceinline__ void dft_low_half(float2 (&v)[P]) {
+  constexpr int H = P / 2;
+  float2 e[H], o[H];
+#pragma unroll
+  for (int r = 0; r < H; ++r) {
+    e[r] = v[2 * r];
+    o[r] = v[2 * r + 1];
+  }
+  dft<H, SG>(e);
+  dft<H, SG>(o);
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const float2 t = rot<P, SG>(o[k], k);
+    v[k] = make_float2(e[k].x + t.x, e[k].y + t.y);
+  }
+}
+
+// sum over the block (every thread calls it), nw = waves per block
+template <int NW>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) t += red[w];
+  return t;
+}
+
+constexpr int kXS = 16 * 16 + 8;   // conv exchange row (float2): 16 columns x 16 lines + pad
+constexpr int kRow = 144;          // pad/crop exchange row (float2): 8 columns x 16 lines + pad
+
+// ---------------------------------------------------------------------------
+// conv: lines along axis 1 of the C x 2N0 x N1 grid (line = coil c of row l0,
+// inner index c + C l0), n = 2 N1 = 16 P.  16 threads per line, thread j holds
+// positions j + 16 k.  Forward: P-point DFT over k (inputs k < P/2, the image
+// half), twiddle W_n^(j k1), one exchange, 16-point DFTs over j; bins
+// k1 + P k2 times That; then the same steps backwards, and only positions
+// j + 16 k < N1 are formed and stored.
+template <int P, int SG>
+__global__ void __launch_bounds__(256) k_toep_conv(ToepArgs a) {
+  constexpr int n = 16 * P, H = P / 2, RPT = P / 16;
+  __shared__ float2 ex[16 * kXS];
+  __shared__ float2 twT[16 * P];   // [j][k1] = exp(SG 2 pi i j k1 / n)
+  const int tid = threadIdx.x;
+  const int q = tid & 15, j = tid >> 4;
+  const int64_t inner = (int64_t)a.C * 2 * a.N0;   // lines per frame
+  const int64_t nb = inner / 16;
+  const int f = (int)(blockIdx.x / nb);
+  const int64_t q0 = ((int64_t)blockIdx.x - (int64_t)f * nb) * 16;
+  if (a.fdone && a.fdone[f]) return;
+  const int64_t l0 = q0 / a.C;
+  float2* gb = a.out + (int64_t)f * a.N1 * inner + q0;
+  const Rsrc rg = rsrc(gb, ((int64_t)a.N1 * inner - q0) * 8);
+  const Rsrc rt = rsrc(a.That + ((int64_t)f * 2 * a.N0 + l0) * n, (int64_t)n * 4);
+  for (int e = tid; e < 16 * P; e += 256) {
+    const float2 w = a.tw[(e / P) * (e % P)];
+    twT[e] = SG < 0 ? w : make_float2(w.x, -w.y);
+  }
+  const int vo = (int)(((int64_t)j * inner + q) * 8);
+  const int so = (int)(16 * inner * 8);
+  float2 v[P];
+#pragma unroll
+  for (int k = 0; k < H; ++k) v[k] = bld2(rg, vo, k * so);
+  __syncthreads();   // twiddle table
+
+  dft_zero_top<P, SG>(v);
+#pragma unroll
+  for (int k1 = 1; k1 < P; ++k1) v[k1] = cmul(v[k1], twT[j * P + k1]);
+  float2 X[RPT][16];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {   // round r: rows k1 = 16 r .. 16 r + 15
+    if (r) __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) ex[kk * kXS + j * 16 + q] = v[16 * r + kk];
+    __syncthreads();
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) X[r][jj] = ex[j * kXS + jj * 16 + q];
+    dft<16, SG>(X[r]);
+  }
+  // thread j holds bins (j + 16 r) + P k2
+  const int vt = j * 4;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r)
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+      const float t = bld1(rt, vt + (16 * r + P * k2) * 4, 0);
+      X[r][k2] = make_float2(X[r][k2].x * t, X[r][k2].y * t);
+    }
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) dft<16, -SG>(X[r]);   // over k2 -> column j'
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    __syncthreads();   // the previous round's (or the forward exchange's) reads are done
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) ex[j * kXS + jj * 16 + q] = cmulc(X[r][jj], twT[jj * P + j + 16 * r]);
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) v[16 * r + kk] = ex[kk * kXS + j * 16 + q];
+  }
+  dft_low_half<P, -SG>(v);   // over k1 -> k < H: position j + 16 k
+#pragma unroll
+  for (int k = 0; k < H; ++k) bst2(rg, v[k], vo, k * so);
+}
+
+// ---------------------------------------------------------------------------
+// pad / crop: lines along axis 0, n = 2 N0 = 512 = 8 x 8 x 8.  A 1024-thread
+// block takes the 16 coils q0 .. q0 + 15 of one grid column (f, i1); 64
+// threads per line, thread t holds positions t + 64 k.  An 8-point DFT over k,
+// twiddle W_512^(t k1), exchange, 8-point DFT over t >> 3, twiddle
+// W_64^((t & 7) m1), exchange, 8-point DFT over t & 7; thread t then holds
+// bins (t >> 3) + 8 (t & 7) + 64 m2.
+template <int SG>
+__device__ __forceinline__ void fill_r8_tables(const float2* tw, float2* twA, float2* twB) {
+  const int tid = threadIdx.x;
+  if (tid < 512) {
+    const float2 w = tw[(tid >> 3) * (tid & 7)];
+    twA[tid] = SG < 0 ? w : make_float2(w.x, -w.y);
+  } else if (tid < 576) {
+    const int e = tid - 512;
+    const float2 w = tw[8 * (e >> 3) * (e & 7)];
+    twB[e] = SG < 0 ? w : make_float2(w.x, -w.y);
+  }
+}
+
+// stages 1 .. 3 after the per-thread 8-point DFT (u holds bins k1 over k)
+__device__ __forceinline__ void r8_exchanges_fwd_tw(float2 (&u)[8], float2 (&v)[8], float2* ex, const float2* twA,
+                                                     const float2* twB, int t, int q) {
+  const int lo = t & 7, hi = t >> 3;
+#pragma unroll
+  for (int k1 = 1; k1 < 8; ++k1) u[k1] = cmul(u[k1], twA[t * 8 + k1]);
+#pragma unroll
+  for (int k1 = 0; k1 < 8; ++k1) ex[(8 * k1 + lo) * kRow + hi * 16 + q] = u[k1];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 8; ++c) v[c] = ex[t * kRow + c * 16 + q];
+  (void)twB;
+}
+
+template <int SG>
+__global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
+  __shared__ float2 ex[64 * kRow];
+  __shared__ float2 twA[512];   // [t][k1] = exp(SG 2 pi i t k1 / 512)
+  __shared__ float2 twB[64];    // [lo][m1] = exp(SG 2 pi i lo m1 / 64)
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  const int q = tid & 15, t = tid >> 4, lo = t & 7, hi = t >> 3;
+  const int nb = a.C / 16;
+  const int64_t o = blockIdx.x / nb;   // f * N1 + i1
+  const int q0 = (int)(blockIdx.x - o * nb) * 16;
+  const int f = (int)(o / a.N1);
+  const int64_t i1 = o - (int64_t)f * a.N1;
+  if (a.fdone && a.fdone[f]) return;
+  fill_r8_tables<SG>(a.tw, twA, twB);
+
+  // image positions i0 = t + 64 k, k < 4 (i0 < N0 = 256); the rest is zero pad
+  const int64_t ib = (int64_t)f * a.Ntot + (int64_t)a.N0 * i1;
+  const int vi = t * 8;
+  const Rsrc rS = rsrc(a.S + (int64_t)a.N0 * i1 * a.C + q0, ((int64_t)a.N0 * a.C - q0) * 8);
+  const int vs = (t * a.C + q) * 8, ss = 64 * a.C * 8;
+  float2 u[8];
+  float pp = 0.f;
+  if (a.pold) {   // p_new = r + beta p_old (CGLS.hpp:55-56), written by the coil-0 lanes
+    const float be = (float)a.cgst[f].beta;
+    const Rsrc rr = rsrc(a.r + ib, (int64_t)a.N0 * 8), rp = rsrc(a.pold + ib, (int64_t)a.N0 * 8);
+    const Rsrc rn = rsrc(a.pnew + ib, (int64_t)a.N0 * 8);
+    const bool c0 = (q0 + q) == 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float2 rv = bld2(rr, vi, k * 512), pv = bld2(rp, vi, k * 512);
+      const float2 xv = make_float2(rv.x + be * pv.x, rv.y + be * pv.y);
+      if (c0) {
+        bst2(rn, xv, vi, k * 512);
+        pp += xv.x * xv.x + xv.y * xv.y;
+      }
+      u[k] = cmul(xv, bld2(rS, vs, k * ss));
+    }
+  } else {
+    const Rsrc rx = rsrc(a.x + ib, (int64_t)a.N0 * 8);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) u[k] = cmul(bld2(rx, vi, k * 512), bld2(rS, vs, k * ss));
+  }
+  if (a.pp_part && q0 == 0) {   // block-uniform
+    const double s = block_sum<16>(pp, red);
+    if (tid == 0) a.pp_part[o] = s;
+  }
+  __syncthreads();   // twiddle tables
+
+  dft_zero_top<8, SG>(u);
+  float2 v[8];
+  r8_exchanges_fwd_tw(u, v, ex, twA, twB, t, q);
+  dft<8, SG>(v);
+#pragma unroll
+  for (int m1 = 1; m1 < 8; ++m1) v[m1] = cmul(v[m1], twB[lo * 8 + m1]);
+  __syncthreads();
+#pragma unroll
+  for (int m1 = 0; m1 < 8; ++m1) ex[(8 * hi + m1) * kRow + lo * 16 + q] = v[m1];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 8; ++c) v[c] = ex[t * kRow + c * 16 + q];
+  dft<8, SG>(v);
+  // bins P = hi + 8 lo + 64 m2 of column o: G[(o * 512 + P) * C + q0 + q]
+  const Rsrc ro = rsrc(a.out + o * 512 * a.C + q0, ((int64_t)512 * a.C - q0) * 8);
+  const int vb = ((hi + 8 * lo) * a.C + q) * 8;
+#pragma unroll
+  for (int m2 = 0; m2 < 8; ++m2) bst2(ro, v[m2], vb, m2 * ss);
+}
+
+template <int SG>
+__global__ void __launch_bounds__(1024) k_toep_crop(ToepArgs a) {
+  __shared__ float2 ex[64 * kRow];
+  __shared__ float2 twA[512];
+  __shared__ float2 twB[64];
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  const int q = tid & 15, t = tid >> 4, lo = t & 7, hi = t >> 3;
+  const int nb = a.C / 16;
+  const int64_t o = blockIdx.x / nb;   // f * N1 + i1
+  const int g = (int)(blockIdx.x - o * nb);
+  const int q0 = g * 16;
+  const int f = (int)(o / a.N1);
+  const int64_t i1 = o - (int64_t)f * a.N1;
+  if (a.fdone && a.fdone[f]) return;
+  fill_r8_tables<SG>(a.tw, twA, twB);
+  const Rsrc rin = rsrc(a.in + o * 512 * a.C + q0, ((int64_t)512 * a.C - q0) * 8);
+  const int vs = (t * a.C + q) * 8, ss = 64 * a.C * 8;
+  float2 u[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) u[k] = bld2(rin, vs, k * ss);
+  __syncthreads();   // twiddle tables
+
+  dft<8, SG>(u);
+  float2 v[8];
+  r8_exchanges_fwd_tw(u, v, ex, twA, twB, t, q);
+  dft<8, SG>(v);
+#pragma unroll
+  for (int m1 = 1; m1 < 8; ++m1) v[m1] = cmul(v[m1], twB[lo * 8 + m1]);
+  __syncthreads();
+#pragma unroll
+  for (int m1 = 0; m1 < 8; ++m1) ex[(8 * hi + m1) * kRow + lo * 16 + q] = v[m1];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 8; ++c) v[c] = ex[t * kRow + c * 16 + q];
+  dft_low_half<8, SG>(v);   // positions hi + 8 lo + 64 m2 < 256 (m2 < 4)
+  // X_c(i0) conj(S_c(i0)) summed over the 16 coils (lanes q)
+  const int pb = hi + 8 * lo;
+  const Rsrc rS = rsrc(a.S + (int64_t)a.N0 * i1 * a.C + q0, ((int64_t)a.N0 * a.C - q0) * 8);
+  const int vb = (pb * a.C + q) * 8;
+  float2 c[4];
+#pragma unroll
+  for (int m2 = 0; m2 < 4; ++m2) {
+    const float2 sv = bld2(rS, vb, m2 * ss);
+    c[m2] = cmulc(v[m2], sv);
+  }
+#pragma unroll
+  for (int m2 = 0; m2 < 4; ++m2)
+#pragma unroll
+    for (int m = 8; m > 0; m >>= 1) {
+      c[m2].x += __shfl_xor(c[m2].x, m);
+      c[m2].y += __shfl_xor(c[m2].y, m);
+    }
+  const int64_t rowb = ((int64_t)f * nb + g) * a.Ntot + (int64_t)a.N0 * i1;
+  const Rsrc ro = rsrc(a.out + rowb, (int64_t)a.N0 * 8);
+  double pq = 0.0;
+  if (q == 0) {
+    const int vp = pb * 8;
+#pragma unroll
+    for (int m2 = 0; m2 < 4; ++m2) bst2(ro, c[m2], vp, m2 * 512);
+    if (a.pdot) {
+      const Rsrc rd = rsrc(a.pdot + (int64_t)f * a.Ntot + (int64_t)a.N0 * i1, (int64_t)a.N0 * 8);
+#pragma unroll
+      for (int m2 = 0; m2 < 4; ++m2) {
+        const float2 pv = bld2(rd, vp, m2 * 512);
+        pq += (double)pv.x * c[m2].x + (double)pv.y * c[m2].y;
+      }
+    }
+  }
+  if (a.pq_part) {   // block-uniform
+    const double s = block_sum<16>(pq, red);
+    if (tid == 0) a.pq_part[((int64_t)f * nb + g) * a.N1 + i1] = s;
+  }
+}
+
+}  // namespace
+
+// 2D only; pad/crop need 2 N0 = 512, the conv 2 N1 in {256, 512}; all need
+// C % 16 == 0 and every per-frame byte offset below 2^31
+bool toep_pad_crop_supported(int N0, int N1, int C) {
+  return N0 == 256 && C % 16 == 0 && N1 > 0 && (int64_t)512 * C * 8 < (int64_t)1 << 31;
+}
+bool toep_conv_supported(int N0, int N1, int C) {
+  return (N1 == 128 || N1 == 256) && C % 16 == 0 && N0 > 0 &&
+         (int64_t)N1 * C * 2 * N0 * 8 < (int64_t)1 << 31;
+}
+
+void launch_toep_pad(const ToepArgs& a, hipStream_t s) {
+  const dim3 grid((unsigned)((int64_t)a.F * a.N1 * (a.C / 16)));
+  CGS_LAUNCH((k_toep_pad<-1>), grid, dim3(1024), 0, s, a);
+}
+
+void launch_toep_conv(const ToepArgs& a, hipStream_t s) {
+  const dim3 grid((unsigned)((int64_t)a.F * a.C * 2 * a.N0 / 16));
+  if (a.N1 == 256) CGS_LAUNCH((k_toep_conv<32, -1>), grid, dim3(256), 0, s, a);
+  else CGS_LAUNCH((k_toep_conv<16, -1>), grid, dim3(256), 0, s, a);
+}
+
+void launch_toep_crop(const ToepArgs& a, hipStream_t s) {
+  const dim3 grid((unsigned)((int64_t)a.F * a.N1 * (a.C / 16)));
+  CGS_LAUNCH((k_toep_crop<1>), grid, dim3(1024), 0, s, a);
+}
+
+}  // namespace cgs

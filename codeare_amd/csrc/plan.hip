@@ -832,9 +832,11 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   p.grid_cg = (C <= 8) ? 8 : (C <= 16 ? 16 : 32);
   p.grid_w = p.grid_cg / 8;
   p.interp_cg = (C <= 8) ? 8 : 16;
+  // 3D: the LDS-staged gather (the (8 + taps - 1)^3 x 2-coil region plus 64
+  // samples' window values, under 64 KB at taps <= 8) measured slower than the
+  // global gather on the koosh ball (most tiles hold few samples), so it is
+  // opt-in (CGS_INTERP_LDS=1)
   p.use_interp_lds = (d == 2);
-  // 3D: the staged (8 + taps - 1)^3 x 2-coil region plus 64 samples' window
-  // values stay under 64 KB of LDS at taps <= 8
   p.tile_item_max = (d == 3) ? 64 : 128;
   p.visits_per_item = (d == 3) ? 128 : 512;
   p.grid_cgr = (C <= 8) ? 8 : 16;
@@ -857,7 +859,7 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   }
   if (const char* e = std::getenv("CGS_VISITS_PER_ITEM")) p.visits_per_item = std::max(16, std::atoi(e));
   if (const char* e = std::getenv("CGS_TILE_ITEM_MAX")) p.tile_item_max = std::max(8, std::atoi(e));
-  if (const char* e = std::getenv("CGS_INTERP_LDS")) p.use_interp_lds = (d == 2) && std::atoi(e) != 0;
+  if (const char* e = std::getenv("CGS_INTERP_LDS")) p.use_interp_lds = std::atoi(e) != 0;
   if (const char* e = std::getenv("CGS_GRID_REG")) p.use_grid_reg = std::atoi(e) != 0;
   if (const char* e = std::getenv("CGS_GRID_CGR")) {
     const int g = std::atoi(e);
@@ -1055,6 +1057,15 @@ static void toep_drop(Plan& p) {
   p.graph_key.clear();
 }
 
+// the lean Toeplitz passes unless CGS_TOEP_LEAN=0
+static bool toep_lean_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("CGS_TOEP_LEAN");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return on;
+}
+
 // per-coil images X_c = crop(IFFT_2N(That .* FFT_2N(pad(S_c x)))) (Toeplitz embedding)
 static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd* pu, int* G_out,
                              bool* tail_out = nullptr) {
@@ -1092,11 +1103,48 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
     launch_fft(a, ld, st, p.stream);
   };
   if (d == 2) {
-    pass(CGS_K_TOEP_PAD, p.ft[0], -1, nullptr, x, p.tt1, (int)N0, (int)M0, C, N1, 4, nullptr, 0,
-         ((double)(1 + C) * N0 * N1 + (double)C * M0 * N1) * 8);
-    pass(CGS_K_TOEP_CONV, p.ft[1], -1, p.tt1, nullptr, p.tt1, (int)N1, (int)N1, C * M0, 1, 0, p.That, M0,
-         2.0 * C * M0 * N1 * 8 + 4.0 * M0 * M1);
-    pass(CGS_K_TOEP_CROP, p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1, 0, nullptr, 0, 0.0, 2);
+    // lean kernels (kernels_toep.hip) where the shape allows; pad and crop
+    // switch together (both leave image positions unshifted along axis 0),
+    // the conv pass is self-contained along axis 1
+    const bool lean_pc = toep_lean_enabled() && toep_pad_crop_supported((int)N0, (int)N1, C);
+    const bool lean_cv = toep_lean_enabled() && toep_conv_supported((int)N0, (int)N1, C);
+    ToepArgs ta{};
+    ta.C = C; ta.N0 = (int)N0; ta.N1 = (int)N1; ta.F = F; ta.Ntot = p.Ntot;
+    ta.S = p.S; ta.fdone = fl;
+    const bool want_tail = pu && tail_out && p.tail_pp;
+    const double pad_bytes = ((double)(1 + C) * N0 * N1 + (double)C * M0 * N1) * 8;
+    if (lean_pc) {
+      ToepArgs a = ta;
+      a.tw = p.ft[0].tw; a.x = x; a.out = p.tt1;
+      if (pu) { a.r = pu->r; a.pold = pu->pold; a.pnew = pu->pnew; a.cgst = p.cgst; }
+      if (want_tail) { a.pp_part = p.tail_pp; ++tail_parts; }
+      Prof pr(p, CGS_K_TOEP_PAD, pad_bytes * F);
+      launch_toep_pad(a, p.stream);
+    } else {
+      pass(CGS_K_TOEP_PAD, p.ft[0], -1, nullptr, x, p.tt1, (int)N0, (int)M0, C, N1, 4, nullptr, 0, pad_bytes);
+    }
+    const double conv_bytes = 2.0 * C * M0 * N1 * 8 + 4.0 * M0 * M1;
+    if (lean_cv) {
+      ToepArgs a = ta;
+      a.tw = p.ft[1].tw; a.out = p.tt1; a.That = p.That;
+      Prof pr(p, CGS_K_TOEP_CONV, conv_bytes * F);
+      launch_toep_conv(a, p.stream);
+    } else {
+      pass(CGS_K_TOEP_CONV, p.ft[1], -1, p.tt1, nullptr, p.tt1, (int)N1, (int)N1, C * M0, 1, 0, p.That, M0,
+           conv_bytes);
+    }
+    if (lean_pc) {
+      ToepArgs a = ta;
+      a.tw = p.ft[0].tw; a.in = p.tt1; a.out = p.Xc;
+      const int G = C / 16;
+      *G_out = G;
+      if (want_tail) { a.pq_part = p.tail_pq; a.pdot = pu->pnew; ++tail_parts; }
+      const double bytes = ((double)C * M0 * N1 + (double)C * N0 * N1 + (double)G * N0 * N1) * 8;
+      Prof pr(p, CGS_K_TOEP_CROP, bytes * F);
+      launch_toep_crop(a, p.stream);
+    } else {
+      pass(CGS_K_TOEP_CROP, p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1, 0, nullptr, 0, 0.0, 2);
+    }
   } else {
     pass(CGS_K_TOEP_PAD, p.ft[0], -1, nullptr, x, p.tt1, (int)N0, (int)M0, C, N1 * N2, 4, nullptr, 0,
          ((double)(1 + C) * N0 * N1 * N2 + (double)C * M0 * N1 * N2) * 8);
