@@ -35,7 +35,15 @@ struct ReduceItem {     // sums the partial slots of one heavy tile
 struct CgState {        // per-frame CG scalars, FP64
   double rn, bn, xn, pq, alpha, beta, rel;
   int it, done, nan, pad;
+  // beta folded into the next pad pass (lean Toeplitz path): the CG tail
+  // snapshots rn and it and sets bpend; the pad pass forms beta and the stop
+  // test from the tail's ||r||^2 partials and commits rn, it, beta, rel, done
+  double rn_snap;
+  int it_snap, bpend;
 };
+
+constexpr int kRedBlocks = 512;   // partial sums per frame for image-domain reductions
+constexpr int kCoilBlocks = 2048; // blocks per frame of the coil combine
 
 struct FftArgs {
   const float2* in;
@@ -95,6 +103,15 @@ struct ToepArgs {
   double* pp_part;        // pad: ||p_new||^2 per (f, i1)
   double* pq_part;        // crop: Re<pdot, P_g> per (f, g, i1)
   const float2* pdot;
+  // pad with the folded beta: ||r||^2 partials of the CG tail (nrpart per
+  // frame), stop rule, residual history; the frame's first block writes cgw,
+  // hist and fdone_w
+  const double* rpart;
+  int nrpart, tolkind, maxit, hcap;
+  double tol;
+  CgState* cgw;
+  double* hist;
+  int* fdone_w;
   int C, N0, N1, F;
   int64_t Ntot;
 };
@@ -234,6 +251,7 @@ struct Plan {
   Stat stat[CGS_K_COUNT];
   unsigned prof_mask = ~0u;           // kernel classes timed with events
   double* tail_pp = nullptr;          // fused CG tail partials (Toeplitz path)
+  bool beta_folded = false;           // the captured / last CG iteration forms beta in the pad pass
   double* tail_pq = nullptr;
   struct Pending { hipEvent_t a, b; int kind; double bytes; };
   std::vector<Pending> pending;
@@ -285,7 +303,9 @@ void launch_interp_lds3(const SampArgs& a, const int4* items, int nitems, int T,
 void launch_grid_reg3(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s);
 size_t interp_lds_bytes(int T, int taps, int CG, int rowstride, int maxs);
 void launch_cg_tail(const Plan& p, float2* x, float2* r, const float2* pv, const float2* part, int G,
-                    float lambda, const double* pq_part, int npq, const double* pp_part, int npp, hipStream_t s);
+                    float lambda, const double* pq_part, int npq, const double* pp_part, int npp, bool snap,
+                    hipStream_t s);
+const double* cg_tail_rpartials(const Plan& p);
 void launch_partial_combine(const Plan& p, const float2* part, int G, float2* q, const float2* pvec, float lambda,
                             double* partials, const int* fl, hipStream_t s);
 void launch_coil_combine(const Plan& p, const float2* Xc, float2* x, const float2* pvec,

@@ -323,19 +323,55 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
   float2 u[8];
   float pp = 0.f;
   if (a.pold) {   // p_new = r + beta p_old (CGLS.hpp:55-56), written by the coil-0 lanes
-    const float be = (float)a.cgst[f].beta;
     const Rsrc rr = rsrc(a.r + ib, (int64_t)a.N0 * 8), rp = rsrc(a.pold + ib, (int64_t)a.N0 * 8);
     const Rsrc rn = rsrc(a.pnew + ib, (int64_t)a.N0 * 8);
+    float2 rv[4], pv[4], sv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      rv[k] = bld2(rr, vi, k * 512);
+      pv[k] = bld2(rp, vi, k * 512);
+      sv[k] = bld2(rS, vs, k * ss);
+    }
+    float be;
+    const CgState& cs = a.cgst[f];
+    if (a.rpart && cs.bpend) {
+      // beta and the stop test of the iteration the CG tail just finished
+      // (CGLS.hpp:53-55): every block sums the same ||r||^2 partials in the
+      // same order; the frame's first block commits the CG state.  No block
+      // reads a field that block writes (the tail left rn_snap, it_snap).
+      double v = 0.0;
+      for (int i = tid; i < a.nrpart; i += 1024) v += a.rpart[(int64_t)f * a.nrpart + i];
+      const double rnew = block_sum<16>(v, red);
+      const int it = cs.it_snap + 1;
+      const double rel = (a.tolkind == CGS_TOL_SQUARED_RATIO) ? rnew / cs.xn : sqrt(rnew / cs.bn);
+      const int nan = (cs.nan || isnan(rel) || isnan(cs.alpha)) ? 1 : 0;
+      const int done = (nan || rel <= a.tol || it >= a.maxit) ? 1 : 0;
+      const double beta = rnew / cs.rn_snap;
+      if (tid == 0 && (int64_t)blockIdx.x == (int64_t)f * a.N1 * nb) {
+        CgState* w = a.cgw + f;
+        w->beta = beta;
+        w->rn = rnew;
+        w->it = it;
+        w->rel = rel;
+        w->nan = nan;
+        w->done = done;
+        if (a.hist && it < a.hcap) a.hist[(int64_t)f * a.hcap + it] = rel;
+        a.fdone_w[f] = done;
+      }
+      if (done) return;   // block-uniform: every thread holds the same sum
+      be = (float)beta;
+    } else {
+      be = (float)cs.beta;
+    }
     const bool c0 = (q0 + q) == 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const float2 rv = bld2(rr, vi, k * 512), pv = bld2(rp, vi, k * 512);
-      const float2 xv = make_float2(rv.x + be * pv.x, rv.y + be * pv.y);
+      const float2 xv = make_float2(rv[k].x + be * pv[k].x, rv[k].y + be * pv[k].y);
       if (c0) {
         bst2(rn, xv, vi, k * 512);
         pp += xv.x * xv.x + xv.y * xv.y;
       }
-      u[k] = cmul(xv, bld2(rS, vs, k * ss));
+      u[k] = cmul(xv, sv[k]);
     }
   } else {
     const Rsrc rx = rsrc(a.x + ib, (int64_t)a.N0 * 8);

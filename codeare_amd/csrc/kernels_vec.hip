@@ -10,8 +10,6 @@
 
 namespace cgs {
 
-constexpr int kRedBlocks = 512;   // partial sums per frame for image-domain reductions
-constexpr int kCoilBlocks = 2048; // blocks per frame of the coil combine (a few pixels per thread group)
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -124,15 +122,32 @@ __global__ void __launch_bounds__(256) k_partial_combine(const float2* __restric
 // Fused CG tail of the Toeplitz path: alpha = rn / (sum pq_part + lambda sum
 // pp_part) (every block sums the same partials in the same order), then
 // q = sum_g part_g + lambda p computed in place, x += alpha p, r -= alpha q and
-// ||r||^2 partials
+// ||r||^2 partials.  The first pixel of every thread is loaded before the
+// partial sums so that both latencies overlap.  snap: record rn and it for the
+// pad pass that forms beta (CgState::bpend)
 __global__ void __launch_bounds__(256) k_cg_tail(float2* x, float2* r, const float2* pv, const float2* part, int G,
                                                  float lambda, CgState* st, const double* pq_part, int npq,
                                                  const double* pp_part, int npp, double* partials, int64_t Ntot,
-                                                 const int* fdone) {
+                                                 const int* fdone, int snap) {
   __shared__ double sh[4];
   __shared__ float alpha_s;
   const int f = blockIdx.y;
   if (fdone[f]) return;
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool has = i0 < Ntot;
+  float2 p0 = make_float2(0.f, 0.f), q0 = p0, x0 = p0, r0 = p0;
+  if (has) {
+    const int64_t o = (int64_t)f * Ntot + i0;
+    p0 = pv[o];
+    q0 = make_float2(lambda * p0.x, lambda * p0.y);
+    for (int g = 0; g < G; ++g) {
+      const float2 v = part[((int64_t)f * G + g) * Ntot + i0];
+      q0.x += v.x;
+      q0.y += v.y;
+    }
+    x0 = x[o];
+    r0 = r[o];
+  }
   {
     double a = 0.0, b = 0.0;
     for (int i = threadIdx.x; i < npq; i += blockDim.x) a += pq_part[(int64_t)f * npq + i];
@@ -141,13 +156,29 @@ __global__ void __launch_bounds__(256) k_cg_tail(float2* x, float2* r, const flo
     if (threadIdx.x == 0) {
       const double al = st[f].rn / pq;   // CGLS.hpp:50
       alpha_s = (float)al;
-      if (blockIdx.x == 0) { st[f].pq = pq; st[f].alpha = al; }
+      if (blockIdx.x == 0) {
+        st[f].pq = pq;
+        st[f].alpha = al;
+        if (snap) {
+          st[f].rn_snap = st[f].rn;
+          st[f].it_snap = st[f].it;
+          st[f].bpend = 1;
+        }
+      }
     }
     __syncthreads();
   }
   const float al = alpha_s;
   double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Ntot; i += (int64_t)gridDim.x * blockDim.x) {
+  if (has) {
+    const int64_t o = (int64_t)f * Ntot + i0;
+    x0.x += al * p0.x; x0.y += al * p0.y;
+    r0.x -= al * q0.x; r0.y -= al * q0.y;
+    x[o] = x0;
+    r[o] = r0;
+    acc += (double)r0.x * r0.x + (double)r0.y * r0.y;
+  }
+  for (int64_t i = i0 + (int64_t)gridDim.x * blockDim.x; i < Ntot; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t o = (int64_t)f * Ntot + i;
     const float2 pp = pv[o];
     float2 qq = make_float2(lambda * pp.x, lambda * pp.y);
@@ -167,10 +198,14 @@ __global__ void __launch_bounds__(256) k_cg_tail(float2* x, float2* r, const flo
   if (threadIdx.x == 0) partials[(int64_t)f * gridDim.x + blockIdx.x] = t;
 }
 
+// ||r||^2 partials of the CG tail: kRedBlocks per frame
+const double* cg_tail_rpartials(const Plan& p) { return p.partials + (size_t)kCoilBlocks * p.F; }
+
 void launch_cg_tail(const Plan& p, float2* x, float2* r, const float2* pv, const float2* part, int G,
-                    float lambda, const double* pq_part, int npq, const double* pp_part, int npp, hipStream_t s) {
+                    float lambda, const double* pq_part, int npq, const double* pp_part, int npp, bool snap,
+                    hipStream_t s) {
   hipLaunchKernelGGL(k_cg_tail, dim3(kRedBlocks, p.F), dim3(256), 0, s, x, r, pv, part, G, lambda, p.cgst, pq_part,
-                     npq, pp_part, npp, p.partials + (size_t)kCoilBlocks * p.F, p.Ntot, p.fdone);
+                     npq, pp_part, npp, p.partials + (size_t)kCoilBlocks * p.F, p.Ntot, p.fdone, snap ? 1 : 0);
 }
 
 void launch_partial_combine(const Plan& p, const float2* part, int G, float2* q, const float2* pvec, float lambda,
@@ -390,6 +425,7 @@ __global__ void k_cg_init(CgState* st, const double* pbn, const double* prn, int
     c.xn = rn;   // codeare: rn = xn = Re<r, p> (CGLS.hpp:35)
     c.pq = 0; c.alpha = 0; c.beta = 0;
     c.it = 0; c.nan = 0; c.pad = 0;
+    c.rn_snap = 0; c.it_snap = 0; c.bpend = 0;
     double rel;
     if (tolkind == CGS_TOL_SQUARED_RATIO) { c.xn = bn; rel = bn > 0 ? rn / bn : 0.0; }
     else rel = bn > 0 ? sqrt(rn / bn) : 0.0;

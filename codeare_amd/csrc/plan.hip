@@ -1068,7 +1068,8 @@ static bool toep_lean_enabled() {
 
 // per-coil images X_c = crop(IFFT_2N(That .* FFT_2N(pad(S_c x)))) (Toeplitz embedding)
 static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd* pu, int* G_out,
-                             bool* tail_out = nullptr) {
+                             bool* tail_out = nullptr, const cgs_solve_opts* fold = nullptr,
+                             bool* folded = nullptr) {
   // inside the CG (pu set) the register FFT passes also produce the partials
   // of the fused CG tail: ||p||^2 per row (pad pass) and <p, q_partial> per
   // (coil group, row) (crop pass)
@@ -1118,6 +1119,19 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
       a.tw = p.ft[0].tw; a.x = x; a.out = p.tt1;
       if (pu) { a.r = pu->r; a.pold = pu->pold; a.pnew = pu->pnew; a.cgst = p.cgst; }
       if (want_tail) { a.pp_part = p.tail_pp; ++tail_parts; }
+      if (want_tail && fold && folded) {
+        // beta and the stop test of the previous iteration move into this pass
+        a.rpart = cg_tail_rpartials(p);
+        a.nrpart = kRedBlocks;
+        a.tolkind = fold->tol_kind;
+        a.tol = fold->tol;
+        a.maxit = fold->maxit;
+        a.cgw = p.cgst;
+        a.hist = p.hist;
+        a.hcap = p.hist_cap;
+        a.fdone_w = p.fdone;
+        *folded = true;
+      }
       Prof pr(p, CGS_K_TOEP_PAD, pad_bytes * F);
       launch_toep_pad(a, p.stream);
     } else {
@@ -1189,14 +1203,17 @@ static int cg_iteration_fused(Plan& p, int normal_op, float2* x, float2* r, cons
   pu.pnew = pnew;
   if (normal_op == CGS_NORMAL_TOEPLITZ) {
     int G = 0;
-    bool tail = false;
-    CHK(toeplitz_to_coils(p, nullptr, p.fdone, &pu, &G, &tail));
+    bool tail = false, folded = false;
+    CHK(toeplitz_to_coils(p, nullptr, p.fdone, &pu, &G, &tail, &o, &folded));
     if (tail) {
-      // alpha from the FFT passes' partials, q formed in place: no combine kernel
+      // alpha from the FFT passes' partials, q formed in place: no combine
+      // kernel; with the lean pad pass beta is formed there (next iteration)
       const int opf = (int)(p.Ntot / p.N[0]);
       Prof pr(p, CGS_K_CG, ((double)G + 6.0) * 8.0 * p.F * p.Ntot);
-      launch_cg_tail(p, x, r, pnew, p.Xc, G, (float)o.lambda, p.tail_pq, G * opf, p.tail_pp, opf, p.stream);
-      launch_cg_beta_after_xr(p, o.tol_kind, o.tol, o.maxit, p.stream);
+      launch_cg_tail(p, x, r, pnew, p.Xc, G, (float)o.lambda, p.tail_pq, G * opf, p.tail_pp, opf, folded,
+                     p.stream);
+      if (!folded) launch_cg_beta_after_xr(p, o.tol_kind, o.tol, o.maxit, p.stream);
+      p.beta_folded = folded;
       return CGS_OK;
     }
     Prof pr(p, CGS_K_COIL, ((double)G + 3.0) * p.Ntot * p.F * 8);
@@ -1326,6 +1343,11 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
       if (all) break;
     }
   }
+  // with beta folded into the pad pass the last iteration's stop test is
+  // still pending when the loop ends at maxit: commit it (frames already done
+  // are skipped)
+  if (p.beta_folded && o.eh_mode == CGS_EH_TRUE && o.normal_op == CGS_NORMAL_TOEPLITZ)
+    launch_cg_beta_after_xr(p, o.tol_kind, o.tol, o.maxit, p.stream);
   HIPCHK(hipMemcpyAsync(xout, x, ib, hipMemcpyDeviceToDevice, p.stream));
   HIPCHK(hipMemcpyAsync(p.h_cgst, p.cgst, sizeof(CgState) * F, hipMemcpyDeviceToHost, p.stream));
   HIPCHK(hipMemsetAsync(p.fdone, 0, sizeof(int) * F, p.stream));
