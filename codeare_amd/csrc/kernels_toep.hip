@@ -336,30 +336,41 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
     const CgState& cs = a.cgst[f];
     if (a.rpart && cs.bpend) {
       // beta and the stop test of the iteration the CG tail just finished
-      // (CGLS.hpp:53-55): every block sums the same ||r||^2 partials in the
-      // same order; the frame's first block commits the CG state.  No block
-      // reads a field that block writes (the tail left rn_snap, it_snap).
-      double v = 0.0;
-      for (int i = tid; i < a.nrpart; i += 1024) v += a.rpart[(int64_t)f * a.nrpart + i];
-      const double rnew = block_sum<16>(v, red);
-      const int it = cs.it_snap + 1;
-      const double rel = (a.tolkind == CGS_TOL_SQUARED_RATIO) ? rnew / cs.xn : sqrt(rnew / cs.bn);
-      const int nan = (cs.nan || isnan(rel) || isnan(cs.alpha)) ? 1 : 0;
-      const int done = (nan || rel <= a.tol || it >= a.maxit) ? 1 : 0;
-      const double beta = rnew / cs.rn_snap;
-      if (tid == 0 && (int64_t)blockIdx.x == (int64_t)f * a.N1 * nb) {
-        CgState* w = a.cgw + f;
-        w->beta = beta;
-        w->rn = rnew;
-        w->it = it;
-        w->rel = rel;
-        w->nan = nan;
-        w->done = done;
-        if (a.hist && it < a.hcap) a.hist[(int64_t)f * a.hcap + it] = rel;
-        a.fdone_w[f] = done;
+      // (CGLS.hpp:53-55): wave 0 of every block sums the same ||r||^2
+      // partials in the same order; the frame's first block commits the CG
+      // state.  No block reads a field that block writes (the tail left
+      // rn_snap and it_snap).
+      __shared__ double fold_s[2];   // beta, done
+      if (tid < 64) {
+        double v = 0.0;
+        for (int i = tid; i < a.nrpart; i += 64) v += a.rpart[(int64_t)f * a.nrpart + i];
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+        if (tid == 0) {
+          const double rnew = v;
+          const int it = cs.it_snap + 1;
+          const double rel = (a.tolkind == CGS_TOL_SQUARED_RATIO) ? rnew / cs.xn : sqrt(rnew / cs.bn);
+          const int nan = (cs.nan || isnan(rel) || isnan(cs.alpha)) ? 1 : 0;
+          const int done = (nan || rel <= a.tol || it >= a.maxit) ? 1 : 0;
+          const double beta = rnew / cs.rn_snap;
+          if ((int64_t)blockIdx.x == (int64_t)f * a.N1 * nb) {
+            CgState* w = a.cgw + f;
+            w->beta = beta;
+            w->rn = rnew;
+            w->it = it;
+            w->rel = rel;
+            w->nan = nan;
+            w->done = done;
+            if (a.hist && it < a.hcap) a.hist[(int64_t)f * a.hcap + it] = rel;
+            a.fdone_w[f] = done;
+          }
+          fold_s[0] = beta;
+          fold_s[1] = done ? 1.0 : 0.0;
+        }
       }
-      if (done) return;   // block-uniform: every thread holds the same sum
-      be = (float)beta;
+      __syncthreads();
+      if (fold_s[1] != 0.0) return;   // block-uniform
+      be = (float)fold_s[0];
     } else {
       be = (float)cs.beta;
     }
@@ -491,20 +502,33 @@ bool toep_conv_supported(int N0, int N1, int C) {
          (int64_t)N1 * C * 2 * N0 * 8 < (int64_t)1 << 31;
 }
 
+// blocks resident per CU: unused dynamic LDS caps it at `occ` (0: no cap), so
+// that blocks of a later round start while others compute or store
+static size_t occ_pad_lds(const char* env, size_t static_lds) {
+  const char* e = std::getenv(env);
+  const int occ = e ? std::atoi(e) : 0;
+  if (occ <= 0) return 0;
+  const size_t per = (size_t)163840 / (size_t)occ;
+  return per > static_lds + 256 ? per - static_lds - 256 : 0;
+}
+
 void launch_toep_pad(const ToepArgs& a, hipStream_t s) {
   const dim3 grid((unsigned)((int64_t)a.F * a.N1 * (a.C / 16)));
-  CGS_LAUNCH((k_toep_pad<-1>), grid, dim3(1024), 0, s, a);
+  static const size_t extra = occ_pad_lds("CGS_TOEP_OCC_PC", 78464);
+  CGS_LAUNCH((k_toep_pad<-1>), grid, dim3(1024), extra, s, a);
 }
 
 void launch_toep_conv(const ToepArgs& a, hipStream_t s) {
   const dim3 grid((unsigned)((int64_t)a.F * a.C * 2 * a.N0 / 16));
-  if (a.N1 == 256) CGS_LAUNCH((k_toep_conv<32, -1>), grid, dim3(256), 0, s, a);
-  else CGS_LAUNCH((k_toep_conv<16, -1>), grid, dim3(256), 0, s, a);
+  static const size_t extra = occ_pad_lds("CGS_TOEP_OCC_CONV", 37888);
+  if (a.N1 == 256) CGS_LAUNCH((k_toep_conv<32, -1>), grid, dim3(256), extra, s, a);
+  else CGS_LAUNCH((k_toep_conv<16, -1>), grid, dim3(256), extra, s, a);
 }
 
 void launch_toep_crop(const ToepArgs& a, hipStream_t s) {
   const dim3 grid((unsigned)((int64_t)a.F * a.N1 * (a.C / 16)));
-  CGS_LAUNCH((k_toep_crop<1>), grid, dim3(1024), 0, s, a);
+  static const size_t extra = occ_pad_lds("CGS_TOEP_OCC_PC", 78464);
+  CGS_LAUNCH((k_toep_crop<1>), grid, dim3(1024), extra, s, a);
 }
 
 }  // namespace cgs

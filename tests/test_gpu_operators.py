@@ -222,6 +222,83 @@ def test_frames_toeplitz_and_per_frame_cg(cgs):
             assert ef <= (1e-5 if rb["iters"][f] == r1["iters"][0] else 1e-3)
 
 
+def _lam_max(P, iters=25, seed=3):
+    """Largest eigenvalue of E^H W E (over all frames of the plan) by power
+    iteration on the Toeplitz operator."""
+    rng = np.random.default_rng(seed)
+    v = rand_img(rng, P._img_shape())
+    lm = 0.0
+    for _ in range(iters):
+        w_ = P.normal(v, lam=0.0, toeplitz=True)
+        lm = float(np.linalg.norm(w_) / np.linalg.norm(v))
+        v = (w_ / np.linalg.norm(w_)).astype(np.complex64)
+    return lm
+
+
+def test_frames_lean_toeplitz(cgs):
+    """Frames on the lean Toeplitz passes (256^2, 16 coils, three frames of
+    uneven spoke counts), whose CG stop tests run inside the pad pass per
+    frame: the joint graph-replayed solve against single-plan solves of each
+    frame (iteration counts and images), and a fixed-iteration solve (the stop
+    test committed after the loop) launched one by one and as graphs."""
+    P, k, w, S, x, off = _frames_problem(cgs, N=256, C=16, R=512, counts=(40, 90, 25), seed=9)
+    C = S.shape[-1]
+    qt = P.normal(x, lam=0.1, toeplitz=True)
+    qn = P.normal(x, lam=0.1)
+    e = rel(qt, qn)
+    print(f"lean frames: toeplitz vs nufft normal {e:.2e}")
+    assert e <= 2e-5
+    y = P.E(x)
+    lam = 1e-2 * _lam_max(P)
+    xb, rb = P.solve(y, maxit=400, tol=1e-4, lam=lam, toeplitz=True, graph_iters=8)
+    for f in range(len(off) - 1):
+        Mf = int(off[f + 1] - off[f])
+        P1 = cgs.Plan(P.N, Mf, 2.0, 3, ncoils=C)
+        P1.kspace(k[:, off[f]:off[f + 1]])
+        P1.weights(w[off[f]:off[f + 1]])
+        P1.sensitivities(S)
+        yf = y[C * off[f]: C * off[f + 1]].reshape((Mf, C), order="F")
+        x1, r1 = P1.solve(yf, maxit=400, tol=1e-4, lam=lam, toeplitz=True)
+        ef = rel(xb[..., f], x1)
+        print(f"  frame {f}: iters {rb['iters'][f]} vs {r1['iters'][0]}, rel resid {rb['final_rel'][f]:.3e}, "
+              f"image {ef:.2e}")
+        assert rb["final_rel"][f] <= 1e-4 and r1["final_rel"][0] <= 1e-4
+        assert abs(rb["iters"][f] - r1["iters"][0]) <= 1
+        assert ef <= (1e-5 if rb["iters"][f] == r1["iters"][0] else 1e-3)
+    xa, ra = P.solve(y, maxit=10, tol=0.0, lam=lam, toeplitz=True)
+    xg, rg = P.solve(y, maxit=10, tol=0.0, lam=lam, toeplitz=True, graph_iters=4)
+    assert list(ra["iters"]) == list(rg["iters"]) == [10, 10, 10]
+    assert rel(xa, xg) <= 1e-6
+    assert np.allclose(ra["final_rel"], rg["final_rel"], rtol=1e-9)
+
+
+@pytest.mark.parametrize("N0,N1,C", [(256, 128, 16), (128, 256, 32), (256, 256, 16)])
+def test_lean_toeplitz_shapes(cgs, N0, N1, C):
+    """Square and non-square images on the lean Toeplitz passes (pad/crop at
+    2 N0 = 512, conv at 2 N1 = 256 or 512): the Toeplitz normal operator
+    against the NUFFT one, and 12 CG iterations through each operator."""
+    from codeare_amd import data
+    rng = np.random.default_rng(N0 + 3 * N1 + C)
+    M = 40000
+    k = rand_traj(rng, 2, M)
+    w = rng.uniform(0.1, 1.0, M).astype(np.float32)
+    S = np.asfortranarray(rand_img(rng, (N0, N1, C)) * 0.2)
+    x = rand_img(rng, (N0, N1))
+    P = cgs.Plan((N0, N1), M, 2.0, 3, ncoils=C)
+    P.kspace(k)
+    P.weights(w)
+    P.sensitivities(S)
+    e = rel(P.normal(x, lam=0.5, toeplitz=True), P.normal(x, lam=0.5))
+    y = P.E(x)
+    lam = 1e-2 * _lam_max(P)
+    xt, rt = P.solve(y, maxit=12, tol=0.0, lam=lam, toeplitz=True, graph_iters=4)
+    xn, rn = P.solve(y, maxit=12, tol=0.0, lam=lam)
+    e2 = rel(xt, xn)
+    print(f"{N0}x{N1} C={C}: toeplitz vs nufft normal {e:.2e}, CG(12) {e2:.2e}")
+    assert e <= 2e-5 and e2 <= 1e-4
+    assert rt["iters"][0] == rn["iters"][0] == 12
+
+
 def test_adjoint_identity_c2_shape(cgs):
     """<E x, W y> == <x, E^H W y> at the full 256^2 x 32-coil golden-angle size."""
     from codeare_amd import data
