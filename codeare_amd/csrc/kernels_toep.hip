@@ -73,7 +73,7 @@ __device__ __forceinline__ float2 rot(float2 a, int t) {
   return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
 }
 
-// one Stockham radix-R stage (R = 2 or 4), span Ns, of an N-point DFT with
+// one Stockham radix-R stage (R = 2, 3 or 4), span Ns, of an N-point DFT with
 // kernel exp(SG 2 pi i / N); natural order after the last stage
 template <int N, int Ns, int R, int SG>
 __device__ __forceinline__ void stg(const float2 (&in)[N], float2 (&out)[N]) {
@@ -90,6 +90,15 @@ __device__ __forceinline__ void stg(const float2 (&in)[N], float2 (&out)[N]) {
       const float2 b1 = make_float2(a[0].x - a[1].x, a[0].y - a[1].y);
       a[0] = b0;
       a[1] = b1;
+    } else if constexpr (R == 3) {
+      // w = exp(SG 2 pi i / 3) = -1/2 + SG i sqrt(3)/2
+      constexpr float h = 0.86602540378443864676f * (float)SG;
+      const float2 t1 = make_float2(a[1].x + a[2].x, a[1].y + a[2].y);
+      const float2 t2 = make_float2(a[1].x - a[2].x, a[1].y - a[2].y);
+      const float2 mm = make_float2(a[0].x - 0.5f * t1.x, a[0].y - 0.5f * t1.y);
+      a[0] = make_float2(a[0].x + t1.x, a[0].y + t1.y);
+      a[1] = make_float2(mm.x - h * t2.y, mm.y + h * t2.x);   // mm + (i h) t2
+      a[2] = make_float2(mm.x + h * t2.y, mm.y - h * t2.x);
     } else {
       const float2 t0 = make_float2(a[0].x + a[2].x, a[0].y + a[2].y);
       const float2 t1 = make_float2(a[0].x - a[2].x, a[0].y - a[2].y);
@@ -115,6 +124,13 @@ __device__ __forceinline__ void dft(float2 (&v)[N]) {
     stg<2, 1, 2, SG>(v, t);
 #pragma unroll
     for (int i = 0; i < N; ++i) v[i] = t[i];
+  } else if constexpr (N == 3) {
+    stg<3, 1, 3, SG>(v, t);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = t[i];
+  } else if constexpr (N == 6) {
+    stg<6, 1, 2, SG>(v, t);
+    stg<6, 2, 3, SG>(t, v);
   } else if constexpr (N == 4) {
     stg<4, 1, 4, SG>(v, t);
 #pragma unroll
@@ -266,44 +282,102 @@ __global__ void __launch_bounds__(256) k_toep_conv(ToepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// pad / crop: lines along axis 0, n = 2 N0 = 512 = 8 x 8 x 8.  A 1024-thread
-// block takes the 16 coils q0 .. q0 + 15 of one grid column (f, i1); 64
-// threads per line, thread t holds positions t + 64 k.  An 8-point DFT over k,
-// twiddle W_512^(t k1), exchange, 8-point DFT over t >> 3, twiddle
-// W_64^((t & 7) m1), exchange, 8-point DFT over t & 7; thread t then holds
-// bins (t >> 3) + 8 (t & 7) + 64 m2.
-template <int SG>
+// 64 threads per line, n = 64 NP (NP = 6: n = 384, NP = 8: n = 512).  A
+// 1024-thread block takes 16 lines (16 coils q0 .. q0 + 15 of one grid row or
+// column); thread t holds positions t + 64 k, k < NP.  Forward: an NP-point
+// DFT over k, twiddle W_n^(t k1), exchange, 8-point DFT over t >> 3, twiddle
+// W_64^((t & 7) m1), exchange, 8-point DFT over t & 7; thread t = 8 k1 + m1
+// (t < 8 NP) then holds bins k1 + NP m1 + 8 NP m2.  Threads t >= 8 NP have no
+// row in the two 8-point stages (NP = 6).
+template <int NP>
+struct R8 {
+  static constexpr int n = 64 * NP;
+  static constexpr int ACT = 8 * NP;   // threads holding a row in the 8 x 8 stages
+};
+
+// twA[t][k1] = exp(SG 2 pi i t k1 / n), twB[lo][m1] = exp(SG 2 pi i lo m1 / 64)
+template <int NP, int SG>
 __device__ __forceinline__ void fill_r8_tables(const float2* tw, float2* twA, float2* twB) {
   const int tid = threadIdx.x;
-  if (tid < 512) {
-    const float2 w = tw[(tid >> 3) * (tid & 7)];
+  if (tid < 64 * NP) {
+    const float2 w = tw[(tid / NP) * (tid % NP)];
     twA[tid] = SG < 0 ? w : make_float2(w.x, -w.y);
-  } else if (tid < 576) {
-    const int e = tid - 512;
-    const float2 w = tw[8 * (e >> 3) * (e & 7)];
+  } else if (tid < 64 * NP + 64) {
+    const int e = tid - 64 * NP;
+    const float2 w = tw[NP * (e >> 3) * (e & 7)];
     twB[e] = SG < 0 ? w : make_float2(w.x, -w.y);
   }
 }
 
-// stages 1 .. 3 after the per-thread 8-point DFT (u holds bins k1 over k)
-__device__ __forceinline__ void r8_exchanges_fwd_tw(float2 (&u)[8], float2 (&v)[8], float2* ex, const float2* twA,
-                                                     const float2* twB, int t, int q) {
+// u: bins k1 of the NP-point DFT over k -> v: the inputs of the last 8-point
+// DFT over t & 7 (see above)
+template <int NP, int SG>
+__device__ __forceinline__ void r8_forward(float2 (&u)[NP], float2 (&v)[8], float2* ex, const float2* twA,
+                                           const float2* twB, int t, int q) {
   const int lo = t & 7, hi = t >> 3;
+  const bool act = R8<NP>::ACT >= 64 || t < R8<NP>::ACT;
 #pragma unroll
-  for (int k1 = 1; k1 < 8; ++k1) u[k1] = cmul(u[k1], twA[t * 8 + k1]);
+  for (int k1 = 1; k1 < NP; ++k1) u[k1] = cmul(u[k1], twA[t * NP + k1]);
 #pragma unroll
-  for (int k1 = 0; k1 < 8; ++k1) ex[(8 * k1 + lo) * kRow + hi * 16 + q] = u[k1];
+  for (int k1 = 0; k1 < NP; ++k1) ex[(8 * k1 + lo) * kRow + hi * 16 + q] = u[k1];
   __syncthreads();
 #pragma unroll
-  for (int c = 0; c < 8; ++c) v[c] = ex[t * kRow + c * 16 + q];
-  (void)twB;
+  for (int c = 0; c < 8; ++c) v[c] = act ? ex[t * kRow + c * 16 + q] : make_float2(0.f, 0.f);
+  dft<8, SG>(v);
+#pragma unroll
+  for (int m1 = 1; m1 < 8; ++m1) v[m1] = cmul(v[m1], twB[lo * 8 + m1]);
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int m1 = 0; m1 < 8; ++m1) ex[(8 * hi + m1) * kRow + lo * 16 + q] = v[m1];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 8; ++c) v[c] = act ? ex[t * kRow + c * 16 + q] : make_float2(0.f, 0.f);
+  // the caller runs the last 8-point DFT over t & 7 (all bins, or m2 < 4)
 }
 
-template <int SG>
+// the same steps backwards with the conjugate kernel: v (bins m2 of thread
+// 8 k1 + m1) -> u[k1] of thread t before the final NP-point DFT over k1
+template <int NP, int SG>
+__device__ __forceinline__ void r8_inverse(float2 (&v)[8], float2 (&u)[NP], float2* ex, const float2* twA,
+                                           const float2* twB, int t, int q) {
+  const int lo = t & 7, hi = t >> 3;
+  const bool act = R8<NP>::ACT >= 64 || t < R8<NP>::ACT;
+  dft<8, -SG>(v);   // over m2 -> j1
+  __syncthreads();  // the forward exchange's reads are done
+  if (act) {
+#pragma unroll
+    for (int j1 = 0; j1 < 8; ++j1) ex[t * kRow + j1 * 16 + q] = v[j1];
+  }
+  __syncthreads();
+  // thread t = 8 k1 + j1 gathers over m1
+#pragma unroll
+  for (int m1 = 0; m1 < 8; ++m1) v[m1] = act ? ex[(8 * hi + m1) * kRow + lo * 16 + q] : make_float2(0.f, 0.f);
+#pragma unroll
+  for (int m1 = 1; m1 < 8; ++m1) v[m1] = cmulc(v[m1], twB[lo * 8 + m1]);
+  dft<8, -SG>(v);   // over m1 -> j2
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int j2 = 0; j2 < 8; ++j2) ex[t * kRow + j2 * 16 + q] = v[j2];
+  }
+  __syncthreads();
+  // thread t = j1 + 8 j2 gathers over k1
+#pragma unroll
+  for (int k1 = 0; k1 < NP; ++k1) u[k1] = ex[(8 * k1 + lo) * kRow + hi * 16 + q];
+#pragma unroll
+  for (int k1 = 1; k1 < NP; ++k1) u[k1] = cmulc(u[k1], twA[t * NP + k1]);
+}
+
+// pad: p = r + beta p_old (or x), times S_c, positions i0 = t + 64 k < N0
+// (k < NP / 2), FFT along axis 0 of column (f, i1)
+template <int NP, int SG>
 __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
-  __shared__ float2 ex[64 * kRow];
-  __shared__ float2 twA[512];   // [t][k1] = exp(SG 2 pi i t k1 / 512)
-  __shared__ float2 twB[64];    // [lo][m1] = exp(SG 2 pi i lo m1 / 64)
+  constexpr int n = R8<NP>::n, H = NP / 2;
+  __shared__ float2 ex[R8<NP>::ACT * kRow];
+  __shared__ float2 twA[64 * NP];
+  __shared__ float2 twB[64];
   __shared__ double red[16];
   const int tid = threadIdx.x;
   const int q = tid & 15, t = tid >> 4, lo = t & 7, hi = t >> 3;
@@ -313,21 +387,20 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
   const int f = (int)(o / a.N1);
   const int64_t i1 = o - (int64_t)f * a.N1;
   if (a.fdone && a.fdone[f]) return;
-  fill_r8_tables<SG>(a.tw, twA, twB);
+  fill_r8_tables<NP, SG>(a.tw, twA, twB);
 
-  // image positions i0 = t + 64 k, k < 4 (i0 < N0 = 256); the rest is zero pad
   const int64_t ib = (int64_t)f * a.Ntot + (int64_t)a.N0 * i1;
   const int vi = t * 8;
   const Rsrc rS = rsrc(a.S + (int64_t)a.N0 * i1 * a.C + q0, ((int64_t)a.N0 * a.C - q0) * 8);
   const int vs = (t * a.C + q) * 8, ss = 64 * a.C * 8;
-  float2 u[8];
+  float2 u[NP];
   float pp = 0.f;
   if (a.pold) {   // p_new = r + beta p_old (CGLS.hpp:55-56), written by the coil-0 lanes
     const Rsrc rr = rsrc(a.r + ib, (int64_t)a.N0 * 8), rp = rsrc(a.pold + ib, (int64_t)a.N0 * 8);
     const Rsrc rn = rsrc(a.pnew + ib, (int64_t)a.N0 * 8);
-    float2 rv[4], pv[4], sv[4];
+    float2 rv[H], pv[H], sv[H];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < H; ++k) {
       rv[k] = bld2(rr, vi, k * 512);
       pv[k] = bld2(rp, vi, k * 512);
       sv[k] = bld2(rS, vs, k * ss);
@@ -376,7 +449,7 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
     }
     const bool c0 = (q0 + q) == 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < H; ++k) {
       const float2 xv = make_float2(rv[k].x + be * pv[k].x, rv[k].y + be * pv[k].y);
       if (c0) {
         bst2(rn, xv, vi, k * 512);
@@ -387,7 +460,7 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
   } else {
     const Rsrc rx = rsrc(a.x + ib, (int64_t)a.N0 * 8);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) u[k] = cmul(bld2(rx, vi, k * 512), bld2(rS, vs, k * ss));
+    for (int k = 0; k < H; ++k) u[k] = cmul(bld2(rx, vi, k * 512), bld2(rS, vs, k * ss));
   }
   if (a.pp_part && q0 == 0) {   // block-uniform
     const double s = block_sum<16>(pp, red);
@@ -395,30 +468,26 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
   }
   __syncthreads();   // twiddle tables
 
-  dft_zero_top<8, SG>(u);
+  dft_zero_top<NP, SG>(u);
   float2 v[8];
-  r8_exchanges_fwd_tw(u, v, ex, twA, twB, t, q);
+  r8_forward<NP, SG>(u, v, ex, twA, twB, t, q);
   dft<8, SG>(v);
+  // bins P = hi + NP lo + 8 NP m2 of column o: G[(o * n + P) * C + q0 + q]
+  if (R8<NP>::ACT >= 64 || t < R8<NP>::ACT) {
+    const Rsrc ro = rsrc(a.out + o * n * a.C + q0, ((int64_t)n * a.C - q0) * 8);
+    const int vb = ((hi + NP * lo) * a.C + q) * 8, sb = 8 * NP * a.C * 8;
 #pragma unroll
-  for (int m1 = 1; m1 < 8; ++m1) v[m1] = cmul(v[m1], twB[lo * 8 + m1]);
-  __syncthreads();
-#pragma unroll
-  for (int m1 = 0; m1 < 8; ++m1) ex[(8 * hi + m1) * kRow + lo * 16 + q] = v[m1];
-  __syncthreads();
-#pragma unroll
-  for (int c = 0; c < 8; ++c) v[c] = ex[t * kRow + c * 16 + q];
-  dft<8, SG>(v);
-  // bins P = hi + 8 lo + 64 m2 of column o: G[(o * 512 + P) * C + q0 + q]
-  const Rsrc ro = rsrc(a.out + o * 512 * a.C + q0, ((int64_t)512 * a.C - q0) * 8);
-  const int vb = ((hi + 8 * lo) * a.C + q) * 8;
-#pragma unroll
-  for (int m2 = 0; m2 < 8; ++m2) bst2(ro, v[m2], vb, m2 * ss);
+    for (int m2 = 0; m2 < 8; ++m2) bst2(ro, v[m2], vb, m2 * sb);
+  }
 }
 
-template <int SG>
+// crop: IFFT along axis 0 of column (f, i1), positions P = hi + NP lo +
+// 8 NP m2 < N0 (m2 < 4), conj(S_c) X_c summed over the 16 coils
+template <int NP, int SG>
 __global__ void __launch_bounds__(1024) k_toep_crop(ToepArgs a) {
-  __shared__ float2 ex[64 * kRow];
-  __shared__ float2 twA[512];
+  constexpr int n = R8<NP>::n;
+  __shared__ float2 ex[R8<NP>::ACT * kRow];
+  __shared__ float2 twA[64 * NP];
   __shared__ float2 twB[64];
   __shared__ double red[16];
   const int tid = threadIdx.x;
@@ -430,36 +499,27 @@ __global__ void __launch_bounds__(1024) k_toep_crop(ToepArgs a) {
   const int f = (int)(o / a.N1);
   const int64_t i1 = o - (int64_t)f * a.N1;
   if (a.fdone && a.fdone[f]) return;
-  fill_r8_tables<SG>(a.tw, twA, twB);
-  const Rsrc rin = rsrc(a.in + o * 512 * a.C + q0, ((int64_t)512 * a.C - q0) * 8);
+  fill_r8_tables<NP, SG>(a.tw, twA, twB);
+  const Rsrc rin = rsrc(a.in + o * n * a.C + q0, ((int64_t)n * a.C - q0) * 8);
   const int vs = (t * a.C + q) * 8, ss = 64 * a.C * 8;
-  float2 u[8];
+  float2 u[NP];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) u[k] = bld2(rin, vs, k * ss);
+  for (int k = 0; k < NP; ++k) u[k] = bld2(rin, vs, k * ss);
   __syncthreads();   // twiddle tables
 
-  dft<8, SG>(u);
+  dft<NP, SG>(u);
   float2 v[8];
-  r8_exchanges_fwd_tw(u, v, ex, twA, twB, t, q);
-  dft<8, SG>(v);
-#pragma unroll
-  for (int m1 = 1; m1 < 8; ++m1) v[m1] = cmul(v[m1], twB[lo * 8 + m1]);
-  __syncthreads();
-#pragma unroll
-  for (int m1 = 0; m1 < 8; ++m1) ex[(8 * hi + m1) * kRow + lo * 16 + q] = v[m1];
-  __syncthreads();
-#pragma unroll
-  for (int c = 0; c < 8; ++c) v[c] = ex[t * kRow + c * 16 + q];
-  dft_low_half<8, SG>(v);   // positions hi + 8 lo + 64 m2 < 256 (m2 < 4)
-  // X_c(i0) conj(S_c(i0)) summed over the 16 coils (lanes q)
-  const int pb = hi + 8 * lo;
+  r8_forward<NP, SG>(u, v, ex, twA, twB, t, q);
+  dft_low_half<8, SG>(v);   // positions hi + NP lo + 8 NP m2 < N0 (m2 < 4)
+  const bool act = R8<NP>::ACT >= 64 || t < R8<NP>::ACT;
+  const int pb = hi + NP * lo;
   const Rsrc rS = rsrc(a.S + (int64_t)a.N0 * i1 * a.C + q0, ((int64_t)a.N0 * a.C - q0) * 8);
-  const int vb = (pb * a.C + q) * 8;
+  const int vb = (pb * a.C + q) * 8, sb = 8 * NP * a.C * 8;
   float2 c[4];
 #pragma unroll
   for (int m2 = 0; m2 < 4; ++m2) {
-    const float2 sv = bld2(rS, vb, m2 * ss);
-    c[m2] = cmulc(v[m2], sv);
+    c[m2] = make_float2(0.f, 0.f);
+    if (act) c[m2] = cmulc(v[m2], bld2(rS, vb, m2 * sb));
   }
 #pragma unroll
   for (int m2 = 0; m2 < 4; ++m2)
@@ -469,17 +529,17 @@ __global__ void __launch_bounds__(1024) k_toep_crop(ToepArgs a) {
       c[m2].y += __shfl_xor(c[m2].y, m);
     }
   const int64_t rowb = ((int64_t)f * nb + g) * a.Ntot + (int64_t)a.N0 * i1;
-  const Rsrc ro = rsrc(a.out + rowb, (int64_t)a.N0 * 8);
   double pq = 0.0;
-  if (q == 0) {
-    const int vp = pb * 8;
+  if (q == 0 && act) {
+    const Rsrc ro = rsrc(a.out + rowb, (int64_t)a.N0 * 8);
+    const int vp = pb * 8, sp = 8 * NP * 8;
 #pragma unroll
-    for (int m2 = 0; m2 < 4; ++m2) bst2(ro, c[m2], vp, m2 * 512);
+    for (int m2 = 0; m2 < 4; ++m2) bst2(ro, c[m2], vp, m2 * sp);
     if (a.pdot) {
       const Rsrc rd = rsrc(a.pdot + (int64_t)f * a.Ntot + (int64_t)a.N0 * i1, (int64_t)a.N0 * 8);
 #pragma unroll
       for (int m2 = 0; m2 < 4; ++m2) {
-        const float2 pv = bld2(rd, vp, m2 * 512);
+        const float2 pv = bld2(rd, vp, m2 * sp);
         pq += (double)pv.x * c[m2].x + (double)pv.y * c[m2].y;
       }
     }
@@ -490,15 +550,61 @@ __global__ void __launch_bounds__(1024) k_toep_crop(ToepArgs a) {
   }
 }
 
+// conv on 64 threads per line (n = 2 N1 = 64 NP): lines along axis 1 as in
+// k_toep_conv, positions i1 = t + 64 k < N1 (k < NP / 2) in and out
+template <int NP, int SG>
+__global__ void __launch_bounds__(1024) k_toep_conv_r8(ToepArgs a) {
+  constexpr int n = R8<NP>::n, H = NP / 2;
+  __shared__ float2 ex[R8<NP>::ACT * kRow];
+  __shared__ float2 twA[64 * NP];
+  __shared__ float2 twB[64];
+  const int tid = threadIdx.x;
+  const int q = tid & 15, t = tid >> 4, lo = t & 7, hi = t >> 3;
+  const int64_t inner = (int64_t)a.C * 2 * a.N0;   // lines per frame
+  const int64_t nb = inner / 16;
+  const int f = (int)(blockIdx.x / nb);
+  const int64_t q0 = ((int64_t)blockIdx.x - (int64_t)f * nb) * 16;
+  if (a.fdone && a.fdone[f]) return;
+  const int64_t l0 = q0 / a.C;
+  fill_r8_tables<NP, SG>(a.tw, twA, twB);
+  float2* gb = a.out + (int64_t)f * a.N1 * inner + q0;
+  const Rsrc rg = rsrc(gb, ((int64_t)a.N1 * inner - q0) * 8);
+  const int vo = (int)(((int64_t)t * inner + q) * 8);
+  const int so = (int)(64 * inner * 8);
+  float2 u[NP];
+#pragma unroll
+  for (int k = 0; k < H; ++k) u[k] = bld2(rg, vo, k * so);
+  __syncthreads();   // twiddle tables
+
+  dft_zero_top<NP, SG>(u);
+  float2 v[8];
+  r8_forward<NP, SG>(u, v, ex, twA, twB, t, q);
+  dft<8, SG>(v);
+  if (R8<NP>::ACT >= 64 || t < R8<NP>::ACT) {
+    const Rsrc rt = rsrc(a.That + ((int64_t)f * 2 * a.N0 + l0) * n, (int64_t)n * 4);
+    const int vt = (hi + NP * lo) * 4;
+#pragma unroll
+    for (int m2 = 0; m2 < 8; ++m2) {
+      const float w = bld1(rt, vt + m2 * 8 * NP * 4, 0);
+      v[m2] = make_float2(v[m2].x * w, v[m2].y * w);
+    }
+  }
+  r8_inverse<NP, SG>(v, u, ex, twA, twB, t, q);
+  dft_low_half<NP, -SG>(u);   // positions t + 64 k, k < H
+#pragma unroll
+  for (int k = 0; k < H; ++k) bst2(rg, u[k], vo, k * so);
+}
+
 }  // namespace
 
-// 2D only; pad/crop need 2 N0 = 512, the conv 2 N1 in {256, 512}; all need
-// C % 16 == 0 and every per-frame byte offset below 2^31
+// 2D only, C % 16 == 0 and every per-frame byte offset below 2^31: pad/crop
+// at 2 N0 = 384 or 512 (64 threads per line), the conv at 2 N1 = 256 or 512
+// (16 threads per line) or 384 (64 threads per line)
 bool toep_pad_crop_supported(int N0, int N1, int C) {
-  return N0 == 256 && C % 16 == 0 && N1 > 0 && (int64_t)512 * C * 8 < (int64_t)1 << 31;
+  return (N0 == 192 || N0 == 256) && C % 16 == 0 && N1 > 0 && (int64_t)2 * N0 * C * 8 < (int64_t)1 << 31;
 }
 bool toep_conv_supported(int N0, int N1, int C) {
-  return (N1 == 128 || N1 == 256) && C % 16 == 0 && N0 > 0 &&
+  return (N1 == 128 || N1 == 192 || N1 == 256) && C % 16 == 0 && N0 > 0 &&
          (int64_t)N1 * C * 2 * N0 * 8 < (int64_t)1 << 31;
 }
 
@@ -515,20 +621,23 @@ static size_t occ_pad_lds(const char* env, size_t static_lds) {
 void launch_toep_pad(const ToepArgs& a, hipStream_t s) {
   const dim3 grid((unsigned)((int64_t)a.F * a.N1 * (a.C / 16)));
   static const size_t extra = occ_pad_lds("CGS_TOEP_OCC_PC", 78464);
-  CGS_LAUNCH((k_toep_pad<-1>), grid, dim3(1024), extra, s, a);
+  if (a.N0 == 256) CGS_LAUNCH((k_toep_pad<8, -1>), grid, dim3(1024), extra, s, a);
+  else CGS_LAUNCH((k_toep_pad<6, -1>), grid, dim3(1024), 0, s, a);
 }
 
 void launch_toep_conv(const ToepArgs& a, hipStream_t s) {
-  const dim3 grid((unsigned)((int64_t)a.F * a.C * 2 * a.N0 / 16));
+  const int64_t lines = (int64_t)a.F * a.C * 2 * a.N0;
   static const size_t extra = occ_pad_lds("CGS_TOEP_OCC_CONV", 37888);
-  if (a.N1 == 256) CGS_LAUNCH((k_toep_conv<32, -1>), grid, dim3(256), extra, s, a);
-  else CGS_LAUNCH((k_toep_conv<16, -1>), grid, dim3(256), extra, s, a);
+  if (a.N1 == 256) CGS_LAUNCH((k_toep_conv<32, -1>), dim3((unsigned)(lines / 16)), dim3(256), extra, s, a);
+  else if (a.N1 == 128) CGS_LAUNCH((k_toep_conv<16, -1>), dim3((unsigned)(lines / 16)), dim3(256), extra, s, a);
+  else CGS_LAUNCH((k_toep_conv_r8<6, -1>), dim3((unsigned)(lines / 16)), dim3(1024), 0, s, a);
 }
 
 void launch_toep_crop(const ToepArgs& a, hipStream_t s) {
   const dim3 grid((unsigned)((int64_t)a.F * a.N1 * (a.C / 16)));
   static const size_t extra = occ_pad_lds("CGS_TOEP_OCC_PC", 78464);
-  CGS_LAUNCH((k_toep_crop<1>), grid, dim3(1024), extra, s, a);
+  if (a.N0 == 256) CGS_LAUNCH((k_toep_crop<8, 1>), grid, dim3(1024), extra, s, a);
+  else CGS_LAUNCH((k_toep_crop<6, 1>), grid, dim3(1024), 0, s, a);
 }
 
 }  // namespace cgs
