@@ -263,12 +263,12 @@ def warm_start_leg(args, plan, y, x, tol, xt):
     xin = torch.empty((1, N1, N0), dtype=torch.complex64, device=dev)
     x0 = torch.empty((1, N1, N0), dtype=torch.complex64, device=dev)
 
-    def plain():
+    def plain(tol=tol):
         _, rep = plan.solve(y, maxit=args.maxit, tol=tol, lam=lam, out=x, toeplitz=True,
                             graph_iters=args.graph_iters)
         return rep
 
-    def warm():
+    def warm(tol=tol):
         plan.EH(y, out=b)
         plan.normal(b, lam=lam, out=Ab, toeplitz=True)
         plan.sync()
@@ -298,6 +298,9 @@ def warm_start_leg(args, plan, y, x, tol, xt):
                      "rel_err_vs_phantom": round(float(np.linalg.norm(xr - xt) / np.linalg.norm(xt)), 5)}
     res["iteration_ratio"] = round(res["plain"]["iters"] / max(1, res["warm"]["iters"]), 3)
     res["speedup"] = round(res["warm"]["solves_per_s"] / res["plain"]["solves_per_s"], 3)
+    # the same comparison at looser stopping tolerances (iterations only)
+    res["iters_plain_warm_by_tol"] = {f"{t:g}": [plain(t)["max_iters"], warm(t)["max_iters"]]
+                                      for t in (1e-3, 1e-4, tol)}
     return res
 
 
