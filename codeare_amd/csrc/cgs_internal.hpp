@@ -306,6 +306,7 @@ void launch_cg_tail(const Plan& p, float2* x, float2* r, const float2* pv, const
                     float lambda, const double* pq_part, int npq, const double* pp_part, int npp, bool snap,
                     hipStream_t s);
 const double* cg_tail_rpartials(const Plan& p);
+int cg_tail_blocks(const Plan& p);
 void launch_partial_combine(const Plan& p, const float2* part, int G, float2* q, const float2* pvec, float lambda,
                             double* partials, const int* fl, hipStream_t s);
 void launch_coil_combine(const Plan& p, const float2* Xc, float2* x, const float2* pvec,
@@ -325,7 +326,7 @@ void launch_coil_alpha(const Plan& p, const float2* Xc, float2* q, const float2*
 void launch_cg_xr_beta(const Plan& p, float2* x, float2* r, const float2* pv, const float2* q, int tolkind,
                        double tol, int maxit, hipStream_t s);
 void launch_cg_xr_alpha(const Plan& p, float2* x, float2* r, const float2* pv, const float2* q, hipStream_t s);
-void launch_cg_beta_after_xr(const Plan& p, int tolkind, double tol, int maxit, hipStream_t s);
+void launch_cg_beta_after_xr(const Plan& p, int nparts, int tolkind, double tol, int maxit, hipStream_t s);
 void launch_axpby_img(const Plan& p, float2* out, const float2* a, float alpha, const float2* b,
                       float beta, hipStream_t s);
 // layout conversions

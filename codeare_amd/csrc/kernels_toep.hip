@@ -415,8 +415,15 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
       // rn_snap and it_snap).
       __shared__ double fold_s[2];   // beta, done
       if (tid < 64) {
+        // eight loads in flight per lane (nrpart <= kRedBlocks = 512)
+        const double* rp = a.rpart + (int64_t)f * a.nrpart;
         double v = 0.0;
-        for (int i = tid; i < a.nrpart; i += 64) v += a.rpart[(int64_t)f * a.nrpart + i];
+        for (int i0 = tid; i0 < a.nrpart; i0 += 512) {
+          double t[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) t[u] = i0 + 64 * u < a.nrpart ? rp[i0 + 64 * u] : 0.0;
+          v += ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+        }
 #pragma unroll
         for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
         if (tid == 0) {

@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(256) k_cg_tail(float2* x, float2* r, const flo
                                                  float lambda, CgState* st, const double* pq_part, int npq,
                                                  const double* pp_part, int npp, double* partials, int64_t Ntot,
                                                  const int* fdone, int snap) {
-  __shared__ double sh[4];
+  __shared__ double sh[4], sh2[4];
   __shared__ float alpha_s;
   const int f = blockIdx.y;
   if (fdone[f]) return;
@@ -149,11 +149,34 @@ __global__ void __launch_bounds__(256) k_cg_tail(float2* x, float2* r, const flo
     r0 = r[o];
   }
   {
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < npq; i += blockDim.x) a += pq_part[(int64_t)f * npq + i];
-    for (int i = threadIdx.x; i < npp; i += blockDim.x) b += pp_part[(int64_t)f * npp + i];
-    const double pq = block_sum(a, sh) + (double)lambda * block_sum(b, sh);
+    // every block sums the partials in the same order; four loads in flight
+    // per thread, then one LDS round (blockDim = 256)
+    const double* pqf = pq_part + (int64_t)f * npq;
+    const double* ppf = pp_part + (int64_t)f * npp;
+    auto sum4 = [](const double* v, int n) {   // v[t], v[t + 256], ... with 4 loads issued together
+      double acc = 0.0;
+      for (int i = threadIdx.x; i < n; i += 1024) {
+        const double t0 = v[i];
+        const double t1 = i + 256 < n ? v[i + 256] : 0.0;
+        const double t2 = i + 512 < n ? v[i + 512] : 0.0;
+        const double t3 = i + 768 < n ? v[i + 768] : 0.0;
+        acc += (t0 + t1) + (t2 + t3);
+      }
+      return acc;
+    };
+    double a = sum4(pqf, npq), b = sum4(ppf, npp);
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+      a += __shfl_xor(a, m);
+      b += __shfl_xor(b, m);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      sh[threadIdx.x >> 6] = a;
+      sh2[threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+      const double pq = ((sh[0] + sh[1]) + (sh[2] + sh[3])) + (double)lambda * ((sh2[0] + sh2[1]) + (sh2[2] + sh2[3]));
       const double al = st[f].rn / pq;   // CGLS.hpp:50
       alpha_s = (float)al;
       if (blockIdx.x == 0) {
@@ -166,8 +189,8 @@ __global__ void __launch_bounds__(256) k_cg_tail(float2* x, float2* r, const flo
         }
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
   const float al = alpha_s;
   double acc = 0.0;
   if (has) {
@@ -198,14 +221,19 @@ __global__ void __launch_bounds__(256) k_cg_tail(float2* x, float2* r, const flo
   if (threadIdx.x == 0) partials[(int64_t)f * gridDim.x + blockIdx.x] = t;
 }
 
-// ||r||^2 partials of the CG tail: kRedBlocks per frame
+// ||r||^2 partials of the CG tail: cg_tail_blocks(p) per frame (one pixel
+// per thread up to kRedBlocks blocks)
 const double* cg_tail_rpartials(const Plan& p) { return p.partials + (size_t)kCoilBlocks * p.F; }
+int cg_tail_blocks(const Plan& p) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (p.Ntot + 255) / 256));
+}
 
 void launch_cg_tail(const Plan& p, float2* x, float2* r, const float2* pv, const float2* part, int G,
                     float lambda, const double* pq_part, int npq, const double* pp_part, int npp, bool snap,
                     hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_tail, dim3(kRedBlocks, p.F), dim3(256), 0, s, x, r, pv, part, G, lambda, p.cgst, pq_part,
-                     npq, pp_part, npp, p.partials + (size_t)kCoilBlocks * p.F, p.Ntot, p.fdone, snap ? 1 : 0);
+  hipLaunchKernelGGL(k_cg_tail, dim3(cg_tail_blocks(p), p.F), dim3(256), 0, s, x, r, pv, part, G, lambda, p.cgst,
+                     pq_part, npq, pp_part, npp, p.partials + (size_t)kCoilBlocks * p.F, p.Ntot, p.fdone,
+                     snap ? 1 : 0);
 }
 
 void launch_partial_combine(const Plan& p, const float2* part, int G, float2* q, const float2* pvec, float lambda,
@@ -335,8 +363,12 @@ void launch_cg_xr_alpha(const Plan& p, float2* x, float2* r, const float2* pv, c
                      kCoilBlocks, p.partials + (size_t)kCoilBlocks * p.F, p.Ntot, p.fdone);
 }
 
-void launch_cg_beta_after_xr(const Plan& p, int tolkind, double tol, int maxit, hipStream_t s) {
-  launch_cg_beta(p, p.partials + (size_t)kCoilBlocks * p.F, tolkind, tol, maxit, s);
+__global__ void k_cg_beta(CgState* st, const double* part, int nb, int tolkind, double tol, int maxit,
+                          double* hist, int hcap, int* fdone);
+
+void launch_cg_beta_after_xr(const Plan& p, int nparts, int tolkind, double tol, int maxit, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_beta, dim3(p.F), dim3(256), 0, s, p.cgst, p.partials + (size_t)kCoilBlocks * p.F, nparts,
+                     tolkind, tol, maxit, p.hist, p.hist_cap, p.fdone);
 }
 
 // x += alpha p ; r -= alpha q ; partial ||r||^2 ; last block: beta, residual, done

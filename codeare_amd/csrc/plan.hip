@@ -1122,7 +1122,7 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
       if (want_tail && fold && folded) {
         // beta and the stop test of the previous iteration move into this pass
         a.rpart = cg_tail_rpartials(p);
-        a.nrpart = kRedBlocks;
+        a.nrpart = cg_tail_blocks(p);
         a.tolkind = fold->tol_kind;
         a.tol = fold->tol;
         a.maxit = fold->maxit;
@@ -1212,7 +1212,7 @@ static int cg_iteration_fused(Plan& p, int normal_op, float2* x, float2* r, cons
       Prof pr(p, CGS_K_CG, ((double)G + 6.0) * 8.0 * p.F * p.Ntot);
       launch_cg_tail(p, x, r, pnew, p.Xc, G, (float)o.lambda, p.tail_pq, G * opf, p.tail_pp, opf, folded,
                      p.stream);
-      if (!folded) launch_cg_beta_after_xr(p, o.tol_kind, o.tol, o.maxit, p.stream);
+      if (!folded) launch_cg_beta_after_xr(p, cg_tail_blocks(p), o.tol_kind, o.tol, o.maxit, p.stream);
       p.beta_folded = folded;
       return CGS_OK;
     }
@@ -1228,7 +1228,7 @@ static int cg_iteration_fused(Plan& p, int normal_op, float2* x, float2* r, cons
   {
     Prof pr(p, CGS_K_CG, 5.0 * 8.0 * p.F * p.Ntot);
     launch_cg_xr_alpha(p, x, r, pnew, q, p.stream);
-    launch_cg_beta_after_xr(p, o.tol_kind, o.tol, o.maxit, p.stream);
+    launch_cg_beta_after_xr(p, kRedBlocks, o.tol_kind, o.tol, o.maxit, p.stream);
   }
   return CGS_OK;
 }
@@ -1347,7 +1347,7 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
   // still pending when the loop ends at maxit: commit it (frames already done
   // are skipped)
   if (p.beta_folded && o.eh_mode == CGS_EH_TRUE && o.normal_op == CGS_NORMAL_TOEPLITZ)
-    launch_cg_beta_after_xr(p, o.tol_kind, o.tol, o.maxit, p.stream);
+    launch_cg_beta_after_xr(p, cg_tail_blocks(p), o.tol_kind, o.tol, o.maxit, p.stream);
   HIPCHK(hipMemcpyAsync(xout, x, ib, hipMemcpyDeviceToDevice, p.stream));
   HIPCHK(hipMemcpyAsync(p.h_cgst, p.cgst, sizeof(CgState) * F, hipMemcpyDeviceToHost, p.stream));
   HIPCHK(hipMemsetAsync(p.fdone, 0, sizeof(int) * F, p.stream));
