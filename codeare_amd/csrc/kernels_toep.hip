@@ -398,6 +398,15 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
   if (a.pold) {   // p_new = r + beta p_old (CGLS.hpp:55-56), written by the coil-0 lanes
     const Rsrc rr = rsrc(a.r + ib, (int64_t)a.N0 * 8), rp = rsrc(a.pold + ib, (int64_t)a.N0 * 8);
     const Rsrc rn = rsrc(a.pnew + ib, (int64_t)a.N0 * 8);
+    // the CG tail's ||r||^2 partials (nrpart <= kRedBlocks = 512, eight per
+    // lane of wave 0) are loaded before the CG state says whether they are
+    // needed, so both latencies overlap
+    double rt[8];
+    if (a.rpart && tid < 64) {
+      const double* rpp = a.rpart + (int64_t)f * a.nrpart;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) rt[u] = tid + 64 * u < a.nrpart ? rpp[tid + 64 * u] : 0.0;
+    }
     float2 rv[H], pv[H], sv[H];
 #pragma unroll
     for (int k = 0; k < H; ++k) {
@@ -415,15 +424,7 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
       // rn_snap and it_snap).
       __shared__ double fold_s[2];   // beta, done
       if (tid < 64) {
-        // eight loads in flight per lane (nrpart <= kRedBlocks = 512)
-        const double* rp = a.rpart + (int64_t)f * a.nrpart;
-        double v = 0.0;
-        for (int i0 = tid; i0 < a.nrpart; i0 += 512) {
-          double t[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) t[u] = i0 + 64 * u < a.nrpart ? rp[i0 + 64 * u] : 0.0;
-          v += ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
-        }
+        double v = ((rt[0] + rt[1]) + (rt[2] + rt[3])) + ((rt[4] + rt[5]) + (rt[6] + rt[7]));
 #pragma unroll
         for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
         if (tid == 0) {
