@@ -169,6 +169,7 @@ struct Plan {
   bool use_interp_lds = true;         // gather from the tile staged in LDS
   bool use_grid_batched = true;       // batched visit lists (else one visit at a time)
   bool use_grid_reg = true;           // 2D: register accumulation, lane = cell of an 8x8 tile
+  bool use_grid_mfma = false;         // 2D: scatter on the matrix cores (k_grid_mfma)
   int grid_cgr = 16;                  // coils per wave in the register kernel
   bool use_grid_atomic = false;       // LDS float atomics variant (kept for comparison)
   int tile_item_max = 64;             // samples per tile-level interp item
@@ -297,6 +298,7 @@ void launch_interp_lds(const SampArgs& a, const int4* items, int nitems, int T, 
 size_t grid_batched_lds(int d, int T, int taps, int CG, int rowstride);
 void launch_grid_atomic(const GridArgs& a, const int4* vrec, int cg, int nitems, int ncgroups, hipStream_t s);
 void launch_grid_reg(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s);
+void launch_grid_mfma(const GridArgs& a, const int4* vrec, int nitems, hipStream_t s);
 size_t interp_lds3_bytes(int T, int taps, int cg, int maxs);
 void launch_interp_lds3(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int nt1, int cg, int maxs,
                         hipStream_t s);

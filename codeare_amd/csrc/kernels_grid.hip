@@ -13,6 +13,8 @@
 // a fixed summation order (deterministic).  Tiles whose visit lists are long
 // (the k-space centre of radial trajectories) are split across several items
 // that write partial tiles, summed afterwards in a fixed order.
+#include <cstdlib>
+
 #include "cgs_internal.hpp"
 
 namespace cgs {
@@ -763,6 +765,171 @@ void launch_grid_reg(const GridArgs& a, const int4* vrec, int cgr, int nitems, i
   if (cgr == 8) grid_reg_taps<8>(a, vrec, nitems, g, shm, s);
   else if (cgr == 16) grid_reg_taps<16>(a, vrec, nitems, g, shm, s);
   else grid_reg_taps<32>(a, vrec, nitems, g, shm, s);
+}
+
+// ---------------------------------------------------------------------------
+// adjoint B^T on the matrix cores (2D, 8x8 tiles).  The tile's grid values are
+// D[cell][col] (64 cells x the real columns re/im of CC = 16 NT coils) and the
+// work item's visit list is a K dimension: D += W[cell][visit] Y[visit][col],
+// with W the visit's separable window weight at the cell (0 outside its
+// footprint) and Y its weighted data.  One wave per work item; every
+// v_mfma_f32_32x32x2f32 (FP32 products and sums) takes two visits for one
+// (32-cell half, 32-column tile).  The zero weights of cells outside a visit's
+// footprint cost matrix-core cycles instead of VALU issue, and the weights are
+// formed while the previous products run.  Operand layout: A lane l =
+// W[l % 32][l / 32], B lane l = Y[l / 32][l % 32], accumulator register r of
+// lane l = D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].  Each cell still has one
+// owner and visits are applied in list order: deterministic, no atomics.
+typedef float gf32x16 __attribute__((ext_vector_type(16)));
+
+template <int NT, int TAPS, int B>
+__global__ void __launch_bounds__(256) k_grid_mfma(GridArgs a, const int4* __restrict__ vrec, int nitems) {
+  constexpr int NPSI = 2 * TAPS, NCOL = 32 * NT, CC = 16 * NT;
+  constexpr int WAVE_FLOATS = B * NCOL + B * NPSI + B * 4;
+  extern __shared__ float shf[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int item = blockIdx.x * 4 + wave;
+  if (item >= nitems) return;          // waves of a block never synchronise with each other
+  const GridItem it = a.items[item];
+  if (a.fdone && a.fdone[it.frame]) return;
+  float* wy = shf + wave * WAVE_FLOATS;           // [B][NCOL]
+  float* wpsi = wy + B * NCOL;                    // [B][NPSI]
+  int4* wrec = (int4*)(wpsi + B * NPSI);          // [B]
+  const int cbase = blockIdx.y * CC;
+  const int t0 = it.tile % a.nt0, t1 = it.tile / a.nt0;
+  const int ts0 = t0 * 8, ts1 = t1 * 8;
+  const int te0 = min(8, a.n0 - ts0), te1 = min(8, a.n1 - ts1);
+  const int i = lane & 31, kk = lane >> 5;
+  const int cx = i & 7, cy0 = i >> 3, cy1 = 4 + (i >> 3);   // A rows: cells i and 32 + i
+  gf32x16 acc[2][NT];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[h][t][r] = 0.f;
+
+  for (int vb = it.vbeg; vb < it.vend; vb += B) {
+    const int nb = min(B, it.vend - vb);
+    if (lane < nb) wrec[lane] = vrec[vb + lane];
+    __builtin_amdgcn_wave_barrier();
+    {
+      // every load of the batch in flight before the first LDS write
+      constexpr int UP = (B * NPSI + 63) / 64, UY = (B * CC + 63) / 64;
+      float pv[UP];
+      float2 yv[UY];
+#pragma unroll
+      for (int u = 0; u < UP; ++u) {
+        const int idx = lane + 64 * u;
+        pv[u] = 0.f;
+        if (idx < nb * NPSI) {
+          const int v = idx / NPSI, q = idx - v * NPSI;
+          pv[u] = a.psi[(int64_t)wrec[v].x * NPSI + q];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UY; ++u) {   // float2 units: (visit, coil)
+        const int idx = lane + 64 * u;
+        yv[u] = make_float2(0.f, 0.f);
+        if (idx < nb * CC) {
+          const int v = idx / CC, c = idx - v * CC;
+          if (cbase + c < a.C) {
+            const int smp = wrec[v].x;
+            yv[u] = a.y[(int64_t)smp * a.C + cbase + c];
+            if (a.w) { const float ww = a.w[smp]; yv[u].x *= ww; yv[u].y *= ww; }
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UP; ++u) {
+        const int idx = lane + 64 * u;
+        if (idx < nb * NPSI) wpsi[idx] = pv[u];
+      }
+#pragma unroll
+      for (int u = 0; u < UY; ++u) {
+        const int idx = lane + 64 * u;
+        if (idx < nb * CC) ((float2*)wy)[idx] = yv[u];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int v = 0; v < nb; v += 2) {
+      const int vk = v + kk;   // this lane's visit of the pair (K index)
+      float w0 = 0.f, w1 = 0.f;
+      float bv[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) bv[t] = 0.f;
+      if (vk < nb) {
+        const int4 rc = wrec[vk];
+        const int rel0 = (int)(short)(rc.y & 0xffff), rel1 = (int)(short)(rc.y >> 16);
+        const float* ps = wpsi + vk * NPSI;
+        const int dx = cx - rel0, dy0 = cy0 - rel1, dy1 = cy1 - rel1;
+        if ((unsigned)dx < (unsigned)TAPS) {
+          const float px = ps[dx];
+          if ((unsigned)dy0 < (unsigned)TAPS) w0 = px * ps[TAPS + dy0];
+          if ((unsigned)dy1 < (unsigned)TAPS) w1 = px * ps[TAPS + dy1];
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) bv[t] = wy[vk * NCOL + 32 * t + i];
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(w0, bv[t], acc[0][t], 0, 0, 0);
+        acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(w1, bv[t], acc[1][t], 0, 0, 0);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+
+  const int ncol = 2 * min(CC, a.C - cbase);   // real columns of this coil group
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int cell = 32 * h + 8 * (r >> 2) + 4 * kk + (r & 3);
+      const int x0 = cell & 7, x1 = cell >> 3;
+      if (x0 >= te0 || x1 >= te1) continue;
+      float* out;
+      if (it.slot < 0) {
+        const int64_t l = (ts0 + x0) + (int64_t)a.n0 * (ts1 + x1);
+        out = (float*)(a.grid + ((int64_t)it.frame * a.ntot + l) * a.C + cbase);
+      } else {
+        out = (float*)(a.partial + ((int64_t)it.slot * a.tilecells + x0 + 8 * x1) * a.C + cbase);
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int col = 32 * t + i;
+        if (col < ncol) out[col] = acc[h][t][r];
+      }
+    }
+}
+
+template <int NT, int B>
+static void grid_mfma_taps(const GridArgs& a, const int4* vrec, int nitems, dim3 g, size_t shm, hipStream_t s) {
+  switch (a.taps) {
+    case 4: hipLaunchKernelGGL((k_grid_mfma<NT, 4, B>), g, dim3(256), shm, s, a, vrec, nitems); break;
+    case 6: hipLaunchKernelGGL((k_grid_mfma<NT, 6, B>), g, dim3(256), shm, s, a, vrec, nitems); break;
+    case 8: hipLaunchKernelGGL((k_grid_mfma<NT, 8, B>), g, dim3(256), shm, s, a, vrec, nitems); break;
+    case 10: hipLaunchKernelGGL((k_grid_mfma<NT, 10, B>), g, dim3(256), shm, s, a, vrec, nitems); break;
+    default: break;
+  }
+}
+
+// NT column tiles of 16 coils per wave (CGS_GRID_MFMA_NT, default 1: more
+// waves per SIMD), visit batches of B (CGS_GRID_MFMA_B, 16 or 32)
+void launch_grid_mfma(const GridArgs& a, const int4* vrec, int nitems, hipStream_t s) {
+  if (nitems == 0) return;
+  static const int nt_env = [] { const char* e = std::getenv("CGS_GRID_MFMA_NT"); return e ? std::atoi(e) : 1; }();
+  static const int b_env = [] { const char* e = std::getenv("CGS_GRID_MFMA_B"); return e ? std::atoi(e) : 32; }();
+  const int NT = (a.C <= 16 || nt_env != 2) ? 1 : 2;
+  const int B = b_env == 16 ? 16 : 32;
+  const int CC = 16 * NT;
+  const dim3 g((unsigned)((nitems + 3) / 4), (unsigned)((a.C + CC - 1) / CC));
+  const int npsi = 2 * a.taps;
+  const size_t shm = sizeof(float) * 4 * (B * 32 * NT + B * npsi + B * 4);
+  if (NT == 1 && B == 16) grid_mfma_taps<1, 16>(a, vrec, nitems, g, shm, s);
+  else if (NT == 1) grid_mfma_taps<1, 32>(a, vrec, nitems, g, shm, s);
+  else if (B == 16) grid_mfma_taps<2, 16>(a, vrec, nitems, g, shm, s);
+  else grid_mfma_taps<2, 32>(a, vrec, nitems, g, shm, s);
 }
 
 size_t grid_atomic_lds(int d, int T, int taps, int CG, int rowstride) {

@@ -432,7 +432,9 @@ static int gridding(Plan& p, const float2* y, bool weighted, const int* fl) {
   const double bytes = (double)p.Mtot * p.d * 4 + (double)p.Mtot * 4 + (double)p.C * p.Mtot * 8 +
                        (double)p.C * p.ntot * p.F * 8;
   Prof pr(p, CGS_K_GRID, bytes);
-  if (p.d == 2 && p.T == 8 && p.use_grid_reg) {
+  if (p.d == 2 && p.T == 8 && p.use_grid_mfma) {
+    launch_grid_mfma(a, p.vrec, p.n_grid_items, p.stream);
+  } else if (p.d == 2 && p.T == 8 && p.use_grid_reg) {
     const int cgr = p.grid_cgr;
     launch_grid_reg(a, p.vrec, cgr, p.n_grid_items, (p.C + cgr - 1) / cgr, p.stream);
   } else if (p.d == 3 && p.T == 8 && p.use_grid_reg) {
@@ -861,6 +863,8 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   if (const char* e = std::getenv("CGS_TILE_ITEM_MAX")) p.tile_item_max = std::max(8, std::atoi(e));
   if (const char* e = std::getenv("CGS_INTERP_LDS")) p.use_interp_lds = std::atoi(e) != 0;
   if (const char* e = std::getenv("CGS_GRID_REG")) p.use_grid_reg = std::atoi(e) != 0;
+  p.use_grid_mfma = (d == 2);
+  if (const char* e = std::getenv("CGS_GRID_MFMA")) p.use_grid_mfma = std::atoi(e) != 0;
   if (const char* e = std::getenv("CGS_GRID_CGR")) {
     const int g = std::atoi(e);
     if (g == 8 || g == 16 || g == 32) p.grid_cgr = g;
@@ -1477,6 +1481,7 @@ namespace cgs {
 struct TuneChoice {
   int vpi, cgr, icg;
   double grid_ms, interp_ms, def_grid_ms, def_interp_ms;
+  bool mfma;
 };
 static std::mutex g_tune_mu;
 static std::vector<std::pair<std::string, TuneChoice>> g_tune_cache;
@@ -1537,22 +1542,28 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
     reps = std::max(1, std::min(reps, 50));
     HIPCHK(hipMemsetAsync(p.ybuf[0], 0, sizeof(float2) * (size_t)p.Mtot * p.C, p.stream));
     HIPCHK(hipMemsetAsync(p.grid, 0, sizeof(float2) * (size_t)p.ntot * p.C * p.F, p.stream));
-    const bool reg = p.d == 2 && p.T == 8 && p.use_grid_reg;
+    const bool tiles8 = p.d == 2 && p.T == 8;
+    const bool reg = tiles8 && p.use_grid_reg;
     std::vector<int> vpis = (p.d == 2) ? std::vector<int>{256, 512, 1024} : std::vector<int>{64, 128, 256};
     std::vector<int> cgrs = (reg && p.C > 8) ? std::vector<int>{8, 16} : std::vector<int>{p.grid_cgr};
     std::vector<int> icgs = (p.C > 8) ? std::vector<int>{8, 16} : std::vector<int>{p.interp_cg};
     const int vpi0 = p.visits_per_item, cgr0 = p.grid_cgr, icg0 = p.interp_cg;
+    const bool mfma0 = p.use_grid_mfma;
     double g0 = 0, i0 = 0;
     CHK(time_kernel(p, reps, true, &g0));
     CHK(time_kernel(p, reps, false, &i0));
-    best = TuneChoice{vpi0, cgr0, icg0, g0, i0, g0, i0};
+    best = TuneChoice{vpi0, cgr0, icg0, g0, i0, g0, i0, mfma0};
+    std::vector<int> kinds = tiles8 ? std::vector<int>{1, 0} : std::vector<int>{0};   // 1: matrix cores
     for (int vpi : vpis) {
       CHK(apply_vpi(p, vpi));
-      for (int cgr : cgrs) {
-        p.grid_cgr = cgr;
-        double t = 0;
-        CHK(time_kernel(p, reps, true, &t));
-        if (t < best.grid_ms) { best.grid_ms = t; best.vpi = vpi; best.cgr = cgr; }
+      for (int kind : kinds) {
+        p.use_grid_mfma = kind == 1;
+        for (int cgr : (kind == 1 ? std::vector<int>{p.grid_cgr} : cgrs)) {
+          p.grid_cgr = cgr;
+          double t = 0;
+          CHK(time_kernel(p, reps, true, &t));
+          if (t < best.grid_ms) { best.grid_ms = t; best.vpi = vpi; best.cgr = cgr; best.mfma = kind == 1; }
+        }
       }
     }
     for (int icg : icgs) {   // the gather does not depend on the visit lists
@@ -1567,6 +1578,7 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
   CHK(apply_vpi(p, best.vpi));
   p.grid_cgr = best.cgr;
   p.interp_cg = best.icg;
+  p.use_grid_mfma = best.mfma;
   // captured graphs hold the old launch shapes
   if (p.graph_exec) (void)hipGraphExecDestroy(p.graph_exec);
   p.graph_exec = nullptr;
@@ -1580,6 +1592,7 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
     rep->interp_ms = best.interp_ms;
     rep->default_grid_ms = best.def_grid_ms;
     rep->default_interp_ms = best.def_interp_ms;
+    rep->grid_mfma = best.mfma ? 1 : 0;
   }
   return CGS_OK;
 }

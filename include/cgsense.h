@@ -154,10 +154,12 @@ typedef struct {
   int from_cache;        /* 1: taken from an earlier plan of the same bucket     */
   double grid_ms, interp_ms;                  /* chosen configuration             */
   double default_grid_ms, default_interp_ms;  /* the plan's defaults              */
+  int grid_mfma;         /* gridding: 1 matrix-core scatter, 0 VALU registers    */
 } cgs_tune_report;
-/* Times the gridding scatter and the window gather of this plan's trajectory
- * over a small grid of launch shapes (`reps` timed runs each) and keeps the
- * fastest.  Results are cached per process under (d, n, C, frames, samples
+/* Times the gridding scatter (2D: matrix-core or VALU-register kernel, visits
+ * per work item, coils per pass) and the window gather of this plan's
+ * trajectory over a small grid of launch shapes (`reps` timed runs each) and
+ * keeps the fastest.  Results are cached per process under (d, n, C, frames, samples
  * per frame rounded to a power of two); with use_cache != 0 a cached bucket is
  * applied without timing.  Needs the trajectory; clobbers the plan's
  * intermediate buffers (not the trajectory, weights, maps or Toeplitz data). */
