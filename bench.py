@@ -39,7 +39,9 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary measurements")
     ap.add_argument("--cpu-iters", type=int, default=2, help="CG iterations in the CPU sample")
     ap.add_argument("--no-dynamic", action="store_true", help="skip the 64-frame batched series")
-    ap.add_argument("--volume", action="store_true", help="also time the 3D koosh-ball workload")
+    ap.add_argument("--volume", dest="volume", action="store_true", default=None,
+                    help="time the 3D koosh-ball workload (default: with the secondary measurements)")
+    ap.add_argument("--no-volume", dest="volume", action="store_false", help="skip the 3D koosh-ball workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with 3 device copies of 256 MiB (counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "codeare_amd", "weights", "unet_golden2d_256_c32.npz"),
@@ -225,7 +227,7 @@ def main():
         out["warm_start_solve"] = warm_start_leg(args, plan, y, x, tol, xt)
         if not args.no_dynamic:
             out["dynamic_batch"] = dynamic_leg(args)
-    if args.volume:
+    if args.volume if args.volume is not None else not args.no_extra:
         out["volume_3d"] = volume_leg(args)
 
     if not args.no_cpu:
