@@ -299,6 +299,7 @@ size_t grid_batched_lds(int d, int T, int taps, int CG, int rowstride);
 void launch_grid_atomic(const GridArgs& a, const int4* vrec, int cg, int nitems, int ncgroups, hipStream_t s);
 void launch_grid_reg(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s);
 void launch_grid_mfma(const GridArgs& a, const int4* vrec, int nitems, hipStream_t s);
+void launch_grid_mfma3(const GridArgs& a, const int4* vrec, int nitems, hipStream_t s);
 size_t interp_lds3_bytes(int T, int taps, int cg, int maxs);
 void launch_interp_lds3(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int nt1, int cg, int maxs,
                         hipStream_t s);

@@ -19,6 +19,8 @@
 
 namespace cgs {
 
+typedef float gf32x16 __attribute__((ext_vector_type(16)));   // MFMA 32x32 accumulator
+
 __device__ __forceinline__ float2 shfl_xor2(float2 v, int m) {
   return make_float2(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
 }
@@ -733,6 +735,140 @@ __global__ void __launch_bounds__(512) k_grid_reg3(GridArgs a, const int4* __res
   }
 }
 
+// 3D on the matrix cores: one block of 8 waves per work item, wave w owning
+// plane w of the 8 x 8 x 8 tile as D[64 cells][32 columns] (16 coils, re/im).
+// Visit batches (records, window values, weighted data) are staged once per
+// block; each wave compacts the visits whose footprint meets its plane (ballot
+// and prefix count) and applies them two at a time with v_mfma_f32_32x32x2f32
+// per 32-cell half.  Visits keep their list order within a wave:
+// deterministic, no atomics.
+template <int TAPS>
+__global__ void __launch_bounds__(512) k_grid_mfma3(GridArgs a, const int4* __restrict__ vrec, int nitems) {
+  constexpr int B = 64, NPSI = 3 * TAPS, CC = 16;
+  __shared__ float2 sy[B * CC];
+  __shared__ float spsi[B * NPSI];
+  __shared__ int4 srec[B];
+  __shared__ unsigned char slist[8][B];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const GridItem it = a.items[blockIdx.x];
+  if (a.fdone && a.fdone[it.frame]) return;   // block-uniform
+  const int cbase = blockIdx.y * CC;
+  const int t0 = it.tile % a.nt0, t1 = (it.tile / a.nt0) % a.nt1, t2 = it.tile / (a.nt0 * a.nt1);
+  const int ts0 = t0 * 8, ts1 = t1 * 8, ts2 = t2 * 8;
+  const int te0 = min(8, a.n0 - ts0), te1 = min(8, a.n1 - ts1), te2 = min(8, a.n2 - ts2);
+  const int i = lane & 31, kk = lane >> 5;
+  const int cx = i & 7, cy0 = i >> 3, cy1 = 4 + (i >> 3);
+  gf32x16 acc0, acc1;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) { acc0[r] = 0.f; acc1[r] = 0.f; }
+
+  for (int vb = it.vbeg; vb < it.vend; vb += B) {
+    const int nb = min(B, it.vend - vb);
+    __syncthreads();   // the previous batch is consumed
+    if (tid < nb) srec[tid] = vrec[vb + tid];
+    __syncthreads();
+    {
+      // every load of the batch in flight before the first LDS write
+      constexpr int UP = (B * NPSI + 511) / 512, UY = (B * CC + 511) / 512;
+      float pv[UP];
+      float2 yv[UY];
+#pragma unroll
+      for (int u = 0; u < UP; ++u) {
+        const int idx = tid + 512 * u;
+        pv[u] = 0.f;
+        if (idx < nb * NPSI) {
+          const int v = idx / NPSI, q = idx - v * NPSI;
+          pv[u] = a.psi[(int64_t)srec[v].x * NPSI + q];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UY; ++u) {
+        const int idx = tid + 512 * u;
+        yv[u] = make_float2(0.f, 0.f);
+        if (idx < nb * CC) {
+          const int v = idx / CC, c = idx - v * CC;
+          if (cbase + c < a.C) {
+            const int smp = srec[v].x;
+            yv[u] = a.y[(int64_t)smp * a.C + cbase + c];
+            if (a.w) { const float ww = a.w[smp]; yv[u].x *= ww; yv[u].y *= ww; }
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UP; ++u) {
+        const int idx = tid + 512 * u;
+        if (idx < nb * NPSI) spsi[idx] = pv[u];
+      }
+#pragma unroll
+      for (int u = 0; u < UY; ++u) {
+        const int idx = tid + 512 * u;
+        if (idx < nb * CC) sy[idx] = yv[u];
+      }
+    }
+    // the visits of this batch whose footprint meets plane w, in list order
+    bool hit = false;
+    if (lane < nb) {
+      const int dz = w - (int)(short)(srec[lane].w & 0xffff);
+      hit = (unsigned)dz < (unsigned)TAPS;
+    }
+    const unsigned long long mask = __ballot(hit);
+    const int cnt = __popcll(mask);
+    if (hit) slist[w][__popcll(mask & ((1ull << lane) - 1ull))] = (unsigned char)lane;
+    __syncthreads();   // staged data and this wave's list are visible
+    for (int p = 0; p < cnt; p += 2) {
+      float w0 = 0.f, w1 = 0.f, bv = 0.f;
+      if (p + kk < cnt) {
+        const int v = slist[w][p + kk];
+        const int4 rc = srec[v];
+        const int rel0 = (int)(short)(rc.y & 0xffff), rel1 = (int)(short)(rc.y >> 16);
+        const int dz = w - (int)(short)(rc.w & 0xffff);
+        const float* ps = spsi + v * NPSI;
+        const int dx = cx - rel0, dy0 = cy0 - rel1, dy1 = cy1 - rel1;
+        if ((unsigned)dx < (unsigned)TAPS) {
+          const float pxz = ps[dx] * ps[2 * TAPS + dz];
+          if ((unsigned)dy0 < (unsigned)TAPS) w0 = pxz * ps[TAPS + dy0];
+          if ((unsigned)dy1 < (unsigned)TAPS) w1 = pxz * ps[TAPS + dy1];
+        }
+        bv = ((const float*)sy)[v * 2 * CC + i];
+      }
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(w0, bv, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(w1, bv, acc1, 0, 0, 0);
+    }
+  }
+
+  if (w >= te2) return;
+  const int ncol = 2 * min(CC, a.C - cbase);
+  if (i >= ncol) return;
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int cell = 32 * h + 8 * (r >> 2) + 4 * kk + (r & 3);
+      const int x0 = cell & 7, x1 = cell >> 3;
+      if (x0 >= te0 || x1 >= te1) continue;
+      float* out;
+      if (it.slot < 0) {
+        const int64_t l = (ts0 + x0) + (int64_t)a.n0 * ((ts1 + x1) + (int64_t)a.n1 * (ts2 + w));
+        out = (float*)(a.grid + ((int64_t)it.frame * a.ntot + l) * a.C + cbase);
+      } else {
+        out = (float*)(a.partial + ((int64_t)it.slot * a.tilecells + x0 + 8 * (x1 + 8 * w)) * a.C + cbase);
+      }
+      out[i] = h == 0 ? acc0[r] : acc1[r];
+    }
+}
+
+void launch_grid_mfma3(const GridArgs& a, const int4* vrec, int nitems, hipStream_t s) {
+  if (nitems == 0) return;
+  const dim3 g((unsigned)nitems, (unsigned)((a.C + 15) / 16));
+  switch (a.taps) {
+    case 4: hipLaunchKernelGGL((k_grid_mfma3<4>), g, dim3(512), 0, s, a, vrec, nitems); break;
+    case 6: hipLaunchKernelGGL((k_grid_mfma3<6>), g, dim3(512), 0, s, a, vrec, nitems); break;
+    case 8: hipLaunchKernelGGL((k_grid_mfma3<8>), g, dim3(512), 0, s, a, vrec, nitems); break;
+    case 10: hipLaunchKernelGGL((k_grid_mfma3<10>), g, dim3(512), 0, s, a, vrec, nitems); break;
+    default: break;
+  }
+}
+
 void launch_grid_reg3(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s) {
   if (nitems == 0) return;
   const dim3 g((unsigned)nitems, (unsigned)ncgroups);
@@ -780,7 +916,6 @@ void launch_grid_reg(const GridArgs& a, const int4* vrec, int cgr, int nitems, i
 // W[l % 32][l / 32], B lane l = Y[l / 32][l % 32], accumulator register r of
 // lane l = D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].  Each cell still has one
 // owner and visits are applied in list order: deterministic, no atomics.
-typedef float gf32x16 __attribute__((ext_vector_type(16)));
 
 template <int NT, int TAPS, int B>
 __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs a, const int4* __restrict__ vrec, int nitems) {

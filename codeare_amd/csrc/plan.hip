@@ -437,6 +437,8 @@ static int gridding(Plan& p, const float2* y, bool weighted, const int* fl) {
   } else if (p.d == 2 && p.T == 8 && p.use_grid_reg) {
     const int cgr = p.grid_cgr;
     launch_grid_reg(a, p.vrec, cgr, p.n_grid_items, (p.C + cgr - 1) / cgr, p.stream);
+  } else if (p.d == 3 && p.T == 8 && p.use_grid_mfma) {
+    launch_grid_mfma3(a, p.vrec, p.n_grid_items, p.stream);
   } else if (p.d == 3 && p.T == 8 && p.use_grid_reg) {
     const int cgr = std::min(p.grid_cgr, 16);
     launch_grid_reg3(a, p.vrec, cgr, p.n_grid_items, (p.C + cgr - 1) / cgr, p.stream);
@@ -840,7 +842,7 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   // opt-in (CGS_INTERP_LDS=1)
   p.use_interp_lds = (d == 2);
   p.tile_item_max = (d == 3) ? 64 : 128;
-  p.visits_per_item = (d == 3) ? 128 : 512;
+  p.visits_per_item = (d == 3) ? 4096 : 512;
   p.grid_cgr = (C <= 8) ? 8 : 16;
   // shape overrides for tuning sweeps (tile side must be a multiple of 8)
   if (const char* e = std::getenv("CGS_TILE")) {
@@ -863,7 +865,7 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   if (const char* e = std::getenv("CGS_TILE_ITEM_MAX")) p.tile_item_max = std::max(8, std::atoi(e));
   if (const char* e = std::getenv("CGS_INTERP_LDS")) p.use_interp_lds = std::atoi(e) != 0;
   if (const char* e = std::getenv("CGS_GRID_REG")) p.use_grid_reg = std::atoi(e) != 0;
-  p.use_grid_mfma = (d == 2);
+  p.use_grid_mfma = true;
   if (const char* e = std::getenv("CGS_GRID_MFMA")) p.use_grid_mfma = std::atoi(e) != 0;
   if (const char* e = std::getenv("CGS_GRID_CGR")) {
     const int g = std::atoi(e);
@@ -1542,9 +1544,9 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
     reps = std::max(1, std::min(reps, 50));
     HIPCHK(hipMemsetAsync(p.ybuf[0], 0, sizeof(float2) * (size_t)p.Mtot * p.C, p.stream));
     HIPCHK(hipMemsetAsync(p.grid, 0, sizeof(float2) * (size_t)p.ntot * p.C * p.F, p.stream));
-    const bool tiles8 = p.d == 2 && p.T == 8;
-    const bool reg = tiles8 && p.use_grid_reg;
-    std::vector<int> vpis = (p.d == 2) ? std::vector<int>{256, 512, 1024} : std::vector<int>{64, 128, 256};
+    const bool tiles8 = p.T == 8;
+    const bool reg = p.d == 2 && tiles8 && p.use_grid_reg;
+    std::vector<int> vpis = (p.d == 2) ? std::vector<int>{256, 512, 1024} : std::vector<int>{1024, 4096, 16384};
     std::vector<int> cgrs = (reg && p.C > 8) ? std::vector<int>{8, 16} : std::vector<int>{p.grid_cgr};
     std::vector<int> icgs = (p.C > 8) ? std::vector<int>{8, 16} : std::vector<int>{p.interp_cg};
     const int vpi0 = p.visits_per_item, cgr0 = p.grid_cgr, icg0 = p.interp_cg;
