@@ -29,6 +29,7 @@ CGS_DEVICE_IO = 0x1
 CGS_PROFILE = 0x2
 CGS_DETERMINISTIC = 0x4
 CGS_AUTOTUNE = 0x8
+CGS_MAPS_PER_FRAME = 0x10
 CGS_EH_TRUE = 0
 CGS_EH_REF = 1
 CGS_TOL_NORM_RATIO = 0
@@ -56,7 +57,8 @@ class CgsError(RuntimeError):
 class SolveOpts(ct.Structure):
     _fields_ = [("maxit", ct.c_int), ("tol", ct.c_double), ("tol_kind", ct.c_int),
                 ("lambda_", ct.c_double), ("eh_mode", ct.c_int), ("ftiter", ct.c_int),
-                ("fteps", ct.c_double), ("normal_op", ct.c_int), ("graph_iters", ct.c_int)]
+                ("fteps", ct.c_double), ("normal_op", ct.c_int), ("graph_iters", ct.c_int),
+                ("joint", ct.c_int)]
 
 
 class TuneReport(ct.Structure):
@@ -139,7 +141,7 @@ class Plan:
 
     def __init__(self, N, M_total: int, sigma: float = 2.0, m: int = 3, ncoils: int = 1,
                  frame_offsets=None, device_io: bool = False, profile: bool = False,
-                 deterministic: bool = False, autotune: bool = False):
+                 deterministic: bool = False, autotune: bool = False, maps_per_frame: bool = False):
         L = load_library()
         self.N = tuple(int(v) for v in N)
         self.d = len(self.N)
@@ -152,8 +154,10 @@ class Plan:
         self.foff = np.ascontiguousarray(frame_offsets, dtype=np.int64)
         self.F = len(self.foff) - 1
         self.device_io = device_io
+        self.maps_per_frame = bool(maps_per_frame)
         flags = ((CGS_DEVICE_IO if device_io else 0) | (CGS_PROFILE if profile else 0)
-                 | (CGS_DETERMINISTIC if deterministic else 0) | (CGS_AUTOTUNE if autotune else 0))
+                 | (CGS_DETERMINISTIC if deterministic else 0) | (CGS_AUTOTUNE if autotune else 0)
+                 | (CGS_MAPS_PER_FRAME if maps_per_frame else 0))
         Narr = (ct.c_int * self.d)(*self.N)
         h = ct.c_void_p()
         _check(L.cgs_plan_create(ct.byref(h), self.d, ct.cast(Narr, ct.c_void_p), self.M,
@@ -200,8 +204,9 @@ class Plan:
 
     def sensitivities(self, S):
         S = _c64(S)
-        if S.size != self.Ntot * self.C:
-            raise ValueError("maps must be (N..., C)")
+        nm = self.F if self.maps_per_frame else 1
+        if S.size != self.Ntot * self.C * nm:
+            raise ValueError("maps must be (N..., C)" if nm == 1 else "maps must be (N..., F, C)")
         self._S = S
         _check(load_library().cgs_set_sensitivities(self._h, _ptr(S)))
 
@@ -283,11 +288,12 @@ class Plan:
 
     def solve(self, y, x0=None, maxit: int = 10, tol: float = 1e-5, tol_kind: int = CGS_TOL_NORM_RATIO,
               lam: float = 0.0, eh_mode: int = CGS_EH_TRUE, ftiter: int = 3, fteps: float = 1e-3,
-              out=None, check_nan: bool = True, toeplitz: bool = False, graph_iters: int = 0):
+              out=None, check_nan: bool = True, toeplitz: bool = False, graph_iters: int = 0,
+              joint: bool = False):
         L = load_library()
         o = SolveOpts(int(maxit), float(tol), int(tol_kind), float(lam), int(eh_mode), int(ftiter),
                       float(fteps), CGS_NORMAL_TOEPLITZ if toeplitz else CGS_NORMAL_NUFFT,
-                      int(graph_iters))
+                      int(graph_iters), int(bool(joint)))
         rep = SolveReport()
         if self.device_io:
             rc = L.cgs_solve(self._h, _ptr(y), _ptr(x0) if x0 is not None else None, _ptr(out),
@@ -348,19 +354,28 @@ def _imsz(p):
 
 class NFFT:
     """NFFT<cxfl> (src/matrix/ft/NFFT.hpp): keys nk, imsz, m (1), alpha (1.0),
-    epsilon (7e-4), maxit (3).  ``*`` is Trafo, ``adjoint`` (->*) the iterative
-    NFFT3 inverse; ``adjoint_true`` is nfft_adjoint."""
+    epsilon (7e-4), maxit (3), 3rd_dim_cart (False).  ``*`` is Trafo,
+    ``adjoint`` (->*) the iterative NFFT3 inverse; ``adjoint_true`` is
+    nfft_adjoint.  With 3rd_dim_cart an imsz (N0, N1, Nz) is Nz 2D slices on
+    the one trajectory (NFFT.hpp:84-103): Trafo maps (N0, N1, Nz) to (nk, Nz)
+    slice by slice and Adjoint back (NFFT.hpp:331-360, :380-412; codeare leaves
+    the Cartesian FFT along the slices commented out, and so does this class)."""
 
     def __init__(self, params: dict):
         if "nk" not in params or "imsz" not in params:
             raise ValueError("NFFT: nk and imsz must be given (NFFT.hpp:72-98)")
         self.M = int(params["nk"])
         self.N = _imsz(params)
+        self.ncart = 1
+        if bool(params.get("3rd_dim_cart", False)) and len(self.N) == 3:
+            self.ncart = self.N[-1]
+            self.N = self.N[:-1]
         self.m = int(params.get("m", 1))
         self.alpha = float(params.get("alpha", 1.0))
         self.epsilon = float(params.get("epsilon", 7.0e-4))
         self.maxit = int(params.get("maxit", 3))
-        self.plan = Plan(self.N, self.M, self.alpha, self.m, ncoils=1)
+        # the slices are the plan's image batch
+        self.plan = Plan(self.N, self.M, self.alpha, self.m, ncoils=self.ncart)
 
     def KSpace(self, k):
         self.plan.kspace(k)
@@ -383,18 +398,26 @@ class NFFT:
 class NCSENSE:
     """NCSENSE<cxfl> (src/matrix/ft/NCSENSE.hpp:69-184): keys sensitivities, nk,
     ftiter (3), cgiter (0), cgeps (1e-6), lambda (1e-6), m (1), alpha (1.0),
-    fteps (1e-3), dim4/dim5 (frames, with frame_offsets for uneven counts).
+    fteps (1e-3), dim4/dim5 (frames, with frame_offsets for uneven counts),
+    3rd_dim_cart (False: with True the maps are (N0, N1, Nz, C), the data
+    (nk, Nz, C), and the Nz slices share the trajectory and the weights,
+    NCSENSE.hpp:81-99, :338-339).
 
     ``*`` = E, ``/`` = reference E^H (ITrafo per coil, intensity-corrected),
     ``Adjoint`` = CGLS solve (codeare semantics: squared-ratio cgeps stop, the
-    reference E^H).  ``mode="true"`` switches E^H to the density-weighted true
-    adjoint for both ``/`` and the solve."""
+    reference E^H, one CG over all frames or slices as CGLS runs on the whole
+    image matrix).  ``mode="true"`` switches E^H to the density-weighted true
+    adjoint for both ``/`` and the solve; ``joint=False`` gives every frame its
+    own CG."""
 
-    def __init__(self, params: dict, mode: str = "ref"):
+    def __init__(self, params: dict, mode: str = "ref", joint: bool = True):
         S = _c64(params["sensitivities"])
-        self.d = S.ndim - 1
+        self.cart = bool(params.get("3rd_dim_cart", False))
+        if self.cart and S.ndim != 4:
+            raise ValueError("NCSENSE: 3rd_dim_cart needs (N0, N1, Nz, C) maps")
+        self.d = S.ndim - (2 if self.cart else 1)
         self.C = S.shape[-1]
-        self.N = tuple(S.shape[:-1])
+        self.N = tuple(S.shape[: self.d])
         self.M = int(params.get("nk", 1))
         self.ftiter = int(params.get("ftiter", 3))
         self.fteps = float(params.get("fteps", 1.0e-3))
@@ -404,34 +427,56 @@ class NCSENSE:
         self.m = int(params.get("m", 1))
         self.alpha = float(params.get("alpha", 1.0))
         self.mode = CGS_EH_REF if mode == "ref" else CGS_EH_TRUE
-        frames = int(params.get("dim4", 1)) * int(params.get("dim5", 1))
+        self.joint = bool(joint)
+        frames = S.shape[2] if self.cart else int(params.get("dim4", 1)) * int(params.get("dim5", 1))
         foff = params.get("frame_offsets")
-        if foff is None:
+        if foff is None or self.cart:
             foff = np.arange(frames + 1, dtype=np.int64) * self.M
         foff = np.asarray(foff, dtype=np.int64)
+        self.F = len(foff) - 1
         self.plan = Plan(self.N, int(foff[-1]), self.alpha, self.m, ncoils=self.C,
-                         frame_offsets=foff)
+                         frame_offsets=foff, maps_per_frame=self.cart)
         self.plan.sensitivities(S)
 
     def Sensitivities(self, S):
         self.plan.sensitivities(S)
 
+    def _per_frame(self, v, block):
+        # one block shared by every frame/slice (NCSENSE.hpp:204-248) -> one per frame
+        v = np.asarray(v, dtype=np.float32)
+        if v.size == block * self.F or self.F == 1:
+            return v
+        if v.size != block:
+            raise ValueError("k-space / weights size does not fit the frames")
+        return np.concatenate([v.reshape(-1, order="F")] * self.F)
+
     def KSpace(self, k):
-        self.plan.kspace(k)
+        k = self._per_frame(k, self.d * self.M)
+        self.plan.kspace(np.asarray(k, dtype=np.float32).reshape((self.d, -1), order="F"))
 
     def Weights(self, w):
-        self.plan.weights(w)
+        self.plan.weights(self._per_frame(w, self.M))
+
+    # codeare's slice data (nk, Nz, C) <-> the plan's frame blocks (nk, C) per slice
+    def _to_plan(self, y):
+        y = _c64(y)
+        if not self.cart:
+            return y
+        return _c64(np.transpose(y.reshape((self.M, self.F, self.C), order="F"), (0, 2, 1)))
 
     def Trafo(self, x):
-        return self.plan.E(x)
+        y = self.plan.E(x)
+        if not self.cart:
+            return y
+        return np.asfortranarray(np.transpose(y.reshape((self.M, self.C, self.F), order="F"), (0, 2, 1)))
 
     __mul__ = Trafo
 
     def __truediv__(self, y):
-        return self.plan.EH(y, self.mode, self.ftiter, self.fteps)
+        return self.plan.EH(self._to_plan(y), self.mode, self.ftiter, self.fteps)
 
     def Adjoint(self, y, x0=None):
         x, self.last_report = self.plan.solve(
-            y, x0=x0, maxit=self.cgiter, tol=self.cgeps, tol_kind=CGS_TOL_SQUARED_RATIO,
-            lam=self.lam, eh_mode=self.mode, ftiter=self.ftiter, fteps=self.fteps)
+            self._to_plan(y), x0=x0, maxit=self.cgiter, tol=self.cgeps, tol_kind=CGS_TOL_SQUARED_RATIO,
+            lam=self.lam, eh_mode=self.mode, ftiter=self.ftiter, fteps=self.fteps, joint=self.joint)
         return x

@@ -62,7 +62,8 @@ struct FftArgs {
   int use_reg;          // register-resident kernel (n = 256 or 512, 16 lines per block)
   // fused operations of the first forward / last adjoint pass (axis 0)
   const float2* x;      // image frames (load mode 1)
-  const float2* S;      // maps (load mode 1)
+  const float2* S;      // maps (load mode 1); frame f's maps start at S + f*Sfs
+  int64_t Sfs;
   const float* dinv0;
   const float* dinv1;
   const float* dinv2;
@@ -96,7 +97,8 @@ struct ToepArgs {
   const float2* pold;
   float2* pnew;
   const CgState* cgst;
-  const float2* S;
+  const float2* S;        // frame f's maps start at S + f*Sfs
+  int64_t Sfs;
   const float* That;      // conv: That[(f*2N0 + l0)*2N1 + bin]
   const float2* tw;       // exp(-2 pi i p / n), n = 2 N of the pass axis
   const int* fdone;
@@ -188,6 +190,8 @@ struct Plan {
   // maps
   float2* S = nullptr;
   float* Imap = nullptr;
+  int64_t Sfs = 0;                    // maps frame stride (Ntot*C with per-frame maps, else 0)
+  int64_t Ifs = 0;                    // intensity-map frame stride (Ntot or 0)
 
   // samples (sorted)
   int* base = nullptr;
@@ -253,6 +257,7 @@ struct Plan {
   unsigned prof_mask = ~0u;           // kernel classes timed with events
   double* tail_pp = nullptr;          // fused CG tail partials (Toeplitz path)
   bool beta_folded = false;           // the captured / last CG iteration forms beta in the pad pass
+  bool graph_folded = false;          // beta_folded of the iterations captured in graph_exec
   double* tail_pq = nullptr;
   struct Pending { hipEvent_t a, b; int kind; double bytes; };
   std::vector<Pending> pending;
@@ -317,11 +322,11 @@ void launch_coil_combine(const Plan& p, const float2* Xc, float2* x, const float
 // vector / scalar kernels
 void launch_sqnorm_partials(const Plan& p, const float2* v, double* partials, hipStream_t s);
 void launch_cg_init(const Plan& p, const double* part_bn, const double* part_rn, int tolkind,
-                    double tol, int maxit, bool warm, hipStream_t s);
-void launch_cg_alpha(const Plan& p, const double* partials, hipStream_t s);
+                    double tol, int maxit, bool warm, bool joint, hipStream_t s);
+void launch_cg_alpha(const Plan& p, const double* partials, bool joint, hipStream_t s);
 void launch_cg_xr(const Plan& p, float2* x, float2* r, const float2* pv, const float2* q,
                   double* partials, hipStream_t s);
-void launch_cg_beta(const Plan& p, const double* partials, int tolkind, double tol, int maxit,
+void launch_cg_beta(const Plan& p, const double* partials, int tolkind, double tol, int maxit, bool joint,
                     hipStream_t s);
 void launch_cg_p(const Plan& p, float2* pv, const float2* r, hipStream_t s);
 void launch_coil_alpha(const Plan& p, const float2* Xc, float2* q, const float2* pv, float lambda,

@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft(FftArgs a) {
             } else {
               xv = a.x[f * a.Ntot + i];
             }
-            const float2 sv = a.S[i * a.C + qq];
+            const float2 sv = a.S[f * a.Sfs + i * a.C + qq];
             float2 t = cmul(xv, sv);
             if (LD == 1) {
               const float dv = deapod(a, p, rest);
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft(FftArgs a) {
           int pos = p - a.nout / 2;
           if (pos < 0) pos += n;
           const float2 xv = lds[q * a.ldn + pidx(pos)];
-          const float2 sv = a.S[(p + (int64_t)a.N0 * rest) * a.C + q0 + q];
+          const float2 sv = a.S[f * a.Sfs + (p + (int64_t)a.N0 * rest) * a.C + q0 + q];
           v[u] = make_float2(sv.x * xv.x + sv.y * xv.y, sv.x * xv.y - sv.y * xv.x);
         }
       }

@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
         } else {
           xv = a.x[f * a.Ntot + i];
         }
-        t = cmul2(xv, a.S[i * a.C + qq]);
+        t = cmul2(xv, a.S[f * a.Sfs + i * a.C + qq]);
         if (LD == 1) {
           const float dv = deapod_r(a, p, rest);
           t = make_float2(t.x * dv, t.y * dv);
@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
         if (p >= n) p -= n;
         float2 c = make_float2(0.f, 0.f);
         if (live && p < a.nout) {
-          const float2 sv = a.S[(p + (int64_t)a.N0 * rest) * a.C + qq];
+          const float2 sv = a.S[f * a.Sfs + (p + (int64_t)a.N0 * rest) * a.C + qq];
           const float2 xv = X[r][k2];
           c = make_float2(sv.x * xv.x + sv.y * xv.y, sv.x * xv.y - sv.y * xv.x);
         }
@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(1024) k_fft_r8(FftArgs a) {
         } else {
           xv = a.x[f * a.Ntot + i];
         }
-        x = cmul2(xv, a.S[i * a.C + qq]);
+        x = cmul2(xv, a.S[f * a.Sfs + i * a.C + qq]);
         if (LD == 1) {
           const float dv = deapod_r(a, p, rest);
           x = make_float2(x.x * dv, x.y * dv);
@@ -594,7 +594,7 @@ __global__ void __launch_bounds__(1024) k_fft_r8(FftArgs a) {
       if (p >= n) p -= n;
       float2 c = make_float2(0.f, 0.f);
       if (live && act && p < a.nout) {
-        const float2 sv = a.S[(p + (int64_t)a.N0 * rest) * a.C + qq];
+        const float2 sv = a.S[f * a.Sfs + (p + (int64_t)a.N0 * rest) * a.C + qq];
         c = make_float2(sv.x * v[m2].x + sv.y * v[m2].y, sv.x * v[m2].y - sv.y * v[m2].x);
       }
 #pragma unroll

@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
 
   const int64_t ib = (int64_t)f * a.Ntot + (int64_t)a.N0 * i1;
   const int vi = t * 8;
-  const Rsrc rS = rsrc(a.S + (int64_t)a.N0 * i1 * a.C + q0, ((int64_t)a.N0 * a.C - q0) * 8);
+  const Rsrc rS = rsrc(a.S + f * a.Sfs + (int64_t)a.N0 * i1 * a.C + q0, ((int64_t)a.N0 * a.C - q0) * 8);
   const int vs = (t * a.C + q) * 8, ss = 64 * a.C * 8;
   float2 u[NP];
   float pp = 0.f;
@@ -521,7 +521,7 @@ __global__ void __launch_bounds__(1024) k_toep_crop(ToepArgs a) {
   dft_low_half<8, SG>(v);   // positions hi + NP lo + 8 NP m2 < N0 (m2 < 4)
   const bool act = R8<NP>::ACT >= 64 || t < R8<NP>::ACT;
   const int pb = hi + NP * lo;
-  const Rsrc rS = rsrc(a.S + (int64_t)a.N0 * i1 * a.C + q0, ((int64_t)a.N0 * a.C - q0) * 8);
+  const Rsrc rS = rsrc(a.S + f * a.Sfs + (int64_t)a.N0 * i1 * a.C + q0, ((int64_t)a.N0 * a.C - q0) * 8);
   const int vb = (pb * a.C + q) * 8, sb = 8 * NP * a.C * 8;
   float2 c[4];
 #pragma unroll

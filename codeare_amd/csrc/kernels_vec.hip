@@ -33,10 +33,12 @@ __global__ void __launch_bounds__(256) k_coil_combine(const float2* X, const flo
                                                       const float* Imap, float2* x,
                                                       const float2* pv, float lambda,
                                                       double* partials, int C, int64_t Ntot,
-                                                      const int* fdone) {
+                                                      const int* fdone, int64_t Sfs, int64_t Ifs) {
   __shared__ double sh[4];
   const int f = blockIdx.y;
   if (fdone && fdone[f]) return;
+  S += f * Sfs;
+  if (Imap) Imap += f * Ifs;
   constexpr int PPB = 256 / CP;
   const int g = threadIdx.x / CP, cl = threadIdx.x % CP;
   double dot = 0.0;
@@ -77,13 +79,17 @@ void launch_coil_combine(const Plan& p, const float2* Xc, float2* x, const float
   const dim3 g(kCoilBlocks, p.F);
   const float* I = use_imap ? p.Imap : nullptr;
   if (p.C <= 8)
-    hipLaunchKernelGGL(k_coil_combine<8>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, fl);
+    hipLaunchKernelGGL(k_coil_combine<8>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, fl,
+                       p.Sfs, p.Ifs);
   else if (p.C <= 16)
-    hipLaunchKernelGGL(k_coil_combine<16>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, fl);
+    hipLaunchKernelGGL(k_coil_combine<16>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, fl,
+                       p.Sfs, p.Ifs);
   else if (p.C <= 32)
-    hipLaunchKernelGGL(k_coil_combine<32>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, fl);
+    hipLaunchKernelGGL(k_coil_combine<32>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, fl,
+                       p.Sfs, p.Ifs);
   else
-    hipLaunchKernelGGL(k_coil_combine<64>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, fl);
+    hipLaunchKernelGGL(k_coil_combine<64>, g, dim3(256), 0, s, Xc, p.S, I, x, pvec, lambda, partials, p.C, p.Ntot, fl,
+                       p.Sfs, p.Ifs);
 }
 
 // q = sum_g part_g + lambda p, with <p, q> partials (kCoilBlocks per frame,
@@ -266,10 +272,11 @@ template <int CP>
 __global__ void __launch_bounds__(256) k_coil_alpha(const float2* X, const float2* S, float2* q,
                                                     const float2* pv, float lambda, double* partials,
                                                     int* tickets, CgState* st, int C, int64_t Ntot,
-                                                    const int* fdone) {
+                                                    const int* fdone, int64_t Sfs) {
   __shared__ double sh[4];
   const int f = blockIdx.y;
   if (fdone[f]) return;
+  S += f * Sfs;
   constexpr int PPB = 256 / CP;
   const int g = threadIdx.x / CP, cl = threadIdx.x % CP;
   double dot = 0.0;
@@ -312,13 +319,17 @@ void launch_coil_alpha(const Plan& p, const float2* Xc, float2* q, const float2*
                        hipStream_t s) {
   const dim3 g(kRedBlocks, p.F);
   if (p.C <= 8)
-    hipLaunchKernelGGL(k_coil_alpha<8>, g, dim3(256), 0, s, Xc, p.S, q, pv, lambda, p.partials, p.tickets, p.cgst, p.C, p.Ntot, p.fdone);
+    hipLaunchKernelGGL(k_coil_alpha<8>, g, dim3(256), 0, s, Xc, p.S, q, pv, lambda, p.partials, p.tickets, p.cgst, p.C, p.Ntot, p.fdone,
+                       p.Sfs);
   else if (p.C <= 16)
-    hipLaunchKernelGGL(k_coil_alpha<16>, g, dim3(256), 0, s, Xc, p.S, q, pv, lambda, p.partials, p.tickets, p.cgst, p.C, p.Ntot, p.fdone);
+    hipLaunchKernelGGL(k_coil_alpha<16>, g, dim3(256), 0, s, Xc, p.S, q, pv, lambda, p.partials, p.tickets, p.cgst, p.C, p.Ntot, p.fdone,
+                       p.Sfs);
   else if (p.C <= 32)
-    hipLaunchKernelGGL(k_coil_alpha<32>, g, dim3(256), 0, s, Xc, p.S, q, pv, lambda, p.partials, p.tickets, p.cgst, p.C, p.Ntot, p.fdone);
+    hipLaunchKernelGGL(k_coil_alpha<32>, g, dim3(256), 0, s, Xc, p.S, q, pv, lambda, p.partials, p.tickets, p.cgst, p.C, p.Ntot, p.fdone,
+                       p.Sfs);
   else
-    hipLaunchKernelGGL(k_coil_alpha<64>, g, dim3(256), 0, s, Xc, p.S, q, pv, lambda, p.partials, p.tickets, p.cgst, p.C, p.Ntot, p.fdone);
+    hipLaunchKernelGGL(k_coil_alpha<64>, g, dim3(256), 0, s, Xc, p.S, q, pv, lambda, p.partials, p.tickets, p.cgst, p.C, p.Ntot, p.fdone,
+                       p.Sfs);
 }
 
 // x += alpha p ; r -= alpha q ; partial ||r||^2, where every block forms
@@ -364,11 +375,11 @@ void launch_cg_xr_alpha(const Plan& p, float2* x, float2* r, const float2* pv, c
 }
 
 __global__ void k_cg_beta(CgState* st, const double* part, int nb, int tolkind, double tol, int maxit,
-                          double* hist, int hcap, int* fdone);
+                          double* hist, int hcap, int* fdone, int nf);
 
 void launch_cg_beta_after_xr(const Plan& p, int nparts, int tolkind, double tol, int maxit, hipStream_t s) {
   hipLaunchKernelGGL(k_cg_beta, dim3(p.F), dim3(256), 0, s, p.cgst, p.partials + (size_t)kCoilBlocks * p.F, nparts,
-                     tolkind, tol, maxit, p.hist, p.hist_cap, p.fdone);
+                     tolkind, tol, maxit, p.hist, p.hist_cap, p.fdone, 1);
 }
 
 // x += alpha p ; r -= alpha q ; partial ||r||^2 ; last block: beta, residual, done
@@ -443,13 +454,17 @@ __device__ double sum_partials(const double* part, int nb, double* sh) {
   return block_sum(acc, sh);
 }
 
+// The scalar kernels below run one block per frame, or, for a joint CG over
+// all frames (nf = F, one block), sum the partials of every frame (frame-major,
+// a fixed order) and give every frame the same scalars and stop flag.
+
 // CG start: bn = ||E^H y||^2, rn = ||r0||^2, residual measure, done test.
 __global__ void k_cg_init(CgState* st, const double* pbn, const double* prn, int nb, int tolkind,
-                          double tol, int maxit, double* hist, int hcap, int* fdone) {
+                          double tol, int maxit, double* hist, int hcap, int* fdone, int nf) {
   __shared__ double sh[4];
   const int f = blockIdx.x;
-  const double bn = sum_partials(pbn + (int64_t)f * nb, nb, sh);
-  const double rn = prn ? sum_partials(prn + (int64_t)f * nb, nb, sh) : bn;
+  const double bn = sum_partials(pbn + (int64_t)f * nb, nb * nf, sh);
+  const double rn = prn ? sum_partials(prn + (int64_t)f * nb, nb * nf, sh) : bn;
   if (threadIdx.x == 0) {
     CgState c;
     c.bn = bn;
@@ -464,31 +479,37 @@ __global__ void k_cg_init(CgState* st, const double* pbn, const double* prn, int
     c.rel = rel;
     c.nan = isnan(rel) ? 1 : 0;
     c.done = (c.nan || rel <= tol || maxit <= 0) ? 1 : 0;
-    st[f] = c;
-    fdone[f] = c.done;
-    if (hist) hist[(int64_t)f * hcap] = rel;
+    for (int g = f; g < f + nf; ++g) {
+      st[g] = c;
+      fdone[g] = c.done;
+      if (hist) hist[(int64_t)g * hcap] = rel;
+    }
   }
 }
 
 void launch_cg_init(const Plan& p, const double* part_bn, const double* part_rn, int tolkind,
-                    double tol, int maxit, bool warm, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_init, dim3(p.F), dim3(256), 0, s, p.cgst, part_bn, warm ? part_rn : nullptr,
-                     kRedBlocks, tolkind, tol, maxit, p.hist, p.hist_cap, p.fdone);
+                    double tol, int maxit, bool warm, bool joint, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_init, dim3(joint ? 1 : p.F), dim3(256), 0, s, p.cgst, part_bn, warm ? part_rn : nullptr,
+                     kRedBlocks, tolkind, tol, maxit, p.hist, p.hist_cap, p.fdone, joint ? p.F : 1);
 }
 
-__global__ void k_cg_alpha(CgState* st, const double* part, int nb, const int* fdone) {
+__global__ void k_cg_alpha(CgState* st, const double* part, int nb, const int* fdone, int nf) {
   __shared__ double sh[4];
   const int f = blockIdx.x;
   if (fdone[f]) return;
-  const double pq = sum_partials(part + (int64_t)f * nb, nb, sh);
+  const double pq = sum_partials(part + (int64_t)f * nb, nb * nf, sh);
   if (threadIdx.x == 0) {
-    st[f].pq = pq;
-    st[f].alpha = st[f].rn / pq;   // CGLS.hpp:50
+    const double al = st[f].rn / pq;   // CGLS.hpp:50
+    for (int g = f; g < f + nf; ++g) {
+      st[g].pq = pq;
+      st[g].alpha = al;
+    }
   }
 }
 
-void launch_cg_alpha(const Plan& p, const double* partials, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_alpha, dim3(p.F), dim3(256), 0, s, p.cgst, partials, kCoilBlocks, p.fdone);
+void launch_cg_alpha(const Plan& p, const double* partials, bool joint, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_alpha, dim3(joint ? 1 : p.F), dim3(256), 0, s, p.cgst, partials, kCoilBlocks, p.fdone,
+                     joint ? p.F : 1);
 }
 
 // x += alpha p ; r -= alpha q ; partial ||r||^2   (CGLS.hpp:51-54)
@@ -519,11 +540,11 @@ void launch_cg_xr(const Plan& p, float2* x, float2* r, const float2* pv, const f
 }
 
 __global__ void k_cg_beta(CgState* st, const double* part, int nb, int tolkind, double tol, int maxit,
-                          double* hist, int hcap, int* fdone) {
+                          double* hist, int hcap, int* fdone, int nf) {
   __shared__ double sh[4];
   const int f = blockIdx.x;
   if (fdone[f]) return;
-  const double rn_new = sum_partials(part + (int64_t)f * nb, nb, sh);
+  const double rn_new = sum_partials(part + (int64_t)f * nb, nb * nf, sh);
   if (threadIdx.x == 0) {
     CgState c = st[f];
     c.beta = rn_new / c.rn;   // CGLS.hpp:53-55
@@ -533,16 +554,18 @@ __global__ void k_cg_beta(CgState* st, const double* part, int nb, int tolkind, 
     c.rel = rel;
     if (isnan(rel) || isnan(c.alpha)) c.nan = 1;
     c.done = (c.nan || rel <= tol || c.it >= maxit) ? 1 : 0;
-    st[f] = c;
-    if (hist && c.it < hcap) hist[(int64_t)f * hcap + c.it] = rel;
-    fdone[f] = c.done;
+    for (int g = f; g < f + nf; ++g) {
+      st[g] = c;
+      if (hist && c.it < hcap) hist[(int64_t)g * hcap + c.it] = rel;
+      fdone[g] = c.done;
+    }
   }
 }
 
-void launch_cg_beta(const Plan& p, const double* partials, int tolkind, double tol, int maxit,
+void launch_cg_beta(const Plan& p, const double* partials, int tolkind, double tol, int maxit, bool joint,
                     hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_beta, dim3(p.F), dim3(256), 0, s, p.cgst, partials, kRedBlocks, tolkind, tol,
-                     maxit, p.hist, p.hist_cap, p.fdone);
+  hipLaunchKernelGGL(k_cg_beta, dim3(joint ? 1 : p.F), dim3(256), 0, s, p.cgst, partials, kRedBlocks, tolkind, tol,
+                     maxit, p.hist, p.hist_cap, p.fdone, joint ? p.F : 1);
 }
 
 // p = r + beta p  (CGLS.hpp:55-56)

@@ -241,6 +241,7 @@ static FftArgs fft_base(const Plan& p, const FftTab& tab, int sign) {
   a.N1 = p.N[1];
   a.Ntot = p.Ntot;
   a.S = p.S;
+  a.Sfs = p.Sfs;
   a.mask_nh = (int)(0.5 * p.N[0]);
   return a;
 }
@@ -906,8 +907,11 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   for (auto& b : p.ybuf) CHK(dalloc(&b, (size_t)Mtot * C));
   for (auto& b : p.img) CHK(dalloc(&b, (size_t)F * p.Ntot));
   CHK(dalloc(&p.w, (size_t)Mtot));
-  CHK(dalloc(&p.S, (size_t)p.Ntot * C));
-  CHK(dalloc(&p.Imap, (size_t)p.Ntot));
+  const int nmaps = (flags & CGS_MAPS_PER_FRAME) ? F : 1;
+  CHK(dalloc(&p.S, (size_t)p.Ntot * C * nmaps));
+  CHK(dalloc(&p.Imap, (size_t)p.Ntot * nmaps));
+  p.Sfs = nmaps > 1 ? p.Ntot * C : 0;
+  p.Ifs = nmaps > 1 ? p.Ntot : 0;
   p.n_partials = std::max<size_t>((size_t)4096 * F, (size_t)256 * F * C);
   CHK(dalloc(&p.partials, p.n_partials));
   CHK(dalloc(&p.cgst, (size_t)F));
@@ -1117,7 +1121,7 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
     const bool lean_cv = toep_lean_enabled() && toep_conv_supported((int)N0, (int)N1, C);
     ToepArgs ta{};
     ta.C = C; ta.N0 = (int)N0; ta.N1 = (int)N1; ta.F = F; ta.Ntot = p.Ntot;
-    ta.S = p.S; ta.fdone = fl;
+    ta.S = p.S; ta.Sfs = p.Sfs; ta.fdone = fl;
     const bool want_tail = pu && tail_out && p.tail_pp;
     const double pad_bytes = ((double)(1 + C) * N0 * N1 + (double)C * M0 * N1) * 8;
     if (lean_pc) {
@@ -1239,6 +1243,26 @@ static int cg_iteration_fused(Plan& p, int normal_op, float2* x, float2* r, cons
   return CGS_OK;
 }
 
+// One CG iteration on separate kernels in codeare's order (CGLS.hpp:48-56):
+// q = (A + lambda) p with <p, q> partials, alpha, x and r with ||r||^2
+// partials, beta and the stop test, p = r + beta p.  The reference E^H takes
+// this path, and so does a joint solve (scalars summed over all frames).
+static int cg_iteration_plain(Plan& p, float2* x, float2* r, float2* pv, float2* q, const cgs_solve_opts& o,
+                              bool joint) {
+  if (o.eh_mode == CGS_EH_TRUE) {
+    CHK(op_normal(p, o.normal_op, pv, q, (float)o.lambda, p.partials, p.fdone));
+  } else {
+    CHK(op_E(p, pv, p.ybuf[1], p.fdone));
+    CHK(op_EH(p, o.eh_mode, p.ybuf[1], q, o.ftiter, o.fteps, pv, (float)o.lambda, p.partials, p.fdone));
+  }
+  Prof pr(p, CGS_K_CG, 7.0 * 8.0 * p.F * p.Ntot);
+  launch_cg_alpha(p, p.partials, joint, p.stream);
+  launch_cg_xr(p, x, r, pv, q, p.partials, p.stream);
+  launch_cg_beta(p, p.partials, o.tol_kind, o.tol, o.maxit, joint, p.stream);
+  launch_cg_p(p, pv, r, p.stream);
+  return CGS_OK;
+}
+
 // ---- CG solve -------------------------------------------------------------------------
 static double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1247,6 +1271,8 @@ static double ms_since(std::chrono::steady_clock::time_point t0) {
 static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, const cgs_solve_opts& o,
                  cgs_solve_report* rep, std::chrono::steady_clock::time_point t_start) {
   const int F = p.F;
+  const bool joint = o.joint != 0 && F > 1;
+  p.beta_folded = false;
   std::vector<double> done_ms((size_t)F, -1.0);
   const size_t ib = sizeof(float2) * (size_t)F * p.Ntot;
   float2 *x = p.img[0], *r = p.img[1], *pv = p.img[2], *q = p.img[3];
@@ -1283,7 +1309,7 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
     HIPCHK(hipMemsetAsync(x, 0, ib, p.stream));
     launch_sqnorm_partials(p, r, partA, p.stream);
   }
-  launch_cg_init(p, partA, partB, o.tol_kind, o.tol, o.maxit, warm, p.stream);
+  launch_cg_init(p, partA, partB, o.tol_kind, o.tol, o.maxit, warm, joint, p.stream);
   HIPCHK(hipMemcpyAsync(pv, r, ib, hipMemcpyDeviceToDevice, p.stream));
   CHK(check_launch());
 
@@ -1297,8 +1323,9 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
     // capture an even number of fused iterations once and replay the graph
     step = o.graph_iters + (o.graph_iters & 1);
     char key[256];
-    snprintf(key, sizeof(key), "%d|%d|%.17g|%d|%.17g|%d|%p|%p", step, o.normal_op, o.lambda, o.tol_kind, o.tol,
-             o.maxit, (void*)p.That, (void*)p.stream);
+    snprintf(key, sizeof(key), "%d|%d|%.17g|%d|%.17g|%d|%p|%p|%d", step, o.normal_op, o.lambda, o.tol_kind, o.tol,
+             o.maxit, (void*)p.That, (void*)p.stream, joint ? 1 : 0);
+    if (p.graph_exec && p.graph_key == key) p.beta_folded = p.graph_folded;
     if (!p.graph_exec || p.graph_key != key) {
       if (p.graph_exec) (void)hipGraphExecDestroy(p.graph_exec);
       p.graph_exec = nullptr;
@@ -1307,7 +1334,8 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
       HIPCHK(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal));
       int rc = CGS_OK;
       for (int k = 0; k < step && rc == CGS_OK; ++k)
-        rc = cg_iteration_fused(p, o.normal_op, x, r, pbuf[k & 1], pbuf[(k + 1) & 1], q, o);
+        rc = joint ? cg_iteration_plain(p, x, r, pv, q, o, true)
+                   : cg_iteration_fused(p, o.normal_op, x, r, pbuf[k & 1], pbuf[(k + 1) & 1], q, o);
       hipError_t ec = hipStreamEndCapture(p.stream, &g);
       p.capturing = false;
       if (rc != CGS_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -1316,21 +1344,16 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
       (void)hipGraphDestroy(g);
       if (ei != hipSuccess) { p.graph_exec = nullptr; set_error(std::string("graph instantiate failed: ") + hipGetErrorString(ei)); return CGS_ERR_HIP; }
       p.graph_key = key;
+      p.graph_folded = p.beta_folded;
     }
   }
   for (int it = 0; it < o.maxit; it += step) {
     if (step > 1) {
       HIPCHK(hipGraphLaunch(p.graph_exec, p.stream));
-    } else if (o.eh_mode == CGS_EH_TRUE) {
+    } else if (o.eh_mode == CGS_EH_TRUE && !joint) {
       CHK(cg_iteration_fused(p, o.normal_op, x, r, pbuf[it & 1], pbuf[(it + 1) & 1], q, o));
     } else {
-      CHK(op_E(p, pv, p.ybuf[1], p.fdone));
-      CHK(op_EH(p, o.eh_mode, p.ybuf[1], q, o.ftiter, o.fteps, pv, (float)o.lambda, p.partials, p.fdone));
-      Prof pr(p, CGS_K_CG, 7.0 * 8.0 * F * p.Ntot);
-      launch_cg_alpha(p, p.partials, p.stream);
-      launch_cg_xr(p, x, r, pv, q, p.partials, p.stream);
-      launch_cg_beta(p, p.partials, o.tol_kind, o.tol, o.maxit, p.stream);
-      launch_cg_p(p, pv, r, p.stream);
+      CHK(cg_iteration_plain(p, x, r, pv, q, o, joint));
     }
     CHK(check_launch());
     // pipelined convergence test: read the flags of the previous iteration
@@ -1628,17 +1651,20 @@ int cgs_set_sensitivities(cgs_plan plan, const void* S) {
   if (!plan || !S) { set_error("null argument"); return CGS_ERR_ARG; }
   Plan& p = plan->p;
   const float2* s = (const float2*)S;
-  std::vector<float2> si((size_t)p.Ntot * p.C);
-  std::vector<float> I((size_t)p.Ntot);
-  for (int64_t i = 0; i < p.Ntot; ++i) {
-    float acc = 0.f;   // IntensityMap accumulates in the maps' precision (MRI.hpp:21-26)
-    for (int c = 0; c < p.C; ++c) {
-      const float2 v = s[i + p.Ntot * c];
-      si[i * p.C + c] = v;
-      acc += v.x * v.x + v.y * v.y;
+  // host (N^d, nm, C) column-major -> device [frame][pixel][coil]
+  const int64_t nm = p.Sfs ? p.F : 1;
+  std::vector<float2> si((size_t)p.Ntot * p.C * nm);
+  std::vector<float> I((size_t)p.Ntot * nm);
+  for (int64_t z = 0; z < nm; ++z)
+    for (int64_t i = 0; i < p.Ntot; ++i) {
+      float acc = 0.f;   // IntensityMap accumulates in the maps' precision (MRI.hpp:21-26)
+      for (int c = 0; c < p.C; ++c) {
+        const float2 v = s[i + p.Ntot * (z + nm * c)];
+        si[(z * p.Ntot + i) * p.C + c] = v;
+        acc += v.x * v.x + v.y * v.y;
+      }
+      I[z * p.Ntot + i] = (float)(1.0 / (std::sqrt((double)acc) + 1.e-9));
     }
-    I[i] = (float)(1.0 / (std::sqrt((double)acc) + 1.e-9));
-  }
   HIPCHK(hipStreamSynchronize(p.stream));
   HIPCHK(hipMemcpy(p.S, si.data(), sizeof(float2) * si.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p.Imap, I.data(), sizeof(float) * I.size(), hipMemcpyHostToDevice));

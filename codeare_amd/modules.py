@@ -41,7 +41,9 @@ class ReconStrategy:
 class CGSENSE(ReconStrategy):
     """Non-Cartesian SENSE: workspace "sensitivities" (N..., C) complex,
     "kspace" (d, nk) float, "weights" (nk) float, "signals" (nk, C) complex
-    -> "image" (N...) complex (CGSENSE.cpp:132-178)."""
+    -> "image" (N...) complex (CGSENSE.cpp:132-178).  With dft_3rd_dim the
+    maps are (N0, N1, Nz, C), the signals (nk, Nz, C) and the image
+    (N0, N1, Nz): stack-of-stars slices on one 2D trajectory, one CG."""
 
     def init(self, **attrs):
         # defaults of CGSENSE::Init (CGSENSE.cpp:61-64) and of the XML examples
@@ -50,15 +52,14 @@ class CGSENSE(ReconStrategy):
         self.attrs.update(attrs)
         if "nk" not in self.attrs:
             raise KeyError("CGSENSE needs the nk attribute")
-        if int(self.attrs["dft_3rd_dim"]):
-            raise NotImplementedError("3rd_dim_cart stack-of-stars is not built")
         self.initialised = True
 
     def prepare(self, ws: dict):
         a = self.attribute
         params = {"sensitivities": ws["sensitivities"], "nk": a("nk", cast=int),
                   "ftiter": a("ftmaxit", cast=int), "cgiter": a("cgmaxit", cast=int),
-                  "cgeps": a("cgeps"), "lambda": a("lambda"), "m": a("m", cast=int)}
+                  "cgeps": a("cgeps"), "lambda": a("lambda"), "m": a("m", cast=int),
+                  "3rd_dim_cart": bool(a("dft_3rd_dim", cast=int))}   # CGSENSE.cpp:147
         # no "alpha" and no "fteps": NCSENSE's defaults (sigma 1, 1e-3) apply
         self.ncs = NCSENSE(params)
         self.ncs.KSpace(ws.pop("kspace"))
@@ -78,7 +79,8 @@ class CGSENSE(ReconStrategy):
 
 class NuFFT(ReconStrategy):
     """Single-coil NFFT: workspace "kspace", "weights", "data" (M*shots)
-    -> "img" = NFFT ->* data (NuFFT.cpp:49-142)."""
+    -> "img" = NFFT ->* data (NuFFT.cpp:49-142).  With dft_3rd_dim (dim 3) the
+    data is (M*shots, Nz) and the image (Nx, Ny, Nz), one 2D slice per column."""
 
     def init(self, **attrs):
         self.attrs = {"m": 1, "alpha": 1.0, "shots": 1, "test_case": 0, "dft_3rd_dim": 0}
@@ -87,11 +89,9 @@ class NuFFT(ReconStrategy):
         sides = ["Nx", "Ny", "Nz"][:d]
         imsz = [int(self.attrs.get(s, 1)) for s in sides]
         nk = int(self.attrs["M"]) * int(self.attrs["shots"])
-        if int(self.attrs["dft_3rd_dim"]):
-            raise NotImplementedError("3rd_dim_cart stack-of-stars is not built")
         # maxit and epsilon are read but not passed on (NuFFT.cpp:80-107)
         self.ft = NFFT({"alpha": float(self.attrs["alpha"]), "m": int(self.attrs["m"]),
-                        "nk": nk, "imsz": imsz})
+                        "nk": nk, "imsz": imsz, "3rd_dim_cart": bool(int(self.attrs["dft_3rd_dim"]))})
         self.initialised = True
 
     def prepare(self, ws: dict):
@@ -101,7 +101,7 @@ class NuFFT(ReconStrategy):
     def process(self, ws: dict):
         if int(self.attrs["test_case"]):
             raise NotImplementedError("the built-in phantom test case is not built; pass data")
-        ws["img"] = self.ft.Adjoint(np.asarray(ws["data"], dtype=np.complex64).reshape(-1))
+        ws["img"] = self.ft.Adjoint(np.asarray(ws["data"], dtype=np.complex64).reshape(-1, order="F"))
         return ws["img"]
 
 

@@ -50,6 +50,9 @@ enum {
 #define CGS_PROFILE        0x2u  /* time every kernel launch with HIP events */
 #define CGS_DETERMINISTIC  0x4u  /* fixed summation order in the gridding (slower) */
 #define CGS_AUTOTUNE       0x8u  /* cgs_set_trajectory runs (or reuses) cgs_autotune */
+#define CGS_MAPS_PER_FRAME 0x10u /* one set of maps per frame: cgs_set_sensitivities
+                                    takes (N^d, F, C), the (N0, N1, Nz, C) maps of
+                                    codeare's 3rd_dim_cart slices (NCSENSE.hpp:98-99) */
 
 /* E^H variants */
 #define CGS_EH_TRUE        0     /* x = sum_c conj(S_c) .* NFFT^H(w .* y_c)   */
@@ -74,6 +77,11 @@ typedef struct {
   int normal_op;      /* CGS_NORMAL_NUFFT or CGS_NORMAL_TOEPLITZ (true E^H) */
   int graph_iters;    /* >0: capture this many CG iterations (true E^H) in one
                          HIP graph and replay it; 0: launch kernels one by one */
+  int joint;          /* 1: one CG over all frames together -- alpha, beta and the
+                         stop test from sums over every frame, as codeare's CGLS
+                         runs on the whole (N^d, F) matrix for nmany frames and
+                         3rd_dim_cart slices (CGLS.hpp:30-59 on NCSENSE.hpp:251-348);
+                         0: every frame has its own scalars and stops on its own */
 } cgs_solve_opts;
 
 typedef struct {
@@ -111,8 +119,9 @@ int cgs_set_trajectory(cgs_plan plan, const float* k);
  * host (M_total) float density weights; also fixes the damping mask
  * (NFFTTraits.hpp:225-260). */
 int cgs_set_weights(cgs_plan plan, const float* w);
-/* NCSENSE::Sensitivities (NCSENSE.hpp:193-197): host (N^d, C) complex64 maps;
- * also computes IntensityMap (MRI.hpp:6-30). */
+/* NCSENSE::Sensitivities (NCSENSE.hpp:193-197): host (N^d, C) complex64 maps,
+ * or (N^d, F, C) with CGS_MAPS_PER_FRAME; also computes IntensityMap
+ * (MRI.hpp:6-30), per frame with per-frame maps. */
 int cgs_set_sensitivities(cgs_plan plan, const void* S);
 
 /* --- NFFT<cxfl> operators (one frame at a time, batch images) -------------- */
@@ -137,7 +146,8 @@ int cgs_normal(cgs_plan plan, const void* x, void* q, double lambda, int normal_
 
 /* CG solve (NCSENSE::Adjoint / operator->* -> CGLS::Solve, NCSENSE.hpp:408-414,
  * CGLS.hpp:30-59).  x0 may be NULL (zero start, as codeare) or a warm start.
- * Every frame has its own scalars and stops on its own. report may be NULL. */
+ * Every frame has its own scalars and stops on its own, or, with opts->joint,
+ * all frames share one CG.  report may be NULL. */
 int cgs_solve(cgs_plan plan, const void* y, const void* x0, void* x,
               const cgs_solve_opts* opts, cgs_solve_report* report);
 

@@ -6,9 +6,14 @@
 //   size_t Size() const
 // works, which includes codeare's Matrix<cxfl> / Matrix<float>.  Parameters use
 // codeare's keys and defaults (NCSENSE.hpp:69-184, NFFT.hpp:66-191):
-//   NFFT:    nk, imsz, m (1), alpha (1.0), epsilon (7e-4), maxit (3)
+//   NFFT:    nk, imsz, m (1), alpha (1.0), epsilon (7e-4), maxit (3),
+//            3rd_dim_cart (0: with 1, imsz (N0, N1, Nz) is Nz 2D slices on one
+//            trajectory, NFFT.hpp:84-103)
 //   NCSENSE: nk, ftiter (3), fteps (1e-3), cgiter (0), cgeps (1e-6),
-//            lambda (1e-6), m (1), alpha (1.0), dim4 (1)
+//            lambda (1e-6), m (1), alpha (1.0), dim4 (1), 3rd_dim_cart (0: with 1,
+//            maps (N0, N1, Nz, C) and data (nk, Nz, C), NCSENSE.hpp:81-99)
+// Frames (dim4) and slices share one CG, as codeare's CGLS runs on the whole
+// image matrix (cgs_solve_opts.joint).
 // Errors from the library are thrown as std::runtime_error carrying the
 // library's message (codeare asserts or throws NCSENSE_KSPACE_DIMENSIONS).
 #pragma once
@@ -56,8 +61,13 @@ class NFFT {
     for (size_t v : p.imsz()) N_.push_back((int)v);
     maxit_ = (int)p.get("maxit", 3);
     eps_ = p.get("epsilon", 7.0e-4);
-    check(cgs_plan_create(&plan_, (int)N_.size(), N_.data(), nk_, p.get("alpha", 1.0), (int)p.get("m", 1), 1, 1,
-                          nullptr, 0));
+    if (p.get("3rd_dim_cart", 0) != 0 && N_.size() == 3) {   // NFFT.hpp:100-103
+      ncart_ = N_.back();
+      N_.pop_back();
+    }
+    // the slices of a Cartesian third axis are the plan's image batch
+    check(cgs_plan_create(&plan_, (int)N_.size(), N_.data(), nk_, p.get("alpha", 1.0), (int)p.get("m", 1), ncart_,
+                          1, nullptr, 0));
   }
   ~NFFT() { cgs_plan_destroy(plan_); }
   NFFT(const NFFT&) = delete;
@@ -66,27 +76,32 @@ class NFFT {
   void KSpace(const RealMatrix& k) { check(cgs_set_trajectory(plan_, k.Ptr())); }
   void Weights(const RealMatrix& w) { check(cgs_set_weights(plan_, w.Ptr())); }
 
-  Matrix Trafo(const Matrix& img) const {            // NFFT.hpp:305-361
-    Matrix out(std::vector<size_t>{(size_t)nk_});
-    check(cgs_nfft_forward(plan_, 0, img.Ptr(), out.Ptr(), 1));
+  Matrix Trafo(const Matrix& img) const {            // NFFT.hpp:305-361 (per slice)
+    Matrix out(ncart_ > 1 ? std::vector<size_t>{(size_t)nk_, (size_t)ncart_} : std::vector<size_t>{(size_t)nk_});
+    check(cgs_nfft_forward(plan_, 0, img.Ptr(), out.Ptr(), ncart_));
     return out;
   }
   Matrix operator*(const Matrix& img) const { return Trafo(img); }
-  Matrix Adjoint(const Matrix& f) const {             // NFFT.hpp:370-436 (iterative inverse)
+  Matrix Adjoint(const Matrix& f) const {             // NFFT.hpp:370-436 (iterative inverse, per slice)
     Matrix out(image_dims());
-    check(cgs_nfft_inverse_ref(plan_, 0, f.Ptr(), out.Ptr(), 1, maxit_, eps_));
+    check(cgs_nfft_inverse_ref(plan_, 0, f.Ptr(), out.Ptr(), ncart_, maxit_, eps_));
     return out;
   }
   Matrix AdjointTrue(const Matrix& f) const {         // nfft_adjoint, NFFTTraits.hpp:201-204
     Matrix out(image_dims());
-    check(cgs_nfft_adjoint(plan_, 0, f.Ptr(), out.Ptr(), 1));
+    check(cgs_nfft_adjoint(plan_, 0, f.Ptr(), out.Ptr(), ncart_));
     return out;
   }
 
  private:
-  std::vector<size_t> image_dims() const { return std::vector<size_t>(N_.begin(), N_.end()); }
+  std::vector<size_t> image_dims() const {
+    std::vector<size_t> d(N_.begin(), N_.end());
+    if (ncart_ > 1) d.push_back((size_t)ncart_);
+    return d;
+  }
   cgs_plan plan_ = nullptr;
   std::vector<int> N_;
+  int ncart_ = 1;
   int64_t nk_ = 0;
   int maxit_ = 3;
   double eps_ = 7.0e-4;
@@ -99,11 +114,15 @@ class NCSENSE {
   // sens: (N0, N1[, N2], C) complex maps, as Params["sensitivities"] in codeare
   NCSENSE(const Params& p, const Matrix& sens, const std::vector<size_t>& sens_dims) {
     if (sens_dims.size() < 3) throw std::runtime_error("NCSENSE: maps must be (N..., C)");
-    d_ = (int)sens_dims.size() - 1;
+    cart_ = p.get("3rd_dim_cart", 0) != 0;
+    if (cart_ && sens_dims.size() != 4) throw std::runtime_error("NCSENSE: 3rd_dim_cart needs (N0, N1, Nz, C) maps");
+    d_ = (int)sens_dims.size() - (cart_ ? 2 : 1);
     C_ = (int)sens_dims.back();
     for (int a = 0; a < d_; ++a) N_.push_back((int)sens_dims[a]);
     nk_ = (int64_t)p.get("nk", 1);
-    nmany_ = (int)p.get("dim4", 1);
+    // frames of the plan: dim4 frames (NCSENSE.hpp:141-151) or the Cartesian
+    // slices, which share the trajectory, the weights and one CG
+    nmany_ = cart_ ? (int)sens_dims[2] : (int)p.get("dim4", 1);
     ftiter_ = (int)p.get("ftiter", 3);
     fteps_ = p.get("fteps", 1.0e-3);
     cgiter_ = (int)p.get("cgiter", 0);
@@ -112,7 +131,7 @@ class NCSENSE {
     std::vector<int64_t> off;
     for (int f = 0; f <= nmany_; ++f) off.push_back(nk_ * f);
     check(cgs_plan_create(&plan_, d_, N_.data(), nk_ * nmany_, p.get("alpha", 1.0), (int)p.get("m", 1), C_,
-                          nmany_, nmany_ > 1 ? off.data() : nullptr, 0));
+                          nmany_, nmany_ > 1 ? off.data() : nullptr, cart_ ? CGS_MAPS_PER_FRAME : 0u));
     check(cgs_set_sensitivities(plan_, sens.Ptr()));
   }
   ~NCSENSE() { cgs_plan_destroy(plan_); }
@@ -120,18 +139,28 @@ class NCSENSE {
   NCSENSE& operator=(const NCSENSE&) = delete;
 
   void Sensitivities(const Matrix& sm) { check(cgs_set_sensitivities(plan_, sm.Ptr())); }
-  void KSpace(const RealMatrix& k) { check(cgs_set_trajectory(plan_, k.Ptr())); }
-  void Weights(const RealMatrix& w) { check(cgs_set_weights(plan_, w.Ptr())); }
+  // k: (d, nk) shared by every frame/slice, or (d, nk, dim4) per frame
+  // (NCSENSE.hpp:204-236); w: (nk) shared by every frame (NCSENSE.hpp:244-248)
+  void KSpace(const RealMatrix& k) {
+    check(cgs_set_trajectory(plan_, per_frame(k.Ptr(), k.Size(), (size_t)d_ * nk_).data()));
+  }
+  void Weights(const RealMatrix& w) { check(cgs_set_weights(plan_, per_frame(w.Ptr(), w.Size(), (size_t)nk_).data())); }
 
   Matrix Trafo(const Matrix& x) const {               // NCSENSE.hpp:310-348
-    Matrix y(std::vector<size_t>{(size_t)nk_, (size_t)C_ * nmany_});
-    check(cgs_E(plan_, x.Ptr(), y.Ptr()));
+    Matrix y(data_dims());
+    if (!cart_) {
+      check(cgs_E(plan_, x.Ptr(), y.Ptr()));
+    } else {
+      std::vector<cxfl> t((size_t)nk_ * C_ * nmany_);
+      check(cgs_E(plan_, x.Ptr(), t.data()));
+      slices_to_codeare(t.data(), y.Ptr());
+    }
     return y;
   }
   Matrix operator*(const Matrix& x) const { return Trafo(x); }
   Matrix operator/(const Matrix& y) const {           // NCSENSE.hpp:251-301
     Matrix x(image_dims());
-    check(cgs_EH(plan_, y.Ptr(), x.Ptr(), CGS_EH_REF, ftiter_, fteps_));
+    check(cgs_EH(plan_, plan_data(y).data(), x.Ptr(), CGS_EH_REF, ftiter_, fteps_));
     return x;
   }
   Matrix Adjoint(const Matrix& y) const {             // NCSENSE.hpp:408-414 -> CGLS.hpp:30-59
@@ -145,8 +174,9 @@ class NCSENSE {
     o.fteps = fteps_;
     o.normal_op = CGS_NORMAL_NUFFT;
     o.graph_iters = 0;
+    o.joint = 1;
     Matrix x(image_dims());
-    const int rc = cgs_solve(plan_, y.Ptr(), nullptr, x.Ptr(), &o, nullptr);
+    const int rc = cgs_solve(plan_, plan_data(y).data(), nullptr, x.Ptr(), &o, nullptr);
     if (rc != CGS_ERR_NAN) check(rc);   // codeare breaks out of CGLS on NaN and returns x
     return x;
   }
@@ -160,7 +190,36 @@ class NCSENSE {
     if (nmany_ > 1) d.push_back((size_t)nmany_);
     return d;
   }
+  std::vector<size_t> data_dims() const {   // (nk, C*dim4), or (nk, Nz, C) for slices
+    if (cart_) return std::vector<size_t>{(size_t)nk_, (size_t)nmany_, (size_t)C_};
+    return std::vector<size_t>{(size_t)nk_, (size_t)C_ * nmany_};
+  }
+  // one block per frame when the caller gives one shared block
+  template <class T>
+  std::vector<T> per_frame(const T* v, size_t n, size_t block) const {
+    if (n != block && n != block * nmany_) throw std::runtime_error("NCSENSE: k-space / weights size mismatch");
+    std::vector<T> out(block * nmany_);
+    for (int f = 0; f < nmany_; ++f)
+      for (size_t i = 0; i < block; ++i) out[f * block + i] = v[n == block ? i : f * block + i];
+    return out;
+  }
+  // codeare's slice data (nk, Nz, C) <-> the plan's frame blocks (nk, C) x Nz
+  void slices_to_codeare(const cxfl* t, cxfl* y) const {
+    for (int z = 0; z < nmany_; ++z)
+      for (int c = 0; c < C_; ++c)
+        for (int64_t j = 0; j < nk_; ++j) y[j + nk_ * (z + (int64_t)nmany_ * c)] = t[j + nk_ * (c + (int64_t)C_ * z)];
+  }
+  std::vector<cxfl> plan_data(const Matrix& y) const {
+    const cxfl* s = reinterpret_cast<const cxfl*>(y.Ptr());
+    std::vector<cxfl> t(s, s + (size_t)nk_ * C_ * nmany_);
+    if (cart_)
+      for (int z = 0; z < nmany_; ++z)
+        for (int c = 0; c < C_; ++c)
+          for (int64_t j = 0; j < nk_; ++j) t[j + nk_ * (c + (int64_t)C_ * z)] = s[j + nk_ * (z + (int64_t)nmany_ * c)];
+    return t;
+  }
   cgs_plan plan_ = nullptr;
+  bool cart_ = false;
   int d_ = 2, C_ = 1, nmany_ = 1;
   std::vector<int> N_;
   int64_t nk_ = 1;

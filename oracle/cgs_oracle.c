@@ -568,6 +568,115 @@ int ora_cgls_ref(const ora_nfft* p, int C, const float* S, const float* w,
   return it;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Frames (NCSENSE nmany, NCSENSE.hpp:141-151, :254-288, :311-333) and Cartesian
+ * slices (3rd_dim_cart, NCSENSE.hpp:81-99, :338-339): frame f has its own plan
+ * (trajectory), maps S + 2*f*Sfs floats, weights w + moff[f] and data block
+ * y + 2*C*moff[f] of shape (M_f, C); the images are concatenated (N^d, F). */
+static void frames_E(const ora_nfft* const* pl, int F, int C, const float* S, int64_t Sfs,
+                     const int64_t* moff, const float* x, float* y) {
+  const int64_t Nt = pl[0]->Ntot;
+  for (int f = 0; f < F; ++f)
+    ora_ncsense_E(pl[f], C, S + 2 * f * Sfs, x + 2 * f * Nt, y + 2 * (int64_t)C * moff[f]);
+}
+
+static void frames_EH(const ora_nfft* const* pl, int F, int C, const float* S, int64_t Sfs,
+                      const float* w, const int64_t* moff, int mode, int ftiter, double fteps,
+                      const float* y, float* x) {
+  const int64_t Nt = pl[0]->Ntot;
+  for (int f = 0; f < F; ++f)
+    apply_EH(pl[f], C, S + 2 * f * Sfs, w + moff[f], mode, ftiter, fteps, y + 2 * (int64_t)C * moff[f],
+             x + 2 * f * Nt);
+}
+
+/* CGLS::Solve (CGLS.hpp:30-59) on the whole (N^d, F) image matrix, as codeare
+ * runs it for nmany frames and 3rd_dim_cart slices: one alpha and beta from
+ * serial complex64 dotc over the concatenated vectors (Lapack.hpp:619-624). */
+int ora_cgls_ref_frames(const ora_nfft* const* pl, int F, int C, const float* S, int64_t Sfs,
+                        const float* w, const int64_t* moff, int mode, int ftiter, double fteps,
+                        int maxit, double cgeps, double lambda, const float* y, float* x_, double* hist) {
+  const int64_t Nt = pl[0]->Ntot * F, Mt = moff[F];
+  float complex* x = (float complex*)x_;
+  float complex* pp = (float complex*)malloc(sizeof(float complex) * (size_t)Nt);
+  float complex* r = (float complex*)malloc(sizeof(float complex) * (size_t)Nt);
+  float complex* q = (float complex*)malloc(sizeof(float complex) * (size_t)Nt);
+  float* yt = (float*)malloc(sizeof(float) * 2 * (size_t)Mt * C);
+  frames_EH(pl, F, C, S, Sfs, w, moff, mode, ftiter, fteps, y, (float*)pp);   /* :31 */
+  if (maxit == 0) {                                                          /* :32-33 */
+    memcpy(x, pp, sizeof(float complex) * (size_t)Nt);
+    free(pp); free(r); free(q); free(yt);
+    return 0;
+  }
+  memcpy(r, pp, sizeof(float complex) * (size_t)Nt);
+  float rn = crealf(dotc_f(r, pp, Nt)), xn = rn, rno;                         /* :35 */
+  memset(x, 0, sizeof(float complex) * (size_t)Nt);
+  int it = 0;
+  for (int i = 0; i < maxit; ++i) {
+    float res = rn / xn;
+    if (hist) hist[i] = res;
+    if (isnan(res) || res <= (float)cgeps) break;                            /* :41 */
+    frames_E(pl, F, C, S, Sfs, moff, (const float*)pp, yt);                  /* :47 */
+    frames_EH(pl, F, C, S, Sfs, w, moff, mode, ftiter, fteps, yt, (float*)q);
+    if (lambda != 0.0)
+      for (int64_t k = 0; k < Nt; ++k) q[k] += (float)lambda * pp[k];
+    float ts = rn / crealf(dotc_f(pp, q, Nt));                               /* :50 */
+    for (int64_t k = 0; k < Nt; ++k) x[k] += ts * pp[k];
+    for (int64_t k = 0; k < Nt; ++k) r[k] -= ts * q[k];
+    rno = rn;
+    rn = crealf(dotc_f(r, r, Nt));
+    for (int64_t k = 0; k < Nt; ++k) pp[k] = pp[k] * (rn / rno) + r[k];      /* :55-56 */
+    ++it;
+  }
+  if (hist && it == maxit) hist[it] = rn / xn;
+  free(pp); free(r); free(q); free(yt);
+  return it;
+}
+
+/* ora_cg_true on the concatenated frames with one set of FP64 scalars (the GPU
+ * library's joint solve). */
+int ora_cg_true_frames(const ora_nfft* const* pl, int F, int C, const float* S, int64_t Sfs,
+                       const float* w, const int64_t* moff, int maxit, double tol, double lambda,
+                       const float* y, float* x_, double* hist) {
+  const int64_t Nt = pl[0]->Ntot * F, Mt = moff[F];
+  float complex* x = (float complex*)x_;
+  float complex* pp = (float complex*)malloc(sizeof(float complex) * (size_t)Nt);
+  float complex* r = (float complex*)malloc(sizeof(float complex) * (size_t)Nt);
+  float complex* q = (float complex*)malloc(sizeof(float complex) * (size_t)Nt);
+  float* yt = (float*)malloc(sizeof(float) * 2 * (size_t)Mt * C);
+  frames_EH(pl, F, C, S, Sfs, w, moff, 0, 0, 0.0, y, (float*)r);
+  double bn = 0;
+  for (int64_t k = 0; k < Nt; ++k) bn += (double)crealf(r[k]) * crealf(r[k]) + (double)cimagf(r[k]) * cimagf(r[k]);
+  memset(x, 0, sizeof(float complex) * (size_t)Nt);
+  memcpy(pp, r, sizeof(float complex) * (size_t)Nt);
+  double rn = bn;
+  int it = 0;
+  for (;;) {
+    double rel = bn > 0 ? sqrt(rn / bn) : 0.0;
+    if (hist) hist[it] = rel;
+    if (isnan(rel) || rel <= tol || it >= maxit) break;
+    frames_E(pl, F, C, S, Sfs, moff, (const float*)pp, yt);
+    frames_EH(pl, F, C, S, Sfs, w, moff, 0, 0, 0.0, yt, (float*)q);
+    double pq = 0;
+    for (int64_t k = 0; k < Nt; ++k) {
+      q[k] += (float)lambda * pp[k];
+      pq += (double)crealf(pp[k]) * crealf(q[k]) + (double)cimagf(pp[k]) * cimagf(q[k]);
+    }
+    double alpha = rn / pq;
+    double rn_new = 0;
+    for (int64_t k = 0; k < Nt; ++k) {
+      x[k] += (float)alpha * pp[k];
+      r[k] -= (float)alpha * q[k];
+      rn_new += (double)crealf(r[k]) * crealf(r[k]) + (double)cimagf(r[k]) * cimagf(r[k]);
+    }
+    double beta = rn_new / rn;
+    rn = rn_new;
+    for (int64_t k = 0; k < Nt; ++k) pp[k] = r[k] + (float)beta * pp[k];
+    ++it;
+  }
+  free(pp); free(r); free(q); free(yt);
+  return it;
+}
+
 /* CG on the true normal equations with FP64 scalars (the GPU solver's algorithm). */
 int ora_cg_true(const ora_nfft* p, int C, const float* S, const float* w,
                 int maxit, double tol, double lambda, const float* y,

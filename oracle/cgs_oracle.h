@@ -92,6 +92,18 @@ int ora_cg_true(const ora_nfft* p, int C, const float* S, const float* w,
                 int maxit, double tol, double lambda, const float* y,
                 const float* x0, float* x, double* res_hist);
 
+/* The two solvers on F frames or Cartesian slices solved together (one CG on
+ * the (N^d, F) image matrix, codeare's nmany and 3rd_dim_cart, NCSENSE.hpp:
+ * 81-99, 141-151, 251-348): frame f uses plans[f], maps S + 2*f*Sfs floats
+ * (Sfs = 0: shared maps), weights w + moff[f] and the (M_f, C) data block at
+ * y + 2*C*moff[f]; x is (N^d, F). */
+int ora_cgls_ref_frames(const ora_nfft* const* plans, int F, int C, const float* S, int64_t Sfs,
+                        const float* w, const int64_t* moff, int mode, int ftiter, double fteps,
+                        int maxit, double cgeps, double lambda, const float* y, float* x, double* res_hist);
+int ora_cg_true_frames(const ora_nfft* const* plans, int F, int C, const float* S, int64_t Sfs,
+                       const float* w, const int64_t* moff, int maxit, double tol, double lambda,
+                       const float* y, float* x, double* res_hist);
+
 /* threads used by the OpenMP regions (0 = runtime default) */
 void ora_set_threads(int n);
 int ora_get_threads(void);

@@ -65,6 +65,10 @@ def lib():
         L.ora_cgls_ref.argtypes = [P, I, P, P, I, I, D, I, D, D, P, P, P]
         L.ora_cg_true.restype = I
         L.ora_cg_true.argtypes = [P, I, P, P, I, D, D, P, P, P, P]
+        L.ora_cgls_ref_frames.restype = I
+        L.ora_cgls_ref_frames.argtypes = [P, I, I, P, I64, P, P, I, I, D, I, D, D, P, P, P]
+        L.ora_cg_true_frames.restype = I
+        L.ora_cg_true_frames.argtypes = [P, I, I, P, I64, P, P, I, D, D, P, P, P]
         L.ora_set_threads.argtypes = [I]
         L.ora_get_threads.restype = I
         _lib = L
@@ -256,3 +260,48 @@ class NFFT:
         it = lib().ora_cg_true(self._p, C, _ptr(S), _ptr(w), int(maxit), float(tol),
                                float(lam), _ptr(y), x0p, _ptr(x), _ptr(hist))
         return x, it, hist[: it + 1]
+
+
+# ---- frames / Cartesian slices solved together ------------------------------
+def _frames_args(plans, S, w, y, maps_per_frame):
+    F = len(plans)
+    arr = (ct.c_void_p * F)(*[p._p for p in plans])
+    moff = np.zeros(F + 1, dtype=np.int64)
+    moff[1:] = np.cumsum([p.M for p in plans])
+    S = _c64(S)
+    C = S.shape[-1]
+    Nt = int(np.prod(plans[0].N))
+    if maps_per_frame:
+        # codeare (N..., F, C) -> one (N..., C) block per frame
+        S = _c64(np.moveaxis(S.reshape((Nt, F, C), order="F"), 1, 2))
+        Sfs = Nt * C
+    else:
+        Sfs = 0
+    w = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
+    y = _c64(y).reshape(-1, order="F")
+    assert w.size == moff[-1] and y.size == moff[-1] * C
+    return F, arr, moff, S, Sfs, C, w, y, Nt
+
+
+def cgls_ref_frames(plans, S, w, y, mode=1, ftiter=3, fteps=1e-3, maxit=10, cgeps=1e-7, lam=0.0,
+                    maps_per_frame=False):
+    """codeare's CGLS on the whole (N^d, F) image of nmany frames / 3rd_dim_cart
+    slices (one oracle NFFT plan per frame).  y holds the (M_f, C) block of every
+    frame in turn; maps are (N..., C) shared or (N..., F, C) per frame."""
+    F, arr, moff, S, Sfs, C, w, y, Nt = _frames_args(plans, S, w, y, maps_per_frame)
+    x = np.zeros(plans[0].N + (F,), dtype=np.complex64, order="F")
+    hist = np.zeros(maxit + 1, dtype=np.float64)
+    it = lib().ora_cgls_ref_frames(ct.cast(arr, ct.c_void_p), F, C, _ptr(S), Sfs, _ptr(w), _ptr(moff),
+                                   int(mode), int(ftiter), float(fteps), int(maxit), float(cgeps),
+                                   float(lam), _ptr(y), _ptr(x), _ptr(hist))
+    return x, it, hist[: it + 1]
+
+
+def cg_true_frames(plans, S, w, y, maxit=50, tol=1e-5, lam=0.0, maps_per_frame=False):
+    """The library's joint true-adjoint CG (FP64 scalars) on frames / slices."""
+    F, arr, moff, S, Sfs, C, w, y, Nt = _frames_args(plans, S, w, y, maps_per_frame)
+    x = np.zeros(plans[0].N + (F,), dtype=np.complex64, order="F")
+    hist = np.zeros(maxit + 1, dtype=np.float64)
+    it = lib().ora_cg_true_frames(ct.cast(arr, ct.c_void_p), F, C, _ptr(S), Sfs, _ptr(w), _ptr(moff),
+                                  int(maxit), float(tol), float(lam), _ptr(y), _ptr(x), _ptr(hist))
+    return x, it, hist[: it + 1]

@@ -39,8 +39,50 @@ static void save(const std::string& path, const M& m) {
   f.write((const char*)m.Ptr(), sizeof(m.data[0]) * m.Size());
 }
 
+// 3rd_dim_cart: Nz slices on the one trajectory (NCSENSE.hpp:81-99, NFFT.hpp:100-103)
+static int run_slices(const std::string& dir, size_t N, size_t C, size_t nk, double m, double alpha, size_t Nz) {
+  codeare_amd::Params p;
+  p.set("nk", (double)nk);
+  p.set("m", m);
+  p.set("alpha", alpha);
+  p.set("ftiter", 3);
+  p.set("fteps", 1e-3);
+  p.set("cgiter", 5);
+  p.set("cgeps", 1e-7);
+  p.set("lambda", 5e-2);
+  p.set("3rd_dim_cart", 1);
+  CMat S({N, N, Nz, C}), x({N, N, Nz}), y({nk, Nz, C});
+  RMat k({2, nk}), w({nk});
+  load(dir + "/S3.bin", S);
+  load(dir + "/x3.bin", x);
+  load(dir + "/y3.bin", y);
+  load(dir + "/k.bin", k);
+  load(dir + "/w.bin", w);
+  codeare_amd::NCSENSE<CMat, RMat> ncs(p, S, {N, N, Nz, C});
+  ncs.KSpace(k);
+  ncs.Weights(w);
+  save(dir + "/E3.bin", ncs * x);
+  save(dir + "/EH3.bin", ncs / y);
+  save(dir + "/solve3.bin", ncs->*y);
+  codeare_amd::Params q;
+  q.set("nk", (double)nk);
+  q.set("m", m);
+  q.set("alpha", alpha);
+  q.set("3rd_dim_cart", 1);
+  q.set_imsz({N, N, Nz});
+  codeare_amd::NFFT<CMat, RMat> ft(q);
+  ft.KSpace(k);
+  ft.Weights(w);
+  save(dir + "/trafo3.bin", ft * x);
+  CMat y0({nk, Nz});
+  for (size_t z = 0; z < Nz; ++z)
+    for (size_t j = 0; j < nk; ++j) y0.data[j + nk * z] = y.data[j + nk * z];   // coil 0 of every slice
+  save(dir + "/inv3.bin", ft.Adjoint(y0));
+  return 0;
+}
+
 int main(int argc, char** argv) {
-  if (argc < 7) { std::fprintf(stderr, "usage: t_ncsense dir N C nk m alpha\n"); return 2; }
+  if (argc < 7) { std::fprintf(stderr, "usage: t_ncsense dir N C nk m alpha [Nz]\n"); return 2; }
   const std::string dir = argv[1];
   const size_t N = std::atoi(argv[2]), C = std::atoi(argv[3]), nk = std::atoi(argv[4]);
   codeare_amd::Params p;
@@ -89,6 +131,8 @@ int main(int argc, char** argv) {
       threw = std::string(e.what()).find("outside") != std::string::npos;
     }
     if (!threw) { std::fprintf(stderr, "bad trajectory not rejected\n"); return 3; }
+    if (argc >= 8 && std::atoi(argv[7]) > 0)
+      run_slices(dir, N, C, nk, std::atof(argv[5]), std::atof(argv[6]), (size_t)std::atoi(argv[7]));
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

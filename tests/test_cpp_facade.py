@@ -52,8 +52,20 @@ def test_cpp_ncsense_matches_oracle(ora, tmp_path):
     _save(os.path.join(d, "y.bin"), y, np.complex64)
     _save(os.path.join(d, "k.bin"), prob.k, np.float32)
     _save(os.path.join(d, "w.bin"), prob.w, np.float32)
-    print(f"t_ncsense N={N} C={C} nk={M} m={wl.m} alpha={wl.sigma}", flush=True)
-    r = subprocess.run([BIN, d, str(N), str(C), str(M), str(wl.m), str(wl.sigma)],
+    # 3rd_dim_cart slices: maps (N, N, Nz, C), data (nk, Nz, C)
+    Nz = 2
+    rng = np.random.default_rng(3)
+    S3 = np.asfortranarray(np.stack([prob.maps, prob.maps * (0.8 + 0.4j)], axis=2).astype(np.complex64))
+    x3 = np.asfortranarray(np.stack([prob.image, np.roll(prob.image, 5, axis=0)], axis=2))
+    y3 = np.zeros((M, Nz, C), dtype=np.complex64, order="F")
+    for z in range(Nz):
+        y3[:, z, :] = O.E(np.asfortranarray(S3[:, :, z, :]), np.asfortranarray(x3[..., z]))
+    y3 += (0.05 * np.abs(y3).max() * rng.normal(size=y3.shape)).astype(np.complex64)
+    _save(os.path.join(d, "S3.bin"), S3, np.complex64)
+    _save(os.path.join(d, "x3.bin"), x3, np.complex64)
+    _save(os.path.join(d, "y3.bin"), y3, np.complex64)
+    print(f"t_ncsense N={N} C={C} nk={M} m={wl.m} alpha={wl.sigma} Nz={Nz}", flush=True)
+    r = subprocess.run([BIN, d, str(N), str(C), str(M), str(wl.m), str(wl.sigma), str(Nz)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
 
@@ -70,3 +82,25 @@ def test_cpp_ncsense_matches_oracle(ora, tmp_path):
           f"NFFT* {e_trafo:.2e}  NFFT::Adjoint {e_inv:.2e}")
     assert e_E <= OP_TOL and e_EH <= OP_TOL and e_trafo <= OP_TOL
     assert e_inv <= 1e-5 and e_solve <= 1e-4
+
+    # slices through the C++ facade
+    e3 = []
+    for z in range(Nz):
+        Sz = np.asfortranarray(S3[:, :, z, :])
+        e3.append(rel(_load(os.path.join(d, "E3.bin"), (M, Nz, C))[:, z, :], O.E(Sz, np.asfortranarray(x3[..., z]))))
+        e3.append(rel(_load(os.path.join(d, "EH3.bin"), (N, N, Nz))[..., z],
+                      O.EH_ref(Sz, prob.w, np.asfortranarray(y3[:, z, :]), ftiter=3, fteps=1e-3)))
+        e3.append(rel(_load(os.path.join(d, "trafo3.bin"), (M, Nz))[:, z], O.trafo(x3[..., z])))
+        io, _, _ = O.itrafo(y3[:, z, 0], prob.w, maxit=3, eps=7e-4)
+        e3.append(rel(_load(os.path.join(d, "inv3.bin"), (N, N, Nz))[..., z], io))
+    plans = []
+    for z in range(Nz):
+        Oz = ora.NFFT((N, N), M, wl.sigma, wl.m)
+        Oz.kspace(prob.k)
+        plans.append(Oz)
+    yb = np.concatenate([y3[:, z, :].reshape(-1, order="F") for z in range(Nz)])
+    xo3, _, _ = ora.cgls_ref_frames(plans, S3, np.tile(prob.w, Nz), yb, mode=1, ftiter=3, fteps=1e-3, maxit=5,
+                                    cgeps=1e-7, lam=5e-2, maps_per_frame=True)
+    e_s3 = rel(_load(os.path.join(d, "solve3.bin"), (N, N, Nz)), xo3)
+    print(f"C++ facade slices: operators max {max(e3):.2e}, ->* {e_s3:.2e}")
+    assert max(e3) <= OP_TOL and e_s3 <= 1e-4

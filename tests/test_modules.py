@@ -96,3 +96,69 @@ def test_nufft_module_matches_oracle(ora):
     e = rel(img, io)
     print(f"NuFFT module: {e:.2e} (oracle ITrafo steps {it})")
     assert e <= 1e-5
+
+
+@pytest.mark.gpu
+def test_cgsense_module_stack_of_stars(ora, tmp_path):
+    """The CGSENSE module with dft_3rd_dim from .cod files: (N0, N1, Nz, C)
+    maps, (nk, Nz, C) signals, one CGLS over the slices -- against the
+    oracle's joint CGLS (sigma 1, fteps 1e-3, as the module passes them)."""
+    from codeare_amd import data, modules
+    rng = np.random.default_rng(12)
+    N, C, Nz, R = 32, 4, 3, 64
+    k = data.radial_2d(32, R, True, 0)
+    M = k.shape[1]
+    w = data.density(k, "k", R)
+    S = np.asfortranarray(np.stack([data.coil_maps(N, 2, C, rng) * (1 + 0.2 * z) for z in range(Nz)], axis=2)
+                          .astype(np.complex64))
+    x = (rng.normal(size=(N, N, Nz)) + 1j * rng.normal(size=(N, N, Nz))).astype(np.complex64)
+    O = ora.NFFT((N, N), M, 1.0, 3)
+    O.kspace(k)
+    y = np.zeros((M, Nz, C), dtype=np.complex64, order="F")
+    for z in range(Nz):
+        y[:, z, :] = O.E(np.asfortranarray(S[:, :, z, :]), np.asfortranarray(x[..., z]))
+    files = {}
+    for name, arr in [("sensitivities", S), ("kspace", k), ("weights", w), ("signals", y)]:
+        files[name] = str(tmp_path / f"{name}.cod")
+        codfile.write_cod(files[name], arr, name)
+    out = str(tmp_path / "image.cod")
+    rc = modules.main(["CGSENSE", f"nk={M}", "m=3", "lambda=5e-2", "cgeps=1e-7", "cgmaxit=6", "ftmaxit=3",
+                       "dft_3rd_dim=1", *[f"{k_}={v}" for k_, v in files.items()], f"image={out}"])
+    assert rc == 0
+    img, name, _ = codfile.read_cod(out)
+    plans = []
+    for z in range(Nz):
+        Oz = ora.NFFT((N, N), M, 1.0, 3)
+        Oz.kspace(k)
+        plans.append(Oz)
+    yb = np.concatenate([y[:, z, :].reshape(-1, order="F") for z in range(Nz)])
+    xo, it, _ = ora.cgls_ref_frames(plans, S, np.tile(w, Nz), yb, mode=1, ftiter=3, fteps=1e-3, maxit=6,
+                                    cgeps=1e-7, lam=5e-2, maps_per_frame=True)
+    e = rel(img, xo)
+    print(f"CGSENSE module, 3 slices: image vs oracle {e:.2e}, oracle iterations {it}")
+    assert img.shape == (N, N, Nz)
+    assert e <= 1e-4
+
+
+@pytest.mark.gpu
+def test_nufft_module_stack_of_stars(ora):
+    from codeare_amd import modules
+    from helpers import rand_traj, rand_img
+    rng = np.random.default_rng(13)
+    N, Nz, M = 32, 3, 1500
+    k = rand_traj(rng, 2, M)
+    w = rng.uniform(0.2, 1.0, M).astype(np.float32)
+    f = rand_img(rng, (M, Nz))
+    s = modules.NuFFT()
+    s.init(dim=3, Nx=N, Ny=N, Nz=Nz, M=M, shots=1, maxit=50, epsilon=1e-9, m=3, alpha=2.0, dft_3rd_dim=1)
+    ws = {"kspace": k, "weights": w, "data": f}
+    s.prepare(ws)
+    img = s.process(ws)
+    assert img.shape == (N, N, Nz)
+    O = ora.NFFT((N, N), M, 2.0, 3)
+    O.kspace(k)
+    for z in range(Nz):
+        io, it, _ = O.itrafo(f[:, z], w, maxit=3, eps=7e-4)
+        e = rel(img[..., z], io)
+        print(f"NuFFT module slice {z}: {e:.2e}")
+        assert e <= 1e-5
