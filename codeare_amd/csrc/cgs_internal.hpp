@@ -110,6 +110,7 @@ struct ToepArgs {
   // hist and fdone_w
   const double* rpart;
   int nrpart, tolkind, maxit, hcap;
+  int row;                // pad/crop: one block per grid row over all coil groups (crop: one partial image)
   double tol;
   CgState* cgw;
   double* hist;

@@ -52,6 +52,18 @@ __device__ __forceinline__ void bst2(Rsrc r, float2 v, int voff, int soff) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
 }
 
+// Workgroup barrier for the LDS exchanges.  lds_barrier() is a release /
+// acquire fence on every address space, so it also waits for the wave's
+// outstanding global loads and stores (vmcnt 0): a store issued before an
+// exchange, or a load of the next coil group issued ahead, would be waited
+// for at every barrier.  No block here passes data through global memory
+// between its waves, so the fences cover LDS only (lgkmcnt 0, then s_barrier).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -196,9 +208,9 @@ template <int NW>
 __device__ __forceinline__ double block_sum(double v, double* red) {
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
-  __syncthreads();
+  lds_barrier();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
+  lds_barrier();
   double t = 0.0;
 #pragma unroll
   for (int w = 0; w < NW; ++w) t += red[w];
@@ -206,6 +218,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 constexpr int kXS = 16 * 16 + 8;   // conv exchange row (float2): 16 columns x 16 lines + pad
+constexpr int kOutOfRange = 0x7ffffff0;   // a vector offset past every buffer's range
 constexpr int kRow = 144;          // pad/crop exchange row (float2): 8 columns x 16 lines + pad
 
 // ---------------------------------------------------------------------------
@@ -240,7 +253,7 @@ __global__ void __launch_bounds__(256) k_toep_conv(ToepArgs a) {
   float2 v[P];
 #pragma unroll
   for (int k = 0; k < H; ++k) v[k] = bld2(rg, vo, k * so);
-  __syncthreads();   // twiddle table
+  lds_barrier();   // twiddle table
 
   dft_zero_top<P, SG>(v);
 #pragma unroll
@@ -248,10 +261,10 @@ __global__ void __launch_bounds__(256) k_toep_conv(ToepArgs a) {
   float2 X[RPT][16];
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {   // round r: rows k1 = 16 r .. 16 r + 15
-    if (r) __syncthreads();
+    if (r) lds_barrier();
 #pragma unroll
     for (int kk = 0; kk < 16; ++kk) ex[kk * kXS + j * 16 + q] = v[16 * r + kk];
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int jj = 0; jj < 16; ++jj) X[r][jj] = ex[j * kXS + jj * 16 + q];
     dft<16, SG>(X[r]);
@@ -269,10 +282,10 @@ __global__ void __launch_bounds__(256) k_toep_conv(ToepArgs a) {
   for (int r = 0; r < RPT; ++r) dft<16, -SG>(X[r]);   // over k2 -> column j'
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
-    __syncthreads();   // the previous round's (or the forward exchange's) reads are done
+    lds_barrier();   // the previous round's (or the forward exchange's) reads are done
 #pragma unroll
     for (int jj = 0; jj < 16; ++jj) ex[j * kXS + jj * 16 + q] = cmulc(X[r][jj], twT[jj * P + j + 16 * r]);
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int kk = 0; kk < 16; ++kk) v[16 * r + kk] = ex[kk * kXS + j * 16 + q];
   }
@@ -320,18 +333,18 @@ __device__ __forceinline__ void r8_forward(float2 (&u)[NP], float2 (&v)[8], floa
   for (int k1 = 1; k1 < NP; ++k1) u[k1] = cmul(u[k1], twA[t * NP + k1]);
 #pragma unroll
   for (int k1 = 0; k1 < NP; ++k1) ex[(8 * k1 + lo) * kRow + hi * 16 + q] = u[k1];
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int c = 0; c < 8; ++c) v[c] = act ? ex[t * kRow + c * 16 + q] : make_float2(0.f, 0.f);
   dft<8, SG>(v);
 #pragma unroll
   for (int m1 = 1; m1 < 8; ++m1) v[m1] = cmul(v[m1], twB[lo * 8 + m1]);
-  __syncthreads();
+  lds_barrier();
   if (act) {
 #pragma unroll
     for (int m1 = 0; m1 < 8; ++m1) ex[(8 * hi + m1) * kRow + lo * 16 + q] = v[m1];
   }
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int c = 0; c < 8; ++c) v[c] = act ? ex[t * kRow + c * 16 + q] : make_float2(0.f, 0.f);
   // the caller runs the last 8-point DFT over t & 7 (all bins, or m2 < 4)
@@ -345,24 +358,24 @@ __device__ __forceinline__ void r8_inverse(float2 (&v)[8], float2 (&u)[NP], floa
   const int lo = t & 7, hi = t >> 3;
   const bool act = R8<NP>::ACT >= 64 || t < R8<NP>::ACT;
   dft<8, -SG>(v);   // over m2 -> j1
-  __syncthreads();  // the forward exchange's reads are done
+  lds_barrier();  // the forward exchange's reads are done
   if (act) {
 #pragma unroll
     for (int j1 = 0; j1 < 8; ++j1) ex[t * kRow + j1 * 16 + q] = v[j1];
   }
-  __syncthreads();
+  lds_barrier();
   // thread t = 8 k1 + j1 gathers over m1
 #pragma unroll
   for (int m1 = 0; m1 < 8; ++m1) v[m1] = act ? ex[(8 * hi + m1) * kRow + lo * 16 + q] : make_float2(0.f, 0.f);
 #pragma unroll
   for (int m1 = 1; m1 < 8; ++m1) v[m1] = cmulc(v[m1], twB[lo * 8 + m1]);
   dft<8, -SG>(v);   // over m1 -> j2
-  __syncthreads();
+  lds_barrier();
   if (act) {
 #pragma unroll
     for (int j2 = 0; j2 < 8; ++j2) ex[t * kRow + j2 * 16 + q] = v[j2];
   }
-  __syncthreads();
+  lds_barrier();
   // thread t = j1 + 8 j2 gathers over k1
 #pragma unroll
   for (int k1 = 0; k1 < NP; ++k1) u[k1] = ex[(8 * k1 + lo) * kRow + hi * 16 + q];
@@ -393,11 +406,10 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
   const int vi = t * 8;
   const Rsrc rS = rsrc(a.S + f * a.Sfs + (int64_t)a.N0 * i1 * a.C + q0, ((int64_t)a.N0 * a.C - q0) * 8);
   const int vs = (t * a.C + q) * 8, ss = 64 * a.C * 8;
-  float2 u[NP];
+  float2 u[NP], pn[H];
   float pp = 0.f;
   if (a.pold) {   // p_new = r + beta p_old (CGLS.hpp:55-56), written by the coil-0 lanes
     const Rsrc rr = rsrc(a.r + ib, (int64_t)a.N0 * 8), rp = rsrc(a.pold + ib, (int64_t)a.N0 * 8);
-    const Rsrc rn = rsrc(a.pnew + ib, (int64_t)a.N0 * 8);
     // the CG tail's ||r||^2 partials (nrpart <= kRedBlocks = 512, eight per
     // lane of wave 0) are loaded before the CG state says whether they are
     // needed, so both latencies overlap
@@ -449,7 +461,7 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
           fold_s[1] = done ? 1.0 : 0.0;
         }
       }
-      __syncthreads();
+      lds_barrier();
       if (fold_s[1] != 0.0) return;   // block-uniform
       be = (float)fold_s[0];
     } else {
@@ -458,12 +470,9 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
     const bool c0 = (q0 + q) == 0;
 #pragma unroll
     for (int k = 0; k < H; ++k) {
-      const float2 xv = make_float2(rv[k].x + be * pv[k].x, rv[k].y + be * pv[k].y);
-      if (c0) {
-        bst2(rn, xv, vi, k * 512);
-        pp += xv.x * xv.x + xv.y * xv.y;
-      }
-      u[k] = cmul(xv, sv[k]);
+      pn[k] = make_float2(rv[k].x + be * pv[k].x, rv[k].y + be * pv[k].y);
+      if (c0) pp += pn[k].x * pn[k].x + pn[k].y * pn[k].y;
+      u[k] = cmul(pn[k], sv[k]);
     }
   } else {
     const Rsrc rx = rsrc(a.x + ib, (int64_t)a.N0 * 8);
@@ -474,7 +483,14 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
     const double s = block_sum<16>(pp, red);
     if (tid == 0) a.pp_part[o] = s;
   }
-  __syncthreads();   // twiddle tables
+  // p_new is stored after the reduction: at the merge of the two paths above
+  // the compiler waits for every outstanding memory operation
+  if (a.pold && q0 + q == 0) {
+    const Rsrc rn = rsrc(a.pnew + ib, (int64_t)a.N0 * 8);
+#pragma unroll
+    for (int k = 0; k < H; ++k) bst2(rn, pn[k], vi, k * 512);
+  }
+  lds_barrier();   // twiddle tables
 
   dft_zero_top<NP, SG>(u);
   float2 v[8];
@@ -513,22 +529,34 @@ __global__ void __launch_bounds__(1024) k_toep_crop(ToepArgs a) {
   float2 u[NP];
 #pragma unroll
   for (int k = 0; k < NP; ++k) u[k] = bld2(rin, vs, k * ss);
-  __syncthreads();   // twiddle tables
+  // the maps at this thread's output positions and, for <p, q>, p there: all
+  // loads are issued before the transform (a buffer of 0 bytes when there is
+  // no p reads zeros; positions beyond N0 of idle threads read zeros)
+  const bool act = R8<NP>::ACT >= 64 || t < R8<NP>::ACT;
+  const int pb = hi + NP * lo;
+  const Rsrc rS = rsrc(a.S + f * a.Sfs + (int64_t)a.N0 * i1 * a.C + q0, ((int64_t)a.N0 * a.C - q0) * 8);
+  const int sb = 8 * NP * a.C * 8, sp = 8 * NP * 8;
+  // idle threads get a vector offset beyond every buffer's range (the range
+  // check covers the vector offset, not the scalar one), so their loads read
+  // zeros without touching memory
+  const int vb = act ? (pb * a.C + q) * 8 : kOutOfRange, vp = act ? pb * 8 : kOutOfRange;
+  const int64_t prow = (int64_t)f * a.Ntot + (int64_t)a.N0 * i1;
+  const Rsrc rd = rsrc(a.pdot ? a.pdot + prow : a.in, a.pdot ? (int64_t)a.N0 * 8 : 0);
+  float2 sv[4], pd[4];
+#pragma unroll
+  for (int m2 = 0; m2 < 4; ++m2) {
+    sv[m2] = bld2(rS, vb, m2 * sb);
+    pd[m2] = bld2(rd, vp, m2 * sp);
+  }
+  lds_barrier();   // twiddle tables
 
   dft<NP, SG>(u);
   float2 v[8];
   r8_forward<NP, SG>(u, v, ex, twA, twB, t, q);
   dft_low_half<8, SG>(v);   // positions hi + NP lo + 8 NP m2 < N0 (m2 < 4)
-  const bool act = R8<NP>::ACT >= 64 || t < R8<NP>::ACT;
-  const int pb = hi + NP * lo;
-  const Rsrc rS = rsrc(a.S + f * a.Sfs + (int64_t)a.N0 * i1 * a.C + q0, ((int64_t)a.N0 * a.C - q0) * 8);
-  const int vb = (pb * a.C + q) * 8, sb = 8 * NP * a.C * 8;
   float2 c[4];
 #pragma unroll
-  for (int m2 = 0; m2 < 4; ++m2) {
-    c[m2] = make_float2(0.f, 0.f);
-    if (act) c[m2] = cmulc(v[m2], bld2(rS, vb, m2 * sb));
-  }
+  for (int m2 = 0; m2 < 4; ++m2) c[m2] = act ? cmulc(v[m2], sv[m2]) : make_float2(0.f, 0.f);
 #pragma unroll
   for (int m2 = 0; m2 < 4; ++m2)
 #pragma unroll
@@ -540,21 +568,228 @@ __global__ void __launch_bounds__(1024) k_toep_crop(ToepArgs a) {
   double pq = 0.0;
   if (q == 0 && act) {
     const Rsrc ro = rsrc(a.out + rowb, (int64_t)a.N0 * 8);
-    const int vp = pb * 8, sp = 8 * NP * 8;
 #pragma unroll
-    for (int m2 = 0; m2 < 4; ++m2) bst2(ro, c[m2], vp, m2 * sp);
-    if (a.pdot) {
-      const Rsrc rd = rsrc(a.pdot + (int64_t)f * a.Ntot + (int64_t)a.N0 * i1, (int64_t)a.N0 * 8);
-#pragma unroll
-      for (int m2 = 0; m2 < 4; ++m2) {
-        const float2 pv = bld2(rd, vp, m2 * sp);
-        pq += (double)pv.x * c[m2].x + (double)pv.y * c[m2].y;
-      }
+    for (int m2 = 0; m2 < 4; ++m2) {
+      bst2(ro, c[m2], vp, m2 * sp);
+      pq += (double)pd[m2].x * c[m2].x + (double)pd[m2].y * c[m2].y;
     }
   }
   if (a.pq_part) {   // block-uniform
     const double s = block_sum<16>(pq, red);
     if (tid == 0) a.pq_part[((int64_t)f * nb + g) * a.N1 + i1] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row variants of pad and crop: one block per grid row (f, i1) that takes the
+// row's coil groups in turn.  The loads of group g + 1 are issued before group
+// g is transformed and stored, so the memory traffic of one group overlaps
+// the arithmetic of the next; with one group per block the resident blocks run
+// load, compute and store in lock-step rounds and the phases do not overlap.
+// The pad forms p = r + beta p_old once per row; the crop sums conj(S_c) X_c
+// over all C coils, so it writes the whole coil sum (one partial image).
+
+template <int NP, int SG, int G>
+__global__ void __launch_bounds__(1024) k_toep_pad_row(ToepArgs a) {
+  constexpr int n = R8<NP>::n, H = NP / 2;
+  __shared__ float2 ex[R8<NP>::ACT * kRow];
+  __shared__ float2 twA[64 * NP];
+  __shared__ float2 twB[64];
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  const int q = tid & 15, t = tid >> 4, lo = t & 7, hi = t >> 3;
+  const int64_t o = blockIdx.x;   // f * N1 + i1
+  const int f = (int)(o / a.N1);
+  const int64_t i1 = o - (int64_t)f * a.N1;
+  if (a.fdone && a.fdone[f]) return;
+  fill_r8_tables<NP, SG>(a.tw, twA, twB);
+
+  const int64_t ib = (int64_t)f * a.Ntot + (int64_t)a.N0 * i1;
+  const int vi = t * 8;
+  // group g's 16 coils sit 128 bytes after group g - 1's
+  const Rsrc rS = rsrc(a.S + f * a.Sfs + (int64_t)a.N0 * i1 * a.C, (int64_t)a.N0 * a.C * 8);
+  const int vs = (t * a.C + q) * 8, ss = 64 * a.C * 8;
+  float2 sv[H], sn[H], xv[H];
+#pragma unroll
+  for (int k = 0; k < H; ++k) sv[k] = bld2(rS, vs, k * ss);
+  float pp = 0.f;
+  if (a.pold) {   // p_new = r + beta p_old (CGLS.hpp:55-56), written by the q = 0 lanes
+    const Rsrc rr = rsrc(a.r + ib, (int64_t)a.N0 * 8), rp = rsrc(a.pold + ib, (int64_t)a.N0 * 8);
+    double rt[8];
+    if (a.rpart && tid < 64) {
+      const double* rpp = a.rpart + (int64_t)f * a.nrpart;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) rt[u] = tid + 64 * u < a.nrpart ? rpp[tid + 64 * u] : 0.0;
+    }
+    float2 rv[H], pv[H];
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      rv[k] = bld2(rr, vi, k * 512);
+      pv[k] = bld2(rp, vi, k * 512);
+    }
+    float be;
+    const CgState& cs = a.cgst[f];
+    if (a.rpart && cs.bpend) {
+      // beta and the stop test of the finished iteration, as in k_toep_pad
+      __shared__ double fold_s[2];
+      if (tid < 64) {
+        double v = ((rt[0] + rt[1]) + (rt[2] + rt[3])) + ((rt[4] + rt[5]) + (rt[6] + rt[7]));
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+        if (tid == 0) {
+          const double rnew = v;
+          const int it = cs.it_snap + 1;
+          const double rel = (a.tolkind == CGS_TOL_SQUARED_RATIO) ? rnew / cs.xn : sqrt(rnew / cs.bn);
+          const int nan = (cs.nan || isnan(rel) || isnan(cs.alpha)) ? 1 : 0;
+          const int done = (nan || rel <= a.tol || it >= a.maxit) ? 1 : 0;
+          const double beta = rnew / cs.rn_snap;
+          if (i1 == 0) {   // the frame's first block commits
+            CgState* w = a.cgw + f;
+            w->beta = beta;
+            w->rn = rnew;
+            w->it = it;
+            w->rel = rel;
+            w->nan = nan;
+            w->done = done;
+            if (a.hist && it < a.hcap) a.hist[(int64_t)f * a.hcap + it] = rel;
+            a.fdone_w[f] = done;
+          }
+          fold_s[0] = beta;
+          fold_s[1] = done ? 1.0 : 0.0;
+        }
+      }
+      lds_barrier();
+      if (fold_s[1] != 0.0) return;   // block-uniform
+      be = (float)fold_s[0];
+    } else {
+      be = (float)cs.beta;
+    }
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      xv[k] = make_float2(rv[k].x + be * pv[k].x, rv[k].y + be * pv[k].y);
+      if (q == 0) pp += xv[k].x * xv[k].x + xv[k].y * xv[k].y;
+    }
+  } else {
+    const Rsrc rx = rsrc(a.x + ib, (int64_t)a.N0 * 8);
+#pragma unroll
+    for (int k = 0; k < H; ++k) xv[k] = bld2(rx, vi, k * 512);
+  }
+  if (a.pp_part) {   // block-uniform
+    const double s = block_sum<16>(pp, red);
+    if (tid == 0) a.pp_part[o] = s;
+  }
+  // p_new is stored after the reduction: at the merge of the two paths above
+  // the compiler waits for every outstanding memory operation
+  if (a.pold && q == 0) {
+    const Rsrc rn = rsrc(a.pnew + ib, (int64_t)a.N0 * 8);
+#pragma unroll
+    for (int k = 0; k < H; ++k) bst2(rn, xv[k], vi, k * 512);
+  }
+  lds_barrier();   // twiddle tables
+
+  const bool act = R8<NP>::ACT >= 64 || t < R8<NP>::ACT;
+  const Rsrc ro = rsrc(a.out + o * n * a.C, (int64_t)n * a.C * 8);
+  const int vb = ((hi + NP * lo) * a.C + q) * 8, sb = 8 * NP * a.C * 8;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    if (g + 1 < G) {
+#pragma unroll
+      for (int k = 0; k < H; ++k) sn[k] = bld2(rS, vs, k * ss + (g + 1) * 128);
+    }
+    float2 u[NP];
+#pragma unroll
+    for (int k = 0; k < H; ++k) u[k] = cmul(xv[k], sv[k]);
+    if (g > 0) lds_barrier();   // the previous group's exchange reads are done
+    dft_zero_top<NP, SG>(u);
+    float2 v[8];
+    r8_forward<NP, SG>(u, v, ex, twA, twB, t, q);
+    dft<8, SG>(v);
+    if (act) {
+#pragma unroll
+      for (int m2 = 0; m2 < 8; ++m2) bst2(ro, v[m2], vb, m2 * sb + g * 128);
+    }
+#pragma unroll
+    for (int k = 0; k < H; ++k) sv[k] = sn[k];
+  }
+}
+
+template <int NP, int SG, int G>
+__global__ void __launch_bounds__(1024) k_toep_crop_row(ToepArgs a) {
+  constexpr int n = R8<NP>::n;
+  __shared__ float2 ex[R8<NP>::ACT * kRow];
+  __shared__ float2 twA[64 * NP];
+  __shared__ float2 twB[64];
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  const int q = tid & 15, t = tid >> 4, lo = t & 7, hi = t >> 3;
+  const int64_t o = blockIdx.x;   // f * N1 + i1
+  const int f = (int)(o / a.N1);
+  const int64_t i1 = o - (int64_t)f * a.N1;
+  if (a.fdone && a.fdone[f]) return;
+  fill_r8_tables<NP, SG>(a.tw, twA, twB);
+  const Rsrc rin = rsrc(a.in + o * n * a.C, (int64_t)n * a.C * 8);
+  const int vs = (t * a.C + q) * 8, ss = 64 * a.C * 8;
+  float2 u[NP], un[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) u[k] = bld2(rin, vs, k * ss);
+  const bool act = R8<NP>::ACT >= 64 || t < R8<NP>::ACT;
+  const int pb = hi + NP * lo;
+  const Rsrc rS = rsrc(a.S + f * a.Sfs + (int64_t)a.N0 * i1 * a.C, (int64_t)a.N0 * a.C * 8);
+  const int sb = 8 * NP * a.C * 8, sp = 8 * NP * 8;
+  const int vb = act ? (pb * a.C + q) * 8 : kOutOfRange, vp = act ? pb * 8 : kOutOfRange;
+  const int64_t rowb = (int64_t)f * a.Ntot + (int64_t)a.N0 * i1;
+  // p at this thread's output positions for <p, q>, loaded ahead (a buffer
+  // of 0 bytes when there is no p reads zeros)
+  const Rsrc rd = rsrc(a.pdot ? a.pdot + rowb : a.in, a.pdot ? (int64_t)a.N0 * 8 : 0);
+  float2 pd[4];
+#pragma unroll
+  for (int m2 = 0; m2 < 4; ++m2) pd[m2] = bld2(rd, vp, m2 * sp);
+  float2 c[4];
+#pragma unroll
+  for (int m2 = 0; m2 < 4; ++m2) c[m2] = make_float2(0.f, 0.f);
+  lds_barrier();   // twiddle tables
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    if (g + 1 < G) {
+#pragma unroll
+      for (int k = 0; k < NP; ++k) un[k] = bld2(rin, vs, k * ss + (g + 1) * 128);
+    }
+    float2 sv[4];
+#pragma unroll
+    for (int m2 = 0; m2 < 4; ++m2) sv[m2] = bld2(rS, vb, m2 * sb + g * 128);
+    if (g > 0) lds_barrier();   // the previous group's exchange reads are done
+    dft<NP, SG>(u);
+    float2 v[8];
+    r8_forward<NP, SG>(u, v, ex, twA, twB, t, q);
+    dft_low_half<8, SG>(v);   // positions hi + NP lo + 8 NP m2 < N0 (m2 < 4)
+#pragma unroll
+    for (int m2 = 0; m2 < 4; ++m2) {
+      const float2 w = act ? cmulc(v[m2], sv[m2]) : make_float2(0.f, 0.f);
+      c[m2].x += w.x;
+      c[m2].y += w.y;
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) u[k] = un[k];
+  }
+#pragma unroll
+  for (int m2 = 0; m2 < 4; ++m2)
+#pragma unroll
+    for (int m = 8; m > 0; m >>= 1) {
+      c[m2].x += __shfl_xor(c[m2].x, m);
+      c[m2].y += __shfl_xor(c[m2].y, m);
+    }
+  double pq = 0.0;
+  if (q == 0 && act) {
+    const Rsrc ro = rsrc(a.out + rowb, (int64_t)a.N0 * 8);
+#pragma unroll
+    for (int m2 = 0; m2 < 4; ++m2) {
+      bst2(ro, c[m2], vp, m2 * sp);
+      pq += (double)pd[m2].x * c[m2].x + (double)pd[m2].y * c[m2].y;
+    }
+  }
+  if (a.pq_part) {   // block-uniform
+    const double s = block_sum<16>(pq, red);
+    if (tid == 0) a.pq_part[o] = s;
   }
 }
 
@@ -582,7 +817,7 @@ __global__ void __launch_bounds__(1024) k_toep_conv_r8(ToepArgs a) {
   float2 u[NP];
 #pragma unroll
   for (int k = 0; k < H; ++k) u[k] = bld2(rg, vo, k * so);
-  __syncthreads();   // twiddle tables
+  lds_barrier();   // twiddle tables
 
   dft_zero_top<NP, SG>(u);
   float2 v[8];
@@ -627,6 +862,20 @@ static size_t occ_pad_lds(const char* env, size_t static_lds) {
 }
 
 void launch_toep_pad(const ToepArgs& a, hipStream_t s) {
+  if (a.row) {
+    const dim3 grid((unsigned)((int64_t)a.F * a.N1));
+    const int G = a.C / 16;
+    if (a.N0 == 256) {
+      if (G == 2) CGS_LAUNCH((k_toep_pad_row<8, -1, 2>), grid, dim3(1024), 0, s, a);
+      else if (G == 3) CGS_LAUNCH((k_toep_pad_row<8, -1, 3>), grid, dim3(1024), 0, s, a);
+      else CGS_LAUNCH((k_toep_pad_row<8, -1, 4>), grid, dim3(1024), 0, s, a);
+    } else {
+      if (G == 2) CGS_LAUNCH((k_toep_pad_row<6, -1, 2>), grid, dim3(1024), 0, s, a);
+      else if (G == 3) CGS_LAUNCH((k_toep_pad_row<6, -1, 3>), grid, dim3(1024), 0, s, a);
+      else CGS_LAUNCH((k_toep_pad_row<6, -1, 4>), grid, dim3(1024), 0, s, a);
+    }
+    return;
+  }
   const dim3 grid((unsigned)((int64_t)a.F * a.N1 * (a.C / 16)));
   static const size_t extra = occ_pad_lds("CGS_TOEP_OCC_PC", 78464);
   if (a.N0 == 256) CGS_LAUNCH((k_toep_pad<8, -1>), grid, dim3(1024), extra, s, a);
@@ -642,6 +891,20 @@ void launch_toep_conv(const ToepArgs& a, hipStream_t s) {
 }
 
 void launch_toep_crop(const ToepArgs& a, hipStream_t s) {
+  if (a.row) {
+    const dim3 grid((unsigned)((int64_t)a.F * a.N1));
+    const int G = a.C / 16;
+    if (a.N0 == 256) {
+      if (G == 2) CGS_LAUNCH((k_toep_crop_row<8, 1, 2>), grid, dim3(1024), 0, s, a);
+      else if (G == 3) CGS_LAUNCH((k_toep_crop_row<8, 1, 3>), grid, dim3(1024), 0, s, a);
+      else CGS_LAUNCH((k_toep_crop_row<8, 1, 4>), grid, dim3(1024), 0, s, a);
+    } else {
+      if (G == 2) CGS_LAUNCH((k_toep_crop_row<6, 1, 2>), grid, dim3(1024), 0, s, a);
+      else if (G == 3) CGS_LAUNCH((k_toep_crop_row<6, 1, 3>), grid, dim3(1024), 0, s, a);
+      else CGS_LAUNCH((k_toep_crop_row<6, 1, 4>), grid, dim3(1024), 0, s, a);
+    }
+    return;
+  }
   const dim3 grid((unsigned)((int64_t)a.F * a.N1 * (a.C / 16)));
   static const size_t extra = occ_pad_lds("CGS_TOEP_OCC_PC", 78464);
   if (a.N0 == 256) CGS_LAUNCH((k_toep_crop<8, 1>), grid, dim3(1024), extra, s, a);
