@@ -320,6 +320,8 @@ def dynamic_leg(args):
     t0 = time.perf_counter()
     plan = cgsense.Plan((wl.N, wl.N), M, wl.sigma, wl.m, ncoils=wl.coils, frame_offsets=prob.frame_offsets,
                         device_io=True, autotune=True)
+    if torch.cuda.current_stream().cuda_stream == 0:
+        torch.cuda.set_stream(torch.cuda.Stream())   # graphs are not captured on the null stream
     plan.set_stream(torch.cuda.current_stream().cuda_stream)
     plan.kspace(prob.k)
     plan.weights(prob.w)
@@ -384,6 +386,8 @@ def volume_leg(args):
     M = prob.k.shape[1]
     t0 = time.perf_counter()
     plan = cgsense.Plan((wl.N,) * 3, M, wl.sigma, wl.m, ncoils=wl.coils, device_io=True, profile=True)
+    if torch.cuda.current_stream().cuda_stream == 0:
+        torch.cuda.set_stream(torch.cuda.Stream())   # graphs are not captured on the null stream
     plan.set_stream(torch.cuda.current_stream().cuda_stream)
     plan.kspace(prob.k)
     plan.weights(prob.w)

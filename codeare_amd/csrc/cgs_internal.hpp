@@ -32,6 +32,19 @@ struct GridItem {       // one gridding work item: part of one tile's visit list
 struct ReduceItem {     // sums the partial slots of one heavy tile
   int frame, tile, slot0, nslots;
 };
+// Workgroup barrier for data exchanged through LDS.  __syncthreads() is a
+// release / acquire fence on every address space, so it also waits for the
+// wave's outstanding global loads and stores (vmcnt 0): a store issued before
+// an exchange, or a load issued ahead for later use, would be waited for at
+// every barrier.  Kernels whose waves pass nothing to each other through
+// global memory use this one: fences on the local address space only
+// (lgkmcnt 0), then s_barrier.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 struct CgState {        // per-frame CG scalars, FP64
   double rn, bn, xn, pq, alpha, beta, rel;
   int it, done, nan, pad;

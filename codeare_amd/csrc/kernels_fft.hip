@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft(FftArgs a) {
     for (int u = 0; u < kUnroll; ++u)
       if (dst[u] >= 0) lds[dst[u]] = v[u];
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- Stockham stages: wave w transforms lines w, w+4, ... ----
   {
@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft(FftArgs a) {
       }
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- store (with crop for adjoint passes) ----
   const bool crop = a.crop_out != 0;

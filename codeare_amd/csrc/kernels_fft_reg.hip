@@ -125,9 +125,9 @@ constexpr int kRegLines = 16;
 __device__ __forceinline__ double block_sum_d(double v, double* red) {
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
-  __syncthreads();
+  lds_barrier();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
+  lds_barrier();
   return red[0] + red[1] + red[2] + red[3];
 }
 
@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
     const double t = block_sum_d(ppacc, red);
     if (tid == 0) a.pp_part[f * a.outer_per_frame + rest] = t;
   }
-  __syncthreads();   // twiddle table
+  lds_barrier();   // twiddle table
 
   // ---- N1-point DFT over k, twiddle W_n^(s j k1), exchange ----
   reg_fft<N1>(v, s);
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
   if (XR == N1) {
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) ex[xidx<N1>(k1, j, q)] = v[k1];
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
 #pragma unroll
@@ -239,10 +239,10 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
   } else {
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {   // round r holds rows 16 r .. 16 r + 15
-      if (r) __syncthreads();
+      if (r) lds_barrier();
 #pragma unroll
       for (int kk = 0; kk < 16; ++kk) ex[xidx<N1>(kk, j, q)] = v[16 * r + kk];
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) X[r][jj] = ex[xidx<N1>(j, jj, q)];
       reg_fft<16>(X[r], s);
@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
 #pragma unroll
     for (int r = 0; r < RPT; ++r) reg_fft<16>(X[r], si);   // over k2 -> index j'
     if (XR == N1) {
-      __syncthreads();   // every thread has read its rows of ex
+      lds_barrier();   // every thread has read its rows of ex
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
         const int k1 = j + 16 * r;
@@ -274,20 +274,20 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
           ex[xidx<N1>(k1, jj, q)] = cmul2(X[r][jj], make_float2(w.x, si < 0 ? w.y : -w.y));
         }
       }
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) v[k1] = ex[xidx<N1>(k1, j, q)];
     } else {
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
         const int k1 = j + 16 * r;
-        __syncthreads();   // the previous round's reads are done
+        lds_barrier();   // the previous round's reads are done
 #pragma unroll
         for (int jj = 0; jj < 16; ++jj) {
           const float2 w = tw[jj * k1];
           ex[xidx<N1>(j, jj, q)] = cmul2(X[r][jj], make_float2(w.x, si < 0 ? w.y : -w.y));
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int kk = 0; kk < 16; ++kk) v[16 * r + kk] = ex[xidx<N1>(kk, j, q)];
       }
@@ -394,9 +394,9 @@ __device__ __forceinline__ float2 twid(const float2* tw, int idx, float s) {
 __device__ __forceinline__ double block_sum_1024(double v, double* red) {
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
-  __syncthreads();
+  lds_barrier();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
+  lds_barrier();
   double t = 0.0;
 #pragma unroll
   for (int w = 0; w < 16; ++w) t += red[w];
@@ -504,7 +504,7 @@ __global__ void __launch_bounds__(1024) k_fft_r8(FftArgs a) {
     const double r = block_sum_1024(ppacc, red);
     if (tid == 0) a.pp_part[f * a.outer_per_frame + rest] = r;
   }
-  __syncthreads();   // twiddle table
+  lds_barrier();   // twiddle table
 
   // stage 1: DFT over k (t = lo + 8 hi), twiddle W_n^(t k1)
   if (N1 > 1) dftN<N1>(u, s);
@@ -513,7 +513,7 @@ __global__ void __launch_bounds__(1024) k_fft_r8(FftArgs a) {
   // exchange 1: element (k1, lo, hi) -> row 8 k1 + lo, column hi
 #pragma unroll
   for (int k1 = 0; k1 < N1; ++k1) ex[(8 * k1 + lo) * kR8Row + hi * 16 + q] = u[k1];
-  __syncthreads();
+  lds_barrier();
   const bool act = t < ACT;   // thread t = 8 k1 + j1 takes row t: values over j2
   float2 v[8];
 #pragma unroll
@@ -522,13 +522,13 @@ __global__ void __launch_bounds__(1024) k_fft_r8(FftArgs a) {
   dft8(v, s);
 #pragma unroll
   for (int m1 = 1; m1 < 8; ++m1) v[m1] = cmul2(v[m1], twid(tw, N1 * lo * m1, s));
-  __syncthreads();
+  lds_barrier();
   // exchange 2: element (k1 = hi, m1, j1 = lo) -> row 8 k1 + m1, column j1
   if (act) {
 #pragma unroll
     for (int m1 = 0; m1 < 8; ++m1) ex[(8 * hi + m1) * kR8Row + lo * 16 + q] = v[m1];
   }
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int c = 0; c < 8; ++c) v[c] = act ? ex[t * kR8Row + c * 16 + q] : make_float2(0.f, 0.f);
   // stage 3: DFT over j1 -> m2; thread t = 8 k1 + m1 holds bins k1 + N1 m1 + 8 N1 m2
@@ -545,24 +545,24 @@ __global__ void __launch_bounds__(1024) k_fft_r8(FftArgs a) {
     const float si = -s;
     // the inverse runs the stages backwards with conjugate twiddles
     dft8(v, si);                                            // over m2 -> j1
-    __syncthreads();
+    lds_barrier();
     if (act) {
 #pragma unroll
       for (int j1 = 0; j1 < 8; ++j1) ex[t * kR8Row + j1 * 16 + q] = v[j1];
     }
-    __syncthreads();
+    lds_barrier();
     // thread t = 8 k1 + j1 gathers over m1
 #pragma unroll
     for (int m1 = 0; m1 < 8; ++m1) v[m1] = act ? ex[(8 * hi + m1) * kR8Row + lo * 16 + q] : make_float2(0.f, 0.f);
 #pragma unroll
     for (int m1 = 1; m1 < 8; ++m1) v[m1] = cmul2(v[m1], twid(tw, N1 * lo * m1, si));
     dft8(v, si);                                            // over m1 -> j2
-    __syncthreads();
+    lds_barrier();
     if (act) {
 #pragma unroll
       for (int j2 = 0; j2 < 8; ++j2) ex[t * kR8Row + j2 * 16 + q] = v[j2];
     }
-    __syncthreads();
+    lds_barrier();
     // thread t = j1 + 8 j2 (lo = j1, hi = j2) gathers over k1
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) u[k1] = ex[(8 * k1 + lo) * kR8Row + hi * 16 + q];

@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs a) {
   // zero the tile
   for (int idx = threadIdx.x; idx < nrows_tile * rowstride; idx += blockDim.x)
     tile[idx] = make_float2(0.f, 0.f);
-  __syncthreads();
+  lds_barrier();
 
   for (int v = it.vbeg; v < it.vend; ++v) {
     const int s = a.visits[v];
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs a) {
       }
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // write the owned tile (all coils of the group), or a partial slot
   const int ncell = te0 * te1 * te2;
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* item
     wrel[2 * k] = r0;
     wrel[2 * k + 1] = r1;
   }
-  __syncthreads();
+  lds_barrier();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cl = lane % CG, rs = lane / CG;
   const int c = cbase + cl;
@@ -346,7 +346,7 @@ __global__ void __launch_bounds__(256) k_grid_batched(GridArgs a) {
 
   for (int vb = it.vbeg; vb < it.vend; vb += B) {
     const int nb = min(B, it.vend - vb);
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x < nb) {
       const int s = a.visits[vb + threadIdx.x];
       const int* bs = a.base + (int64_t)s * D;
@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(256) k_grid_batched(GridArgs a) {
 #pragma unroll
       for (int q = 0; q < NPSI; ++q) bpsi[threadIdx.x * NPSI + q] = ps[q];
     }
-    __syncthreads();
+    lds_barrier();
     for (int idx = threadIdx.x; idx < nb * CG; idx += blockDim.x) {
       const int v = idx / CG, cc = idx % CG;
       const int s = bmeta[v * 8];
@@ -375,7 +375,7 @@ __global__ void __launch_bounds__(256) k_grid_batched(GridArgs a) {
       }
       ybat[idx] = yv;
     }
-    __syncthreads();
+    lds_barrier();
     for (int v = 0; v < nb; ++v) {
       const int* mt = bmeta + v * 8;
       const int rel0 = mt[1], rel1 = mt[2], rel2 = mt[3];
@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(256) k_grid_batched(GridArgs a) {
       }
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   const int ncell = te0 * te1 * te2;
   for (int idx = threadIdx.x; idx < ncell * CG; idx += blockDim.x) {
@@ -468,9 +468,9 @@ __global__ void __launch_bounds__(256) k_grid_atomic(GridArgs a, const int4* __r
 
   for (int vb = it.vbeg; vb < it.vend; vb += B) {
     const int nb = min(B, it.vend - vb);
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x < nb) brec[threadIdx.x] = vrec[vb + threadIdx.x];
-    __syncthreads();
+    lds_barrier();
     {
       // window values: NPSI * nb <= 64 * 24 floats, 6 per thread at most
       float pv[(B * NPSI + 255) / 256];
@@ -508,7 +508,7 @@ __global__ void __launch_bounds__(256) k_grid_atomic(GridArgs a, const int4* __r
         if (idx < nb * CG) ybat[idx] = yv[u];
       }
     }
-    __syncthreads();
+    lds_barrier();
     for (int v = wave; v < nb; v += 4) {
       const int4 r = brec[v];
       const int rel0 = (int)(short)(r.y & 0xffff), rel1 = (int)(short)(r.y >> 16);
@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(256) k_grid_atomic(GridArgs a, const int4* __r
       }
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   const int ncell = te0 * te1 * te2;
   for (int idx = threadIdx.x; idx < ncell * CG; idx += blockDim.x) {
@@ -683,9 +683,9 @@ __global__ void __launch_bounds__(512) k_grid_reg3(GridArgs a, const int4* __res
 
   for (int vb = it.vbeg; vb < it.vend; vb += B) {
     const int nb = min(B, it.vend - vb);
-    __syncthreads();   // the previous batch is consumed
+    lds_barrier();   // the previous batch is consumed
     if (tid < nb) srec[tid] = vrec[vb + tid];
-    __syncthreads();
+    lds_barrier();
     for (int idx = tid; idx < nb * NPSI; idx += 512) {
       const int v = idx / NPSI, q = idx - v * NPSI;
       spsi[idx] = a.psi[(int64_t)srec[v].x * NPSI + q];
@@ -700,7 +700,7 @@ __global__ void __launch_bounds__(512) k_grid_reg3(GridArgs a, const int4* __res
       }
       sy[idx] = yv;
     }
-    __syncthreads();
+    lds_barrier();
     for (int v = 0; v < nb; ++v) {
       const int4 r = srec[v];
       const int dz = w - (int)(short)(r.w & 0xffff);
@@ -764,9 +764,9 @@ __global__ void __launch_bounds__(512) k_grid_mfma3(GridArgs a, const int4* __re
 
   for (int vb = it.vbeg; vb < it.vend; vb += B) {
     const int nb = min(B, it.vend - vb);
-    __syncthreads();   // the previous batch is consumed
+    lds_barrier();   // the previous batch is consumed
     if (tid < nb) srec[tid] = vrec[vb + tid];
-    __syncthreads();
+    lds_barrier();
     {
       // every load of the batch in flight before the first LDS write
       constexpr int UP = (B * NPSI + 511) / 512, UY = (B * CC + 511) / 512;
@@ -814,7 +814,7 @@ __global__ void __launch_bounds__(512) k_grid_mfma3(GridArgs a, const int4* __re
     const unsigned long long mask = __ballot(hit);
     const int cnt = __popcll(mask);
     if (hit) slist[w][__popcll(mask & ((1ull << lane) - 1ull))] = (unsigned char)lane;
-    __syncthreads();   // staged data and this wave's list are visible
+    lds_barrier();   // staged data and this wave's list are visible
     for (int p = 0; p < cnt; p += 2) {
       float w0 = 0.f, w1 = 0.f, bv = 0.f;
       if (p + kk < cnt) {
@@ -1222,7 +1222,7 @@ __global__ void __launch_bounds__(256) k_interp_lds3(SampArgs a, const int4* ite
     wrel[3 * k + 1] = r1;
     wrel[3 * k + 2] = r2;
   }
-  __syncthreads();
+  lds_barrier();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cl = lane % CG, rs = lane / CG;
   const int c = cbase + cl;

@@ -52,18 +52,6 @@ __device__ __forceinline__ void bst2(Rsrc r, float2 v, int voff, int soff) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
 }
 
-// Workgroup barrier for the LDS exchanges.  lds_barrier() is a release /
-// acquire fence on every address space, so it also waits for the wave's
-// outstanding global loads and stores (vmcnt 0): a store issued before an
-// exchange, or a load of the next coil group issued ahead, would be waited
-// for at every barrier.  No block here passes data through global memory
-// between its waves, so the fences cover LDS only (lgkmcnt 0, then s_barrier).
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -386,7 +374,7 @@ __device__ __forceinline__ void r8_inverse(float2 (&v)[8], float2 (&u)[NP], floa
 // pad: p = r + beta p_old (or x), times S_c, positions i0 = t + 64 k < N0
 // (k < NP / 2), FFT along axis 0 of column (f, i1)
 template <int NP, int SG>
-__global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
+__global__ void __launch_bounds__(1024, 8) k_toep_pad(ToepArgs a) {
   constexpr int n = R8<NP>::n, H = NP / 2;
   __shared__ float2 ex[R8<NP>::ACT * kRow];
   __shared__ float2 twA[64 * NP];
@@ -505,10 +493,14 @@ __global__ void __launch_bounds__(1024) k_toep_pad(ToepArgs a) {
   }
 }
 
+// (The group pad and crop keep two 1024-thread blocks per CU: at most 64
+// VGPRs, which __launch_bounds__(1024, 8) -- eight waves per SIMD -- holds the
+// compiler to.)
+
 // crop: IFFT along axis 0 of column (f, i1), positions P = hi + NP lo +
 // 8 NP m2 < N0 (m2 < 4), conj(S_c) X_c summed over the 16 coils
 template <int NP, int SG>
-__global__ void __launch_bounds__(1024) k_toep_crop(ToepArgs a) {
+__global__ void __launch_bounds__(1024, 8) k_toep_crop(ToepArgs a) {
   constexpr int n = R8<NP>::n;
   __shared__ float2 ex[R8<NP>::ACT * kRow];
   __shared__ float2 twA[64 * NP];
