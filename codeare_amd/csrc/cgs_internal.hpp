@@ -124,6 +124,7 @@ struct ToepArgs {
   const double* rpart;
   int nrpart, tolkind, maxit, hcap;
   int row;                // pad/crop: one block per grid row over all coil groups (crop: one partial image)
+  int ppf;                // half passes: planes (f, i2) per frame
   double tol;
   CgState* cgw;
   double* hist;
@@ -305,6 +306,8 @@ bool fft_reg_supported(const FftArgs& a, int ld_mode, int st_mode);
 void launch_fft_reg(const FftArgs& a, int ld_mode, int st_mode, hipStream_t s);
 bool toep_pad_crop_supported(int N0, int N1, int C);
 bool toep_conv_supported(int N0, int N1, int C);
+bool toep_half_supported(int N0, int N1, int C);
+void launch_toep_half(const ToepArgs& a, int dir, hipStream_t s);
 void launch_toep_pad(const ToepArgs& a, hipStream_t s);
 void launch_toep_conv(const ToepArgs& a, hipStream_t s);
 void launch_toep_crop(const ToepArgs& a, hipStream_t s);

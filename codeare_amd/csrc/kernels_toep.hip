@@ -282,6 +282,80 @@ __global__ void __launch_bounds__(256) k_toep_conv(ToepArgs a) {
   for (int k = 0; k < H; ++k) bst2(rg, v[k], vo, k * so);
 }
 
+// One half of k_toep_conv as a pass of its own: the middle axis (axis 1) of
+// the 3D operator, lines over the inner index c + C l0 (16 per block) of each
+// plane (f, i2), rows spaced inner = C 2 N0 apart.  DIR = +1: the N rows of
+// `in` (the image half of the zero-padded line) -> all n = 2 N bins in `out`
+// (forward, as the conv's first half); DIR = -1: the n bins of `in` -> the
+// first N positions of the inverse transform in `out` (the conv's second half).
+template <int P, int SG, int DIR>
+__global__ void __launch_bounds__(256) k_toep_half(ToepArgs a) {
+  constexpr int n = 16 * P, H = P / 2, RPT = P / 16;
+  __shared__ float2 ex[16 * kXS];
+  __shared__ float2 twT[16 * P];   // [j][k1] = exp(SG 2 pi i j k1 / n)
+  const int tid = threadIdx.x;
+  const int q = tid & 15, j = tid >> 4;
+  const int64_t inner = (int64_t)a.C * 2 * a.N0;   // lines per plane
+  const int64_t nb = inner / 16;
+  const int64_t pl = blockIdx.x / nb;               // plane f * ppf + i2
+  const int64_t q0 = ((int64_t)blockIdx.x - pl * nb) * 16;
+  if (a.fdone && a.fdone[pl / a.ppf]) return;
+  constexpr int64_t rin = DIR > 0 ? n / 2 : n, rout = DIR > 0 ? n : n / 2;
+  const Rsrc ri = rsrc(a.in + pl * rin * inner + q0, (rin * inner - q0) * 8);
+  const Rsrc ro = rsrc(a.out + pl * rout * inner + q0, (rout * inner - q0) * 8);
+  for (int e = tid; e < 16 * P; e += 256) {
+    const float2 w = a.tw[(e / P) * (e % P)];
+    twT[e] = SG < 0 ? w : make_float2(w.x, -w.y);
+  }
+  const int vo = (int)(((int64_t)j * inner + q) * 8);
+  const int so = (int)(16 * inner * 8);
+  float2 v[P];
+  float2 X[RPT][16];
+  if (DIR > 0) {
+#pragma unroll
+    for (int k = 0; k < H; ++k) v[k] = bld2(ri, vo, k * so);
+    lds_barrier();   // twiddle table
+    dft_zero_top<P, SG>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < P; ++k1) v[k1] = cmul(v[k1], twT[j * P + k1]);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {   // round r: rows k1 = 16 r .. 16 r + 15
+      if (r) lds_barrier();
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) ex[kk * kXS + j * 16 + q] = v[16 * r + kk];
+      lds_barrier();
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) X[r][jj] = ex[j * kXS + jj * 16 + q];
+      dft<16, SG>(X[r]);
+    }
+    // thread j holds bins (j + 16 r) + P k2
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+#pragma unroll
+      for (int k2 = 0; k2 < 16; ++k2) bst2(ro, X[r][k2], vo, (int)((16 * r + P * k2) * inner * 8));
+  } else {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+#pragma unroll
+      for (int k2 = 0; k2 < 16; ++k2) X[r][k2] = bld2(ri, vo, (int)((16 * r + P * k2) * inner * 8));
+    lds_barrier();   // twiddle table
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) dft<16, -SG>(X[r]);   // over k2 -> column j'
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      if (r) lds_barrier();   // the previous round's reads are done
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) ex[j * kXS + jj * 16 + q] = cmulc(X[r][jj], twT[jj * P + j + 16 * r]);
+      lds_barrier();
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) v[16 * r + kk] = ex[kk * kXS + j * 16 + q];
+    }
+    dft_low_half<P, -SG>(v);   // over k1 -> k < H: position j + 16 k
+#pragma unroll
+    for (int k = 0; k < H; ++k) bst2(ro, v[k], vo, k * so);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 64 threads per line, n = 64 NP (NP = 6: n = 384, NP = 8: n = 512).  A
 // 1024-thread block takes 16 lines (16 coils q0 .. q0 + 15 of one grid row or
@@ -832,11 +906,17 @@ __global__ void __launch_bounds__(1024) k_toep_conv_r8(ToepArgs a) {
 
 }  // namespace
 
-// 2D only, C % 16 == 0 and every per-frame byte offset below 2^31: pad/crop
-// at 2 N0 = 384 or 512 (64 threads per line), the conv at 2 N1 = 256 or 512
+// C % 16 == 0 and every per-frame byte offset below 2^31: pad/crop at 2 N0 =
+// 256, 384 or 512 (64 threads per line), the conv at 2 N1 = 256 or 512
 // (16 threads per line) or 384 (64 threads per line)
 bool toep_pad_crop_supported(int N0, int N1, int C) {
-  return (N0 == 192 || N0 == 256) && C % 16 == 0 && N1 > 0 && (int64_t)2 * N0 * C * 8 < (int64_t)1 << 31;
+  return (N0 == 128 || N0 == 192 || N0 == 256) && C % 16 == 0 && N1 > 0 &&
+         (int64_t)2 * N0 * C * 8 < (int64_t)1 << 31;
+}
+// the 3D middle-axis passes: n = 2 N1 = 256 or 512, every per-plane offset
+// below 2^31
+bool toep_half_supported(int N0, int N1, int C) {
+  return (N1 == 128 || N1 == 256) && C % 16 == 0 && (int64_t)2 * N1 * C * 2 * N0 * 8 < (int64_t)1 << 31;
 }
 bool toep_conv_supported(int N0, int N1, int C) {
   return (N1 == 128 || N1 == 192 || N1 == 256) && C % 16 == 0 && N0 > 0 &&
@@ -853,25 +933,26 @@ static size_t occ_pad_lds(const char* env, size_t static_lds) {
   return per > static_lds + 256 ? per - static_lds - 256 : 0;
 }
 
-void launch_toep_pad(const ToepArgs& a, hipStream_t s) {
+// NP = N0 / 32 points per thread (n = 2 N0 = 64 NP): 4, 6 or 8
+template <int NP>
+static void launch_pad_np(const ToepArgs& a, hipStream_t s) {
   if (a.row) {
     const dim3 grid((unsigned)((int64_t)a.F * a.N1));
     const int G = a.C / 16;
-    if (a.N0 == 256) {
-      if (G == 2) CGS_LAUNCH((k_toep_pad_row<8, -1, 2>), grid, dim3(1024), 0, s, a);
-      else if (G == 3) CGS_LAUNCH((k_toep_pad_row<8, -1, 3>), grid, dim3(1024), 0, s, a);
-      else CGS_LAUNCH((k_toep_pad_row<8, -1, 4>), grid, dim3(1024), 0, s, a);
-    } else {
-      if (G == 2) CGS_LAUNCH((k_toep_pad_row<6, -1, 2>), grid, dim3(1024), 0, s, a);
-      else if (G == 3) CGS_LAUNCH((k_toep_pad_row<6, -1, 3>), grid, dim3(1024), 0, s, a);
-      else CGS_LAUNCH((k_toep_pad_row<6, -1, 4>), grid, dim3(1024), 0, s, a);
-    }
+    if (G == 2) CGS_LAUNCH((k_toep_pad_row<NP, -1, 2>), grid, dim3(1024), 0, s, a);
+    else if (G == 3) CGS_LAUNCH((k_toep_pad_row<NP, -1, 3>), grid, dim3(1024), 0, s, a);
+    else CGS_LAUNCH((k_toep_pad_row<NP, -1, 4>), grid, dim3(1024), 0, s, a);
     return;
   }
   const dim3 grid((unsigned)((int64_t)a.F * a.N1 * (a.C / 16)));
-  static const size_t extra = occ_pad_lds("CGS_TOEP_OCC_PC", 78464);
-  if (a.N0 == 256) CGS_LAUNCH((k_toep_pad<8, -1>), grid, dim3(1024), extra, s, a);
-  else CGS_LAUNCH((k_toep_pad<6, -1>), grid, dim3(1024), 0, s, a);
+  static const size_t extra = NP == 8 ? occ_pad_lds("CGS_TOEP_OCC_PC", 78464) : 0;
+  CGS_LAUNCH((k_toep_pad<NP, -1>), grid, dim3(1024), extra, s, a);
+}
+
+void launch_toep_pad(const ToepArgs& a, hipStream_t s) {
+  if (a.N0 == 256) launch_pad_np<8>(a, s);
+  else if (a.N0 == 192) launch_pad_np<6>(a, s);
+  else launch_pad_np<4>(a, s);
 }
 
 void launch_toep_conv(const ToepArgs& a, hipStream_t s) {
@@ -882,25 +963,39 @@ void launch_toep_conv(const ToepArgs& a, hipStream_t s) {
   else CGS_LAUNCH((k_toep_conv_r8<6, -1>), dim3((unsigned)(lines / 16)), dim3(1024), 0, s, a);
 }
 
-void launch_toep_crop(const ToepArgs& a, hipStream_t s) {
+template <int NP>
+static void launch_crop_np(const ToepArgs& a, hipStream_t s) {
   if (a.row) {
     const dim3 grid((unsigned)((int64_t)a.F * a.N1));
     const int G = a.C / 16;
-    if (a.N0 == 256) {
-      if (G == 2) CGS_LAUNCH((k_toep_crop_row<8, 1, 2>), grid, dim3(1024), 0, s, a);
-      else if (G == 3) CGS_LAUNCH((k_toep_crop_row<8, 1, 3>), grid, dim3(1024), 0, s, a);
-      else CGS_LAUNCH((k_toep_crop_row<8, 1, 4>), grid, dim3(1024), 0, s, a);
-    } else {
-      if (G == 2) CGS_LAUNCH((k_toep_crop_row<6, 1, 2>), grid, dim3(1024), 0, s, a);
-      else if (G == 3) CGS_LAUNCH((k_toep_crop_row<6, 1, 3>), grid, dim3(1024), 0, s, a);
-      else CGS_LAUNCH((k_toep_crop_row<6, 1, 4>), grid, dim3(1024), 0, s, a);
-    }
+    if (G == 2) CGS_LAUNCH((k_toep_crop_row<NP, 1, 2>), grid, dim3(1024), 0, s, a);
+    else if (G == 3) CGS_LAUNCH((k_toep_crop_row<NP, 1, 3>), grid, dim3(1024), 0, s, a);
+    else CGS_LAUNCH((k_toep_crop_row<NP, 1, 4>), grid, dim3(1024), 0, s, a);
     return;
   }
   const dim3 grid((unsigned)((int64_t)a.F * a.N1 * (a.C / 16)));
-  static const size_t extra = occ_pad_lds("CGS_TOEP_OCC_PC", 78464);
-  if (a.N0 == 256) CGS_LAUNCH((k_toep_crop<8, 1>), grid, dim3(1024), extra, s, a);
-  else CGS_LAUNCH((k_toep_crop<6, 1>), grid, dim3(1024), 0, s, a);
+  static const size_t extra = NP == 8 ? occ_pad_lds("CGS_TOEP_OCC_PC", 78464) : 0;
+  CGS_LAUNCH((k_toep_crop<NP, 1>), grid, dim3(1024), extra, s, a);
+}
+
+void launch_toep_crop(const ToepArgs& a, hipStream_t s) {
+  if (a.N0 == 256) launch_crop_np<8>(a, s);
+  else if (a.N0 == 192) launch_crop_np<6>(a, s);
+  else launch_crop_np<4>(a, s);
+}
+
+// the middle-axis half passes of the 3D operator: lines of n = 2 N1 = 256 or
+// 512 points over the planes (f, i2)
+void launch_toep_half(const ToepArgs& a, int dir, hipStream_t s) {
+  const int64_t lines = (int64_t)a.F * a.ppf * a.C * 2 * a.N0;
+  const dim3 grid((unsigned)(lines / 16));
+  if (dir > 0) {
+    if (a.N1 == 256) CGS_LAUNCH((k_toep_half<32, -1, 1>), grid, dim3(256), 0, s, a);
+    else CGS_LAUNCH((k_toep_half<16, -1, 1>), grid, dim3(256), 0, s, a);
+  } else {
+    if (a.N1 == 256) CGS_LAUNCH((k_toep_half<32, -1, -1>), grid, dim3(256), 0, s, a);
+    else CGS_LAUNCH((k_toep_half<16, -1, -1>), grid, dim3(256), 0, s, a);
+  }
 }
 
 }  // namespace cgs

@@ -1134,18 +1134,24 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
     Prof pr(p, kind, bytes * F);
     launch_fft(a, ld, st, p.stream);
   };
-  if (d == 2) {
+  // 3D on the lean kernels too when every pass has one: the axis-0 pad and
+  // crop treat the N1 N2 rows of a frame like the 2D kernels' N1 rows, the
+  // middle axis runs the half passes, the last axis the conv kernel
+  const int64_t rows = (d == 2) ? N1 : N1 * N2;
+  const bool lean3 = d == 3 && toep_lean_enabled() && toep_pad_crop_supported((int)N0, (int)rows, C) &&
+                     toep_half_supported((int)N0, (int)N1, C) && toep_conv_supported((int)(M0 * N1), (int)N2, C);
+  if (d == 2 || lean3) {
     // lean kernels (kernels_toep.hip) where the shape allows; pad and crop
     // switch together (both leave image positions unshifted along axis 0),
-    // the conv pass is self-contained along axis 1
-    const bool lean_pc = toep_lean_enabled() && toep_pad_crop_supported((int)N0, (int)N1, C);
-    const bool lean_cv = toep_lean_enabled() && toep_conv_supported((int)N0, (int)N1, C);
+    // the conv pass is self-contained along its axis
+    const bool lean_pc = lean3 || (toep_lean_enabled() && toep_pad_crop_supported((int)N0, (int)N1, C));
+    const bool lean_cv = lean3 || (toep_lean_enabled() && toep_conv_supported((int)N0, (int)N1, C));
     ToepArgs ta{};
-    ta.C = C; ta.N0 = (int)N0; ta.N1 = (int)N1; ta.F = F; ta.Ntot = p.Ntot;
+    ta.C = C; ta.N0 = (int)N0; ta.N1 = (int)rows; ta.F = F; ta.Ntot = p.Ntot;
     ta.S = p.S; ta.Sfs = p.Sfs; ta.fdone = fl;
     ta.row = 0;
     const bool want_tail = pu && tail_out && p.tail_pp;
-    const double pad_bytes = ((double)(1 + C) * N0 * N1 + (double)C * M0 * N1) * 8;
+    const double pad_bytes = ((double)(1 + C) * N0 * rows + (double)C * M0 * rows) * 8;
     if (lean_pc) {
       ToepArgs a = ta;
       a.row = toep_pad_row_enabled(C) ? 1 : 0;
@@ -1171,7 +1177,30 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
       pass(CGS_K_TOEP_PAD, p.ft[0], -1, nullptr, x, p.tt1, (int)N0, (int)M0, C, N1, 4, nullptr, 0, pad_bytes);
     }
     const double conv_bytes = 2.0 * C * M0 * N1 * 8 + 4.0 * M0 * M1;
-    if (lean_cv) {
+    if (lean3) {
+      // axis 1: tt1 (N1 rows per plane) -> tt2 (2 N1 rows), the conv along
+      // axis 2 in tt2, axis 1 back into tt1
+      ToepArgs h{};
+      h.C = C; h.N0 = (int)N0; h.N1 = (int)N1; h.F = F; h.ppf = (int)N2; h.fdone = fl; h.tw = p.ft[1].tw;
+      const double half_bytes = (double)C * M0 * (N1 + M1) * N2 * 8;
+      {
+        h.in = p.tt1; h.out = p.tt2;
+        Prof pr(p, CGS_K_TOEP_PAD, half_bytes * F);
+        launch_toep_half(h, +1, p.stream);
+      }
+      {
+        ToepArgs a{};
+        a.C = C; a.N0 = (int)(M0 * N1); a.N1 = (int)N2; a.F = F; a.Ntot = p.Ntot;
+        a.fdone = fl; a.tw = p.ft[2].tw; a.out = p.tt2; a.That = p.That;
+        Prof pr(p, CGS_K_TOEP_CONV, (2.0 * C * M0 * M1 * N2 * 8 + 4.0 * M0 * M1 * 2 * N2) * F);
+        launch_toep_conv(a, p.stream);
+      }
+      {
+        h.in = p.tt2; h.out = p.tt1;
+        Prof pr(p, CGS_K_TOEP_CROP, half_bytes * F);
+        launch_toep_half(h, -1, p.stream);
+      }
+    } else if (lean_cv) {
       ToepArgs a = ta;
       a.tw = p.ft[1].tw; a.out = p.tt1; a.That = p.That;
       Prof pr(p, CGS_K_TOEP_CONV, conv_bytes * F);
@@ -1187,7 +1216,7 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
       const int G = a.row ? 1 : C / 16;
       *G_out = G;
       if (want_tail) { a.pq_part = p.tail_pq; a.pdot = pu->pnew; ++tail_parts; }
-      const double bytes = ((double)C * M0 * N1 + (double)C * N0 * N1 + (double)G * N0 * N1) * 8;
+      const double bytes = ((double)C * M0 * rows + (double)C * N0 * rows + (double)G * N0 * rows) * 8;
       Prof pr(p, CGS_K_TOEP_CROP, bytes * F);
       launch_toep_crop(a, p.stream);
     } else {
@@ -1198,8 +1227,20 @@ static int toeplitz_to_coils(Plan& p, const float2* x, const int* fl, const PUpd
          ((double)(1 + C) * N0 * N1 * N2 + (double)C * M0 * N1 * N2) * 8);
     pass(CGS_K_TOEP_PAD, p.ft[1], -1, p.tt1, nullptr, p.tt2, (int)N1, (int)M1, C * M0, N2, 0, nullptr, 0,
          (double)C * M0 * (N1 + M1) * N2 * 8);
-    pass(CGS_K_TOEP_CONV, p.ft[2], -1, p.tt2, nullptr, p.tt2, (int)N2, (int)N2, C * M0 * M1, 1, 0, p.That, M0 * M1,
-         2.0 * C * M0 * M1 * N2 * 8 + 4.0 * M0 * M1 * 2 * N2);
+    // the conv along axis 2 on the lean kernel: its lines run over the
+    // (2 N0 x 2 N1) cells x C coils like the 2D kernel's over 2 N0 x C, so it
+    // takes N0' = 2 N0 N1 (2 N0' cells) and N1' = N2
+    const double conv3_bytes = 2.0 * C * M0 * M1 * N2 * 8 + 4.0 * M0 * M1 * 2 * N2;
+    if (toep_lean_enabled() && toep_conv_supported((int)(M0 * N1), (int)N2, C)) {
+      ToepArgs a{};
+      a.C = C; a.N0 = (int)(M0 * N1); a.N1 = (int)N2; a.F = F; a.Ntot = p.Ntot;
+      a.fdone = fl; a.tw = p.ft[2].tw; a.out = p.tt2; a.That = p.That;
+      Prof pr(p, CGS_K_TOEP_CONV, conv3_bytes * F);
+      launch_toep_conv(a, p.stream);
+    } else {
+      pass(CGS_K_TOEP_CONV, p.ft[2], -1, p.tt2, nullptr, p.tt2, (int)N2, (int)N2, C * M0 * M1, 1, 0, p.That, M0 * M1,
+           conv3_bytes);
+    }
     pass(CGS_K_TOEP_CROP, p.ft[1], +1, p.tt2, nullptr, p.tt1, (int)M1, (int)N1, C * M0, N2, 0, nullptr, 0,
          (double)C * M0 * (M1 + N1) * N2 * 8);
     pass(CGS_K_TOEP_CROP, p.ft[0], +1, p.tt1, nullptr, p.Xc, (int)M0, (int)N0, C, N1 * N2, 0, nullptr, 0, 0.0, 2);

@@ -484,3 +484,53 @@ def test_koosh_3d_medium(cgs, ora):
     e4 = rel(xg, xo)
     print(f"3D 64^3: E {e1:.2e} E^H {e2:.2e} toeplitz normal {e3:.2e} CG(5) {e4:.2e}")
     assert e1 <= OP_TOL and e2 <= OP_TOL and e3 <= 5e-5 and e4 <= 1e-4
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 128), (32, 48, 192), (32, 32, 256)])
+def test_toeplitz_3d_lean_conv(cgs, shape):
+    """3D Toeplitz operator whose axis-2 convolution runs on the lean kernel
+    (N2 = 128 / 192 / 256, 16 coils): against the NUFFT normal operator, and
+    8 CG iterations through each operator."""
+    rng = np.random.default_rng(sum(shape))
+    M, C = 60000, 16
+    k = rand_traj(rng, 3, M)
+    w = rng.uniform(0.1, 1.0, M).astype(np.float32)
+    S = np.asfortranarray(rand_img(rng, shape + (C,)) * 0.2)
+    x = rand_img(rng, shape)
+    P = cgs.Plan(shape, M, 2.0, 3, ncoils=C)
+    P.kspace(k)
+    P.weights(w)
+    P.sensitivities(S)
+    e = rel(P.normal(x, lam=0.5, toeplitz=True), P.normal(x, lam=0.5))
+    y = P.E(x)
+    xt, rt = P.solve(y, maxit=8, tol=0.0, lam=1.0, toeplitz=True, graph_iters=4)
+    xn, rn = P.solve(y, maxit=8, tol=0.0, lam=1.0)
+    e2 = rel(xt, xn)
+    print(f"{shape} C={C}: toeplitz vs nufft normal {e:.2e}, CG(8) {e2:.2e}")
+    assert e <= 2e-5 and e2 <= 1e-4
+    assert rt["iters"][0] == rn["iters"][0] == 8
+
+
+def test_toeplitz_3d_lean_all_passes(cgs):
+    """3D Toeplitz operator on the lean passes throughout (128^3, 16 coils:
+    axis-0 pad/crop at n = 256, the middle-axis half passes, the axis-2
+    conv): against the NUFFT normal operator, and 6 CG iterations (fused
+    tail, graph-replayed) against 6 through the NUFFT operator."""
+    rng = np.random.default_rng(128)
+    shape, M, C = (128, 128, 128), 200000, 16
+    k = rand_traj(rng, 3, M)
+    w = rng.uniform(0.1, 1.0, M).astype(np.float32)
+    S = np.asfortranarray(rand_img(rng, shape + (C,)) * 0.2)
+    x = rand_img(rng, shape)
+    P = cgs.Plan(shape, M, 2.0, 3, ncoils=C)
+    P.kspace(k)
+    P.weights(w)
+    P.sensitivities(S)
+    e = rel(P.normal(x, lam=0.5, toeplitz=True), P.normal(x, lam=0.5))
+    y = P.E(x)
+    xt, rt = P.solve(y, maxit=6, tol=0.0, lam=1.0, toeplitz=True, graph_iters=2)
+    xn, rn = P.solve(y, maxit=6, tol=0.0, lam=1.0)
+    e2 = rel(xt, xn)
+    print(f"128^3 C=16 lean: toeplitz vs nufft normal {e:.2e}, CG(6) {e2:.2e}")
+    assert e <= 2e-5 and e2 <= 1e-4
+    assert rt["iters"][0] == rn["iters"][0] == 6
