@@ -238,3 +238,62 @@ def test_nfft_3rd_dim_cart(cgs, ora):
         e2 = rel(back[..., z], o)
         print(f"slice {z}: trafo {e1:.2e} adjoint {e2:.2e}")
         assert e1 <= OP_TOL and e2 <= OP_TOL
+
+
+def test_empty_and_ragged_frames(cgs):
+    """Frames of 7, 0, 13 and 1 spokes in one plan: the empty frame gets no
+    data, a zero image from E^H and a zero solution after 0 iterations; the
+    others match single-plan operators and solves (per-frame CG), and a joint
+    solve equals the joint solve of the plan without the empty frame."""
+    from codeare_amd import data
+    rng = np.random.default_rng(41)
+    N, C, R = 48, 4, 96
+    counts = (7, 0, 13, 1)
+    ks, first = [], 0
+    for c in counts:
+        ks.append(data.radial_2d(c, R, True, first))
+        first += c
+    k = np.asfortranarray(np.concatenate(ks, axis=1))
+    off = np.concatenate([[0], np.cumsum(np.array(counts) * R)]).astype(np.int64)
+    w = data.density(k, "k", R)
+    S = data.coil_maps(N, 2, C, rng)
+    x = rand_img(rng, (N, N, len(counts)))
+    P = cgs.Plan((N, N), k.shape[1], 2.0, 3, ncoils=C, frame_offsets=off)
+    P.kspace(k)
+    P.weights(w)
+    P.sensitivities(S)
+    y = P.E(x)
+    assert y.size == C * off[-1]
+    xe = P.EH(y)
+    assert np.all(xe[..., 1] == 0)
+    for toep in (False, True):
+        xs, rep = P.solve(y, maxit=200, tol=1e-4, lam=0.5, toeplitz=toep, graph_iters=4 if toep else 0)
+        assert rep["iters"][1] == 0 and np.all(xs[..., 1] == 0)
+        for f in (0, 2, 3):
+            Mf = int(off[f + 1] - off[f])
+            P1 = cgs.Plan((N, N), Mf, 2.0, 3, ncoils=C)
+            P1.kspace(k[:, off[f]:off[f + 1]])
+            P1.weights(w[off[f]:off[f + 1]])
+            P1.sensitivities(S)
+            yf = y[C * off[f]: C * off[f + 1]].reshape((Mf, C), order="F")
+            assert rel(P1.E(x[..., f]), yf) <= 1e-6
+            x1, r1 = P1.solve(yf, maxit=200, tol=1e-4, lam=0.5, toeplitz=toep)
+            ef = rel(xs[..., f], x1)
+            print(f"toeplitz={toep} frame {f} ({counts[f]} spokes): iters {rep['iters'][f]} vs {r1['iters'][0]}, "
+                  f"image {ef:.2e}")
+            assert abs(rep["iters"][f] - r1["iters"][0]) <= 1
+            assert ef <= (1e-5 if rep["iters"][f] == r1["iters"][0] else 1e-3)
+    # joint: the empty frame adds nothing to the sums
+    xj, rj = P.solve(y, maxit=30, tol=0.0, lam=0.5, joint=True)
+    keep = [0, 2, 3]
+    off2 = np.concatenate([[0], np.cumsum(np.array([counts[f] for f in keep]) * R)]).astype(np.int64)
+    k2 = np.asfortranarray(np.concatenate([ks[f] for f in keep], axis=1))
+    P2 = cgs.Plan((N, N), k2.shape[1], 2.0, 3, ncoils=C, frame_offsets=off2)
+    P2.kspace(k2)
+    P2.weights(data.density(k2, "k", R))
+    P2.sensitivities(S)
+    y2 = np.concatenate([y[C * off[f]: C * off[f + 1]] for f in keep])
+    x2, r2 = P2.solve(y2, maxit=30, tol=0.0, lam=0.5, joint=True)
+    ej = rel(xj[..., keep], x2)
+    print(f"joint with / without the empty frame: {ej:.2e}")
+    assert ej <= 1e-5 and np.all(xj[..., 1] == 0)
