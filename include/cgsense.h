@@ -147,7 +147,8 @@ int cgs_normal(cgs_plan plan, const void* x, void* q, double lambda, int normal_
 /* CG solve (NCSENSE::Adjoint / operator->* -> CGLS::Solve, NCSENSE.hpp:408-414,
  * CGLS.hpp:30-59).  x0 may be NULL (zero start, as codeare) or a warm start.
  * Every frame has its own scalars and stops on its own, or, with opts->joint,
- * all frames share one CG.  report may be NULL. */
+ * all frames share one CG.  maxit = 0 returns E^H y as codeare's CGLS does
+ * (CGLS.hpp:31-33), with or without x0.  report may be NULL. */
 int cgs_solve(cgs_plan plan, const void* y, const void* x0, void* x,
               const cgs_solve_opts* opts, cgs_solve_report* report);
 

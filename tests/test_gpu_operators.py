@@ -534,3 +534,27 @@ def test_toeplitz_3d_lean_all_passes(cgs):
     print(f"128^3 C=16 lean: toeplitz vs nufft normal {e:.2e}, CG(6) {e2:.2e}")
     assert e <= 2e-5 and e2 <= 1e-4
     assert rt["iters"][0] == rn["iters"][0] == 6
+
+
+def test_zero_iterations_return_eh(cgs, ora):
+    """codeare's CGLS returns A/y itself when cgiter is 0 (CGLS.hpp:31-33),
+    which is NCSENSE's default in the Params constructor: ->* is then the
+    reference E^H.  The same for the true adjoint, and with a warm start
+    (codeare has none; zero iterations still return E^H y)."""
+    prob, y, P, O = _c1_problem(cgs, ora, N=64, coils=4, spokes=48, samples=128)
+    for mode in (cgs.CGS_EH_REF, cgs.CGS_EH_TRUE):
+        x0, rep = P.solve(y, maxit=0, eh_mode=mode, ftiter=3, fteps=1e-3)
+        assert rep["iters"][0] == 0
+        assert rel(x0, P.EH(y, mode=mode, ftiter=3, fteps=1e-3)) <= 1e-7
+    xo, it, _ = O.cgls_ref(prob.maps, prob.w, y, mode=1, ftiter=3, fteps=1e-3, maxit=0)
+    assert it == 0
+    ncs = cgs.NCSENSE({"sensitivities": prob.maps, "nk": prob.k.shape[1], "m": prob.wl.m,
+                       "alpha": prob.wl.sigma})   # cgiter defaults to 0
+    ncs.KSpace(prob.k)
+    ncs.Weights(prob.w)
+    e = rel(ncs.Adjoint(y), xo)
+    print(f"->* with cgiter 0 vs the oracle's CGLS(0) = E^H: {e:.2e}")
+    assert e <= 1e-5
+    xw = rand_img(np.random.default_rng(3), P._img_shape())
+    xr, _ = P.solve(y, x0=xw, maxit=0)
+    assert rel(xr, P.EH(y)) <= 1e-7
