@@ -1209,11 +1209,18 @@ __global__ void __launch_bounds__(256) k_interp_lds3(SampArgs a, const int4* ite
         if (dst[u] >= 0) ext[dst[u]] = v[u];
     }
   }
-  const int ns = it.w - it.z;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cl = lane % CG, rs = lane / CG;
+  const int c = cbase + cl;
+  // the item's samples in batches of maxs: window values and offsets staged
+  // per batch, the grid region once for the whole item
+  for (int sb = it.z; sb < it.w; sb += maxs) {
+  const int ns = min(maxs, it.w - sb);
+  if (sb > it.z) lds_barrier();   // the previous batch is consumed
   for (int idx = threadIdx.x; idx < ns * 3 * TAPS; idx += blockDim.x)
-    wpsi[idx] = a.psi[(int64_t)it.z * 3 * TAPS + idx];
+    wpsi[idx] = a.psi[(int64_t)sb * 3 * TAPS + idx];
   for (int k = threadIdx.x; k < ns; k += blockDim.x) {
-    const int smp = it.z + k;
+    const int smp = sb + k;
     int r0 = a.base[3 * smp] - org0, r1 = a.base[3 * smp + 1] - org1, r2 = a.base[3 * smp + 2] - org2;
     if (r0 < 0) r0 += a.n0; else if (r0 >= a.n0) r0 -= a.n0;
     if (r1 < 0) r1 += a.n1; else if (r1 >= a.n1) r1 -= a.n1;
@@ -1223,9 +1230,6 @@ __global__ void __launch_bounds__(256) k_interp_lds3(SampArgs a, const int4* ite
     wrel[3 * k + 2] = r2;
   }
   lds_barrier();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int cl = lane % CG, rs = lane / CG;
-  const int c = cbase + cl;
   for (int k = wave; k < ns; k += 4) {
     const float* ps = wpsi + k * 3 * TAPS;
     const int r0 = wrel[3 * k], r1 = wrel[3 * k + 1], r2 = wrel[3 * k + 2];
@@ -1251,7 +1255,8 @@ __global__ void __launch_bounds__(256) k_interp_lds3(SampArgs a, const int4* ite
       acc.x += o.x;
       acc.y += o.y;
     }
-    if (rs == 0 && c < a.C) a.y[(int64_t)(it.z + k) * a.C + c] = acc;
+    if (rs == 0 && c < a.C) a.y[(int64_t)(sb + k) * a.C + c] = acc;
+  }
   }
 }
 

@@ -392,6 +392,7 @@ static int fft_adjoint(Plan& p, float2* X, const int* fl) {
 }
 
 // ---- window gather / scatter --------------------------------------------------------
+constexpr int kInterp3Batch = 64;   // samples per staged batch of the 3D LDS gather
 static int interp(Plan& p, float2* y, const int* fl) {
   SampArgs a{};
   a.base = p.base; a.psi = p.psi; a.w = nullptr;
@@ -407,9 +408,10 @@ static int interp(Plan& p, float2* y, const int* fl) {
     const int ng = (p.C + p.interp_cg - 1) / p.interp_cg;
     launch_interp_lds(a, p.tile_items, p.n_tile_items, p.T, p.nt[0], p.interp_cg, p.tile_item_max, ng, p.stream);
   } else if (p.d == 3 && p.use_interp_lds && p.T == 8 && p.taps <= 8 &&
-             interp_lds3_bytes(p.T, p.taps, 2, p.tile_item_max) <= 65536) {
-    // (8 + taps - 1)^3 cells x 2 coils staged per item (54 KB at taps 8)
-    launch_interp_lds3(a, p.tile_items, p.n_tile_items, p.T, p.nt[0], p.nt[1], 2, p.tile_item_max, p.stream);
+             interp_lds3_bytes(p.T, p.taps, 2, kInterp3Batch) <= 65536) {
+    // (8 + taps - 1)^3 cells x 2 coils staged once per item (54 KB at taps
+    // 8), its samples taken kInterp3Batch at a time
+    launch_interp_lds3(a, p.tile_items, p.n_tile_items, p.T, p.nt[0], p.nt[1], 2, kInterp3Batch, p.stream);
   } else {
     launch_interp(a, p.interp_cg, p.stream);
   }
@@ -837,12 +839,12 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   p.grid_cg = (C <= 8) ? 8 : (C <= 16 ? 16 : 32);
   p.grid_w = p.grid_cg / 8;
   p.interp_cg = (C <= 8) ? 8 : 16;
-  // 3D: the LDS-staged gather (the (8 + taps - 1)^3 x 2-coil region plus 64
-  // samples' window values, under 64 KB at taps <= 8) measured slower than the
-  // global gather on the koosh ball (most tiles hold few samples), so it is
-  // opt-in (CGS_INTERP_LDS=1)
-  p.use_interp_lds = (d == 2);
-  p.tile_item_max = (d == 3) ? 64 : 128;
+  // 3D: the LDS-staged gather stages the (8 + taps - 1)^3 x 2-coil region once
+  // per item and streams the item's samples through it 64 at a time; items of
+  // up to 2048 samples amortise the staging (dense koosh-ball tiles hold
+  // thousands of samples)
+  p.use_interp_lds = true;
+  p.tile_item_max = (d == 3) ? 2048 : 128;
   p.visits_per_item = (d == 3) ? 4096 : 512;
   p.grid_cgr = (C <= 8) ? 8 : 16;
   // shape overrides for tuning sweeps (tile side must be a multiple of 8)
