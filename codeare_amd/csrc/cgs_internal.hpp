@@ -190,6 +190,7 @@ struct Plan {
   int grid_cgr = 16;                  // coils per wave in the register kernel
   bool use_grid_atomic = false;       // LDS float atomics variant (kept for comparison)
   int tile_item_max = 64;             // samples per tile-level interp item
+  int interp3_threads = 1024;         // 3D LDS gather: threads per block
 
   // stream / state
   hipStream_t stream = nullptr;
@@ -324,7 +325,7 @@ void launch_grid_mfma(const GridArgs& a, const int4* vrec, int nitems, hipStream
 void launch_grid_mfma3(const GridArgs& a, const int4* vrec, int nitems, hipStream_t s);
 size_t interp_lds3_bytes(int T, int taps, int cg, int maxs);
 void launch_interp_lds3(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int nt1, int cg, int maxs,
-                        hipStream_t s);
+                        int threads, hipStream_t s);
 void launch_grid_reg3(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s);
 size_t interp_lds_bytes(int T, int taps, int CG, int rowstride, int maxs);
 void launch_cg_tail(const Plan& p, float2* x, float2* r, const float2* pv, const float2* part, int G,

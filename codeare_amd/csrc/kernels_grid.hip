@@ -1165,9 +1165,9 @@ void launch_interp_lds(const SampArgs& a, const int4* items, int nitems, int T, 
 // coils) staged in LDS.  Work item = (frame, tile, samples whose floor(n x)
 // lies in the tile); lanes = (coil, tap row (s1, s2)), a shuffle reduction
 // over the rows.
-template <int CG, int TAPS>
-__global__ void __launch_bounds__(256) k_interp_lds3(SampArgs a, const int4* items, int T, int maxs, int nt0,
-                                                     int nt1) {
+template <int CG, int TAPS, int NT>
+__global__ void __launch_bounds__(NT, NT / 128) k_interp_lds3(SampArgs a, const int4* items, int T, int maxs,
+                                                              int nt0, int nt1) {
   constexpr int RS = 64 / CG;
   extern __shared__ float2 sh[];
   const int4 it = items[blockIdx.x];            // frame, tile, sbeg, send
@@ -1183,7 +1183,7 @@ __global__ void __launch_bounds__(256) k_interp_lds3(SampArgs a, const int4* ite
   const int org0 = t0 * T - m, org1 = t1 * T - m, org2 = t2 * T - m;
   const int64_t gbase = (int64_t)it.x * a.ntot;
   {
-    constexpr int U = 8, STEP = 256 / CG;
+    constexpr int U = 8, STEP = NT / CG;
     const int c = threadIdx.x % CG;
     const bool cok = cbase + c < a.C;
     for (int e00 = threadIdx.x / CG; e00 < E3; e00 += STEP * U) {
@@ -1230,7 +1230,7 @@ __global__ void __launch_bounds__(256) k_interp_lds3(SampArgs a, const int4* ite
     wrel[3 * k + 2] = r2;
   }
   lds_barrier();
-  for (int k = wave; k < ns; k += 4) {
+  for (int k = wave; k < ns; k += NT / 64) {
     const float* ps = wpsi + k * 3 * TAPS;
     const int r0 = wrel[3 * k], r1 = wrel[3 * k + 1], r2 = wrel[3 * k + 2];
     float2 acc = make_float2(0.f, 0.f);
@@ -1265,12 +1265,11 @@ size_t interp_lds3_bytes(int T, int taps, int cg, int maxs) {
   return sizeof(float2) * E * E * E * cg + sizeof(float) * (size_t)maxs * 3 * taps + sizeof(int) * (size_t)maxs * 3;
 }
 
-void launch_interp_lds3(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int nt1, int cg, int maxs,
-                        hipStream_t s) {
-  if (nitems == 0) return;
-  const size_t shm = interp_lds3_bytes(T, a.taps, cg, maxs);
-  const dim3 g((unsigned)nitems, (unsigned)((a.C + cg - 1) / cg));
-#define CGS_IL3(CG_, TP) hipLaunchKernelGGL((k_interp_lds3<CG_, TP>), g, dim3(256), shm, s, a, items, T, maxs, nt0, nt1)
+template <int NT>
+static void interp_lds3_launch(const SampArgs& a, const int4* items, dim3 g, size_t shm, int T, int nt0, int nt1,
+                               int cg, int maxs, hipStream_t s) {
+#define CGS_IL3(CG_, TP) \
+  hipLaunchKernelGGL((k_interp_lds3<CG_, TP, NT>), g, dim3(NT), shm, s, a, items, T, maxs, nt0, nt1)
   if (cg == 2) {
     switch (a.taps) { case 4: CGS_IL3(2, 4); break; case 6: CGS_IL3(2, 6); break;
                       case 8: CGS_IL3(2, 8); break; case 10: CGS_IL3(2, 10); break; default: break; }
@@ -1279,6 +1278,20 @@ void launch_interp_lds3(const SampArgs& a, const int4* items, int nitems, int T,
                       case 8: CGS_IL3(4, 8); break; case 10: CGS_IL3(4, 10); break; default: break; }
   }
 #undef CGS_IL3
+}
+
+// threads per block: the staged region (54 KB at taps 8) allows two blocks per
+// CU, so 1024-thread blocks give 8 waves per SIMD to cover the LDS latency of
+// the tap reads (256-thread blocks: 2 waves per SIMD, 1.34x slower on the
+// 128^3 koosh ball)
+void launch_interp_lds3(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int nt1, int cg, int maxs,
+                        int threads, hipStream_t s) {
+  if (nitems == 0) return;
+  const size_t shm = interp_lds3_bytes(T, a.taps, cg, maxs);
+  const dim3 g((unsigned)nitems, (unsigned)((a.C + cg - 1) / cg));
+  if (threads >= 1024) interp_lds3_launch<1024>(a, items, g, shm, T, nt0, nt1, cg, maxs, s);
+  else if (threads >= 512) interp_lds3_launch<512>(a, items, g, shm, T, nt0, nt1, cg, maxs, s);
+  else interp_lds3_launch<256>(a, items, g, shm, T, nt0, nt1, cg, maxs, s);
 }
 
 // ---------------------------------------------------------------------------

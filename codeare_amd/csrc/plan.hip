@@ -411,7 +411,8 @@ static int interp(Plan& p, float2* y, const int* fl) {
              interp_lds3_bytes(p.T, p.taps, 2, kInterp3Batch) <= 65536) {
     // (8 + taps - 1)^3 cells x 2 coils staged once per item (54 KB at taps
     // 8), its samples taken kInterp3Batch at a time
-    launch_interp_lds3(a, p.tile_items, p.n_tile_items, p.T, p.nt[0], p.nt[1], 2, kInterp3Batch, p.stream);
+    launch_interp_lds3(a, p.tile_items, p.n_tile_items, p.T, p.nt[0], p.nt[1], 2, kInterp3Batch, p.interp3_threads,
+                       p.stream);
   } else {
     launch_interp(a, p.interp_cg, p.stream);
   }
@@ -867,6 +868,7 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   if (const char* e = std::getenv("CGS_VISITS_PER_ITEM")) p.visits_per_item = std::max(16, std::atoi(e));
   if (const char* e = std::getenv("CGS_TILE_ITEM_MAX")) p.tile_item_max = std::max(8, std::atoi(e));
   if (const char* e = std::getenv("CGS_INTERP_LDS")) p.use_interp_lds = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CGS_INTERP3_THREADS")) p.interp3_threads = std::atoi(e);
   if (const char* e = std::getenv("CGS_GRID_REG")) p.use_grid_reg = std::atoi(e) != 0;
   p.use_grid_mfma = true;
   if (const char* e = std::getenv("CGS_GRID_MFMA")) p.use_grid_mfma = std::atoi(e) != 0;
