@@ -25,6 +25,15 @@ __device__ __forceinline__ float2 shfl_xor2(float2 v, int m) {
   return make_float2(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
 }
 
+// v + (v of the lane that the DPP control CTRL selects); lanes whose source
+// lies outside the 16-lane row add zero
+template <int CTRL>
+__device__ __forceinline__ float2 dpp_add2(float2 v) {
+  const float x = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.x), CTRL, 0xF, 0xF, true));
+  const float y = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.y), CTRL, 0xF, 0xF, true));
+  return make_float2(v.x + x, v.y + y);
+}
+
 // ---------------------------------------------------------------------------
 // forward B: y[s][c] = sum_taps psi * G[cell][c]
 template <int CG, int TAPS, int D>
@@ -1249,11 +1258,26 @@ __global__ void __launch_bounds__(NT, NT / 128) k_interp_lds3(SampArgs a, const 
       acc.x += wr * racc.x;
       acc.y += wr * racc.y;
     }
+    if (CG == 2) {
+      // within each 16-lane row: lanes l and l ^ 2 (quad permutation), then
+      // l + 4 and l + 8 (row shifts) -- lanes 0 and 1 of a row hold its sums
+      // for coils 0 and 1; then across the four rows
+      acc = dpp_add2<0x4E>(acc);
+      acc = dpp_add2<0x104>(acc);
+      acc = dpp_add2<0x108>(acc);
 #pragma unroll
-    for (int msk = CG; msk < 64; msk <<= 1) {
-      const float2 o = shfl_xor2(acc, msk);
-      acc.x += o.x;
-      acc.y += o.y;
+      for (int msk = 16; msk < 64; msk <<= 1) {
+        const float2 o = shfl_xor2(acc, msk);
+        acc.x += o.x;
+        acc.y += o.y;
+      }
+    } else {
+#pragma unroll
+      for (int msk = CG; msk < 64; msk <<= 1) {
+        const float2 o = shfl_xor2(acc, msk);
+        acc.x += o.x;
+        acc.y += o.y;
+      }
     }
     if (rs == 0 && c < a.C) a.y[(int64_t)(sb + k) * a.C + c] = acc;
   }
