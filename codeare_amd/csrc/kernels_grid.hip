@@ -34,6 +34,21 @@ __device__ __forceinline__ float2 dpp_add2(float2 v) {
   return make_float2(v.x + x, v.y + y);
 }
 
+// v + v of lane l ^ 16 (SPAN 16) or l ^ 32 (SPAN 32), by the row swaps of
+// gfx950 (v_permlane16_swap / v_permlane32_swap, no LDS): with both operands
+// v, the two results hold the lane's own value and its partner's
+template <int SPAN>
+__device__ __forceinline__ float swap_sum(float v) {
+  const unsigned u = __float_as_uint(v);
+  if (SPAN == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  } else {
+    const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // forward B: y[s][c] = sum_taps psi * G[cell][c]
 template <int CG, int TAPS, int D>
@@ -1261,16 +1276,12 @@ __global__ void __launch_bounds__(NT, NT / 128) k_interp_lds3(SampArgs a, const 
     if (CG == 2) {
       // within each 16-lane row: lanes l and l ^ 2 (quad permutation), then
       // l + 4 and l + 8 (row shifts) -- lanes 0 and 1 of a row hold its sums
-      // for coils 0 and 1; then across the four rows
+      // for coils 0 and 1; then across the four rows by row swaps
       acc = dpp_add2<0x4E>(acc);
       acc = dpp_add2<0x104>(acc);
       acc = dpp_add2<0x108>(acc);
-#pragma unroll
-      for (int msk = 16; msk < 64; msk <<= 1) {
-        const float2 o = shfl_xor2(acc, msk);
-        acc.x += o.x;
-        acc.y += o.y;
-      }
+      acc = make_float2(swap_sum<16>(acc.x), swap_sum<16>(acc.y));
+      acc = make_float2(swap_sum<32>(acc.x), swap_sum<32>(acc.y));
     } else {
 #pragma unroll
       for (int msk = CG; msk < 64; msk <<= 1) {
