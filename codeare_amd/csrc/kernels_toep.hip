@@ -191,6 +191,21 @@ __device__ __forceinline__ void dft_low_half(float2 (&v)[P]) {
   }
 }
 
+// sum over the 16 lanes of a row (every lane ends with the same sum), by DPP
+// lane moves instead of LDS permutes: quad permutations [1,0,3,2] and
+// [2,3,0,1], then the half-row and row mirrors
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float row16_sum(float v) {
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  v += dpp_mov<0x141>(v);
+  v += dpp_mov<0x140>(v);
+  return v;
+}
+
 // sum over the block (every thread calls it), nw = waves per block
 template <int NW>
 __device__ __forceinline__ double block_sum(double v, double* red) {
@@ -624,12 +639,10 @@ __global__ void __launch_bounds__(1024, 8) k_toep_crop(ToepArgs a) {
 #pragma unroll
   for (int m2 = 0; m2 < 4; ++m2) c[m2] = act ? cmulc(v[m2], sv[m2]) : make_float2(0.f, 0.f);
 #pragma unroll
-  for (int m2 = 0; m2 < 4; ++m2)
-#pragma unroll
-    for (int m = 8; m > 0; m >>= 1) {
-      c[m2].x += __shfl_xor(c[m2].x, m);
-      c[m2].y += __shfl_xor(c[m2].y, m);
-    }
+  for (int m2 = 0; m2 < 4; ++m2) {   // over the 16 coils (lanes q of a row)
+    c[m2].x = row16_sum(c[m2].x);
+    c[m2].y = row16_sum(c[m2].y);
+  }
   const int64_t rowb = ((int64_t)f * nb + g) * a.Ntot + (int64_t)a.N0 * i1;
   double pq = 0.0;
   if (q == 0 && act) {
@@ -838,12 +851,10 @@ __global__ void __launch_bounds__(1024) k_toep_crop_row(ToepArgs a) {
     for (int k = 0; k < NP; ++k) u[k] = un[k];
   }
 #pragma unroll
-  for (int m2 = 0; m2 < 4; ++m2)
-#pragma unroll
-    for (int m = 8; m > 0; m >>= 1) {
-      c[m2].x += __shfl_xor(c[m2].x, m);
-      c[m2].y += __shfl_xor(c[m2].y, m);
-    }
+  for (int m2 = 0; m2 < 4; ++m2) {   // over the 16 coils (lanes q of a row)
+    c[m2].x = row16_sum(c[m2].x);
+    c[m2].y = row16_sum(c[m2].y);
+  }
   double pq = 0.0;
   if (q == 0 && act) {
     const Rsrc ro = rsrc(a.out + rowb, (int64_t)a.N0 * 8);
