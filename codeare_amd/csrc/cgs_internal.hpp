@@ -45,6 +45,51 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Lane sums without LDS traffic: DPP lane moves within 16-lane rows and the
+// gfx950 row swaps (v_permlane16_swap / v_permlane32_swap) across rows.  All
+// lanes of the wave take part.
+template <int CTRL>   // v of the lane that DPP control CTRL selects (0 outside the row)
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+// v + v of lane l ^ 16 (SPAN 16) or l ^ 32 (SPAN 32): with both operands v,
+// the two results of a row swap hold the lane's own value and its partner's
+template <int SPAN>
+__device__ __forceinline__ float swap_sum(float v) {
+  const unsigned u = __float_as_uint(v);
+  if (SPAN == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  } else {
+    const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+}
+
+// sum over the 16 lanes of a row (every lane ends with the same sum): quad
+// permutations [1,0,3,2] and [2,3,0,1], then the half-row and row mirrors
+__device__ __forceinline__ float row16_sum(float v) {
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  v += dpp_mov<0x141>(v);
+  v += dpp_mov<0x140>(v);
+  return v;
+}
+
+// sum over the lanes l' with l' % CG == l % CG (CG a power of two <= 32):
+// quad permutations for lane distances 1 and 2, row rotations by 4 and 8,
+// then the row swaps; every lane ends with its class's sum
+template <int CG>
+__device__ __forceinline__ float class_sum(float v) {
+  if (CG < 2) v += dpp_mov<0xB1>(v);
+  if (CG < 4) v += dpp_mov<0x4E>(v);
+  if (CG < 8) v += dpp_mov<0x124>(v);
+  if (CG < 16) v += dpp_mov<0x128>(v);
+  if (CG < 32) v = swap_sum<16>(v);
+  return swap_sum<32>(v);
+}
+
 struct CgState {        // per-frame CG scalars, FP64
   double rn, bn, xn, pq, alpha, beta, rel;
   int it, done, nan, pad;

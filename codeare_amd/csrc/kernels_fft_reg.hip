@@ -328,11 +328,7 @@ __global__ void __launch_bounds__(256) k_fft_reg(FftArgs a) {
           const float2 xv = X[r][k2];
           c = make_float2(sv.x * xv.x + sv.y * xv.y, sv.x * xv.y - sv.y * xv.x);
         }
-#pragma unroll
-        for (int m = 8; m > 0; m >>= 1) {
-          c.x += __shfl_xor(c.x, m);
-          c.y += __shfl_xor(c.y, m);
-        }
+        c = make_float2(row16_sum(c.x), row16_sum(c.y));   // over the 16 coils (lanes q of a row)
         if (q == 0 && p < a.nout) {
           outp[p] = c;
           if (pd) {
@@ -597,11 +593,7 @@ __global__ void __launch_bounds__(1024) k_fft_r8(FftArgs a) {
         const float2 sv = a.S[f * a.Sfs + (p + (int64_t)a.N0 * rest) * a.C + qq];
         c = make_float2(sv.x * v[m2].x + sv.y * v[m2].y, sv.x * v[m2].y - sv.y * v[m2].x);
       }
-#pragma unroll
-      for (int m = 8; m > 0; m >>= 1) {
-        c.x += __shfl_xor(c.x, m);
-        c.y += __shfl_xor(c.y, m);
-      }
+      c = make_float2(row16_sum(c.x), row16_sum(c.y));   // over the 16 coils (lanes q of a row)
       if (q == 0 && act && p < a.nout) {
         outp[p] = c;
         if (pd) {

@@ -34,20 +34,6 @@ __device__ __forceinline__ float2 dpp_add2(float2 v) {
   return make_float2(v.x + x, v.y + y);
 }
 
-// v + v of lane l ^ 16 (SPAN 16) or l ^ 32 (SPAN 32), by the row swaps of
-// gfx950 (v_permlane16_swap / v_permlane32_swap, no LDS): with both operands
-// v, the two results hold the lane's own value and its partner's
-template <int SPAN>
-__device__ __forceinline__ float swap_sum(float v) {
-  const unsigned u = __float_as_uint(v);
-  if (SPAN == 16) {
-    const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
-    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
-  } else {
-    const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
-    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
-  }
-}
 
 // ---------------------------------------------------------------------------
 // forward B: y[s][c] = sum_taps psi * G[cell][c]
@@ -322,12 +308,7 @@ __global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* item
       acc.x += wr * racc.x;
       acc.y += wr * racc.y;
     }
-#pragma unroll
-    for (int msk = CG; msk < 64; msk <<= 1) {
-      const float2 o = shfl_xor2(acc, msk);
-      acc.x += o.x;
-      acc.y += o.y;
-    }
+    acc = make_float2(class_sum<CG>(acc.x), class_sum<CG>(acc.y));   // over the row slots
     if (rs == 0 && c < a.C) a.y[(int64_t)(it.z + k) * a.C + c] = acc;
   }
 }
@@ -1283,12 +1264,7 @@ __global__ void __launch_bounds__(NT, NT / 128) k_interp_lds3(SampArgs a, const 
       acc = make_float2(swap_sum<16>(acc.x), swap_sum<16>(acc.y));
       acc = make_float2(swap_sum<32>(acc.x), swap_sum<32>(acc.y));
     } else {
-#pragma unroll
-      for (int msk = CG; msk < 64; msk <<= 1) {
-        const float2 o = shfl_xor2(acc, msk);
-        acc.x += o.x;
-        acc.y += o.y;
-      }
+      acc = make_float2(class_sum<CG>(acc.x), class_sum<CG>(acc.y));
     }
     if (rs == 0 && c < a.C) a.y[(int64_t)(sb + k) * a.C + c] = acc;
   }

@@ -191,21 +191,6 @@ __device__ __forceinline__ void dft_low_half(float2 (&v)[P]) {
   }
 }
 
-// sum over the 16 lanes of a row (every lane ends with the same sum), by DPP
-// lane moves instead of LDS permutes: quad permutations [1,0,3,2] and
-// [2,3,0,1], then the half-row and row mirrors
-template <int CTRL>
-__device__ __forceinline__ float dpp_mov(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
-}
-__device__ __forceinline__ float row16_sum(float v) {
-  v += dpp_mov<0xB1>(v);
-  v += dpp_mov<0x4E>(v);
-  v += dpp_mov<0x141>(v);
-  v += dpp_mov<0x140>(v);
-  return v;
-}
-
 // sum over the block (every thread calls it), nw = waves per block
 template <int NW>
 __device__ __forceinline__ double block_sum(double v, double* red) {
