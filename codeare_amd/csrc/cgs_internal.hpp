@@ -77,6 +77,20 @@ __device__ __forceinline__ float row16_sum(float v) {
   return v;
 }
 
+// sum over the aligned group of GS consecutive lanes (GS a power of two
+// <= 64; every lane ends with its group's sum); only the group's own lanes
+// are read, so groups that have left a loop may be inactive
+template <int GS>
+__device__ __forceinline__ float group_sum(float v) {
+  if (GS >= 2) v += dpp_mov<0xB1>(v);
+  if (GS >= 4) v += dpp_mov<0x4E>(v);
+  if (GS >= 8) v += dpp_mov<0x141>(v);
+  if (GS >= 16) v += dpp_mov<0x140>(v);
+  if (GS >= 32) v = swap_sum<16>(v);
+  if (GS >= 64) v = swap_sum<32>(v);
+  return v;
+}
+
 // sum over the lanes l' with l' % CG == l % CG (CG a power of two <= 32):
 // quad permutations for lane distances 1 and 2, row rotations by 4 and 8,
 // then the row swaps; every lane ends with its class's sum

@@ -50,11 +50,7 @@ __global__ void __launch_bounds__(256) k_coil_combine(const float2* X, const flo
       acc.x += s.x * v.x + s.y * v.y;
       acc.y += s.x * v.y - s.y * v.x;
     }
-#pragma unroll
-    for (int m = CP / 2; m > 0; m >>= 1) {
-      acc.x += __shfl_xor(acc.x, m);
-      acc.y += __shfl_xor(acc.y, m);
-    }
+    acc = make_float2(group_sum<CP>(acc.x), group_sum<CP>(acc.y));   // over the pixel's CP lanes
     if (cl == 0) {
       if (Imap) { const float iv = Imap[i]; acc.x *= iv; acc.y *= iv; }
       const int64_t o = (int64_t)f * Ntot + i;
@@ -288,11 +284,7 @@ __global__ void __launch_bounds__(256) k_coil_alpha(const float2* X, const float
       acc.x += s.x * v.x + s.y * v.y;
       acc.y += s.x * v.y - s.y * v.x;
     }
-#pragma unroll
-    for (int m = CP / 2; m > 0; m >>= 1) {
-      acc.x += __shfl_xor(acc.x, m);
-      acc.y += __shfl_xor(acc.y, m);
-    }
+    acc = make_float2(group_sum<CP>(acc.x), group_sum<CP>(acc.y));   // over the pixel's CP lanes
     if (cl == 0) {
       const int64_t o = (int64_t)f * Ntot + i;
       const float2 p = pv[o];
