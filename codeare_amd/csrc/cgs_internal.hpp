@@ -67,6 +67,43 @@ __device__ __forceinline__ float swap_sum(float v) {
   }
 }
 
+// FP64 forms: the two 32-bit halves move separately (exact)
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov_d(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, true);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <int SPAN>
+__device__ __forceinline__ double swap_sum_d(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+  unsigned l0, l1, h0, h1;
+  if (SPAN == 16) {
+    const auto rl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto rh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    l0 = rl[0]; l1 = rl[1]; h0 = rh[0]; h1 = rh[1];
+  } else {
+    const auto rl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto rh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    l0 = rl[0]; l1 = rl[1]; h0 = rh[0]; h1 = rh[1];
+  }
+  return __longlong_as_double((long long)(((unsigned long long)h0 << 32) | l0)) +
+         __longlong_as_double((long long)(((unsigned long long)h1 << 32) | l1));
+}
+
+// sum over all 64 lanes of the wave (every lane ends with the same sum)
+__device__ __forceinline__ double wave_sum_d(double v) {
+  v += dpp_mov_d<0xB1>(v);
+  v += dpp_mov_d<0x4E>(v);
+  v += dpp_mov_d<0x141>(v);
+  v += dpp_mov_d<0x140>(v);
+  v = swap_sum_d<16>(v);
+  return swap_sum_d<32>(v);
+}
+
 // sum over the 16 lanes of a row (every lane ends with the same sum): quad
 // permutations [1,0,3,2] and [2,3,0,1], then the half-row and row mirrors
 __device__ __forceinline__ float row16_sum(float v) {

@@ -123,8 +123,7 @@ constexpr int kRegLines = 16;
 
 // sum over the 256-thread block (every thread calls it)
 __device__ __forceinline__ double block_sum_d(double v, double* red) {
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+  v = wave_sum_d(v);
   lds_barrier();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   lds_barrier();
@@ -388,8 +387,7 @@ __device__ __forceinline__ float2 twid(const float2* tw, int idx, float s) {
 }
 
 __device__ __forceinline__ double block_sum_1024(double v, double* red) {
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+  v = wave_sum_d(v);
   lds_barrier();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   lds_barrier();

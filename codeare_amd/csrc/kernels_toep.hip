@@ -194,8 +194,7 @@ __device__ __forceinline__ void dft_low_half(float2 (&v)[P]) {
 // sum over the block (every thread calls it), nw = waves per block
 template <int NW>
 __device__ __forceinline__ double block_sum(double v, double* red) {
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+  v = wave_sum_d(v);
   lds_barrier();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   lds_barrier();
@@ -499,8 +498,7 @@ __global__ void __launch_bounds__(1024, 8) k_toep_pad(ToepArgs a) {
       __shared__ double fold_s[2];   // beta, done
       if (tid < 64) {
         double v = ((rt[0] + rt[1]) + (rt[2] + rt[3])) + ((rt[4] + rt[5]) + (rt[6] + rt[7]));
-#pragma unroll
-        for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+        v = wave_sum_d(v);
         if (tid == 0) {
           const double rnew = v;
           const int it = cs.it_snap + 1;
@@ -698,8 +696,7 @@ __global__ void __launch_bounds__(1024) k_toep_pad_row(ToepArgs a) {
       __shared__ double fold_s[2];
       if (tid < 64) {
         double v = ((rt[0] + rt[1]) + (rt[2] + rt[3])) + ((rt[4] + rt[5]) + (rt[6] + rt[7]));
-#pragma unroll
-        for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+        v = wave_sum_d(v);
         if (tid == 0) {
           const double rnew = v;
           const int it = cs.it_snap + 1;

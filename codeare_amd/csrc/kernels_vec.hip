@@ -13,7 +13,7 @@ namespace cgs {
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+  v = wave_sum_d(v);
   if (lane == 0) sh[wave] = v;
   __syncthreads();
   double t = 0.0;
@@ -167,11 +167,8 @@ __global__ void __launch_bounds__(256) k_cg_tail(float2* x, float2* r, const flo
       return acc;
     };
     double a = sum4(pqf, npq), b = sum4(ppf, npp);
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1) {
-      a += __shfl_xor(a, m);
-      b += __shfl_xor(b, m);
-    }
+    a = wave_sum_d(a);
+    b = wave_sum_d(b);
     if ((threadIdx.x & 63) == 0) {
       sh[threadIdx.x >> 6] = a;
       sh2[threadIdx.x >> 6] = b;
