@@ -21,10 +21,6 @@ namespace cgs {
 
 typedef float gf32x16 __attribute__((ext_vector_type(16)));   // MFMA 32x32 accumulator
 
-__device__ __forceinline__ float2 shfl_xor2(float2 v, int m) {
-  return make_float2(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
-}
-
 // v + (v of the lane that the DPP control CTRL selects); lanes whose source
 // lies outside the 16-lane row add zero
 template <int CTRL>
@@ -85,12 +81,7 @@ __global__ void __launch_bounds__(256) k_interp(SampArgs a) {
           acc.y += wr * racc.y;
         }
       }
-#pragma unroll
-      for (int m = CG; m < 64; m <<= 1) {
-        const float2 o = shfl_xor2(acc, m);
-        acc.x += o.x;
-        acc.y += o.y;
-      }
+      acc = make_float2(class_sum<CG>(acc.x), class_sum<CG>(acc.y));   // over the row slots
       if (rs == 0 && c < a.C) a.y[(int64_t)s * a.C + c] = acc;
     }
   }
