@@ -1094,10 +1094,10 @@ static bool toep_row_enabled(int C) {
   }();
   return on && toep_row_shape(C);
 }
-static bool toep_pad_row_enabled(int C) {
+static bool toep_pad_row_enabled(int C) {   // CGS_TOEP_PAD_ROW=0: one block per coil group
   static const bool on = [] {
     const char* e = std::getenv("CGS_TOEP_PAD_ROW");
-    return e && std::atoi(e) != 0;
+    return !(e && std::atoi(e) == 0);
   }();
   return on && toep_row_shape(C);
 }
