@@ -441,21 +441,22 @@ class NCSENSE:
     def Sensitivities(self, S):
         self.plan.sensitivities(S)
 
-    def _per_frame(self, v, block):
-        # one block shared by every frame/slice (NCSENSE.hpp:204-248) -> one per frame
+    def _per_frame(self, v, block, per_sample):
+        # one block shared by every frame/slice (NCSENSE.hpp:204-248) -> one per
+        # frame; per-frame blocks (uneven ones with frame_offsets) pass through
         v = np.asarray(v, dtype=np.float32)
-        if v.size == block * self.F or self.F == 1:
+        if v.size == block * self.F or v.size == per_sample * self.plan.M or self.F == 1:
             return v
         if v.size != block:
             raise ValueError("k-space / weights size does not fit the frames")
         return np.concatenate([v.reshape(-1, order="F")] * self.F)
 
     def KSpace(self, k):
-        k = self._per_frame(k, self.d * self.M)
+        k = self._per_frame(k, self.d * self.M, self.d)
         self.plan.kspace(np.asarray(k, dtype=np.float32).reshape((self.d, -1), order="F"))
 
     def Weights(self, w):
-        self.plan.weights(self._per_frame(w, self.M))
+        self.plan.weights(self._per_frame(w, self.M, 1))
 
     # codeare's slice data (nk, Nz, C) <-> the plan's frame blocks (nk, C) per slice
     def _to_plan(self, y):

@@ -1349,6 +1349,11 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
   double* partB = p.partials + 512 * F;
   const int hcap = o.maxit + 1;
   if (hcap > p.hist_cap) {
+    // a captured CG graph writes the history through the old pointer and
+    // capacity: drop it with the buffer it refers to
+    if (p.graph_exec) (void)hipGraphExecDestroy(p.graph_exec);
+    p.graph_exec = nullptr;
+    p.graph_key.clear();
     dfree(p.hist);
     CHK(dalloc(&p.hist, (size_t)F * hcap));
     p.hist_cap = hcap;
@@ -1392,8 +1397,8 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
     // capture an even number of fused iterations once and replay the graph
     step = o.graph_iters + (o.graph_iters & 1);
     char key[256];
-    snprintf(key, sizeof(key), "%d|%d|%.17g|%d|%.17g|%d|%p|%p|%d", step, o.normal_op, o.lambda, o.tol_kind, o.tol,
-             o.maxit, (void*)p.That, (void*)p.stream, joint ? 1 : 0);
+    snprintf(key, sizeof(key), "%d|%d|%.17g|%d|%.17g|%d|%p|%p|%d|%p|%d", step, o.normal_op, o.lambda, o.tol_kind,
+             o.tol, o.maxit, (void*)p.That, (void*)p.stream, joint ? 1 : 0, (void*)p.hist, p.hist_cap);
     if (p.graph_exec && p.graph_key == key) p.beta_folded = p.graph_folded;
     if (!p.graph_exec || p.graph_key != key) {
       if (p.graph_exec) (void)hipGraphExecDestroy(p.graph_exec);

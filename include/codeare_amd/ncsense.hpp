@@ -10,10 +10,12 @@
 //            3rd_dim_cart (0: with 1, imsz (N0, N1, Nz) is Nz 2D slices on one
 //            trajectory, NFFT.hpp:84-103)
 //   NCSENSE: nk, ftiter (3), fteps (1e-3), cgiter (0), cgeps (1e-6),
-//            lambda (1e-6), m (1), alpha (1.0), dim4 (1), 3rd_dim_cart (0: with 1,
-//            maps (N0, N1, Nz, C) and data (nk, Nz, C), NCSENSE.hpp:81-99)
-// Frames (dim4) and slices share one CG, as codeare's CGLS runs on the whole
-// image matrix (cgs_solve_opts.joint).
+//            lambda (1e-6), m (1), alpha (1.0), dim4 (1), dim5 (1),
+//            3rd_dim_cart (0: with 1, maps (N0, N1, Nz, C) and data (nk, Nz, C),
+//            NCSENSE.hpp:81-99)
+// Frames (dim4 x dim5 of them, frame l + dim4*n for index (l, n),
+// NCSENSE.hpp:141-151, :222-233) and slices share one CG, as codeare's CGLS
+// runs on the whole image matrix (cgs_solve_opts.joint).
 // Errors from the library are thrown as std::runtime_error carrying the
 // library's message (codeare asserts or throws NCSENSE_KSPACE_DIMENSIONS).
 #pragma once
@@ -120,9 +122,12 @@ class NCSENSE {
     C_ = (int)sens_dims.back();
     for (int a = 0; a < d_; ++a) N_.push_back((int)sens_dims[a]);
     nk_ = (int64_t)p.get("nk", 1);
-    // frames of the plan: dim4 frames (NCSENSE.hpp:141-151) or the Cartesian
-    // slices, which share the trajectory, the weights and one CG
-    nmany_ = cart_ ? (int)sens_dims[2] : (int)p.get("dim4", 1);
+    // frames of the plan: dim4 x dim5 frames (m_nmany, NCSENSE.hpp:141-151) or
+    // the Cartesian slices, which share the trajectory, the weights and one CG
+    dim4_ = cart_ ? 1 : (int)p.get("dim4", 1);
+    dim5_ = cart_ ? 1 : (int)p.get("dim5", 1);
+    if (dim4_ < 1 || dim5_ < 1) throw std::runtime_error("NCSENSE: dim4 and dim5 must be >= 1");
+    nmany_ = cart_ ? (int)sens_dims[2] : dim4_ * dim5_;
     ftiter_ = (int)p.get("ftiter", 3);
     fteps_ = p.get("fteps", 1.0e-3);
     cgiter_ = (int)p.get("cgiter", 0);
@@ -139,7 +144,7 @@ class NCSENSE {
   NCSENSE& operator=(const NCSENSE&) = delete;
 
   void Sensitivities(const Matrix& sm) { check(cgs_set_sensitivities(plan_, sm.Ptr())); }
-  // k: (d, nk) shared by every frame/slice, or (d, nk, dim4) per frame
+  // k: (d, nk) shared by every frame/slice, or (d, nk, dim4[, dim5]) per frame
   // (NCSENSE.hpp:204-236); w: (nk) shared by every frame (NCSENSE.hpp:244-248)
   void KSpace(const RealMatrix& k) {
     check(cgs_set_trajectory(plan_, per_frame(k.Ptr(), k.Size(), (size_t)d_ * nk_).data()));
@@ -147,6 +152,9 @@ class NCSENSE {
   void Weights(const RealMatrix& w) { check(cgs_set_weights(plan_, per_frame(w.Ptr(), w.Size(), (size_t)nk_).data())); }
 
   Matrix Trafo(const Matrix& x) const {               // NCSENSE.hpp:310-348
+    size_t nx = (size_t)nmany_;
+    for (int v : N_) nx *= (size_t)v;
+    if ((size_t)x.Size() != nx) throw std::runtime_error("NCSENSE: image size does not match N x frames");
     Matrix y(data_dims());
     if (!cart_) {
       check(cgs_E(plan_, x.Ptr(), y.Ptr()));
@@ -185,14 +193,24 @@ class NCSENSE {
   cgs_plan plan() const { return plan_; }
 
  private:
+  // (N..., Nz) for slices; (N..., dim4[, dim5]) for frames (m_bwd_out, NCSENSE.hpp:172-179)
   std::vector<size_t> image_dims() const {
     std::vector<size_t> d(N_.begin(), N_.end());
-    if (nmany_ > 1) d.push_back((size_t)nmany_);
+    if (cart_) {
+      d.push_back((size_t)nmany_);
+    } else if (nmany_ > 1) {
+      d.push_back((size_t)dim4_);
+      if (dim5_ > 1) d.push_back((size_t)dim5_);
+    }
     return d;
   }
-  std::vector<size_t> data_dims() const {   // (nk, C*dim4), or (nk, Nz, C) for slices
+  // (nk, Nz, C) for slices; (nk, C[, dim4[, dim5]]) for frames (m_fwd_out, NCSENSE.hpp:171)
+  std::vector<size_t> data_dims() const {
     if (cart_) return std::vector<size_t>{(size_t)nk_, (size_t)nmany_, (size_t)C_};
-    return std::vector<size_t>{(size_t)nk_, (size_t)C_ * nmany_};
+    std::vector<size_t> d{(size_t)nk_, (size_t)C_};
+    if (dim4_ > 1) d.push_back((size_t)dim4_);
+    if (dim5_ > 1) d.push_back((size_t)dim5_);
+    return d;
   }
   // one block per frame when the caller gives one shared block
   template <class T>
@@ -210,6 +228,8 @@ class NCSENSE {
         for (int64_t j = 0; j < nk_; ++j) y[j + nk_ * (z + (int64_t)nmany_ * c)] = t[j + nk_ * (c + (int64_t)C_ * z)];
   }
   std::vector<cxfl> plan_data(const Matrix& y) const {
+    if ((size_t)y.Size() != (size_t)nk_ * C_ * nmany_)
+      throw std::runtime_error("NCSENSE: data size does not match nk x coils x frames");
     const cxfl* s = reinterpret_cast<const cxfl*>(y.Ptr());
     std::vector<cxfl> t(s, s + (size_t)nk_ * C_ * nmany_);
     if (cart_)
@@ -220,7 +240,7 @@ class NCSENSE {
   }
   cgs_plan plan_ = nullptr;
   bool cart_ = false;
-  int d_ = 2, C_ = 1, nmany_ = 1;
+  int d_ = 2, C_ = 1, nmany_ = 1, dim4_ = 1, dim5_ = 1;
   std::vector<int> N_;
   int64_t nk_ = 1;
   int ftiter_ = 3, cgiter_ = 0;

@@ -77,9 +77,11 @@ def normal(x, S, Khat, lam=0.0):
     return torch.sum(torch.conj(S).permute(2, 0, 1) * v, dim=0) + lam * x
 
 
-def cg(S, Khat, b, lam=0.0, maxit=100, tol=0.0, x0=None):
+def cg(S, Khat, b, lam=0.0, maxit=100, tol=0.0, x0=None, keep=None):
     """CG on (E^H W E + lam) x = b, FP64, norm-ratio stop ||r|| / ||b|| <= tol.
-    Returns (x, iterations, residual history)."""
+    Returns (x, iterations, residual history).  With keep (a dict whose keys
+    are iteration counts) the iterate after each of those counts is stored
+    under its key."""
     x = torch.zeros_like(b) if x0 is None else x0.clone().to(torch.complex128)
     r = b - normal(x, S, Khat, lam) if x0 is not None else b.clone()
     p = r.clone()
@@ -97,4 +99,6 @@ def cg(S, Khat, b, lam=0.0, maxit=100, tol=0.0, x0=None):
         rn = rn_new
         it += 1
         hist.append(math.sqrt(float(rn / bn)))
+        if keep is not None and it in keep:
+            keep[it] = x.clone()
     return x, it, hist

@@ -81,8 +81,41 @@ static int run_slices(const std::string& dir, size_t N, size_t C, size_t nk, dou
   return 0;
 }
 
+// GRASP-style frames: NCSENSE with dim4 x dim5 frames, per-frame trajectories
+// (2, nk, dim4, dim5), shared weights, cgiter 0 and ftiter 1, so that ->* is the
+// one-step reference E^H of every frame (GRASP.cpp:205-219 through
+// CS_XSENSE.hpp:93-114; NCSENSE.hpp:141-151, :204-236, :254-288)
+static int run_frames(const std::string& dir, size_t N, size_t C, size_t nk, double m, double alpha, size_t d4,
+                      size_t d5) {
+  codeare_amd::Params p;
+  p.set("nk", (double)nk);
+  p.set("m", m);
+  p.set("alpha", alpha);
+  p.set("ftiter", 1);
+  p.set("fteps", 7e-4);
+  p.set("cgiter", 0);
+  p.set("dim4", (double)d4);
+  p.set("dim5", (double)d5);
+  CMat S({N, N, C}), x({N, N, d4, d5}), y({nk, C, d4, d5});
+  RMat k({2, nk, d4, d5}), w({nk});
+  load(dir + "/Sf.bin", S);
+  load(dir + "/xf.bin", x);
+  load(dir + "/yf.bin", y);
+  load(dir + "/kf.bin", k);
+  load(dir + "/wf.bin", w);
+  codeare_amd::NCSENSE<CMat, RMat> ncs(p, S, {N, N, C});
+  ncs.KSpace(k);
+  ncs.Weights(w);
+  CMat e = ncs * x;
+  if (e.Size() != nk * C * d4 * d5) { std::fprintf(stderr, "frames: E has the wrong size\n"); return 4; }
+  save(dir + "/Ef.bin", e);
+  save(dir + "/EHf.bin", ncs / y);
+  save(dir + "/solvef.bin", ncs->*y);
+  return 0;
+}
+
 int main(int argc, char** argv) {
-  if (argc < 7) { std::fprintf(stderr, "usage: t_ncsense dir N C nk m alpha [Nz]\n"); return 2; }
+  if (argc < 7) { std::fprintf(stderr, "usage: t_ncsense dir N C nk m alpha [Nz [dim4 dim5]]\n"); return 2; }
   const std::string dir = argv[1];
   const size_t N = std::atoi(argv[2]), C = std::atoi(argv[3]), nk = std::atoi(argv[4]);
   codeare_amd::Params p;
@@ -133,6 +166,11 @@ int main(int argc, char** argv) {
     if (!threw) { std::fprintf(stderr, "bad trajectory not rejected\n"); return 3; }
     if (argc >= 8 && std::atoi(argv[7]) > 0)
       run_slices(dir, N, C, nk, std::atof(argv[5]), std::atof(argv[6]), (size_t)std::atoi(argv[7]));
+    if (argc >= 10 && std::atoi(argv[8]) > 0 && std::atoi(argv[9]) > 0) {
+      const int rc = run_frames(dir, N, C, nk, std::atof(argv[5]), std::atof(argv[6]), (size_t)std::atoi(argv[8]),
+                                (size_t)std::atoi(argv[9]));
+      if (rc != 0) return rc;
+    }
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

@@ -64,8 +64,31 @@ def test_cpp_ncsense_matches_oracle(ora, tmp_path):
     _save(os.path.join(d, "S3.bin"), S3, np.complex64)
     _save(os.path.join(d, "x3.bin"), x3, np.complex64)
     _save(os.path.join(d, "y3.bin"), y3, np.complex64)
-    print(f"t_ncsense N={N} C={C} nk={M} m={wl.m} alpha={wl.sigma} Nz={Nz}", flush=True)
-    r = subprocess.run([BIN, d, str(N), str(C), str(M), str(wl.m), str(wl.sigma), str(Nz)],
+    # GRASP-style frames: dim4 x dim5 = 2 x 2, per-frame golden-angle
+    # trajectories (2, nk, dim4, dim5), shared weights, data (nk, C, dim4, dim5)
+    d4, d5 = 2, 2
+    spokes, R = wl.spokes, wl.samples
+    kf = np.stack([data.radial_2d(spokes, R, True, f * spokes) for f in range(d4 * d5)], axis=-1)
+    kf = np.asfortranarray(kf.reshape((2, M, d4, d5), order="F"))
+    wf = data.density(kf[:, :, 0, 0], "k", R)
+    xf = np.asfortranarray(np.stack([np.roll(prob.image, 3 * f, axis=1) * (1 + 0.2 * f) for f in range(d4 * d5)],
+                                    axis=-1).reshape((N, N, d4, d5), order="F").astype(np.complex64))
+    Of = []
+    yf = np.zeros((M, C, d4, d5), dtype=np.complex64, order="F")
+    for f in range(d4 * d5):
+        l, n = f % d4, f // d4
+        Oi = ora.NFFT((N, N), M, wl.sigma, wl.m)
+        Oi.kspace(np.asfortranarray(kf[:, :, l, n]))
+        Of.append(Oi)
+        yf[:, :, l, n] = Oi.E(prob.maps, np.asfortranarray(xf[:, :, l, n]))
+    yf += (0.05 * np.abs(yf).max() * rng.normal(size=yf.shape)).astype(np.complex64)
+    _save(os.path.join(d, "Sf.bin"), prob.maps, np.complex64)
+    _save(os.path.join(d, "xf.bin"), xf, np.complex64)
+    _save(os.path.join(d, "yf.bin"), yf, np.complex64)
+    _save(os.path.join(d, "kf.bin"), kf, np.float32)
+    _save(os.path.join(d, "wf.bin"), wf, np.float32)
+    print(f"t_ncsense N={N} C={C} nk={M} m={wl.m} alpha={wl.sigma} Nz={Nz} dim4={d4} dim5={d5}", flush=True)
+    r = subprocess.run([BIN, d, str(N), str(C), str(M), str(wl.m), str(wl.sigma), str(Nz), str(d4), str(d5)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
 
@@ -104,3 +127,16 @@ def test_cpp_ncsense_matches_oracle(ora, tmp_path):
     e_s3 = rel(_load(os.path.join(d, "solve3.bin"), (N, N, Nz)), xo3)
     print(f"C++ facade slices: operators max {max(e3):.2e}, ->* {e_s3:.2e}")
     assert max(e3) <= OP_TOL and e_s3 <= 1e-4
+
+    # dim4 x dim5 frames: E and the one-step reference E^H (cgiter 0, ftiter 1)
+    Ef = _load(os.path.join(d, "Ef.bin"), (M, C, d4, d5))
+    EHf = _load(os.path.join(d, "EHf.bin"), (N, N, d4, d5))
+    Sf = _load(os.path.join(d, "solvef.bin"), (N, N, d4, d5))
+    ef = []
+    for f in range(d4 * d5):
+        l, n = f % d4, f // d4
+        eh = Of[f].EH_ref(prob.maps, wf, np.asfortranarray(yf[:, :, l, n]), ftiter=1, fteps=7e-4)
+        ef += [rel(Ef[:, :, l, n], Of[f].E(prob.maps, np.asfortranarray(xf[:, :, l, n]))),
+               rel(EHf[:, :, l, n], eh), rel(Sf[:, :, l, n], eh)]
+    print(f"C++ facade dim4 x dim5 frames (cgiter 0, ftiter 1): max {max(ef):.2e}")
+    assert max(ef) <= OP_TOL

@@ -171,6 +171,100 @@ def test_itrafo_vs_dense_twin(ora):
             assert rel(f.reshape(-1, order="F"), fd) <= 1e-8
 
 
+def _dense_nfft(P, N):
+    """Columns A e_i of the oracle's own NFFT (FP64), image index column-major."""
+    A = np.zeros((P.M, N * N), dtype=np.complex128)
+    for i in range(N * N):
+        e = np.zeros(N * N, dtype=np.complex128)
+        e[i] = 1.0
+        A[:, i] = P.trafo(e.reshape((N, N), order="F"))
+    return A
+
+
+def _krylov_lsq(B, rhs, k):
+    """argmin ||rhs - B u|| over u in K_k(B^H B, B^H rhs) (orthonormal Arnoldi
+    basis with re-orthogonalisation, then a dense least-squares solve)."""
+    G = B.conj().T @ B
+    s = B.conj().T @ rhs
+    Q = [s / np.linalg.norm(s)]
+    for _ in range(k - 1):
+        v = G @ Q[-1]
+        for _ in range(2):
+            for q in Q:
+                v = v - np.vdot(q, v) * q
+        Q.append(v / np.linalg.norm(v))
+    Q = np.stack(Q, axis=1)
+    c, *_ = np.linalg.lstsq(B @ Q, rhs, rcond=None)
+    return Q @ c
+
+
+def test_itrafo_is_the_krylov_minimiser(ora):
+    """NFFT3's weighted damped CGNR (NFFTTraits.hpp:122-138) is CGLS on
+    B = W^1/2 A W_hat^1/2 in u, with f_hat = W_hat^1/2 u: its k-th iterate
+    minimises ||W^1/2 (y - A f_hat)|| over f_hat in W_hat K_k(B^H B, B^H W^1/2 y).
+    A is the oracle's own FP64 NFFT (m = 3, sigma = 2) assembled column by
+    column, so this pins the recurrence -- where w and w_hat enter, the step
+    and direction updates -- without restating it.  Run long, the iterate is
+    the masked weighted least-squares solution."""
+    rng = np.random.default_rng(12)
+    N, M = 12, 180
+    k, w, y = _tiny_cgnr_setup(rng, N, M)
+    P = ora.NFFT((N, N), M, 2.0, 3)
+    P.kspace(k)
+    A = _dense_nfft(P, N)
+    f = rng.normal(size=M) + 1j * rng.normal(size=M)
+    assert rel(P.adjoint(f).reshape(-1, order="F"), A.conj().T @ f) <= 1e-12   # the CGNR's A^H is A's adjoint
+    what = ora.damping_mask(2, [N, N]).reshape(-1, order="F")
+    assert 0 < what.sum() < N * N                                                 # corners are damped to zero
+    sw = np.sqrt(w)
+    B = sw[:, None] * A * np.sqrt(what)[None, :]
+    for kk in (1, 2, 3, 4, 6):
+        fo, it, _ = P.itrafo(y, w, maxit=kk, eps=0.0)
+        assert it == kk
+        fk = np.sqrt(what) * _krylov_lsq(B, sw * y, kk)
+        assert rel(fo.reshape(-1, order="F"), fk) <= 1e-11
+    mask = what > 0
+    us, *_ = np.linalg.lstsq(B[:, mask], sw * y, rcond=None)
+    fs = np.zeros(N * N, dtype=np.complex128)
+    fs[mask] = us
+    fo, it, _ = P.itrafo(y, w, maxit=400, eps=0.0)
+    assert rel(fo.reshape(-1, order="F"), fs) <= 1e-6
+
+
+def test_itrafo_krylov_check_catches_a_misplaced_damping(ora):
+    """The check above has teeth: the same dense CGNR with w_hat dropped from
+    the f_hat update (a plausible slip) leaves the Krylov minimiser at once."""
+    rng = np.random.default_rng(12)
+    N, M = 12, 180
+    k, w, y = _tiny_cgnr_setup(rng, N, M)
+    P = ora.NFFT((N, N), M, 2.0, 3)
+    P.kspace(k)
+    A = _dense_nfft(P, N)
+    what = ora.damping_mask(2, [N, N]).reshape(-1, order="F")
+    sw = np.sqrt(w)
+    B = sw[:, None] * A * np.sqrt(what)[None, :]
+    for kk in (2, 3):
+        good, _ = ndft.cgnr_dense(A, y, w, what, kk, 0.0)
+        # planted mistake: f_hat += alpha p instead of alpha w_hat p
+        f = np.zeros(N * N, dtype=np.complex128)
+        r = y.copy()
+        z = A.conj().T @ (w * r)
+        dot_z = np.sum(what * np.abs(z) ** 2)
+        p = z.copy()
+        for _ in range(kk):
+            v = A @ (what * p)
+            alpha = dot_z / np.sum(w * np.abs(v) ** 2)
+            f = f + alpha * p
+            r = r - alpha * v
+            z = A.conj().T @ (w * r)
+            dz_new = np.sum(what * np.abs(z) ** 2)
+            p = (dz_new / dot_z) * p + z
+            dot_z = dz_new
+        fk = np.sqrt(what) * _krylov_lsq(B, sw * y, kk)
+        assert rel(good, fk) <= 1e-11
+        assert rel(f, fk) >= 1e-2
+
+
 def test_itrafo_eps_is_absolute(ora):
     rng = np.random.default_rng(13)
     N, M = 12, 180

@@ -175,3 +175,46 @@ def test_unet_training_step_deterministic_and_learns():
     print(f"loss {losses[0][0]:.4e} -> {losses[0][-1]:.4e}")
     assert np.array_equal(finals[0], finals[1])          # fixed-order reductions
     assert losses[0][-1] <= 0.3 * losses[0][0]
+
+
+@pytest.mark.gpu
+def test_unet_gradients_match_oracle_full_depth():
+    """The warm-start network's own depth (4 levels, 32 base channels) at
+    128^2: every layer's weight and bias gradient against the restatement with
+    the GPU's BF16 storage points, and against the FP64 network -- the GPU may
+    be no further from the FP64 gradients than the BF16 restatement is (BF16
+    rounding flips compound with depth, as in the forward pass)."""
+    rng = np.random.default_rng(32)
+    L, H, W, B = 4, 128, 128, 1
+    net, theta = _net(L, H, W, B, seed=13)
+    x = _rand_batch(rng, B, H, W)
+    t = _rand_batch(rng, B, H, W, 0.5)
+    loss = net.gradients(x, t)
+    g = net.grads()
+    ref = U.UNetRef(L, theta, bf16=True)
+    _, lo = ref.forward(x.astype(np.complex128), t.astype(np.complex128))
+    go = ref.backward()
+    r64 = U.UNetRef(L, theta, bf16=False)
+    r64.forward(x.astype(np.complex128), t.astype(np.complex128))
+    g64 = r64.backward()
+    print(f"loss GPU {loss:.6e} restatement {lo:.6e}")
+    assert abs(loss - lo) <= 1e-3 * lo
+
+    def e(a, b):
+        return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+    off = 0
+    worst = 0.0
+    for k, (ca, cb, co) in enumerate(U.layer_table(L)):
+        nw = 9 * co * (ca + cb)
+        for name, sl in (("W", slice(off, off + nw)), ("b", slice(off + nw, off + nw + co))):
+            e_bf, e_gpu64, e_emu64 = e(g[sl], go[sl]), e(g[sl], g64[sl]), e(go[sl], g64[sl])
+            worst = max(worst, e_bf)
+            print(f"layer {k} {name}: vs BF16 restatement {e_bf:.2e}; vs FP64 GPU {e_gpu64:.2e} "
+                  f"restatement {e_emu64:.2e}")
+            assert e_bf <= 5e-2
+            assert e_gpu64 <= 1.5 * e_emu64 + 1e-2
+        off += nw + co
+    eh = e(g[off:], go[off:])
+    print(f"head: rel err {eh:.2e}; worst layer {worst:.2e}")
+    assert eh <= 1e-2
