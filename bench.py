@@ -11,8 +11,10 @@ embedding (the same operator as E then E^H, to NFFT accuracy).
 The JSON line also carries: the same solve with the operator applied through
 E and E^H (window gather, FFTs, gridding scatter), the HIP-event roofline of
 the dominant kernel in the timed region, the NuFFT kernels' GB/s, the same
-problem with lambda scaled to 1e-3 of the operator norm, and a CPU baseline
-(the FP64 C port of the algorithm on the host cores, bounded sample, scaled).
+problem with lambda scaled to 1e-3 of the operator norm, codeare's own
+algorithm (CGLS with the reference E^H) on the GPU, a CPU baseline (the FP64 C
+port on the host cores, bounded sample, scaled) in both modes, and the error of
+the GPU images against the FP64 direct-NUDFT CG at equal iteration counts.
 """
 import argparse
 import json
@@ -39,7 +41,10 @@ def parse():
                          "secondary solves use at most 8")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary measurements")
-    ap.add_argument("--cpu-iters", type=int, default=2, help="CG iterations in the CPU sample")
+    ap.add_argument("--cpu-iters", type=int, default=10,
+                    help="timed CG iterations in the CPU sample (plus one warm-up); the median is scaled")
+    ap.add_argument("--ref-maxit", type=int, default=3000,
+                    help="iteration cap of the reference-algorithm (ref-compat CGLS) solve")
     ap.add_argument("--no-dynamic", action="store_true", help="skip the 64-frame batched series")
     ap.add_argument("--volume", dest="volume", action="store_true", default=None,
                     help="time the 3D koosh-ball workload (default: with the secondary measurements)")
@@ -140,6 +145,7 @@ def main():
     iters = [r["max_iters"] for r in reps]
     final_rel = reps[-1]["final_rel"][0]
     xr = x.cpu().numpy()
+    x_head = xr.copy()
     xt = x_true.cpu().numpy()
     err_phantom = float(np.linalg.norm(xr - xt) / np.linalg.norm(xt))
 
@@ -225,6 +231,8 @@ def main():
                                       "solves_per_s": round(1.0 / t_s, 3), "iters": rep["max_iters"],
                                       "final_rel_resid": rep["final_rel"][0],
                                       "rel_err_vs_phantom": round(float(np.linalg.norm(xr - xt) / np.linalg.norm(xt)), 5)}
+        scaled = (lam_s, xr.copy(), rep["max_iters"])
+        out["reference_algorithm"] = {"gpu": ref_compat_leg(args, plan, y, x, lam)}
 
         out["warm_start_solve"] = warm_start_leg(args, plan, y, x, tol, xt)
         if not args.no_dynamic:
@@ -233,8 +241,17 @@ def main():
         out["volume_3d"] = volume_leg(args)
 
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(prob, y_noisy, iters[-1], lam, args.cpu_iters)
-        out["speedup_vs_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 1)
+        ref = out.get("reference_algorithm", {}).get("gpu")
+        cb = cpu_baseline(prob, y_noisy, iters[-1], lam, args.cpu_iters,
+                          ref_iters_gpu=ref["iters"] if ref else None)
+        cref = cb.pop("reference_algorithm", None)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu_same_mode"] = round(out["value"] / cb["value"], 1)
+        if ref and cref:
+            out["reference_algorithm"]["cpu"] = cref
+            out["reference_algorithm"]["gpu_vs_cpu"] = round(ref["solves_per_s"] / cref["solves_per_s"], 1)
+        sc = scaled if not args.no_extra else (None, None, None)
+        out["error_vs_fp64_nudft"] = fp64_errors(prob, y_noisy, x_head, iters[-1], lam, sc[0], sc[1], sc[2])
     if args.copy_calibration:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -243,6 +260,33 @@ def main():
         torch.cuda.synchronize()
         out["copy_calibration_bytes"] = 2 * src.numel() * 4
     print(json.dumps(out))
+
+
+def ref_compat_leg(args, plan, y, x, lam, ftiter=3, fteps=1e-3, cgeps=1e-10):
+    """codeare's own solve on the GPU: CGLS (CGLS.hpp:30-59) with the reference
+    E^H (ftiter steps of the NFFT3 weighted, damped CGNR per coil, then conj(S),
+    coil sum and intensity map, NCSENSE.hpp:251-301), squared-ratio stop at
+    cgeps = 1e-10 (= rel-resid 1e-5 as a norm ratio), same lambda and data as
+    the headline; kernels launched one by one (no graph)."""
+    from codeare_amd import cgsense
+
+    def run(maxit):
+        _, rep = plan.solve(y, maxit=maxit, tol=cgeps, tol_kind=cgsense.CGS_TOL_SQUARED_RATIO, lam=lam,
+                            eh_mode=cgsense.CGS_EH_REF, ftiter=ftiter, fteps=fteps, out=x, check_nan=False)
+        return rep
+
+    plan.profile_mask([])
+    run(3)
+    plan.sync()
+    t0 = time.perf_counter()
+    rep = run(args.ref_maxit)
+    plan.sync()
+    el = time.perf_counter() - t0
+    plan.kernel_stats(reset=True)
+    return {"algorithm": f"CGLS, reference E^H (ftiter {ftiter}, fteps {fteps:g}), squared-ratio cgeps {cgeps:g}",
+            "iters": rep["max_iters"], "final_squared_ratio": rep["final_rel"][0],
+            "converged": bool(rep["final_rel"][0] <= cgeps), "maxit": args.ref_maxit,
+            "solves_per_s": round(1.0 / el, 4), "ms_per_iter": round(1e3 * el / max(1, rep["max_iters"]), 4)}
 
 
 def warm_start_leg(args, plan, y, x, tol, xt):
@@ -415,33 +459,134 @@ def volume_leg(args):
     return res
 
 
-def cpu_baseline(prob, y, gpu_iters, lam, cpu_iters):
-    """The oracle's FP64 CG on the same inputs (the port of the reference
-    algorithm: NFFT3-style FP64 NFFT with precomputed window, complex64 CG
-    vectors, one OpenMP thread per coil), timed on a bounded sample: E^H y plus
-    `cpu_iters` CG iterations, scaled to one solve of gpu_iters iterations."""
+def host_cpu():
+    """(nproc, CPU model, threads the CPU legs use): the job's thread allowance
+    (OMP_NUM_THREADS when set) or every core."""
+    nproc = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or nproc
+    except ValueError:
+        threads = nproc
+    return nproc, model, max(1, min(threads, nproc))
+
+
+def _median_iter(marks, skip=1):
+    """Median per-iteration time from the oracle's time marks, skipping the
+    first `skip` iterations (warm-up)."""
+    import numpy as np
+    d = np.diff(marks)
+    d = d[skip:] if d.size > skip else d
+    return float(np.median(d)), int(d.size)
+
+
+def cpu_baseline(prob, y, gpu_iters, lam, cpu_iters, ref_iters_gpu=None, ftiter=3, fteps=1e-3):
+    """The oracle (the FP64-NFFT C port of the reference path: PRE_PSI-style
+    precomputed window, FP32 CG vectors, OpenMP over coils) on the same inputs
+    and the host's cores, on a bounded sample: E^H y plus `cpu_iters`
+    iterations, each timed (steady clock); the first iteration is a warm-up and
+    the solve time is E^H + (the GPU's iteration count) x the median iteration.
+    Two modes, each against the GPU solve of the same mode: true adjoint (the
+    headline's system) and codeare's own algorithm (reference E^H = ftiter
+    steps of the NFFT3 CGNR per coil, CGLS with the squared-ratio stop,
+    NCSENSE.hpp:251-301, CGLS.hpp:30-59)."""
     from oracle import ora
     ora.build()
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    nproc, model, threads = host_cpu()
     ora.set_threads(threads)
     wl = prob.wl
     O = ora.NFFT((wl.N,) * wl.d, prob.k.shape[1], wl.sigma, wl.m)
     t0 = time.perf_counter()
     O.kspace(prob.k)
     t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.cg_true(prob.maps, prob.w, y, maxit=0, tol=0.0, lam=lam)
-    t_b = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.cg_true(prob.maps, prob.w, y, maxit=cpu_iters, tol=0.0, lam=lam)
-    t_k = time.perf_counter() - t0
-    t_iter = max(1e-9, (t_k - t_b) / cpu_iters)
-    t_solve = t_b + gpu_iters * t_iter
-    return {"value": round(1.0 / t_solve, 6), "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"E^H y + {cpu_iters} CG iterations of the same workload timed "
-                      f"({t_b:.2f} s + {cpu_iters}x{t_iter:.2f} s), scaled to the {gpu_iters} iterations the "
-                      f"GPU solve needed; trajectory setup {t_setup:.2f} s excluded",
-            "s_per_iter": round(t_iter, 3)}
+    O.cg_true(prob.maps, prob.w, y, maxit=1, tol=0.0, lam=lam)          # warm-up
+    O.cg_true(prob.maps, prob.w, y, maxit=cpu_iters + 1, tol=0.0, lam=lam)
+    mk = ora.time_marks()
+    t_b = float(mk[0])
+    t_it, n_it = _median_iter(mk)
+    t_solve = t_b + gpu_iters * t_it
+    out = {"value": round(1.0 / t_solve, 6), "unit": "solves/s", "cores": threads, "kind": "port",
+           "mode": "true adjoint (the headline's system, through E then E^H)",
+           "nproc": nproc, "cpu_model": model,
+           "sample": f"E^H W y ({t_b:.3f} s) + {n_it + 1} CG iterations of the same 256^2 x 32-coil workload, "
+                     f"each timed; median iteration {t_it:.3f} s (first one dropped as warm-up) x the "
+                     f"{gpu_iters} iterations the GPU solve needed; trajectory setup {t_setup:.2f} s excluded",
+           "eh_s": round(t_b, 4), "s_per_iter_median": round(t_it, 4)}
+    if ref_iters_gpu:
+        O.cgls_ref(prob.maps, prob.w, y, mode=1, ftiter=ftiter, fteps=fteps, maxit=1, cgeps=0.0, lam=lam)
+        nref = max(3, cpu_iters // 2)
+        O.cgls_ref(prob.maps, prob.w, y, mode=1, ftiter=ftiter, fteps=fteps, maxit=nref + 1, cgeps=0.0, lam=lam)
+        mk = ora.time_marks()
+        tb_r = float(mk[0])
+        ti_r, n_r = _median_iter(mk)
+        out["reference_algorithm"] = {
+            "eh_s": round(tb_r, 4), "s_per_iter_median": round(ti_r, 4), "iters_timed": n_r,
+            "solves_per_s": round(1.0 / (tb_r + ref_iters_gpu * ti_r), 6),
+            "scaled_to_iters": ref_iters_gpu}
+    return out
+
+
+def fp64_errors(prob, y_noisy, x_gpu, iters, lam, lam_s=None, x_gpu_s=None, iters_s=None):
+    """Error of the GPU images against the FP64 direct-NUDFT CG
+    (oracle/nudft_torch.py) at the GPU's own iteration counts: the headline
+    solve, the scaled-lambda solve and workload 1 (128^2, 8 coils, 10 CG
+    iterations, lambda 0); for the headline also the part its data determine
+    (DTFT inside the sampled disk |k| <= 0.5 on the 2N grid)."""
+    import numpy as np
+    import torch
+    from codeare_amd import cgsense, data
+    from oracle import nudft_torch as T
+    dev = torch.device("cuda:0")
+    wl = prob.wl
+    N = wl.N
+    S = torch.from_numpy(np.ascontiguousarray(prob.maps.astype(np.complex128))).to(dev)
+    b = T.eh(prob.k, prob.w, y_noisy, S)
+    Kh = T.kernel_spectrum(prob.k, prob.w, N, N, dev)
+
+    def rel(a, b_):
+        a = torch.as_tensor(a).to(dev).to(torch.complex128).reshape(-1)
+        b_ = b_.reshape(-1)
+        return float(torch.linalg.vector_norm(a - b_) / torch.linalg.vector_norm(b_))
+
+    def disk(x):
+        X = torch.fft.fft2(x, s=(2 * N, 2 * N))
+        f = torch.fft.fftfreq(2 * N, device=dev, dtype=torch.float64)
+        return X[(f[:, None] ** 2 + f[None, :] ** 2) <= 0.25]
+
+    xg = torch.from_numpy(np.ascontiguousarray(x_gpu.reshape((N, N), order="F").astype(np.complex128))).to(dev)
+    xo, n, _ = T.cg(S, Kh, b, lam, maxit=iters, tol=0.0)
+    bn = float(torch.linalg.vector_norm(b))
+    out = {"headline": {"iters": n, "rel_l2": round(rel(xg, xo), 6),
+                        "rel_l2_inside_sampled_disk": round(float(torch.linalg.vector_norm(disk(xg) - disk(xo)) /
+                                                                  torch.linalg.vector_norm(disk(xo))), 8),
+                        "true_rel_resid_fp64": float(torch.linalg.vector_norm(b - T.normal(xg, S, Kh, lam))) / bn}}
+    if x_gpu_s is not None:
+        xgs = torch.from_numpy(np.ascontiguousarray(x_gpu_s.reshape((N, N), order="F").astype(np.complex128))).to(dev)
+        xos, ns, _ = T.cg(S, Kh, b, lam_s, maxit=iters_s, tol=0.0)
+        out["scaled_lambda"] = {"iters": ns, "rel_l2": round(rel(xgs, xos), 8)}
+    # workload 1
+    p1 = data.make_problem(1, 0)
+    w1 = p1.wl
+    P1 = cgsense.Plan((w1.N, w1.N), p1.k.shape[1], w1.sigma, w1.m, ncoils=w1.coils)
+    P1.kspace(p1.k)
+    P1.weights(p1.w)
+    P1.sensitivities(p1.maps)
+    y1 = data.make_data(p1, P1.E)
+    x1, _ = P1.solve(y1, maxit=w1.maxit, tol=0.0, lam=w1.lam)
+    S1 = torch.from_numpy(np.ascontiguousarray(p1.maps.astype(np.complex128))).to(dev)
+    b1 = T.eh(p1.k, p1.w, y1, S1)
+    K1 = T.kernel_spectrum(p1.k, p1.w, w1.N, w1.N, dev)
+    xo1, n1, _ = T.cg(S1, K1, b1, w1.lam, maxit=w1.maxit, tol=0.0)
+    out["workload1_cg10"] = {"iters": n1, "rel_l2": round(rel(np.ascontiguousarray(x1), xo1), 8)}
+    return out
 
 
 if __name__ == "__main__":

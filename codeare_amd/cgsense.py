@@ -42,7 +42,7 @@ EXPORTED = [
     "cgs_plan_create", "cgs_plan_destroy", "cgs_set_stream", "cgs_sync",
     "cgs_set_trajectory", "cgs_set_weights", "cgs_set_sensitivities",
     "cgs_nfft_forward", "cgs_nfft_adjoint", "cgs_nfft_inverse_ref",
-    "cgs_E", "cgs_EH", "cgs_normal", "cgs_solve",
+    "cgs_E", "cgs_EH", "cgs_normal", "cgs_solve", "cgs_solve_noalloc",
     "cgs_kernel_stats", "cgs_plan_info", "cgs_status_string", "cgs_last_error",
     "cgs_residual_history", "cgs_autotune", "cgs_profile_mask",
 ]
@@ -65,7 +65,7 @@ class TuneReport(ct.Structure):
     _fields_ = [("visits_per_item", ct.c_int), ("grid_coils", ct.c_int), ("interp_coils", ct.c_int),
                 ("from_cache", ct.c_int), ("grid_ms", ct.c_double), ("interp_ms", ct.c_double),
                 ("default_grid_ms", ct.c_double), ("default_interp_ms", ct.c_double),
-                ("grid_mfma", ct.c_int)]
+                ("grid_mfma", ct.c_int), ("grid_sweep", ct.c_int)]
 
 
 class SolveReport(ct.Structure):
@@ -102,6 +102,7 @@ def load_library(path: str | None = None):
     L.cgs_EH.argtypes = [P, P, P, I, I, D]
     L.cgs_normal.argtypes = [P, P, P, D, I]
     L.cgs_solve.argtypes = [P, P, P, P, ct.POINTER(SolveOpts), ct.POINTER(SolveReport)]
+    L.cgs_solve_noalloc.argtypes = [P, P, P, P, ct.POINTER(SolveOpts), ct.POINTER(SolveReport)]
     L.cgs_kernel_stats.argtypes = [P, I, ct.POINTER(I64), ct.POINTER(D), ct.POINTER(D), I]
     L.cgs_plan_info.argtypes = [P, ct.POINTER(I), ct.POINTER(I), ct.POINTER(I64), ct.POINTER(I)]
     L.cgs_residual_history.argtypes = [P, I, P, I]
@@ -289,15 +290,18 @@ class Plan:
     def solve(self, y, x0=None, maxit: int = 10, tol: float = 1e-5, tol_kind: int = CGS_TOL_NORM_RATIO,
               lam: float = 0.0, eh_mode: int = CGS_EH_TRUE, ftiter: int = 3, fteps: float = 1e-3,
               out=None, check_nan: bool = True, toeplitz: bool = False, graph_iters: int = 0,
-              joint: bool = False):
+              joint: bool = False, noalloc: bool = False):
+        """CG solve (cgs_solve); with noalloc the allocation-free variant
+        cgs_solve_noalloc (device-I/O plans, graph_iters 0)."""
         L = load_library()
+        fn = L.cgs_solve_noalloc if noalloc else L.cgs_solve
         o = SolveOpts(int(maxit), float(tol), int(tol_kind), float(lam), int(eh_mode), int(ftiter),
                       float(fteps), CGS_NORMAL_TOEPLITZ if toeplitz else CGS_NORMAL_NUFFT,
                       int(graph_iters), int(bool(joint)))
         rep = SolveReport()
         if self.device_io:
-            rc = L.cgs_solve(self._h, _ptr(y), _ptr(x0) if x0 is not None else None, _ptr(out),
-                             ct.byref(o), ct.byref(rep))
+            rc = fn(self._h, _ptr(y), _ptr(x0) if x0 is not None else None, _ptr(out),
+                    ct.byref(o), ct.byref(rep))
             x = out
         else:
             y = _c64(y)
@@ -306,7 +310,7 @@ class Plan:
             if x0 is not None:
                 x0 = _c64(x0)
                 x0p = _ptr(x0)
-            rc = L.cgs_solve(self._h, _ptr(y), x0p, _ptr(x), ct.byref(o), ct.byref(rep))
+            rc = fn(self._h, _ptr(y), x0p, _ptr(x), ct.byref(o), ct.byref(rep))
         if rc != CGS_ERR_NAN or check_nan:
             _check(rc)
         F = min(self.F, 1024)

@@ -258,6 +258,21 @@ struct GridArgs {
   int64_t ntot, tilecells;
 };
 
+constexpr int kSweepPad = 16;   // padding records after the line-sweep entries (>= the chunk size)
+
+// 2D gridding as a sweep along grid lines (kernels_sweep.hip): one wave per
+// (frame, line l1, segment of W cells along axis 0)
+struct SweepArgs {
+  const int4* items;      // {entry begin, entry end, frame, l1 * nseg + segment}
+  const int4* ent;        // {sorted sample, r0, bits(phi1[t1]), bits(phi1[t1] * w)}, by r0 per item
+  const float* psi_rot;   // [s * taps + k]: axis-0 window value of the tap on a cell == k mod taps
+  const float2* y;        // Y[s * C + c]
+  float2* grid;
+  const int* fdone;
+  int nitems, taps, C, W, nseg, n0, weighted;
+  int64_t ntot;
+};
+
 struct FftTab {          // tables of one FFT length
   int n = 0;
   float2* tw = nullptr;   // exp(-2 pi i p / n)
@@ -285,6 +300,9 @@ struct Plan {
   bool use_grid_mfma = false;         // 2D: scatter on the matrix cores (k_grid_mfma)
   int grid_cgr = 16;                  // coils per wave in the register kernel
   bool use_grid_atomic = false;       // LDS float atomics variant (kept for comparison)
+  bool use_grid_sweep = false;        // 2D: line-sweep scatter (kernels_sweep.hip) where supported
+  bool sweep_allowed = false;         // build the line-sweep tables (CGS_GRID_SWEEP=1)
+  int sweep_W = 64;                   // cells per line segment (a multiple of taps)
   int tile_item_max = 64;             // samples per tile-level interp item
   int interp3_threads = 1024;         // 3D LDS gather: threads per block
 
@@ -325,6 +343,13 @@ struct Plan {
   int64_t n_visits = 0;
   int n_slots = 0;
   float2* partial = nullptr;
+  // line-sweep scatter tables (2D)
+  int4* sw_items = nullptr;
+  int n_sw_items = 0, sw_nseg = 0;
+  int4* sw_ent = nullptr;
+  int64_t n_sw_ent = 0;
+  float* psi_rot = nullptr;
+  std::vector<int4> h_sw_ent;         // host copy (the weighted field is refilled by cgs_set_weights)
 
   // work buffers
   float2* grid = nullptr;             // F*ntot*C
@@ -357,6 +382,16 @@ struct Plan {
   std::vector<float> h_w;             // weights copy (host, original order)
   CgState* h_cgst = nullptr;          // pinned mirror
   int* h_done = nullptr;              // pinned
+
+  // solve bookkeeping without allocation in the call: polling events and
+  // per-frame completion times made at plan creation; the history the last
+  // solve wrote (the plan's buffer, or the allocation-free solve's scratch)
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
+  std::vector<double> done_ms;
+  double* last_hist = nullptr;
+  int last_hist_cap = 0;
+  bool no_prof = false;               // no timing events (allocation-free solve)
+  int device = 0;
 
   // captured CG iterations (replayed by graph-mode solves)
   bool capturing = false;
@@ -419,6 +454,9 @@ void launch_grid_atomic(const GridArgs& a, const int4* vrec, int cg, int nitems,
 void launch_grid_reg(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s);
 void launch_grid_mfma(const GridArgs& a, const int4* vrec, int nitems, hipStream_t s);
 void launch_grid_mfma3(const GridArgs& a, const int4* vrec, int nitems, hipStream_t s);
+int sweep_groups(int C);
+bool grid_sweep_supported(int taps, int C);
+void launch_grid_sweep(const SweepArgs& a, hipStream_t s);
 size_t interp_lds3_bytes(int T, int taps, int cg, int maxs);
 void launch_interp_lds3(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int nt1, int cg, int maxs,
                         int threads, hipStream_t s);
@@ -472,6 +510,7 @@ void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 void launch_toep_layout(const float2* Khat, float* That, int64_t cells, int nl, int F, float scale,
                         hipStream_t s);
 void launch_fill_int(int* dst, int value, int n, hipStream_t s);
+void launch_zero(void* dst, size_t bytes, hipStream_t s);   // bytes a multiple of 4
 
 // errors
 void set_error(const std::string& msg);

@@ -6,6 +6,8 @@
 // The CG scalars (rn, <p,q>, alpha, beta, residual) live on the device in FP64
 // (CgState); vector arithmetic stays FP32.  Every kernel skips frames whose
 // done flag is set, so finished frames cost only a flag read.
+#include <algorithm>
+
 #include "cgs_internal.hpp"
 
 namespace cgs {
@@ -873,6 +875,19 @@ __global__ void k_fill_int(int* dst, int v, int n) {
 }
 void launch_fill_int(int* dst, int value, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, s, dst, value, n);
+}
+
+// zero-fill of n 32-bit words (a kernel, not a memset node: safe where a
+// caller captures the stream into a graph)
+__global__ void k_zero_u32(uint32_t* dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = 0u;
+}
+void launch_zero(void* dst, size_t bytes, hipStream_t s) {
+  const int64_t n = (int64_t)(bytes / 4);
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t*)dst, n);
 }
 
 }  // namespace cgs

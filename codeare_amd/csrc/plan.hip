@@ -144,7 +144,7 @@ struct Prof {
            k == CGS_K_TOEP_CROP;
   }
   Prof(Plan& p_, int kind_, double bytes_) : p(p_), kind(kind_), bytes(bytes_) {
-    if (!(p.flags & CGS_PROFILE) || p.capturing) return;
+    if (!(p.flags & CGS_PROFILE) || p.capturing || p.no_prof) return;
     if (!((p.prof_mask >> kind) & 1u)) {
       p.stat[kind].launches += 1;
       p.stat[kind].bytes += bytes;
@@ -436,6 +436,17 @@ static int gridding(Plan& p, const float2* y, bool weighted, const int* fl) {
   const double bytes = (double)p.Mtot * p.d * 4 + (double)p.Mtot * 4 + (double)p.C * p.Mtot * 8 +
                        (double)p.C * p.ntot * p.F * 8;
   Prof pr(p, CGS_K_GRID, bytes);
+  if (p.d == 2 && p.use_grid_sweep && p.sw_items) {
+    SweepArgs sa{};
+    sa.items = p.sw_items; sa.ent = p.sw_ent; sa.psi_rot = p.psi_rot;
+    sa.y = y; sa.grid = p.grid; sa.fdone = fl;
+    sa.nitems = p.n_sw_items; sa.taps = p.taps; sa.C = p.C;
+    sa.W = std::max(1, p.sweep_W / p.taps) * p.taps;
+    sa.nseg = p.sw_nseg; sa.n0 = p.n[0]; sa.weighted = weighted ? 1 : 0;
+    sa.ntot = p.ntot;
+    launch_grid_sweep(sa, p.stream);
+    return CGS_OK;
+  }
   if (p.d == 2 && p.T == 8 && p.use_grid_mfma) {
     launch_grid_mfma(a, p.vrec, p.n_grid_items, p.stream);
   } else if (p.d == 2 && p.T == 8 && p.use_grid_reg) {
@@ -502,7 +513,7 @@ static int cgnr(Plan& p, const float2* y, int maxit, double eps, const int* fl) 
     Prof pr(p, CGS_K_CG, 3.0 * xbytes);
     launch_cnr_dot_image(p, p.Xc, true, p.cnr + 1 * FC, p.stream);             // dot_z
     HIPCHK(hipMemcpyAsync(p.Xc3, p.Xc, xbytes, hipMemcpyDeviceToDevice, p.stream));   // p = z
-    HIPCHK(hipMemsetAsync(p.Xc2, 0, xbytes, p.stream));                          // f_hat = 0
+    launch_zero(p.Xc2, xbytes, p.stream);                                       // f_hat = 0
   }
   for (int l = 0; l < maxit; ++l) {
     launch_cnr_scalars(p, 0, eps, p.stream);                                   // dot_r < eps -> stop
@@ -550,7 +561,16 @@ static void toep_drop(Plan& p);
 static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep);
 
 // ---- trajectory presort -----------------------------------------------------------------
-static void free_samples(Plan& p) {
+static void free_sweep(Plan& p) {
+  dfree(p.sw_items); dfree(p.sw_ent); dfree(p.psi_rot);
+  p.n_sw_items = 0; p.n_sw_ent = 0; p.sw_nseg = 0;
+  p.h_sw_ent.clear();
+  p.h_sw_ent.shrink_to_fit();
+}
+
+// frees the tile-sorted sample tables (and, with sweep, the line-sweep tables)
+static void free_samples(Plan& p, bool sweep = true) {
+  if (sweep) free_sweep(p);
   dfree(p.base); dfree(p.psi); dfree(p.perm); dfree(p.sframe);
   dfree(p.interp_items); dfree(p.tile_items); dfree(p.grid_items);
   p.n_tile_items = 0; dfree(p.reduce_items); dfree(p.visits); dfree(p.vrec); dfree(p.partial);
@@ -559,7 +579,133 @@ static void free_samples(Plan& p) {
   p.n_slots = 0;
 }
 
-static int build_samples(Plan& p, const float* k) {
+// Line-sweep scatter tables (2D, kernels_sweep.hip).  Work items are the
+// (frame, line l1, segment [S, S + W)) triples, all of them (an item without
+// entries writes its cells' zeros).  An entry is a sample whose footprint covers
+// line l1 and meets the segment, with its first footprint cell r0 relative to
+// S in [-(taps - 1), W): for each segment the footprint (mod n0) meets, the
+// aliases b0 - S and b0 - S -/+ n0 that fall in that range (two of them only
+// when the footprint wraps around a grid shorter than W + taps).  Entries are
+// stable-sorted by r0 (sample order within one r0); items heaviest first.
+static int build_sweep(Plan& p, const std::vector<int>& hbase, const std::vector<float>& hpsi,
+                       const std::vector<int>& hframe) {
+  const int taps = p.taps;
+  if (p.d != 2 || !p.sweep_allowed || p.n[0] % taps != 0 || !grid_sweep_supported(taps, p.C)) return CGS_OK;
+  const int n0 = p.n[0], n1 = p.n[1];
+  const int W = std::max(1, p.sweep_W / taps) * taps;
+  const int nseg = (n0 + W - 1) / W;
+  const int64_t M = p.Mtot;
+  const int64_t nitems = (int64_t)p.F * n1 * nseg;
+  if (nitems > INT32_MAX / 2 || M * taps * 4 > INT32_MAX) {
+    set_error("line-sweep tables exceed 32-bit indices");
+    return CGS_ERR_UNSUPPORTED;
+  }
+  // segments and start cells of one sample's axis-0 footprint
+  auto for_segments = [&](int b0, auto&& fn) {
+    int sg[2], ns = 0;
+    const int ends[2] = {b0, (b0 + taps - 1) % n0};
+    for (int z = 0; z < 2; ++z) {
+      const int v = ends[z] / W;
+      if (ns == 0 || sg[0] != v) sg[ns++] = v;
+    }
+    for (int z = 0; z < ns; ++z) {
+      const int S = sg[z] * W, E = std::min(W, n0 - S);
+      const int r = b0 - S;
+      const int cand[3] = {r, r - n0, r + n0};
+      for (int q = 0; q < 3; ++q)
+        if (cand[q] >= -(taps - 1) && cand[q] < E) fn(sg[z], cand[q]);
+    }
+  };
+  std::vector<int64_t> cnt((size_t)nitems + 1, 0);
+  for (int64_t s = 0; s < M; ++s) {
+    const int b0 = hbase[s * 2], b1 = hbase[s * 2 + 1];
+    const int64_t fb = (int64_t)hframe[s] * n1;
+    for (int t1 = 0; t1 < taps; ++t1) {
+      const int l1 = (b1 + t1) % n1;
+      for_segments(b0, [&](int sg, int) { cnt[(size_t)((fb + l1) * nseg + sg) + 1]++; });
+    }
+  }
+  for (int64_t q = 0; q < nitems; ++q) cnt[q + 1] += cnt[q];
+  const int64_t nent = cnt[nitems];
+  if (nent > INT32_MAX) {
+    set_error("too many line-sweep entries for 32-bit indices");
+    return CGS_ERR_UNSUPPORTED;
+  }
+  // entries grouped by item in sample order, then stable-sorted by r0 per item
+  std::vector<int4> ent((size_t)nent);
+  {
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t s = 0; s < M; ++s) {
+      const int b0 = hbase[s * 2], b1 = hbase[s * 2 + 1];
+      const int64_t fb = (int64_t)hframe[s] * n1;
+      for (int t1 = 0; t1 < taps; ++t1) {
+        const int l1 = (b1 + t1) % n1;
+        const float w1 = hpsi[(s * 2 + 1) * taps + t1];
+        for_segments(b0, [&](int sg, int r0) {
+          const float wf = w1;
+          int4 e;
+          e.x = (int)s;
+          e.y = r0;
+          std::memcpy(&e.z, &wf, 4);
+          std::memcpy(&e.w, &wf, 4);
+          ent[(size_t)pos[(size_t)((fb + l1) * nseg + sg)]++] = e;
+        });
+      }
+    }
+  }
+  std::vector<int> order((size_t)nitems);
+  for (int64_t q = 0; q < nitems; ++q) {
+    order[q] = (int)q;
+    std::stable_sort(ent.begin() + cnt[q], ent.begin() + cnt[q + 1],
+                     [](const int4& a, const int4& b) { return a.y < b.y; });
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return cnt[a + 1] - cnt[a] > cnt[b + 1] - cnt[b]; });
+  std::vector<int4> items((size_t)nitems);
+  for (int64_t q = 0; q < nitems; ++q) {
+    const int64_t k = order[q];
+    const int f = (int)(k / ((int64_t)n1 * nseg));
+    const int line = (int)(k % ((int64_t)n1 * nseg));
+    items[q] = make_int4((int)cnt[k], (int)cnt[k + 1], f, line);
+  }
+  // axis-0 window values rotated to the slot (cell mod taps) each tap lands on
+  std::vector<float> rot((size_t)M * taps);
+  for (int64_t s = 0; s < M; ++s) {
+    const int b0 = hbase[s * 2];
+    for (int k = 0; k < taps; ++k) rot[s * taps + k] = hpsi[(s * 2) * taps + (((k - b0) % taps) + taps) % taps];
+  }
+  // kSweepPad padding records (sample 0, zero weights) after the last entry:
+  // the kernel loads whole chunks of records and skips those past an item's end
+  ent.resize(ent.size() + kSweepPad, make_int4(0, 0, 0, 0));
+  CHK(dalloc(&p.sw_items, items.size()));
+  CHK(dalloc(&p.sw_ent, ent.size()));
+  CHK(dalloc(&p.psi_rot, rot.size()));
+  HIPCHK(hipMemcpy(p.sw_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
+  if (!ent.empty()) HIPCHK(hipMemcpy(p.sw_ent, ent.data(), sizeof(int4) * ent.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.psi_rot, rot.data(), sizeof(float) * rot.size(), hipMemcpyHostToDevice));
+  p.n_sw_items = (int)nitems;
+  p.n_sw_ent = nent;
+  p.sw_nseg = nseg;
+  p.h_sw_ent = std::move(ent);
+  return CGS_OK;
+}
+
+// the weighted field of the line-sweep entries: phi1 * w of the entry's sample
+static int sweep_weights(Plan& p, const std::vector<float>& ws) {
+  if (!p.sw_ent || p.h_sw_ent.empty()) return CGS_OK;
+  for (auto& e : p.h_sw_ent) {
+    float w1;
+    std::memcpy(&w1, &e.z, 4);
+    const float wf = w1 * ws[(size_t)e.x];
+    std::memcpy(&e.w, &wf, 4);
+  }
+  HIPCHK(hipMemcpy(p.sw_ent, p.h_sw_ent.data(), sizeof(int4) * p.h_sw_ent.size(), hipMemcpyHostToDevice));
+  return CGS_OK;
+}
+
+// sweep: also (re)build the line-sweep tables, which depend on the trajectory
+// only (not on the visit-list chunking the autotuner varies)
+static int build_samples(Plan& p, const float* k, bool sweep = true) {
   const int d = p.d, m = p.m, taps = p.taps, T = p.T;
   const int64_t M = p.Mtot;
   // base cell and fractional offset of every sample (FP64)
@@ -732,7 +878,7 @@ static int build_samples(Plan& p, const float* k) {
   std::stable_sort(gitems.begin(), gitems.end(),
                    [](const GridItem& x, const GridItem& y) { return (x.vend - x.vbeg) > (y.vend - y.vbeg); });
 
-  free_samples(p);
+  free_samples(p, sweep);
   CHK(dalloc(&p.base, hbase.size()));
   CHK(dalloc(&p.psi, hpsi.size()));
   CHK(dalloc(&p.perm, perm.size()));
@@ -760,6 +906,7 @@ static int build_samples(Plan& p, const float* k) {
   if (!ritems.empty())
     HIPCHK(hipMemcpy(p.reduce_items, ritems.data(), sizeof(ReduceItem) * ritems.size(), hipMemcpyHostToDevice));
   if (!vis.empty()) HIPCHK(hipMemcpy(p.visits, vis.data(), sizeof(int) * vis.size(), hipMemcpyHostToDevice));
+  if (sweep) CHK(build_sweep(p, hbase, hpsi, hframe));
   p.n_interp_items = (int)items.size();
   p.n_grid_items = (int)gitems.size();
   p.n_reduce_items = (int)ritems.size();
@@ -788,6 +935,10 @@ static void destroy_plan(Plan& p) {
   p.pending.clear();
   if (p.graph_exec) (void)hipGraphExecDestroy(p.graph_exec);
   p.graph_exec = nullptr;
+  for (auto& e : p.poll_ev) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
   for (auto e : p.event_pool) (void)hipEventDestroy(e);
   p.event_pool.clear();
   if (p.own_stream && p.stream) (void)hipStreamDestroy(p.stream);
@@ -877,11 +1028,17 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
     if (g == 8 || g == 16 || g == 32) p.grid_cgr = g;
   }
   if (const char* e = std::getenv("CGS_GRID_ATOMIC")) p.use_grid_atomic = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CGS_GRID_SWEEP")) p.use_grid_sweep = p.sweep_allowed = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CGS_SWEEP_W")) p.sweep_W = std::max(4, std::atoi(e));
 
   if (flags & 0x8000u) {   // internal: caller-provided stream set later
   }
+  HIPCHK(hipGetDevice(&p.device));
   HIPCHK(hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking));
   p.own_stream = true;
+  HIPCHK(hipEventCreateWithFlags(&p.poll_ev[0], hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&p.poll_ev[1], hipEventDisableTiming));
+  p.done_ms.assign((size_t)F, -1.0);
 
   // tables
   p.sigma = sigma;
@@ -1342,24 +1499,18 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
   const int F = p.F;
   const bool joint = o.joint != 0 && F > 1;
   p.beta_folded = false;
-  std::vector<double> done_ms((size_t)F, -1.0);
+  std::vector<double>& done_ms = p.done_ms;
+  std::fill(done_ms.begin(), done_ms.end(), -1.0);
   const size_t ib = sizeof(float2) * (size_t)F * p.Ntot;
   float2 *x = p.img[0], *r = p.img[1], *pv = p.img[2], *q = p.img[3];
   double* partA = p.partials;
   double* partB = p.partials + 512 * F;
-  const int hcap = o.maxit + 1;
-  if (hcap > p.hist_cap) {
-    // a captured CG graph writes the history through the old pointer and
-    // capacity: drop it with the buffer it refers to
-    if (p.graph_exec) (void)hipGraphExecDestroy(p.graph_exec);
-    p.graph_exec = nullptr;
-    p.graph_key.clear();
-    dfree(p.hist);
-    CHK(dalloc(&p.hist, (size_t)F * hcap));
-    p.hist_cap = hcap;
-  }
-  HIPCHK(hipMemsetAsync(p.hist, 0, sizeof(double) * F * p.hist_cap, p.stream));
-  HIPCHK(hipMemsetAsync(p.fdone, 0, sizeof(int) * F, p.stream));
+  if (o.maxit + 1 > p.hist_cap) { set_error("residual history smaller than maxit + 1"); return CGS_ERR_STATE; }
+  p.last_hist = p.hist;
+  p.last_hist_cap = p.hist_cap;
+  // fills are kernels (no memset nodes where a caller captures the stream)
+  launch_zero(p.hist, sizeof(double) * F * p.hist_cap, p.stream);
+  launch_fill_int(p.fdone, 0, F, p.stream);
   {
     Prof pr(p, CGS_K_OTHER, 2.0 * sizeof(float2) * p.Mtot * p.C);
     launch_y_ext2int(p, y_ext, p.ybuf[0], p.C, -1, p.stream);
@@ -1380,16 +1531,14 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
     launch_axpby_img(p, r, r, 1.f, q, -1.f, p.stream);          // r = b - (A + lambda) x0
     launch_sqnorm_partials(p, r, partB, p.stream);
   } else {
-    HIPCHK(hipMemsetAsync(x, 0, ib, p.stream));
+    launch_zero(x, ib, p.stream);
     launch_sqnorm_partials(p, r, partA, p.stream);
   }
   launch_cg_init(p, partA, partB, o.tol_kind, o.tol, o.maxit, warm, joint, p.stream);
   HIPCHK(hipMemcpyAsync(pv, r, ib, hipMemcpyDeviceToDevice, p.stream));
   CHK(check_launch());
 
-  hipEvent_t ev[2];
-  HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  hipEvent_t* ev = p.poll_ev;   // made at plan creation
   int status = CGS_OK;
   float2* pbuf[2] = {pv, p.img[6]};   // p double buffer for the fused direction update
   int step = 1;
@@ -1453,10 +1602,8 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
     launch_cg_beta_after_xr(p, cg_tail_blocks(p), o.tol_kind, o.tol, o.maxit, p.stream);
   HIPCHK(hipMemcpyAsync(xout, x, ib, hipMemcpyDeviceToDevice, p.stream));
   HIPCHK(hipMemcpyAsync(p.h_cgst, p.cgst, sizeof(CgState) * F, hipMemcpyDeviceToHost, p.stream));
-  HIPCHK(hipMemsetAsync(p.fdone, 0, sizeof(int) * F, p.stream));
+  launch_fill_int(p.fdone, 0, F, p.stream);
   HIPCHK(hipStreamSynchronize(p.stream));
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
   const double t_end = ms_since(t_start);
   for (int f = 0; f < F; ++f)
     if (done_ms[f] < 0) done_ms[f] = t_end;
@@ -1581,6 +1728,7 @@ struct TuneChoice {
   int vpi, cgr, icg;
   double grid_ms, interp_ms, def_grid_ms, def_interp_ms;
   bool mfma;
+  bool sweep;
 };
 static std::mutex g_tune_mu;
 static std::vector<std::pair<std::string, TuneChoice>> g_tune_cache;
@@ -1618,7 +1766,7 @@ static int apply_vpi(Plan& p, int vpi) {
   if (p.visits_per_item == vpi) return CGS_OK;
   p.visits_per_item = vpi;
   HIPCHK(hipStreamSynchronize(p.stream));
-  CHK(build_samples(p, p.h_k.data()));
+  CHK(build_samples(p, p.h_k.data(), false));
   if (p.have_w) {   // weights are stored in the sorted sample order
     std::vector<float> ws((size_t)p.Mtot);
     for (int64_t s = 0; s < p.Mtot; ++s) ws[s] = p.h_w[p.h_perm[s]];
@@ -1648,10 +1796,18 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
     std::vector<int> icgs = (p.C > 8) ? std::vector<int>{8, 16} : std::vector<int>{p.interp_cg};
     const int vpi0 = p.visits_per_item, cgr0 = p.grid_cgr, icg0 = p.interp_cg;
     const bool mfma0 = p.use_grid_mfma;
+    const bool sweep0 = p.use_grid_sweep && p.sw_items != nullptr;
     double g0 = 0, i0 = 0;
     CHK(time_kernel(p, reps, true, &g0));
     CHK(time_kernel(p, reps, false, &i0));
-    best = TuneChoice{vpi0, cgr0, icg0, g0, i0, g0, i0, mfma0};
+    best = TuneChoice{vpi0, cgr0, icg0, g0, i0, g0, i0, mfma0, sweep0};
+    if (p.sw_items) {   // the line sweep (independent of the visit-list chunking)
+      p.use_grid_sweep = true;
+      double t = 0;
+      CHK(time_kernel(p, reps, true, &t));
+      if (t < best.grid_ms) { best.grid_ms = t; best.sweep = true; }
+    }
+    p.use_grid_sweep = false;   // the tile kernels below
     std::vector<int> kinds = tiles8 ? std::vector<int>{1, 0} : std::vector<int>{0};   // 1: matrix cores
     for (int vpi : vpis) {
       CHK(apply_vpi(p, vpi));
@@ -1661,7 +1817,9 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
           p.grid_cgr = cgr;
           double t = 0;
           CHK(time_kernel(p, reps, true, &t));
-          if (t < best.grid_ms) { best.grid_ms = t; best.vpi = vpi; best.cgr = cgr; best.mfma = kind == 1; }
+          if (t < best.grid_ms) {
+            best.grid_ms = t; best.vpi = vpi; best.cgr = cgr; best.mfma = kind == 1; best.sweep = false;
+          }
         }
       }
     }
@@ -1678,6 +1836,7 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
   p.grid_cgr = best.cgr;
   p.interp_cg = best.icg;
   p.use_grid_mfma = best.mfma;
+  p.use_grid_sweep = best.sweep && p.sw_items != nullptr;
   // captured graphs hold the old launch shapes
   if (p.graph_exec) (void)hipGraphExecDestroy(p.graph_exec);
   p.graph_exec = nullptr;
@@ -1692,6 +1851,7 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
     rep->default_grid_ms = best.def_grid_ms;
     rep->default_interp_ms = best.def_interp_ms;
     rep->grid_mfma = best.mfma ? 1 : 0;
+    rep->grid_sweep = best.sweep ? 1 : 0;
   }
   return CGS_OK;
 }
@@ -1713,6 +1873,7 @@ int cgs_set_weights(cgs_plan plan, const float* w) {
   for (int64_t s = 0; s < p.Mtot; ++s) ws[s] = w[p.h_perm[s]];
   HIPCHK(hipStreamSynchronize(p.stream));
   HIPCHK(hipMemcpy(p.w, ws.data(), sizeof(float) * ws.size(), hipMemcpyHostToDevice));
+  CHK(sweep_weights(p, ws));
   p.h_w.assign(w, w + (size_t)p.Mtot);
   toep_drop(p);
   p.have_w = true;
@@ -1870,6 +2031,144 @@ int cgs_nfft_inverse_ref(cgs_plan plan, int frame, const void* f, void* img, int
   GUARD_END
 }
 
+}  // extern "C"
+
+namespace cgs {
+
+// ---- allocation-free solve ---------------------------------------------------------------
+// The CG bookkeeping of cgs_solve_noalloc (per-frame CgState, stop flags,
+// reduction partials, residual history) lives in this array: one copy per GPU,
+// held until the process ends.  Host code never passes `scratch` itself (in
+// host code the name is a stand-in, not a device address): it takes the
+// device address once per device with hipGetSymbolAddress and uses that.
+static __device__ __align__(256) unsigned char scratch[CGS_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_dev[kMaxDevices] = {};
+
+static int scratch_address(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) { set_error("device index out of range"); return CGS_ERR_ARG; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !ptr) {
+      set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+      return CGS_ERR_HIP;
+    }
+    g_scratch_dev[dev] = (unsigned char*)ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return CGS_OK;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// bytes of scratch a solve of this plan with `maxit` iterations needs
+static size_t noalloc_bytes(const Plan& p, int maxit) {
+  return align256(sizeof(CgState) * p.F) + align256(sizeof(int) * p.F) + align256(sizeof(double) * p.n_partials) +
+         sizeof(double) * (size_t)p.F * (size_t)(maxit + 1);
+}
+
+// points the plan's CG bookkeeping at the scratch for one call
+struct ScratchSwap {
+  Plan& p;
+  CgState* cgst;
+  int* fdone;
+  double* partials;
+  double* hist;
+  int hist_cap;
+  ScratchSwap(Plan& p_, unsigned char* base, int hcap)
+      : p(p_), cgst(p_.cgst), fdone(p_.fdone), partials(p_.partials), hist(p_.hist), hist_cap(p_.hist_cap) {
+    size_t off = 0;
+    p.cgst = (CgState*)(base + off);
+    off += align256(sizeof(CgState) * p.F);
+    p.fdone = (int*)(base + off);
+    off += align256(sizeof(int) * p.F);
+    p.partials = (double*)(base + off);
+    off += align256(sizeof(double) * p.n_partials);
+    p.hist = (double*)(base + off);
+    p.hist_cap = hcap;
+    p.no_prof = true;
+  }
+  ~ScratchSwap() {
+    p.cgst = cgst;
+    p.fdone = fdone;
+    p.partials = partials;
+    p.hist = hist;
+    p.hist_cap = hist_cap;
+    p.no_prof = false;
+  }
+};
+
+static int check_solve_opts(const cgs_solve_opts& o) {
+  if (o.maxit < 0 || o.ftiter < 0 || !(o.tol >= 0)) { set_error("bad solver options"); return CGS_ERR_ARG; }
+  if (o.eh_mode != CGS_EH_TRUE && o.eh_mode != CGS_EH_REF) { set_error("bad E^H mode"); return CGS_ERR_ARG; }
+  if (o.normal_op != CGS_NORMAL_NUFFT && o.normal_op != CGS_NORMAL_TOEPLITZ) { set_error("bad normal operator"); return CGS_ERR_ARG; }
+  if (o.normal_op == CGS_NORMAL_TOEPLITZ && o.eh_mode != CGS_EH_TRUE) {
+    set_error("the Toeplitz normal operator exists for the true adjoint only");
+    return CGS_ERR_ARG;
+  }
+  return CGS_OK;
+}
+
+}  // namespace cgs
+
+extern "C" {
+
+int cgs_solve_noalloc(cgs_plan plan, const void* y, const void* x0, void* x, const cgs_solve_opts* opts,
+                      cgs_solve_report* report) {
+  GUARD_BEGIN
+  const auto t_start = std::chrono::steady_clock::now();
+  if (!plan || !y || !x || !opts) { set_error("null argument"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  CHK(require_ready(p, true));
+  const cgs_solve_opts o = *opts;
+  CHK(check_solve_opts(o));
+  if (!(p.flags & CGS_DEVICE_IO)) {
+    set_error("cgs_solve_noalloc needs a CGS_DEVICE_IO plan (host buffers would need staging copies)");
+    return CGS_ERR_ARG;
+  }
+  if (o.graph_iters != 0) {
+    set_error("cgs_solve_noalloc launches kernels one by one: graph_iters must be 0 (a graph allocates)");
+    return CGS_ERR_ARG;
+  }
+  if (o.eh_mode == CGS_EH_TRUE && o.normal_op == CGS_NORMAL_TOEPLITZ && !p.have_toep) {
+    set_error("the Toeplitz spectrum is not prepared (it allocates): call cgs_normal or cgs_solve with "
+              "CGS_NORMAL_TOEPLITZ once first");
+    return CGS_ERR_STATE;
+  }
+  const size_t need = noalloc_bytes(p, o.maxit);
+  if (need > (size_t)CGS_NOALLOC_SCRATCH_BYTES) {
+    set_error("cgs_solve_noalloc: " + std::to_string(need) + " bytes of CG bookkeeping (frames x (maxit + 1) "
+              "history, partials) exceed the " + std::to_string((size_t)CGS_NOALLOC_SCRATCH_BYTES) +
+              "-byte scratch");
+    return CGS_ERR_ARG;
+  }
+  unsigned char* base = nullptr;
+  CHK(scratch_address(p.device, &base));
+  const float2* yd = (const float2*)y;
+  float2* xd = (float2*)x;
+  if (o.maxit == 0) {   // CGLS.hpp:32-33: return A/y
+    ScratchSwap sw(p, base, 1);
+    launch_y_ext2int(p, yd, p.ybuf[0], p.C, -1, p.stream);
+    CHK(op_EH(p, o.eh_mode, p.ybuf[0], xd, o.ftiter, o.fteps, nullptr, 0.f, nullptr, nullptr));
+    if (report) { std::memset(report, 0, sizeof(*report)); report->nframes = p.F; }
+  } else {
+    ScratchSwap sw(p, base, o.maxit + 1);
+    CHK(solve(p, yd, (const float2*)x0, xd, o, report, t_start));
+  }
+  CHK(check_launch());
+  HIPCHK(hipStreamSynchronize(p.stream));
+  if (report) report->total_ms = ms_since(t_start);
+  return (report && report->status == CGS_ERR_NAN) ? CGS_ERR_NAN : CGS_OK;
+  GUARD_END
+}
+
 int cgs_solve(cgs_plan plan, const void* y, const void* x0, void* x, const cgs_solve_opts* opts,
               cgs_solve_report* report) {
   GUARD_BEGIN
@@ -1878,12 +2177,19 @@ int cgs_solve(cgs_plan plan, const void* y, const void* x0, void* x, const cgs_s
   Plan& p = plan->p;
   CHK(require_ready(p, true));
   cgs_solve_opts o = *opts;
-  if (o.maxit < 0 || o.ftiter < 0 || !(o.tol >= 0)) { set_error("bad solver options"); return CGS_ERR_ARG; }
-  if (o.eh_mode != CGS_EH_TRUE && o.eh_mode != CGS_EH_REF) { set_error("bad E^H mode"); return CGS_ERR_ARG; }
-  if (o.normal_op != CGS_NORMAL_NUFFT && o.normal_op != CGS_NORMAL_TOEPLITZ) { set_error("bad normal operator"); return CGS_ERR_ARG; }
-  if (o.normal_op == CGS_NORMAL_TOEPLITZ && o.eh_mode != CGS_EH_TRUE) {
-    set_error("the Toeplitz normal operator exists for the true adjoint only");
-    return CGS_ERR_ARG;
+  CHK(check_solve_opts(o));
+  const int hcap = o.maxit + 1;
+  if (hcap > p.hist_cap) {
+    // a captured CG graph writes the history through the old pointer and
+    // capacity: drop it with the buffer it refers to
+    HIPCHK(hipStreamSynchronize(p.stream));
+    if (p.graph_exec) (void)hipGraphExecDestroy(p.graph_exec);
+    p.graph_exec = nullptr;
+    p.graph_key.clear();
+    if (p.last_hist == p.hist) { p.last_hist = nullptr; p.last_hist_cap = 0; }
+    dfree(p.hist);
+    CHK(dalloc(&p.hist, (size_t)p.F * hcap));
+    p.hist_cap = hcap;
   }
   const size_t ib = sizeof(float2) * (size_t)p.Ntot * p.F;
   const float2* yd;
@@ -1937,12 +2243,14 @@ int cgs_residual_history(cgs_plan plan, int frame, double* out, int cap) {
   GUARD_BEGIN
   if (!plan || !out || cap < 0) { set_error("null argument"); return -CGS_ERR_ARG; }
   Plan& p = plan->p;
-  if (frame < 0 || frame >= p.F || !p.hist) { set_error("no history for this frame"); return -CGS_ERR_STATE; }
+  if (frame < 0 || frame >= p.F || !p.last_hist) { set_error("no history for this frame"); return -CGS_ERR_STATE; }
   HIPCHK(hipStreamSynchronize(p.stream));
-  CgState st;
-  HIPCHK(hipMemcpy(&st, p.cgst + frame, sizeof(CgState), hipMemcpyDeviceToHost));
-  const int n = std::min(cap, std::min(st.it + 1, p.hist_cap));
-  if (n > 0) HIPCHK(hipMemcpy(out, p.hist + (size_t)frame * p.hist_cap, sizeof(double) * n, hipMemcpyDeviceToHost));
+  // the iteration count of the last solve (its report copy; the device state
+  // of an allocation-free solve lives in the shared scratch)
+  const int it = p.h_cgst ? p.h_cgst[frame].it : 0;
+  const int n = std::min(cap, std::min(it + 1, p.last_hist_cap));
+  if (n > 0)
+    HIPCHK(hipMemcpy(out, p.last_hist + (size_t)frame * p.last_hist_cap, sizeof(double) * n, hipMemcpyDeviceToHost));
   return n;
   GUARD_END
 }

@@ -152,6 +152,25 @@ int cgs_normal(cgs_plan plan, const void* x, void* q, double lambda, int normal_
 int cgs_solve(cgs_plan plan, const void* y, const void* x0, void* x,
               const cgs_solve_opts* opts, cgs_solve_report* report);
 
+/* Allocation-free cgs_solve.  No counterpart in codeare: it is for callers that
+ * cannot allocate device memory in the call (tight serving loops, code that
+ * must not call hipMalloc).  Same arguments and results as cgs_solve on a
+ * CGS_DEVICE_IO plan; nothing is allocated: the CG bookkeeping (per-frame
+ * scalars and stop flags, reduction partials, the residual history) lives in a
+ * fixed device array of CGS_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends, and every other buffer is the plan's.  Limits: graph_iters
+ * must be 0 (instantiating a graph allocates); the Toeplitz operator needs its
+ * spectrum prepared beforehand (one cgs_normal or cgs_solve with
+ * CGS_NORMAL_TOEPLITZ on the plan); frames x (maxit + 1) history entries plus
+ * the partials must fit the array (CGS_ERR_ARG otherwise; 8 MiB hold e.g. one
+ * frame up to ~1e6 iterations, or 64 frames of 16 coils up to ~9000).
+ * Every call on one GPU shares that array (it keeps the last call's
+ * contents, and cgs_residual_history reads the last call's history from it):
+ * calls that overlap on one GPU race. */
+#define CGS_NOALLOC_SCRATCH_BYTES (8u << 20)
+int cgs_solve_noalloc(cgs_plan plan, const void* y, const void* x0, void* x,
+                      const cgs_solve_opts* opts, cgs_solve_report* report);
+
 /* Residual measure (tol_kind units) of the last cgs_solve for one frame, one
  * entry per iteration starting with the initial one (CGLS's _res history,
  * CGLS.hpp:38, here per solve).  Returns the number of entries written. */
@@ -166,9 +185,10 @@ typedef struct {
   double grid_ms, interp_ms;                  /* chosen configuration             */
   double default_grid_ms, default_interp_ms;  /* the plan's defaults              */
   int grid_mfma;         /* gridding: 1 matrix-core scatter, 0 VALU registers    */
+  int grid_sweep;        /* gridding: 1 line sweep (2D), 0 the tile kernels      */
 } cgs_tune_report;
-/* Times the gridding scatter (2D: matrix-core or VALU-register kernel, visits
- * per work item, coils per pass) and the window gather of this plan's
+/* Times the gridding scatter (2D: line sweep, or the tile kernels -- matrix-core
+ * or VALU-register, visits per work item, coils per pass) and the window gather of this plan's
  * trajectory over a small grid of launch shapes (`reps` timed runs each) and
  * keeps the fastest.  Results are cached per process under (d, n, C, frames, samples
  * per frame rounded to a power of two); with use_cache != 0 a cached bucket is

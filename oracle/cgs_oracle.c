@@ -8,6 +8,7 @@
 #include "cgs_oracle.h"
 
 #include <math.h>
+#include <time.h>
 #include <stdlib.h>
 #include <string.h>
 #include <complex.h>
@@ -18,6 +19,27 @@
 typedef double complex cplx;
 
 static int g_threads = 0;
+
+/* wall-clock marks of the last ora_cg_true / ora_cgls_ref call (the CPU-baseline
+ * leg times them): t[0] after E^H y, t[i] after iteration i */
+#define ORA_TMARKS 4096
+static double g_tmark[ORA_TMARKS + 1];
+static int g_ntmark = 0;
+static double g_tstart = 0.0;
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+static void tmark_reset(void) { g_ntmark = 0; g_tstart = now_s(); }
+static void tmark(void) {
+  if (g_ntmark <= ORA_TMARKS) g_tmark[g_ntmark++] = now_s() - g_tstart;
+}
+int ora_time_marks(double* out, int cap) {
+  const int n = g_ntmark < cap ? g_ntmark : cap;
+  for (int i = 0; i < n; ++i) out[i] = g_tmark[i];
+  return n;
+}
 void ora_set_threads(int n) { g_threads = n; }
 int ora_get_threads(void) {
 #ifdef _OPENMP
@@ -537,7 +559,9 @@ int ora_cgls_ref(const ora_nfft* p, int C, const float* S, const float* w,
   float complex* r = (float complex*)malloc(sizeof(float complex) * (size_t)Nt);
   float complex* q = (float complex*)malloc(sizeof(float complex) * (size_t)Nt);
   float* yt = (float*)malloc(sizeof(float) * 2 * (size_t)M * C);
+  tmark_reset();
   apply_EH(p, C, S, w, mode, ftiter, fteps, y, (float*)pp);        /* :31 */
+  tmark();
   if (maxit == 0) {                                                /* :32-33 */
     memcpy(x, pp, sizeof(float complex) * (size_t)Nt);
     free(pp); free(r); free(q); free(yt);
@@ -562,6 +586,7 @@ int ora_cgls_ref(const ora_nfft* p, int C, const float* S, const float* w,
     rn = crealf(dotc_f(r, r, Nt));
     for (int64_t k = 0; k < Nt; ++k) pp[k] = pp[k] * (rn / rno) + r[k];   /* :55-56 */
     ++it;
+    tmark();
   }
   if (hist && it == maxit) hist[it] = rn / xn;
   free(pp); free(r); free(q); free(yt);
@@ -688,7 +713,9 @@ int ora_cg_true(const ora_nfft* p, int C, const float* S, const float* w,
   float complex* r = (float complex*)malloc(sizeof(float complex) * (size_t)Nt);
   float complex* q = (float complex*)malloc(sizeof(float complex) * (size_t)Nt);
   float* yt = (float*)malloc(sizeof(float) * 2 * (size_t)M * C);
+  tmark_reset();
   ora_ncsense_EH_true(p, C, S, w, y, (float*)b);
+  tmark();
   double bn = 0;
   for (int64_t k = 0; k < Nt; ++k) bn += (double)crealf(b[k]) * crealf(b[k]) + (double)cimagf(b[k]) * cimagf(b[k]);
   if (x0) {
@@ -726,6 +753,7 @@ int ora_cg_true(const ora_nfft* p, int C, const float* S, const float* w,
     rn = rn_new;
     for (int64_t k = 0; k < Nt; ++k) pp[k] = r[k] + (float)beta * pp[k];
     ++it;
+    tmark();
   }
   free(b); free(pp); free(r); free(q); free(yt);
   return it;

@@ -88,6 +88,9 @@ int ora_cgls_ref(const ora_nfft* p, int C, const float* S, const float* w,
 /* CG on (E^H W E + lambda) x = E^H W y with FP64 scalars and complex64 vectors,
  * optional warm start x0, stop when ||r||/||E^H W y|| <= tol (norm ratio).
  * Returns iterations run; res_hist (nullable) gets the norm ratio (maxit+1). */
+/* Seconds since the start of the last ora_cg_true / ora_cgls_ref call: entry 0
+ * after E^H y, entry i after iteration i.  Returns the number written. */
+int ora_time_marks(double* out, int cap);
 int ora_cg_true(const ora_nfft* p, int C, const float* S, const float* w,
                 int maxit, double tol, double lambda, const float* y,
                 const float* x0, float* x, double* res_hist);

@@ -69,6 +69,8 @@ def lib():
         L.ora_cgls_ref_frames.argtypes = [P, I, I, P, I64, P, P, I, I, D, I, D, D, P, P, P]
         L.ora_cg_true_frames.restype = I
         L.ora_cg_true_frames.argtypes = [P, I, I, P, I64, P, P, I, D, D, P, P, P]
+        L.ora_time_marks.restype = I
+        L.ora_time_marks.argtypes = [P, I]
         L.ora_set_threads.argtypes = [I]
         L.ora_get_threads.restype = I
         _lib = L
@@ -85,6 +87,14 @@ def _c128(a):
 
 def _c64(a):
     return np.asfortranarray(a, dtype=np.complex64)
+
+
+def time_marks():
+    """Seconds since the start of the last cg_true / cgls_ref call: [0] after
+    E^H y, [i] after iteration i."""
+    buf = np.zeros(4097, dtype=np.float64)
+    n = lib().ora_time_marks(_ptr(buf), buf.size)
+    return buf[:n]
 
 
 def set_threads(n: int) -> None:

@@ -45,3 +45,59 @@ def test_graph_solve_after_history_regrowth(cgs, N, coils):
     assert np.all(h2 > 0)
     assert np.allclose(h2, h3, rtol=1e-9, atol=0) and np.allclose(h1, h3, rtol=1e-9, atol=0)
     assert rel(x2, x3) <= 1e-6 and rel(x1, x3) <= 1e-6
+
+
+def _plan_env(cgs, env, N, M, sigma, m, C, k, w, S, foff=None):
+    import os
+    old = {k_: os.environ.get(k_) for k_ in env}
+    os.environ.update(env)
+    try:
+        P = cgs.Plan(N, M, sigma, m, ncoils=C, frame_offsets=foff)
+    finally:
+        for k_, v in old.items():
+            if v is None:
+                os.environ.pop(k_, None)
+            else:
+                os.environ[k_] = v
+    P.kspace(k)
+    P.weights(w)
+    P.sensitivities(S)
+    return P
+
+
+@pytest.mark.parametrize("N,sigma,m,C,F", [
+    (64, 2.0, 3, 32, 1),     # 32 coils: one lane group per wave
+    (96, 2.0, 3, 16, 2),     # 16 coils: two lane groups, two frames
+    (128, 2.0, 3, 8, 1),     # 8 coils: four lane groups
+    (48, 2.0, 2, 16, 1),     # m = 2: 6 taps (n = 96)
+    (64, 1.0, 1, 12, 1),     # sigma 1, m 1 (codeare's defaults): 4 taps, 12 coils (padded lanes)
+    (16, 1.5, 3, 48, 1),     # n = 24 < one segment: footprints wrap inside it; 48 coils: two waves
+])
+def test_line_sweep_scatter_matches_tile_scatter(cgs, N, sigma, m, C, F):
+    """The line-sweep scatter (2D default) against the tile-owner scatter, on
+    radial and random trajectories: E^H (weighted) and the unweighted adjoint
+    NFFT of every coil, one frame or several."""
+    from codeare_amd import data
+    from helpers import rand_img
+    rng = np.random.default_rng(N + C + m)
+    per = []
+    for f in range(F):
+        kr = data.radial_2d(24, 2 * N, True, 24 * f)
+        kx = np.asfortranarray(rng.uniform(-0.5, 0.5, size=(2, 700)).astype(np.float32))
+        per.append(np.concatenate([kr, kx], axis=1))
+    k = np.asfortranarray(np.concatenate(per, axis=1))
+    Mf = per[0].shape[1]
+    M = k.shape[1]
+    foff = np.arange(F + 1, dtype=np.int64) * Mf if F > 1 else None
+    w = rng.uniform(0.1, 1.0, M).astype(np.float32)
+    S = data.coil_maps(N, 2, C, rng)
+    y = rand_img(rng, (M * C,))
+    Ps = _plan_env(cgs, {"CGS_GRID_SWEEP": "1"}, (N, N), M, sigma, m, C, k, w, S, foff)
+    Pt = _plan_env(cgs, {"CGS_GRID_SWEEP": "0"}, (N, N), M, sigma, m, C, k, w, S, foff)
+    e1 = rel(Ps.EH(y), Pt.EH(y))
+    e2 = 0.0
+    if F == 1:
+        yb = y.reshape((M, C), order="F")
+        e2 = rel(Ps.nfft_adjoint(yb), Pt.nfft_adjoint(yb))
+    print(f"N={N} sigma={sigma} m={m} C={C} F={F}: E^H {e1:.2e}, adjoint NFFT {e2:.2e}")
+    assert e1 <= 1e-6 and e2 <= 1e-6
