@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--volume", dest="volume", action="store_true", default=None,
                     help="time the 3D koosh-ball workload (default: with the secondary measurements)")
     ap.add_argument("--no-volume", dest="volume", action="store_false", help="skip the 3D koosh-ball workload")
+    ap.add_argument("--nufft-only", action="store_true",
+                    help="only time the NuFFT operators (E, then E^H W) at 256^2 x 32 coils and the 128^3 x 16-coil "
+                         "koosh ball, `steps` applications each (for kernel profiles)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with 3 device copies of 256 MiB (counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "codeare_amd", "weights", "unet_golden2d_256_c32.npz"),
@@ -90,6 +93,9 @@ def main():
     import torch
     from codeare_amd import cgsense, data
 
+    if args.nufft_only:
+        print(json.dumps(nufft_leg(args)))
+        return
     prob = data.make_problem(args.workload, 0)
     wl = prob.wl
     M = prob.k.shape[1]
@@ -260,6 +266,44 @@ def main():
         torch.cuda.synchronize()
         out["copy_calibration_bytes"] = 2 * src.numel() * 4
     print(json.dumps(out))
+
+
+def nufft_leg(args):
+    """E and E^H W (window gather / gridding scatter with their FFT passes) at
+    256^2 x 32 coils (402 x 512 golden angle) and the 128^3 x 16-coil koosh
+    ball, `steps` applications each after `warmup`; per-kernel HIP-event
+    times and algorithmic GB/s."""
+    import numpy as np
+    import torch
+    from codeare_amd import cgsense, data
+    out = {"metric": "NuFFT operator GB/s (algorithmic) vs HBM", "unit": "GB/s"}
+    for cfg in (2, 3):
+        prob = data.make_problem(cfg, 0)
+        wl = prob.wl
+        M = prob.k.shape[1]
+        plan = cgsense.Plan((wl.N,) * wl.d, M, wl.sigma, wl.m, ncoils=wl.coils, device_io=True, profile=True)
+        plan.kspace(prob.k)
+        plan.weights(prob.w)
+        plan.sensitivities(prob.maps)
+        dev = torch.device("cuda:0")
+        x = torch.from_numpy(np.ascontiguousarray(prob.image.reshape(-1, order="F"))).to(dev)
+        y = torch.empty(M * wl.coils, dtype=torch.complex64, device=dev)
+        xb = torch.empty_like(x)
+        for _ in range(max(1, args.warmup)):
+            plan.E(x, out=y)
+            plan.EH(y, out=xb)
+        plan.sync()
+        plan.kernel_stats(reset=True)
+        for _ in range(args.steps):
+            plan.E(x, out=y)
+            plan.EH(y, out=xb)
+        plan.sync()
+        per = per_kernel(plan.kernel_stats(reset=True))
+        out[wl.name] = {"applications": args.steps, "kernels": per,
+                        "frac_hbm": {k: round(v["GBps"] / HBM_PEAK_GBS, 4) for k, v in per.items()}}
+        del plan
+        torch.cuda.empty_cache()
+    return out
 
 
 def ref_compat_leg(args, plan, y, x, lam, ftiter=3, fteps=1e-3, cgeps=1e-10):

@@ -40,11 +40,11 @@ KERNELS = ["fft_fwd", "interp", "grid", "fft_adj", "coil", "cg", "other", "toep_
 
 EXPORTED = [
     "cgs_plan_create", "cgs_plan_destroy", "cgs_set_stream", "cgs_sync",
-    "cgs_set_trajectory", "cgs_set_weights", "cgs_set_sensitivities",
+    "cgs_set_trajectory", "cgs_set_weights", "cgs_set_sensitivities", "cgs_set_sensitivities_device",
     "cgs_nfft_forward", "cgs_nfft_adjoint", "cgs_nfft_inverse_ref",
     "cgs_E", "cgs_EH", "cgs_normal", "cgs_solve", "cgs_solve_noalloc",
     "cgs_kernel_stats", "cgs_plan_info", "cgs_status_string", "cgs_last_error",
-    "cgs_residual_history", "cgs_autotune", "cgs_profile_mask",
+    "cgs_residual_history", "cgs_autotune", "cgs_profile_mask", "cgs_set_b0", "cgs_b0_info",
 ]
 
 
@@ -95,6 +95,7 @@ def load_library(path: str | None = None):
     L.cgs_set_trajectory.argtypes = [P, P]
     L.cgs_set_weights.argtypes = [P, P]
     L.cgs_set_sensitivities.argtypes = [P, P]
+    L.cgs_set_sensitivities_device.argtypes = [P, P]
     L.cgs_nfft_forward.argtypes = [P, I, P, P, I]
     L.cgs_nfft_adjoint.argtypes = [P, I, P, P, I]
     L.cgs_nfft_inverse_ref.argtypes = [P, I, P, P, I, I, D]
@@ -108,6 +109,8 @@ def load_library(path: str | None = None):
     L.cgs_residual_history.argtypes = [P, I, P, I]
     L.cgs_autotune.argtypes = [P, I, I, ct.POINTER(TuneReport)]
     L.cgs_profile_mask.argtypes = [P, ct.c_uint]
+    L.cgs_set_b0.argtypes = [P, P, P]
+    L.cgs_b0_info.argtypes = [P, ct.POINTER(I), ct.POINTER(D), ct.POINTER(D)]
     L.cgs_status_string.restype = ct.c_char_p
     L.cgs_status_string.argtypes = [I]
     L.cgs_last_error.restype = ct.c_char_p
@@ -210,6 +213,35 @@ class Plan:
             raise ValueError("maps must be (N..., C)" if nm == 1 else "maps must be (N..., F, C)")
         self._S = S
         _check(load_library().cgs_set_sensitivities(self._h, _ptr(S)))
+
+    def sensitivities_device(self, S):
+        """Maps already on the device (a torch tensor, the layout of
+        `sensitivities`), loaded on the plan's stream; keep S alive until then."""
+        nm = self.F if self.maps_per_frame else 1
+        if S.numel() != self.Ntot * self.C * nm or not S.is_cuda:
+            raise ValueError("maps must be a device tensor of (N..., C) entries")
+        _check(load_library().cgs_set_sensitivities_device(self._h, ct.c_void_p(S.data_ptr())))
+
+    def b0(self, b0, t):
+        """Off-resonance model of the NFFT operators (codeare's NFFT "b0" and
+        "timing", NFFT.hpp:117-173): b0 (N0, N1) in cycles per unit of t,
+        t (M,) sample times; None removes it."""
+        L = load_library()
+        if b0 is None:
+            _check(L.cgs_set_b0(self._h, None, None))
+            return
+        b0 = np.asfortranarray(b0, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        if b0.size != self.Ntot or t.size != self.M:
+            raise ValueError("b0 must have N0*N1 entries and timing M_total")
+        _check(L.cgs_set_b0(self._h, _ptr(b0), _ptr(t)))
+
+    def b0_info(self):
+        n3 = ct.c_int()
+        om = ct.c_double()
+        ts = ct.c_double()
+        _check(load_library().cgs_b0_info(self._h, ct.byref(n3), ct.byref(om), ct.byref(ts)))
+        return {"N3": n3.value, "omega": om.value, "ts": ts.value}
 
     def info(self):
         n = (ct.c_int * 3)()
@@ -358,7 +390,9 @@ def _imsz(p):
 
 class NFFT:
     """NFFT<cxfl> (src/matrix/ft/NFFT.hpp): keys nk, imsz, m (1), alpha (1.0),
-    epsilon (7e-4), maxit (3), 3rd_dim_cart (False).  ``*`` is Trafo,
+    epsilon (7e-4), maxit (3), 3rd_dim_cart (False), b0 and timing (the
+    off-resonance model, NFFT.hpp:117-173: b0 (N0, N1) in cycles per unit of
+    timing, timing (nk,); applied once KSpace is set).  ``*`` is Trafo,
     ``adjoint`` (->*) the iterative NFFT3 inverse; ``adjoint_true`` is
     nfft_adjoint.  With 3rd_dim_cart an imsz (N0, N1, Nz) is Nz 2D slices on
     the one trajectory (NFFT.hpp:84-103): Trafo maps (N0, N1, Nz) to (nk, Nz)
@@ -380,9 +414,22 @@ class NFFT:
         self.maxit = int(params.get("maxit", 3))
         # the slices are the plan's image batch
         self.plan = Plan(self.N, self.M, self.alpha, self.m, ncoils=self.ncart)
+        self._b0 = None
+        if params.get("b0") is not None:
+            b0 = np.asarray(params["b0"], dtype=np.float32)
+            t = params.get("timing")
+            # NFFT.hpp:117-141: a b0 that does not fit the image, or timing that
+            # does not fit the trajectory, falls back to b0 = 0 with a warning
+            if b0.size != int(np.prod(self.N)) or t is None or np.asarray(t).size != self.M:
+                import warnings
+                warnings.warn("NFFT: b0 / timing do not fit image / trajectory; defaulting to b0 = 0")
+            else:
+                self._b0 = (b0, np.asarray(t, dtype=np.float32))
 
     def KSpace(self, k):
         self.plan.kspace(k)
+        if self._b0 is not None:
+            self.plan.b0(*self._b0)
 
     def Weights(self, w):
         self.plan.weights(w)

@@ -258,18 +258,23 @@ struct GridArgs {
   int64_t ntot, tilecells;
 };
 
-constexpr int kSweepPad = 16;   // padding records after the line-sweep entries (>= the chunk size)
+constexpr int kSweepPad = 24;   // padding records after the line-sweep entries (>= 3 x the chunk size)
 
 // 2D gridding as a sweep along grid lines (kernels_sweep.hip): one wave per
 // (frame, line l1, segment of W cells along axis 0)
 struct SweepArgs {
-  const int4* items;      // {entry begin, entry end, frame, l1 * nseg + segment}
+  const int4* items;      // two int4 per item: {entry begin, entry end, frame (-1: padding),
+                          // l1 * nseg + segment}, {partial offset in floats (-1: the grid),
+                          // first cell, end cell, 0}
   const int4* ent;        // {sorted sample, r0, bits(phi1[t1]), bits(phi1[t1] * w)}, by r0 per item
   const float* psi_rot;   // [s * taps + k]: axis-0 window value of the tap on a cell == k mod taps
   const float2* y;        // Y[s * C + c]
   float2* grid;
+  float* part;            // cell ranges of the pieces of split line segments
+  const int4* ritems;     // split segments: {piece begin, piece end, frame, l1 * nseg + segment}
+  const int4* rpieces;    // their pieces in order: {partial offset, first cell, end cell, 0}
   const int* fdone;
-  int nitems, taps, C, W, nseg, n0, weighted;
+  int nitems, nritems, taps, C, W, nseg, n0, weighted;
   int64_t ntot;
 };
 
@@ -344,8 +349,13 @@ struct Plan {
   int n_slots = 0;
   float2* partial = nullptr;
   // line-sweep scatter tables (2D)
-  int4* sw_items = nullptr;
+  int4* sw_items = nullptr;           // two int4 per item (SweepArgs::items, items2)
   int n_sw_items = 0, sw_nseg = 0;
+  int4* sw_ritems = nullptr;          // split line segments and their pieces
+  int4* sw_rpieces = nullptr;
+  int n_sw_ritems = 0;
+  float* sw_part = nullptr;           // partial cell ranges of the pieces
+  int sweep_K = 256;                  // entries per piece of a split line segment
   int4* sw_ent = nullptr;
   int64_t n_sw_ent = 0;
   float* psi_rot = nullptr;
@@ -382,6 +392,19 @@ struct Plan {
   std::vector<float> h_w;             // weights copy (host, original order)
   CgState* h_cgst = nullptr;          // pinned mirror
   int* h_done = nullptr;              // pinned
+
+  // off-resonance (B0) model of the NFFT operators (cgs_set_b0): a helper 3D
+  // plan over (N0, N1, N3) with nodes (k_j, tau_j), per-pixel phase
+  // exp(-2 pi i b0 ts) and third-axis position nu = N3 b0 / omega, per-sample
+  // 1 / phihat(tau_j) in the helper's sample order
+  Plan* b0q = nullptr;
+  float2* b0_phase = nullptr;
+  float* b0_nu = nullptr;
+  float* b0_phinv = nullptr;
+  int b0_N3 = 0;
+  float b0_b3 = 0.f;
+  double b0_ts = 0.0, b0_omega = 0.0;
+  std::vector<float> h_b0, h_t;       // host copies (the helper is rebuilt by a new trajectory)
 
   // solve bookkeeping without allocation in the call: polling events and
   // per-frame completion times made at plan creation; the history the last
@@ -457,6 +480,7 @@ void launch_grid_mfma3(const GridArgs& a, const int4* vrec, int nitems, hipStrea
 int sweep_groups(int C);
 bool grid_sweep_supported(int taps, int C);
 void launch_grid_sweep(const SweepArgs& a, hipStream_t s);
+void launch_grid_sweep_reduce(const SweepArgs& a, hipStream_t s);
 size_t interp_lds3_bytes(int T, int taps, int cg, int maxs);
 void launch_interp_lds3(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int nt1, int cg, int maxs,
                         int threads, hipStream_t s);
@@ -510,7 +534,14 @@ void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 void launch_toep_layout(const float2* Khat, float* That, int64_t cells, int nl, int F, float scale,
                         hipStream_t s);
 void launch_fill_int(int* dst, int value, int n, hipStream_t s);
+// off-resonance model (kernels_b0.hip)
+void launch_b0_spread(const float2* x, const float2* phase, const float* nu, float2* X3, int64_t Ntot, int C, int N3,
+                      int m, float b3, int mask_nh, int N0, int N1, hipStream_t s);
+void launch_b0_gather(const float2* X3, const float2* phase, const float* nu, float2* x, int64_t Ntot, int C, int N3,
+                      int m, float b3, hipStream_t s);
+void launch_b0_scale(const float2* y, const float* scale, float2* out, int64_t M, int C, hipStream_t s);
 void launch_zero(void* dst, size_t bytes, hipStream_t s);   // bytes a multiple of 4
+void launch_maps_load(const float2* S, float2* Sd, float* I, int64_t Ntot, int64_t nm, int C, hipStream_t s);
 
 // errors
 void set_error(const std::string& msg);

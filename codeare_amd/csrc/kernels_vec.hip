@@ -890,4 +890,27 @@ void launch_zero(void* dst, size_t bytes, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t*)dst, n);
 }
 
+// Maps already on the device, (Ntot, nm, C) column-major -> [z][pixel][coil],
+// and the intensity map 1 / (sqrt(sum_c |S_c|^2) + 1e-9) accumulated in FP32 in
+// coil order with separately rounded products and sums, as the host loader does.
+__global__ void k_maps_load(const float2* __restrict__ S, float2* __restrict__ Sd, float* __restrict__ I,
+                            int64_t Ntot, int64_t nm, int C) {
+  const int64_t n = Ntot * nm;
+  for (int64_t zi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; zi < n; zi += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t z = zi / Ntot, i = zi - z * Ntot;
+    float acc = 0.f;
+    for (int c = 0; c < C; ++c) {
+      const float2 v = S[i + Ntot * (z + nm * c)];
+      Sd[zi * C + c] = v;
+      acc = __fadd_rn(acc, __fadd_rn(__fmul_rn(v.x, v.x), __fmul_rn(v.y, v.y)));
+    }
+    I[zi] = (float)(1.0 / (sqrt((double)acc) + 1.e-9));
+  }
+}
+void launch_maps_load(const float2* S, float2* Sd, float* I, int64_t Ntot, int64_t nm, int C, hipStream_t s) {
+  const int64_t n = Ntot * nm;
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192));
+  hipLaunchKernelGGL(k_maps_load, dim3((unsigned)blocks), dim3(256), 0, s, S, Sd, I, Ntot, nm, C);
+}
+
 }  // namespace cgs

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -439,12 +440,14 @@ static int gridding(Plan& p, const float2* y, bool weighted, const int* fl) {
   if (p.d == 2 && p.use_grid_sweep && p.sw_items) {
     SweepArgs sa{};
     sa.items = p.sw_items; sa.ent = p.sw_ent; sa.psi_rot = p.psi_rot;
-    sa.y = y; sa.grid = p.grid; sa.fdone = fl;
+    sa.y = y; sa.grid = p.grid; sa.part = p.sw_part; sa.fdone = fl;
+    sa.ritems = p.sw_ritems; sa.rpieces = p.sw_rpieces; sa.nritems = p.n_sw_ritems;
     sa.nitems = p.n_sw_items; sa.taps = p.taps; sa.C = p.C;
     sa.W = std::max(1, p.sweep_W / p.taps) * p.taps;
     sa.nseg = p.sw_nseg; sa.n0 = p.n[0]; sa.weighted = weighted ? 1 : 0;
     sa.ntot = p.ntot;
     launch_grid_sweep(sa, p.stream);
+    launch_grid_sweep_reduce(sa, p.stream);
     return CGS_OK;
   }
   if (p.d == 2 && p.T == 8 && p.use_grid_mfma) {
@@ -489,9 +492,17 @@ static int op_EH_true(Plan& p, const float2* y, float2* x, const float2* pv, flo
   return CGS_OK;
 }
 
+// The CGNR's operator pair: A^H W (samples -> p.Xc) and A w_hat (image -> samples).
+// Default: the plan's NFFT (gridding + adjoint FFT passes, masked forward FFT
+// passes + gather); the off-resonance model supplies its own.
+struct CnrOps {
+  std::function<int(const float2* r)> adj_w;
+  std::function<int(const float2* x, float2* v)> fwd_masked;
+};
+
 // NFFT3 weighted damped CGNR (NFFTTraits.hpp:122-138) for every (frame, coil):
 // f_hat = ITrafo(y), written to p.Xc2.  Frames flagged in fl are skipped.
-static int cgnr(Plan& p, const float2* y, int maxit, double eps, const int* fl) {
+static int cgnr(Plan& p, const float2* y, int maxit, double eps, const int* fl, const CnrOps* ops = nullptr) {
   const size_t Fb = sizeof(int) * p.F;
   if (fl && fl != p.fmask) HIPCHK(hipMemcpyAsync(p.fmask, fl, Fb, hipMemcpyDeviceToDevice, p.stream));
   if (!fl) launch_fill_int(p.fmask, 0, p.F, p.stream);
@@ -507,8 +518,17 @@ static int cgnr(Plan& p, const float2* y, int maxit, double eps, const int* fl) 
     launch_cnr_scalars(p, 3, eps, p.stream);
     launch_cnr_dot_samples(p, r, p.cnr + 0 * FC, p.stream);                    // dot_r
   }
-  CHK(gridding(p, r, true, m));                                                 // z = A^H W r
-  CHK(fft_adjoint(p, p.Xc, m));
+  auto adj_w = [&](const float2* rr) -> int {                                   // z = A^H W r
+    if (ops) return ops->adj_w(rr);
+    CHK(gridding(p, rr, true, m));
+    return fft_adjoint(p, p.Xc, m);
+  };
+  auto fwd_masked = [&](const float2* xx, float2* vv) -> int {                 // v = A(w_hat p)
+    if (ops) return ops->fwd_masked(xx, vv);
+    CHK(fft_forward(p, 3, xx, m));
+    return interp(p, vv, m);
+  };
+  CHK(adj_w(r));
   {
     Prof pr(p, CGS_K_CG, 3.0 * xbytes);
     launch_cnr_dot_image(p, p.Xc, true, p.cnr + 1 * FC, p.stream);             // dot_z
@@ -517,8 +537,7 @@ static int cgnr(Plan& p, const float2* y, int maxit, double eps, const int* fl) 
   }
   for (int l = 0; l < maxit; ++l) {
     launch_cnr_scalars(p, 0, eps, p.stream);                                   // dot_r < eps -> stop
-    CHK(fft_forward(p, 3, p.Xc3, m));                                           // v = A(w_hat p)
-    CHK(interp(p, v, m));
+    CHK(fwd_masked(p.Xc3, v));                                                  // v = A(w_hat p)
     {
       Prof pr(p, CGS_K_CG, 4.0 * xbytes + 4.0 * ybytes);
       launch_cnr_dot_samples(p, v, p.cnr + 2 * FC, p.stream);                  // dot_v
@@ -527,8 +546,7 @@ static int cgnr(Plan& p, const float2* y, int maxit, double eps, const int* fl) 
       launch_cnr_update_samples(p, r, v, p.stream);                            // r -= a v
       launch_cnr_dot_samples(p, r, p.cnr + 0 * FC, p.stream);                  // dot_r
     }
-    CHK(gridding(p, r, true, m));                                               // z = A^H W r
-    CHK(fft_adjoint(p, p.Xc, m));
+    CHK(adj_w(r));                                                              // z = A^H W r
     {
       Prof pr(p, CGS_K_CG, 4.0 * xbytes);
       launch_cnr_dot_image(p, p.Xc, true, p.cnr + 6 * FC, p.stream);           // dot_z (new)
@@ -563,7 +581,8 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep);
 // ---- trajectory presort -----------------------------------------------------------------
 static void free_sweep(Plan& p) {
   dfree(p.sw_items); dfree(p.sw_ent); dfree(p.psi_rot);
-  p.n_sw_items = 0; p.n_sw_ent = 0; p.sw_nseg = 0;
+  dfree(p.sw_ritems); dfree(p.sw_rpieces); dfree(p.sw_part);
+  p.n_sw_items = 0; p.n_sw_ent = 0; p.sw_nseg = 0; p.n_sw_ritems = 0;
   p.h_sw_ent.clear();
   p.h_sw_ent.shrink_to_fit();
 }
@@ -653,21 +672,65 @@ static int build_sweep(Plan& p, const std::vector<int>& hbase, const std::vector
       }
     }
   }
-  std::vector<int> order((size_t)nitems);
-  for (int64_t q = 0; q < nitems; ++q) {
-    order[q] = (int)q;
+  for (int64_t q = 0; q < nitems; ++q)
     std::stable_sort(ent.begin() + cnt[q], ent.begin() + cnt[q + 1],
                      [](const int4& a, const int4& b) { return a.y < b.y; });
-  }
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return cnt[a + 1] - cnt[a] > cnt[b + 1] - cnt[b]; });
-  std::vector<int4> items((size_t)nitems);
+  // Work items: a line segment with at most K entries is one item writing its
+  // cells to the grid; a longer one is split into pieces of about K entries
+  // (contiguous in r0 order) that write the cells they touch to partial
+  // ranges, summed in piece order by the reduce pass.  Items go to the eight
+  // XCDs (a block's XCD is its index mod 8) by bands of 8 lines, so that the
+  // lines sharing samples share an L2; heaviest first within an XCD.
+  const int K = std::max(16, p.sweep_K);
+  constexpr int kXcd = 8, kWavesPerBlock = 4;
+  struct Piece { int4 a, b; int64_t n; };
+  std::vector<std::vector<Piece>> per_xcd(kXcd);
+  std::vector<int4> ritems, rpieces;
+  int64_t part_floats = 0;
+  const int ncomp = 2 * p.C;
   for (int64_t q = 0; q < nitems; ++q) {
-    const int64_t k = order[q];
-    const int f = (int)(k / ((int64_t)n1 * nseg));
-    const int line = (int)(k % ((int64_t)n1 * nseg));
-    items[q] = make_int4((int)cnt[k], (int)cnt[k + 1], f, line);
+    const int f = (int)(q / ((int64_t)n1 * nseg));
+    const int line = (int)(q % ((int64_t)n1 * nseg));
+    const int l1 = line / nseg;
+    const int E = std::min(W, n0 - (line % nseg) * W);
+    const int64_t e0 = cnt[q], e1 = cnt[q + 1], ne = e1 - e0;
+    auto& dst = per_xcd[(size_t)((l1 / 8) % kXcd)];
+    if (ne <= K) {
+      dst.push_back(Piece{make_int4((int)e0, (int)e1, f, line), make_int4(-1, 0, E, 0), ne});
+      continue;
+    }
+    const int npc = (int)((ne + K - 1) / K);
+    ritems.push_back(make_int4((int)rpieces.size(), (int)rpieces.size() + npc, f, line));
+    for (int j = 0; j < npc; ++j) {
+      const int64_t a = e0 + ne * j / npc, b = e0 + ne * (j + 1) / npc;
+      const int ca = std::max(0, ent[(size_t)a].y), cb = std::min(E, ent[(size_t)(b - 1)].y + taps);
+      const int4 pc = make_int4((int)part_floats, ca, cb, 0);
+      rpieces.push_back(pc);
+      dst.push_back(Piece{make_int4((int)a, (int)b, f, line), pc, b - a});
+      part_floats += (int64_t)(cb - ca) * ncomp;
+    }
   }
+  if (part_floats > INT32_MAX) {
+    set_error("line-sweep partial ranges exceed 32-bit offsets");
+    return CGS_ERR_UNSUPPORTED;
+  }
+  size_t per_max = 0;
+  for (auto& v : per_xcd) {
+    std::stable_sort(v.begin(), v.end(), [](const Piece& x, const Piece& y) { return x.n > y.n; });
+    per_max = std::max(per_max, v.size());
+  }
+  // item i runs in block i / 4 on XCD (i / 4) % 8: the j-th piece of XCD x
+  // takes slot 4 (8 (j / 4) + x) + j % 4; empty slots are padding (frame -1)
+  const size_t rounds = (per_max + kWavesPerBlock - 1) / kWavesPerBlock;
+  const size_t nslots = rounds * kXcd * kWavesPerBlock;
+  std::vector<int4> items(2 * nslots, make_int4(0, 0, -1, 0));
+  for (int x = 0; x < kXcd; ++x)
+    for (size_t j = 0; j < per_xcd[x].size(); ++j) {
+      const size_t slot = kWavesPerBlock * (kXcd * (j / kWavesPerBlock) + x) + j % kWavesPerBlock;
+      items[2 * slot] = per_xcd[x][j].a;
+      items[2 * slot + 1] = per_xcd[x][j].b;
+    }
+  const int64_t nslots_used = (int64_t)nslots;
   // axis-0 window values rotated to the slot (cell mod taps) each tap lands on
   std::vector<float> rot((size_t)M * taps);
   for (int64_t s = 0; s < M; ++s) {
@@ -683,7 +746,15 @@ static int build_sweep(Plan& p, const std::vector<int>& hbase, const std::vector
   HIPCHK(hipMemcpy(p.sw_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
   if (!ent.empty()) HIPCHK(hipMemcpy(p.sw_ent, ent.data(), sizeof(int4) * ent.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p.psi_rot, rot.data(), sizeof(float) * rot.size(), hipMemcpyHostToDevice));
-  p.n_sw_items = (int)nitems;
+  if (!ritems.empty()) {
+    CHK(dalloc(&p.sw_ritems, ritems.size()));
+    CHK(dalloc(&p.sw_rpieces, rpieces.size()));
+    CHK(dalloc(&p.sw_part, (size_t)part_floats));
+    HIPCHK(hipMemcpy(p.sw_ritems, ritems.data(), sizeof(int4) * ritems.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p.sw_rpieces, rpieces.data(), sizeof(int4) * rpieces.size(), hipMemcpyHostToDevice));
+  }
+  p.n_sw_ritems = (int)ritems.size();
+  p.n_sw_items = (int)nslots_used;
   p.n_sw_ent = nent;
   p.sw_nseg = nseg;
   p.h_sw_ent = std::move(ent);
@@ -916,9 +987,129 @@ static int build_samples(Plan& p, const float* k, bool sweep = true) {
   return CGS_OK;
 }
 
+// ---- off-resonance (B0) model ------------------------------------------------------------
+// f_j = sum_r x(r) exp(-2 pi i k_j.r) exp(-2 pi i b0(r) t_j) with the time-frequency
+// term as a third NFFT dimension (the mri_inh_3d plan codeare's NFFT builds for
+// "b0" and "timing", NFFT.hpp:117-173, NFFTTraits.hpp:91-108): kernels_b0.hip.
+constexpr double kB0Sigma = 1.2;   // oversampling of the third axis (NFFT.hpp:147)
+
+static void b0_free(Plan& p) {
+  if (p.b0q) {
+    destroy_plan(*p.b0q);
+    delete p.b0q;
+    p.b0q = nullptr;
+  }
+  dfree(p.b0_phase); dfree(p.b0_nu); dfree(p.b0_phinv);
+  p.b0_N3 = 0;
+}
+
+// weights of the helper plan in its sample order
+static int b0_weights(Plan& p) {
+  if (!p.b0q || p.h_w.empty()) return CGS_OK;
+  Plan& q = *p.b0q;
+  std::vector<float> ws((size_t)q.Mtot);
+  for (int64_t s = 0; s < q.Mtot; ++s) ws[s] = p.h_w[q.h_perm[s]];
+  HIPCHK(hipMemcpy(q.w, ws.data(), sizeof(float) * ws.size(), hipMemcpyHostToDevice));
+  q.h_w = p.h_w;
+  q.have_w = true;
+  return CGS_OK;
+}
+
+// (re)builds the helper plan and the tables from p.h_b0, p.h_t and the trajectory
+static int b0_build(Plan& p) {
+  b0_free(p);
+  if (p.h_b0.empty()) return CGS_OK;
+  const int m = p.m;
+  double tmin = p.h_t[0], tmax = p.h_t[0], B = 0.0;
+  for (float v : p.h_t) { tmin = std::min(tmin, (double)v); tmax = std::max(tmax, (double)v); }
+  for (float v : p.h_b0) B = std::max(B, std::fabs((double)v));
+  const double ts = 0.5 * (tmin + tmax), T = 0.5 * (tmax - tmin);
+  // third axis: N3 >= 4 sigma3 T B + 2m keeps |tau| <= 1/(2 sigma3) with every
+  // window tap inside [-N3/2, N3/2) (the size rule of NFFT.hpp:150-155); even,
+  // with an oversampled length the FFT passes support and two windows wide
+  int N3 = (int)std::ceil(4.0 * kB0Sigma * T * B + 2.0 * m);
+  N3 = std::max(N3 + (N3 & 1), 2 * m + 2);
+  for (;; N3 += 2) {
+    const int n3 = (int)std::ceil(p.sigma * N3);
+    int rad[8], nst = 0;
+    if (n3 >= 2 * p.taps && factorize(n3, rad, &nst)) break;
+    if (N3 > 4096) { set_error("off-resonance axis too long (b0 x timing span)"); return CGS_ERR_UNSUPPORTED; }
+  }
+  const double omega = (B > 0.0) ? B / (0.5 - (double)m / N3) : 1.0;
+  const double b3 = M_PI * (2.0 - 1.0 / kB0Sigma);
+  Plan* q = new Plan;
+  const int N3d[3] = {p.N[0], p.N[1], N3};
+  int rc = create_plan(*q, 3, N3d, p.Mtot, p.sigma, m, p.C, 1, nullptr, CGS_DEVICE_IO);
+  if (rc != CGS_OK) { destroy_plan(*q); delete q; return rc; }
+  (void)hipStreamDestroy(q->stream);   // the helper runs on the plan's stream
+  q->stream = p.stream;
+  q->own_stream = false;
+  p.b0q = q;
+  // nodes (tau_j, k(0, j), k(1, j)): tau pairs with the third image axis
+  std::vector<float> k3((size_t)3 * p.Mtot);
+  std::vector<double> tau((size_t)p.Mtot);
+  for (int64_t j = 0; j < p.Mtot; ++j) {
+    tau[j] = ((double)p.h_t[j] - ts) * omega / N3;
+    k3[3 * j + 0] = (float)tau[j];
+    k3[3 * j + 1] = p.h_k[2 * j + 0];
+    k3[3 * j + 2] = p.h_k[2 * j + 1];
+  }
+  CHK(build_samples(*q, k3.data(), false));
+  q->h_k = k3;
+  q->have_k = true;
+  CHK(b0_weights(p));
+  std::vector<float> phinv((size_t)p.Mtot);
+  for (int64_t s = 0; s < p.Mtot; ++s) {
+    const double a = 2.0 * M_PI * tau[q->h_perm[s]];
+    const double s2 = b3 * b3 - a * a;
+    phinv[s] = (float)(1.0 / bessel_i0(m * std::sqrt(s2 > 0 ? s2 : 0.0)));
+  }
+  std::vector<float2> ph((size_t)p.Ntot);
+  std::vector<float> nu((size_t)p.Ntot);
+  for (int64_t i = 0; i < p.Ntot; ++i) {
+    const double b = p.h_b0[i];
+    ph[i] = make_float2((float)std::cos(-2.0 * M_PI * b * ts), (float)std::sin(-2.0 * M_PI * b * ts));
+    nu[i] = (float)(N3 * b / omega);
+  }
+  CHK(dalloc(&p.b0_phase, ph.size()));
+  CHK(dalloc(&p.b0_nu, nu.size()));
+  CHK(dalloc(&p.b0_phinv, phinv.size()));
+  HIPCHK(hipMemcpy(p.b0_phase, ph.data(), sizeof(float2) * ph.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.b0_nu, nu.data(), sizeof(float) * nu.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.b0_phinv, phinv.data(), sizeof(float) * phinv.size(), hipMemcpyHostToDevice));
+  p.b0_N3 = N3;
+  p.b0_b3 = (float)b3;
+  p.b0_ts = ts;
+  p.b0_omega = omega;
+  return CGS_OK;
+}
+
+// y (helper's sample order) = B0-NFFT of the coil images X (p's coil layout),
+// with the damping mask on X when masked
+static int b0_forward(Plan& p, const float2* X, float2* y, bool masked) {
+  Plan& q = *p.b0q;
+  launch_b0_spread(X, p.b0_phase, p.b0_nu, q.Xc, p.Ntot, p.C, p.b0_N3, p.m, p.b0_b3, masked ? p.N[0] / 2 : 0,
+                   p.N[0], p.N[1], p.stream);
+  CHK(fft_forward(q, 2, q.Xc, nullptr));
+  CHK(interp(q, y, nullptr));
+  launch_b0_scale(y, p.b0_phinv, y, q.Mtot, q.C, p.stream);
+  return CGS_OK;
+}
+
+// X (p's coil layout) = adjoint B0-NFFT of y (helper's order), optionally of w .* y
+static int b0_adjoint(Plan& p, const float2* y, bool weighted, float2* X) {
+  Plan& q = *p.b0q;
+  launch_b0_scale(y, p.b0_phinv, q.ybuf[2], q.Mtot, q.C, p.stream);
+  CHK(gridding(q, q.ybuf[2], weighted, nullptr));
+  CHK(fft_adjoint(q, q.Xc, nullptr));
+  launch_b0_gather(q.Xc, p.b0_phase, p.b0_nu, X, p.Ntot, p.C, p.b0_N3, p.m, p.b0_b3, p.stream);
+  return CGS_OK;
+}
+
 // ---- plan life cycle --------------------------------------------------------------------
 static void destroy_plan(Plan& p) {
   if (p.stream) (void)hipStreamSynchronize(p.stream);
+  b0_free(p);
   free_samples(p);
   for (int a = 0; a < 3; ++a) { dfree(p.fa[a].tw); dfree(p.ft[a].tw); dfree(p.dinv[a]); }
   toep_drop(p);
@@ -1030,6 +1221,7 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   if (const char* e = std::getenv("CGS_GRID_ATOMIC")) p.use_grid_atomic = std::atoi(e) != 0;
   if (const char* e = std::getenv("CGS_GRID_SWEEP")) p.use_grid_sweep = p.sweep_allowed = std::atoi(e) != 0;
   if (const char* e = std::getenv("CGS_SWEEP_W")) p.sweep_W = std::max(4, std::atoi(e));
+  if (const char* e = std::getenv("CGS_SWEEP_K")) p.sweep_K = std::max(16, std::atoi(e));
 
   if (flags & 0x8000u) {   // internal: caller-provided stream set later
   }
@@ -1096,6 +1288,12 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
 
 static int require_ready(const Plan& p, bool need_S) {
   if (!p.have_k) { set_error("trajectory not set (cgs_set_trajectory)"); return CGS_ERR_STATE; }
+  if (need_S && p.b0q) {
+    // codeare's NCSENSE reads "b0" but passes it to none of its NFFTs
+    // (NCSENSE.hpp:100-103): the coil operators have no off-resonance model
+    set_error("the NCSENSE operators do not model off-resonance: clear b0 (cgs_set_b0 with NULL)");
+    return CGS_ERR_UNSUPPORTED;
+  }
   if (!p.have_w) { set_error("weights not set (cgs_set_weights)"); return CGS_ERR_STATE; }
   if (need_S && !p.have_S) { set_error("sensitivities not set (cgs_set_sensitivities)"); return CGS_ERR_STATE; }
   return CGS_OK;
@@ -1695,6 +1893,7 @@ int cgs_set_stream(cgs_plan plan, void* stream) {
   if (p.own_stream) (void)hipStreamDestroy(p.stream);
   p.stream = (hipStream_t)stream;
   p.own_stream = false;
+  if (p.b0q) p.b0q->stream = p.stream;
   return CGS_OK;
   GUARD_END
 }
@@ -1715,6 +1914,7 @@ int cgs_set_trajectory(cgs_plan plan, const float* k) {
   CHK(build_samples(p, k));
   p.h_k.assign(k, k + (size_t)p.d * p.Mtot);
   toep_drop(p);
+  if (!p.h_b0.empty()) CHK(b0_build(p));   // the off-resonance helper follows the trajectory
   p.have_k = true;
   p.have_w = false;   // weights follow the sample order (NFFT.hpp:258-296: Weights after KSpace)
   if (p.flags & CGS_AUTOTUNE) CHK(autotune(p, 5, true, nullptr));
@@ -1864,6 +2064,44 @@ int cgs_autotune(cgs_plan plan, int reps, int use_cache, cgs_tune_report* report
   GUARD_END
 }
 
+int cgs_set_b0(cgs_plan plan, const float* b0, const float* t) {
+  GUARD_BEGIN
+  if (!plan) { set_error("null argument"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  HIPCHK(hipStreamSynchronize(p.stream));
+  if (!b0) {   // back to the plain NFFT
+    b0_free(p);
+    p.h_b0.clear();
+    p.h_t.clear();
+    return CGS_OK;
+  }
+  if (!t) { set_error("timing required with b0"); return CGS_ERR_ARG; }
+  if (p.d != 2 || p.F != 1) { set_error("the off-resonance model is 2D, one frame (codeare's mri_inh_3d plan)"); return CGS_ERR_UNSUPPORTED; }
+  if (!p.have_k) { set_error("set the trajectory before b0 and timing"); return CGS_ERR_STATE; }
+  for (int64_t i = 0; i < p.Ntot; ++i)
+    if (!std::isfinite(b0[i])) { set_error("b0 is not finite"); return CGS_ERR_ARG; }
+  for (int64_t j = 0; j < p.Mtot; ++j)
+    if (!std::isfinite(t[j])) { set_error("timing is not finite"); return CGS_ERR_ARG; }
+  p.h_b0.assign(b0, b0 + (size_t)p.Ntot);
+  p.h_t.assign(t, t + (size_t)p.Mtot);
+  const int rc = b0_build(p);
+  if (rc != CGS_OK) { b0_free(p); p.h_b0.clear(); p.h_t.clear(); }
+  return rc;
+  GUARD_END
+}
+
+int cgs_b0_info(cgs_plan plan, int* N3, double* omega, double* ts) {
+  GUARD_BEGIN
+  if (!plan) { set_error("null argument"); return CGS_ERR_ARG; }
+  const Plan& p = plan->p;
+  if (!p.b0q) { set_error("no off-resonance model on this plan"); return CGS_ERR_STATE; }
+  if (N3) *N3 = p.b0_N3;
+  if (omega) *omega = p.b0_omega;
+  if (ts) *ts = p.b0_ts;
+  return CGS_OK;
+  GUARD_END
+}
+
 int cgs_set_weights(cgs_plan plan, const float* w) {
   GUARD_BEGIN
   if (!plan || !w) { set_error("null argument"); return CGS_ERR_ARG; }
@@ -1875,6 +2113,7 @@ int cgs_set_weights(cgs_plan plan, const float* w) {
   HIPCHK(hipMemcpy(p.w, ws.data(), sizeof(float) * ws.size(), hipMemcpyHostToDevice));
   CHK(sweep_weights(p, ws));
   p.h_w.assign(w, w + (size_t)p.Mtot);
+  CHK(b0_weights(p));
   toep_drop(p);
   p.have_w = true;
   return CGS_OK;
@@ -1903,6 +2142,18 @@ int cgs_set_sensitivities(cgs_plan plan, const void* S) {
   HIPCHK(hipStreamSynchronize(p.stream));
   HIPCHK(hipMemcpy(p.S, si.data(), sizeof(float2) * si.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p.Imap, I.data(), sizeof(float) * I.size(), hipMemcpyHostToDevice));
+  p.have_S = true;
+  return CGS_OK;
+  GUARD_END
+}
+
+int cgs_set_sensitivities_device(cgs_plan plan, const void* S) {
+  GUARD_BEGIN
+  if (!plan || !S) { set_error("null argument"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  const int64_t nm = p.Sfs ? p.F : 1;
+  launch_maps_load((const float2*)S, p.S, p.Imap, p.Ntot, nm, p.C, p.stream);
+  CHK(check_launch());
   p.have_S = true;
   return CGS_OK;
   GUARD_END
@@ -1981,9 +2232,14 @@ int cgs_nfft_forward(cgs_plan plan, int frame, const void* img, void* f, int bat
   float2* fd = (p.flags & CGS_DEVICE_IO) ? (float2*)f : (float2*)p.ext_a;
   mask_one_frame(p, frame);
   launch_img_ext2coil(p, id, p.Xc, batch, frame, p.stream);
-  CHK(fft_forward(p, 2, p.Xc, p.fmask));
-  CHK(interp(p, p.ybuf[1], p.fmask));
-  launch_y_int2ext(p, p.ybuf[1], fd, batch, frame, p.stream);
+  if (p.b0q) {   // off-resonance model: samples in the helper plan's order
+    CHK(b0_forward(p, p.Xc, p.b0q->ybuf[1], false));
+    launch_y_int2ext(*p.b0q, p.b0q->ybuf[1], fd, batch, frame, p.stream);
+  } else {
+    CHK(fft_forward(p, 2, p.Xc, p.fmask));
+    CHK(interp(p, p.ybuf[1], p.fmask));
+    launch_y_int2ext(p, p.ybuf[1], fd, batch, frame, p.stream);
+  }
   CHK(check_launch());
   CHK(stage_out(p, f, fd, sizeof(float2) * Mf * batch));
   return CGS_OK;
@@ -2001,9 +2257,14 @@ int cgs_nfft_adjoint(cgs_plan plan, int frame, const void* f, void* img, int bat
   CHK(stage_in(p, f, (float2*)p.ext_a, sizeof(float2) * Mf * batch, &fd));
   float2* id = (p.flags & CGS_DEVICE_IO) ? (float2*)img : (float2*)p.ext_b;
   mask_one_frame(p, frame);
-  launch_y_ext2int(p, fd, p.ybuf[1], batch, frame, p.stream);
-  CHK(gridding(p, p.ybuf[1], false, p.fmask));
-  CHK(fft_adjoint(p, p.Xc, p.fmask));
+  if (p.b0q) {
+    launch_y_ext2int(*p.b0q, fd, p.b0q->ybuf[1], batch, frame, p.stream);
+    CHK(b0_adjoint(p, p.b0q->ybuf[1], false, p.Xc));
+  } else {
+    launch_y_ext2int(p, fd, p.ybuf[1], batch, frame, p.stream);
+    CHK(gridding(p, p.ybuf[1], false, p.fmask));
+    CHK(fft_adjoint(p, p.Xc, p.fmask));
+  }
   launch_img_coil2ext(p, p.Xc, id, batch, frame, p.stream);
   CHK(check_launch());
   CHK(stage_out(p, img, id, sizeof(float2) * p.Ntot * batch));
@@ -2022,8 +2283,26 @@ int cgs_nfft_inverse_ref(cgs_plan plan, int frame, const void* f, void* img, int
   CHK(stage_in(p, f, (float2*)p.ext_a, sizeof(float2) * Mf * batch, &fd));
   float2* id = (p.flags & CGS_DEVICE_IO) ? (float2*)img : (float2*)p.ext_b;
   mask_one_frame(p, frame);
-  launch_y_ext2int(p, fd, p.ybuf[0], batch, frame, p.stream);
-  CHK(cgnr(p, p.ybuf[0], maxit, eps, p.fmask));
+  if (p.b0q) {
+    // the CGNR on the off-resonance model: its sample vectors in the helper's
+    // order, so the sample reductions read the helper's weights and frames
+    Plan& q = *p.b0q;
+    launch_y_ext2int(q, fd, p.ybuf[0], batch, frame, p.stream);
+    CnrOps ops;
+    ops.adj_w = [&](const float2* r) { return b0_adjoint(p, r, true, p.Xc); };
+    ops.fwd_masked = [&](const float2* x, float2* v) { return b0_forward(p, x, v, true); };
+    float* w_keep = p.w;
+    int* sframe_keep = p.sframe;
+    p.w = q.w;
+    p.sframe = q.sframe;
+    const int rc = cgnr(p, p.ybuf[0], maxit, eps, p.fmask, &ops);
+    p.w = w_keep;
+    p.sframe = sframe_keep;
+    CHK(rc);
+  } else {
+    launch_y_ext2int(p, fd, p.ybuf[0], batch, frame, p.stream);
+    CHK(cgnr(p, p.ybuf[0], maxit, eps, p.fmask));
+  }
   launch_img_coil2ext(p, p.Xc2, id, batch, frame, p.stream);
   CHK(check_launch());
   CHK(stage_out(p, img, id, sizeof(float2) * p.Ntot * batch));

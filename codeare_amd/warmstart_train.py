@@ -129,11 +129,20 @@ def _as_hw(v, N):
     return v.view(N, N).cpu().numpy()
 
 
-def generate_pairs(pipe: Pipeline, items, lam, target_tol=1e-7, maxit=2000, log=print):
+def generate_pairs(pipe: Pipeline, items, lam, target_tol=1e-7, maxit=2000, log=print, keep_maps=False):
+    """(X, T, iterations, maps): network inputs and CG targets; with keep_maps
+    the items' maps as one device tensor (n, Ntot * C) (16.8 MB per item at
+    256^2 x 32 coils)."""
     X, T, its = [], [], []
     t0 = time.perf_counter()
     N = pipe.N
+    maps = None
+    if keep_maps:
+        t = pipe.torch
+        maps = t.empty((len(items), N * N * pipe.C), dtype=t.complex64, device=pipe.dev)
     for n, (it, prob) in enumerate(_problems(items)):
+        if keep_maps:
+            maps[n].copy_(pipe.torch.from_numpy(np.asarray(prob.maps).reshape(-1, order="F")))
         pipe.load(prob, it)
         xa, s = pipe.net_input(lam)
         X.append(_as_hw(xa, N) / s)
@@ -143,22 +152,69 @@ def generate_pairs(pipe: Pipeline, items, lam, target_tol=1e-7, maxit=2000, log=
         if (n + 1) % 64 == 0:
             log(f"  pairs {n + 1}/{len(items)}: {time.perf_counter() - t0:.1f} s, "
                 f"CG iterations to {target_tol:g}: mean {np.mean(its):.0f}")
-    return np.stack(X).astype(np.complex64), np.stack(T).astype(np.complex64), its
+    return np.stack(X).astype(np.complex64), np.stack(T).astype(np.complex64), its, maps
 
 
-def train(net: UNet, X, T, epochs=20, batch=16, lr=1e-3, seed=0, log=print):
+def train(net: UNet, X, T, epochs=20, batch=16, lr=1e-3, seed=0, log=print, loss="l2", pipe=None, lam=None,
+          lmax=None, maps=None):
+    """Adam on mini-batches.
+
+    loss "l2": the image error |x0 - x*|^2 (the library's own training step).
+    loss "anorm": the A-norm error e^H A e, e = x0 - x*, A = (E^H W E +
+    lambda) / lambda_max with each item's own maps; its gradient 2 A e takes
+    one Toeplitz normal-operator application per image.  loss "resid": the
+    normal-equation residual |A e|^2 = |A x0 - A x*|^2, CG's stopping
+    quantity; gradient 2 A^2 e, two applications (A is self-adjoint).  Both
+    enter the library's L2 step through the shifted target t' = x0 - g / 2
+    (g the loss gradient), whose L2 gradient is g up to a constant factor.
+    X, T: (n, H, W) complex64 host arrays; maps: device tensor (n, Ntot * C)
+    of the items' maps for the operator losses."""
     rng = np.random.default_rng(seed)
     n = X.shape[0]
     hist = []
     t0 = time.perf_counter()
+    if loss != "l2":
+        torch = pipe.torch
+        dev = pipe.dev
+        Xd = torch.from_numpy(X).to(dev)
+        Td = torch.from_numpy(T).to(dev)
+        H, W = Xd.shape[1], Xd.shape[2]
+        out = torch.empty((batch, H, W), dtype=torch.complex64, device=dev)
+        tp = torch.empty_like(out)
+        q = torch.empty(H * W, dtype=torch.complex64, device=dev)
+        q2 = torch.empty_like(q)
     for ep in range(epochs):
         order = rng.permutation(n)
         ls = []
         for s in range(0, n - batch + 1, batch):
             idx = np.sort(order[s:s + batch])
-            ls.append(net.train_step(X[idx], T[idx], lr=lr))
+            if loss == "l2":
+                ls.append(net.train_step(X[idx], T[idx], lr=lr))
+                continue
+            ii = torch.from_numpy(idx).to(dev)
+            xb = Xd.index_select(0, ii).contiguous()
+            torch.cuda.synchronize()
+            net.forward(xb, out=out)
+            torch.cuda.synchronize()
+            e = out - Td.index_select(0, ii)
+            val = torch.zeros((), dtype=torch.float64, device=dev)
+            for b in range(batch):
+                eb = e[b].reshape(-1).contiguous()
+                pipe.P.sensitivities_device(maps[int(idx[b])])
+                pipe.P.normal(eb, lam=lam, out=q, toeplitz=True)
+                if loss == "anorm":
+                    q.div_(lmax)
+                    val += torch.vdot(eb, q).real.double()
+                else:
+                    pipe.P.normal(q, lam=lam, out=q2, toeplitz=True)
+                    val += (q.abs() ** 2).sum().double() / (lmax * lmax)
+                    q.copy_(q2).div_(lmax * lmax)
+                tp[b] = out[b] - q.view(H, W)
+            torch.cuda.synchronize()
+            net.train_step(xb, tp, lr=lr)
+            ls.append(float(val) / (batch * H * W))
         hist.append(float(np.mean(ls)))
-        log(f"  epoch {ep + 1}/{epochs}: loss {hist[-1]:.4e}  ({time.perf_counter() - t0:.1f} s)")
+        log(f"  epoch {ep + 1}/{epochs}: loss ({loss}) {hist[-1]:.4e}  ({time.perf_counter() - t0:.1f} s)")
     return hist
 
 
@@ -234,6 +290,8 @@ def main(argv=None):
     ap.add_argument("--target-tol", type=float, default=1e-7)
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--lambda-rel", type=float, default=1e-3, help="lambda = this x lambda_max")
+    ap.add_argument("--loss", choices=("l2", "anorm", "resid"), default="l2",
+                    help="training loss: image L2 error, the A-norm error, or the normal-equation residual")
     ap.add_argument("--out", default="")
     ap.add_argument("--report", default="")
     a = ap.parse_args(argv)
@@ -249,22 +307,26 @@ def main(argv=None):
     log(f"U-Net: {net.nparams} parameters")
     log(f"generating {a.train} training pairs (CG to {a.target_tol:g})")
     t0 = time.perf_counter()
-    X, T, its = generate_pairs(pipe, list(range(a.train)), lam, a.target_tol, log=log)
+    X, T, its, maps = generate_pairs(pipe, list(range(a.train)), lam, a.target_tol, log=log,
+                                     keep_maps=a.loss != "l2")
     t_gen = time.perf_counter() - t0
     log(f"training: {a.epochs} epochs, batch {a.batch}")
     net.profile(1)
     t0 = time.perf_counter()
-    hist = train(net, X, T, a.epochs, a.batch, a.lr, log=log)
+    hist = train(net, X, T, a.epochs, a.batch, a.lr, log=log, loss=a.loss, pipe=pipe, lam=lam, lmax=lmax,
+                 maps=maps)
+    del maps
     t_train = time.perf_counter() - t0
     prof = net.profile(0)
     if a.out:
-        np.savez(a.out, theta=net.params(), H=net.H, W=net.W, levels=net.levels, lam=lam, lambda_max=lmax)
+        np.savez(a.out, theta=net.params(), H=net.H, W=net.W, levels=net.levels, lam=lam, lambda_max=lmax,
+                 loss=a.loss, train_items=a.train, epochs=a.epochs)
     log(f"evaluating on {a.eval} held-out items")
     rows = evaluate(net, pipe, list(range(EVAL_BASE, EVAL_BASE + a.eval)), lam, a.tol, log=log)
     summ = summarize(rows)
     mfma = {k: (prof[k + "_flop"] / (prof[k + "_ms"] * 1e-3) / 1e12 if prof[k + "_ms"] > 0 else 0.0)
             for k in ("fwd", "dgrad", "wgrad")}
-    rep = {"train_items": a.train, "eval_items": a.eval, "epochs": a.epochs, "batch": a.batch,
+    rep = {"train_items": a.train, "eval_items": a.eval, "epochs": a.epochs, "batch": a.batch, "loss": a.loss,
            "lr": a.lr, "target_tol": a.target_tol, "tol": a.tol, "lambda": lam, "lambda_max": lmax,
            "pair_generation_s": t_gen, "train_s": t_train, "target_cg_iters_mean": float(np.mean(its)),
            "loss_history": hist, "conv_TFLOPs": mfma, "kernel_ms": prof, "eval": summ, "rows": rows}

@@ -123,6 +123,28 @@ int cgs_set_weights(cgs_plan plan, const float* w);
  * or (N^d, F, C) with CGS_MAPS_PER_FRAME; also computes IntensityMap
  * (MRI.hpp:6-30), per frame with per-frame maps. */
 int cgs_set_sensitivities(cgs_plan plan, const void* S);
+/* The same with S a device pointer (same layout), loaded and the intensity map
+ * computed on the plan's stream, bit-identical to cgs_set_sensitivities; the
+ * caller keeps S alive until the stream has passed the call.  No counterpart
+ * in the reference: for callers that keep many map sets on the device and
+ * switch between them per solve (training loops, batched series). */
+int cgs_set_sensitivities_device(cgs_plan plan, const void* S);
+
+/* Off-resonance model of the NFFT operators: NFFT<cxfl> with "b0" and "timing"
+ * (NFFT.hpp:117-173, NFFTTraits.hpp:91-108,150-166,188-216, NFFT3's
+ * mri_inh_3d plan).  b0: host (N0, N1) float field (cycles per unit of t), t:
+ * host (M_total) float sample times.  cgs_nfft_forward / _adjoint /
+ * _inverse_ref then apply
+ *   f_j = sum_r x(r) exp(-2 pi i k_j.r) exp(-2 pi i b0(r) t_j)
+ * (and its adjoint, and the CGNR on it), with the time-frequency term as a
+ * third NFFT dimension of oversampling 1.2 and window cut-off m.  2D, one
+ * frame; needs the trajectory first (a later cgs_set_trajectory rebuilds the
+ * model).  b0 = NULL removes it.  The NCSENSE operators (cgs_E, cgs_EH,
+ * cgs_normal, cgs_solve) refuse a plan with b0 set: codeare's NCSENSE reads b0
+ * but gives it to none of its NFFTs (NCSENSE.hpp:100-103). */
+int cgs_set_b0(cgs_plan plan, const float* b0, const float* t);
+/* Third-axis length N3, frequency scale omega and mid time ts of the model. */
+int cgs_b0_info(cgs_plan plan, int* N3, double* omega, double* ts);
 
 /* --- NFFT<cxfl> operators (one frame at a time, batch images) -------------- */
 /* nfft_trafo of `batch` images (N^d, batch) -> (M_f, batch) for frame `frame`

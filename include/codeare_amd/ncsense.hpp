@@ -75,8 +75,23 @@ class NFFT {
   NFFT(const NFFT&) = delete;
   NFFT& operator=(const NFFT&) = delete;
 
-  void KSpace(const RealMatrix& k) { check(cgs_set_trajectory(plan_, k.Ptr())); }
+  void KSpace(const RealMatrix& k) {
+    check(cgs_set_trajectory(plan_, k.Ptr()));
+    if (!b0_.empty()) check(cgs_set_b0(plan_, b0_.data(), t_.data()));
+  }
   void Weights(const RealMatrix& w) { check(cgs_set_weights(plan_, w.Ptr())); }
+  // codeare's "b0" (N0, N1) and "timing" (nk) Params (NFFT.hpp:117-173): the
+  // off-resonance model of Trafo and Adjoint, applied once KSpace is set
+  void B0(const RealMatrix& b0, const RealMatrix& timing) {
+    b0_.assign(b0.Ptr(), b0.Ptr() + b0.Size());
+    t_.assign(timing.Ptr(), timing.Ptr() + timing.Size());
+    if (b0_.size() != (size_t)N_[0] * (size_t)N_[1] || t_.size() != (size_t)nk_) {
+      b0_.clear();
+      t_.clear();
+      throw std::runtime_error("NFFT: b0 must fit the image and timing the trajectory");
+    }
+    check(cgs_set_b0(plan_, b0_.data(), t_.data()));
+  }
 
   Matrix Trafo(const Matrix& img) const {            // NFFT.hpp:305-361 (per slice)
     Matrix out(ncart_ > 1 ? std::vector<size_t>{(size_t)nk_, (size_t)ncart_} : std::vector<size_t>{(size_t)nk_});
@@ -103,6 +118,7 @@ class NFFT {
   }
   cgs_plan plan_ = nullptr;
   std::vector<int> N_;
+  std::vector<float> b0_, t_;
   int ncart_ = 1;
   int64_t nk_ = 0;
   int maxit_ = 3;

@@ -65,18 +65,21 @@ def _plan_env(cgs, env, N, M, sigma, m, C, k, w, S, foff=None):
     return P
 
 
-@pytest.mark.parametrize("N,sigma,m,C,F", [
-    (64, 2.0, 3, 32, 1),     # 32 coils: one lane group per wave
-    (96, 2.0, 3, 16, 2),     # 16 coils: two lane groups, two frames
-    (128, 2.0, 3, 8, 1),     # 8 coils: four lane groups
-    (48, 2.0, 2, 16, 1),     # m = 2: 6 taps (n = 96)
-    (64, 1.0, 1, 12, 1),     # sigma 1, m 1 (codeare's defaults): 4 taps, 12 coils (padded lanes)
-    (16, 1.5, 3, 48, 1),     # n = 24 < one segment: footprints wrap inside it; 48 coils: two waves
+@pytest.mark.parametrize("N,sigma,m,C,F,K", [
+    (64, 2.0, 3, 32, 1, 256),     # 32 coils: one lane group per wave
+    (64, 2.0, 3, 32, 1, 32),      # line segments split into pieces of 32 entries + reduce
+    (96, 2.0, 3, 16, 2, 256),     # 16 coils: two lane groups, two frames
+    (96, 2.0, 3, 16, 2, 40),      # ... with split segments
+    (128, 2.0, 3, 8, 1, 256),     # 8 coils: four lane groups
+    (48, 2.0, 2, 16, 1, 64),      # m = 2: 6 taps (n = 96)
+    (64, 1.0, 1, 12, 1, 256),     # sigma 1, m 1 (codeare's defaults): 4 taps, 12 coils (padded lanes)
+    (16, 1.5, 3, 48, 1, 24),      # n = 24 < one segment: footprints wrap inside it; 48 coils: two waves
 ])
-def test_line_sweep_scatter_matches_tile_scatter(cgs, N, sigma, m, C, F):
-    """The line-sweep scatter (2D default) against the tile-owner scatter, on
-    radial and random trajectories: E^H (weighted) and the unweighted adjoint
-    NFFT of every coil, one frame or several."""
+def test_line_sweep_scatter_matches_tile_scatter(cgs, N, sigma, m, C, F, K):
+    """The line-sweep scatter against the tile-owner scatter, on radial and
+    random trajectories: E^H (weighted) and the unweighted adjoint NFFT of
+    every coil, one frame or several, with line segments whole or split into
+    pieces of K entries."""
     from codeare_amd import data
     from helpers import rand_img
     rng = np.random.default_rng(N + C + m)
@@ -92,12 +95,49 @@ def test_line_sweep_scatter_matches_tile_scatter(cgs, N, sigma, m, C, F):
     w = rng.uniform(0.1, 1.0, M).astype(np.float32)
     S = data.coil_maps(N, 2, C, rng)
     y = rand_img(rng, (M * C,))
-    Ps = _plan_env(cgs, {"CGS_GRID_SWEEP": "1"}, (N, N), M, sigma, m, C, k, w, S, foff)
+    Ps = _plan_env(cgs, {"CGS_GRID_SWEEP": "1", "CGS_SWEEP_K": str(K)}, (N, N), M, sigma, m, C, k, w, S, foff)
     Pt = _plan_env(cgs, {"CGS_GRID_SWEEP": "0"}, (N, N), M, sigma, m, C, k, w, S, foff)
     e1 = rel(Ps.EH(y), Pt.EH(y))
     e2 = 0.0
     if F == 1:
         yb = y.reshape((M, C), order="F")
         e2 = rel(Ps.nfft_adjoint(yb), Pt.nfft_adjoint(yb))
-    print(f"N={N} sigma={sigma} m={m} C={C} F={F}: E^H {e1:.2e}, adjoint NFFT {e2:.2e}")
+    print(f"N={N} sigma={sigma} m={m} C={C} F={F} K={K}: E^H {e1:.2e}, adjoint NFFT {e2:.2e}")
     assert e1 <= 1e-6 and e2 <= 1e-6
+
+
+@pytest.mark.parametrize("per_frame", [False, True])
+def test_device_maps_load_matches_host_load(cgs, per_frame):
+    """cgs_set_sensitivities_device (maps already on the device, loaded and the
+    intensity map formed on the GPU) gives bit-identical E and E^H to the host
+    loader, for shared and per-frame maps."""
+    import torch
+    from codeare_amd import data
+    rng = np.random.default_rng(3)
+    N, C, F, spokes, R = 48, 5, 3, 21, 96
+    ks = [data.radial_2d(spokes, R, True, f * spokes) for f in range(F)]
+    k = np.asfortranarray(np.concatenate(ks, axis=1))
+    off = np.arange(F + 1, dtype=np.int64) * spokes * R
+    w = np.abs(np.sqrt(k[0] ** 2 + k[1] ** 2)).astype(np.float32) + 1e-3
+    nm = F if per_frame else 1
+    S = np.asfortranarray(np.stack([data.coil_maps(N, 2, C, rng) for _ in range(nm)], axis=2)
+                          .reshape(N, N, nm, C) if per_frame else data.coil_maps(N, 2, C, rng))
+    x = torch.from_numpy((rng.normal(size=N * N * F) + 1j * rng.normal(size=N * N * F)).astype(np.complex64)).cuda()
+    out = {}
+    for mode in ("host", "device"):
+        P = cgs.Plan((N, N), int(k.shape[1]), 2.0, 3, ncoils=C, frame_offsets=off, maps_per_frame=per_frame,
+                     device_io=True)
+        P.kspace(k)
+        P.weights(w)
+        if mode == "host":
+            P.sensitivities(S)
+        else:
+            P.sensitivities_device(torch.from_numpy(np.asarray(S).reshape(-1, order="F")).cuda())
+        y = torch.empty(int(k.shape[1]) * C, dtype=torch.complex64, device="cuda")
+        xb = torch.empty_like(x)
+        P.E(x, out=y)
+        P.EH(y, out=xb)
+        P.sync()
+        out[mode] = (y.cpu().numpy(), xb.cpu().numpy())
+    assert np.array_equal(out["host"][0], out["device"][0])
+    assert np.array_equal(out["host"][1], out["device"][1])
