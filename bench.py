@@ -52,8 +52,11 @@ def parse():
     ap.add_argument("--nufft-only", action="store_true",
                     help="only time the NuFFT operators (E, then E^H W) at 256^2 x 32 coils and the 128^3 x 16-coil "
                          "koosh ball, `steps` applications each (for kernel profiles)")
+    ap.add_argument("--variants", action="store_true",
+                    help="with --nufft-only: also run every alternative gridding / gather kernel (one plan each)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with 3 device copies of 256 MiB (counter calibration under rocprofv3 --pmc)")
+    ap.add_argument("--warm-seeds", type=int, default=16, help="held-out seeds the warm-start leg evaluates")
     ap.add_argument("--weights", default=os.path.join(ROOT, "codeare_amd", "weights", "unet_golden2d_256_c32.npz"),
                     help="trained warm-start network (skipped if absent)")
     return ap.parse_args()
@@ -94,7 +97,10 @@ def main():
     from codeare_amd import cgsense, data
 
     if args.nufft_only:
-        print(json.dumps(nufft_leg(args)))
+        out = nufft_leg(args)
+        if args.copy_calibration:
+            copy_calibration(out)
+        print(json.dumps(out))
         return
     prob = data.make_problem(args.workload, 0)
     wl = prob.wl
@@ -259,13 +265,37 @@ def main():
         sc = scaled if not args.no_extra else (None, None, None)
         out["error_vs_fp64_nudft"] = fp64_errors(prob, y_noisy, x_head, iters[-1], lam, sc[0], sc[1], sc[2])
     if args.copy_calibration:
-        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = 2 * src.numel() * 4
+        copy_calibration(out)
     print(json.dumps(out))
+
+
+def copy_calibration(out):
+    """Three device copies of 256 MiB: the reference dispatch that scales the
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    out["copy_calibration_bytes"] = 2 * src.numel() * 4
+
+
+# alternative kernels of the NuFFT operators, selected by the plan's
+# environment switches at plan creation: (label, workload, switches)
+NUFFT_VARIANTS = [
+    ("2d_grid_bin", 2, {"CGS_GRID_BIN": "1"}),
+    ("2d_grid_bin_cg4", 2, {"CGS_GRID_BIN": "1", "CGS_BIN_CG": "4", "CGS_BIN_THREADS": "512"}),
+    ("3d_grid_bin", 3, {"CGS_GRID_BIN": "1"}),
+    ("2d_grid_sweep", 2, {"CGS_GRID_SWEEP": "1"}),
+    ("2d_grid_reg", 2, {"CGS_GRID_MFMA": "0"}),
+    ("2d_grid_batched", 2, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0"}),
+    ("2d_grid_atomic", 2, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0", "CGS_GRID_ATOMIC": "1"}),
+    ("2d_grid_plain", 2, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0", "CGS_GRID_BATCHED": "0"}),
+    ("2d_interp_global", 2, {"CGS_INTERP_LDS": "0"}),
+    ("3d_grid_reg3", 3, {"CGS_GRID_MFMA": "0"}),
+    ("3d_interp_global", 3, {"CGS_INTERP_LDS": "0"}),
+]
 
 
 def nufft_leg(args):
@@ -277,11 +307,25 @@ def nufft_leg(args):
     import torch
     from codeare_amd import cgsense, data
     out = {"metric": "NuFFT operator GB/s (algorithmic) vs HBM", "unit": "GB/s"}
-    for cfg in (2, 3):
-        prob = data.make_problem(cfg, 0)
+    runs = [(None, 2, {}), (None, 3, {})] + (NUFFT_VARIANTS if args.variants else [])
+    probs = {}
+    for label, cfg, env in runs:
+        if cfg not in probs:
+            probs[cfg] = data.make_problem(cfg, 0)
+        prob = probs[cfg]
         wl = prob.wl
         M = prob.k.shape[1]
-        plan = cgsense.Plan((wl.N,) * wl.d, M, wl.sigma, wl.m, ncoils=wl.coils, device_io=True, profile=True)
+        saved = {k: os.environ.get(k) for k in env}
+        os.environ.update(env)
+        try:
+            plan = cgsense.Plan((wl.N,) * wl.d, M, wl.sigma, wl.m, ncoils=wl.coils, device_io=True, profile=True)
+        finally:
+            for k, v in saved.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
+        print(f"nufft leg: {label or 'default'} ({wl.name})", file=sys.stderr, flush=True)
         plan.kspace(prob.k)
         plan.weights(prob.w)
         plan.sensitivities(prob.maps)
@@ -299,8 +343,8 @@ def nufft_leg(args):
             plan.EH(y, out=xb)
         plan.sync()
         per = per_kernel(plan.kernel_stats(reset=True))
-        out[wl.name] = {"applications": args.steps, "kernels": per,
-                        "frac_hbm": {k: round(v["GBps"] / HBM_PEAK_GBS, 4) for k, v in per.items()}}
+        out[label or wl.name] = {"applications": args.steps, "kernels": per,
+                                 "frac_hbm": {k: round(v["GBps"] / HBM_PEAK_GBS, 4) for k, v in per.items()}}
         del plan
         torch.cuda.empty_cache()
     return out
@@ -393,6 +437,40 @@ def warm_start_leg(args, plan, y, x, tol, xt):
     # the same comparison at looser stopping tolerances (iterations only)
     res["iters_plain_warm_by_tol"] = {f"{t:g}": [plain(t)["max_iters"], warm(t)["max_iters"]]
                                       for t in (1e-3, 1e-4, tol)}
+    res["note"] = ("workload item 0 is a training item; held_out_live are unseen seeds (items 5000+), "
+                   "c5_protocol_500 the 500-seed evaluation of the training run")
+    # held-out seeds, measured here (each with its own maps and noise)
+    cur = torch.cuda.current_stream()
+    try:
+        from codeare_amd import warmstart_train as WT
+        pipe = WT.Pipeline()   # makes its stream torch's current one
+        rows = WT.evaluate(net, pipe, list(range(WT.EVAL_BASE, WT.EVAL_BASE + args.warm_seeds)), lam, tol,
+                           log=lambda s: None)
+        summ = WT.summarize(rows)
+        res["held_out_live"] = {k: summ[k] for k in ("n", "iters_plain_mean", "iters_warm_mean", "iteration_ratio",
+                                                     "solves_per_s_plain", "solves_per_s_warm", "r0_warm_median",
+                                                     "r0_xa_median", "iters_saved_histogram")}
+        res["held_out_live"]["speedup"] = round(summ["solves_per_s_warm"] / summ["solves_per_s_plain"], 3)
+        del pipe
+    except Exception as e:   # the headline numbers stand without it
+        res["held_out_live"] = {"error": str(e)}
+    finally:
+        torch.cuda.synchronize()
+        torch.cuda.set_stream(cur)
+    rep_path = os.path.splitext(args.weights)[0] + "_report.json"
+    if os.path.exists(rep_path):
+        with open(rep_path) as f:
+            rep = json.load(f)
+        ev = rep["eval"]
+        res["c5_protocol_500"] = {
+            "train_items": rep["train_items"], "epochs": rep["epochs"], "loss": rep.get("loss", "l2"),
+            "eval_items": ev["n"], "iters_plain_mean": ev["iters_plain_mean"], "iters_warm_mean": ev["iters_warm_mean"],
+            "iters_plain_percentiles": ev.get("iters_plain_percentiles"),
+            "iters_warm_percentiles": ev.get("iters_warm_percentiles"),
+            "iters_saved_histogram": ev.get("iters_saved_histogram"),
+            "r0_warm_median": ev.get("r0_warm_median"), "r0_xa_median": ev.get("r0_xa_median"),
+            "warm_over_plain_time": round(ev["solves_per_s_warm"] / ev["solves_per_s_plain"], 3),
+            "source": os.path.relpath(rep_path, ROOT)}
     return res
 
 

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT_DIR, "libcgsense_hip.so")
 SOURCES = ["plan.hip", "kernels_fft.hip", "kernels_fft_reg.hip", "kernels_toep.hip", "kernels_grid.hip", "kernels_vec.hip",
-           "kernels_sweep.hip", "kernels_b0.hip",
+           "kernels_sweep.hip", "kernels_b0.hip", "kernels_bin.hip",
            "unet.hip"]
 ARCH = os.environ.get("CGS_OFFLOAD_ARCH", "gfx950")
 

@@ -45,6 +45,7 @@ EXPORTED = [
     "cgs_E", "cgs_EH", "cgs_normal", "cgs_solve", "cgs_solve_noalloc",
     "cgs_kernel_stats", "cgs_plan_info", "cgs_status_string", "cgs_last_error",
     "cgs_residual_history", "cgs_autotune", "cgs_profile_mask", "cgs_set_b0", "cgs_b0_info",
+    "cgs_gridding_kind",
 ]
 
 
@@ -65,7 +66,7 @@ class TuneReport(ct.Structure):
     _fields_ = [("visits_per_item", ct.c_int), ("grid_coils", ct.c_int), ("interp_coils", ct.c_int),
                 ("from_cache", ct.c_int), ("grid_ms", ct.c_double), ("interp_ms", ct.c_double),
                 ("default_grid_ms", ct.c_double), ("default_interp_ms", ct.c_double),
-                ("grid_mfma", ct.c_int), ("grid_sweep", ct.c_int)]
+                ("grid_mfma", ct.c_int), ("grid_sweep", ct.c_int), ("grid_bin", ct.c_int)]
 
 
 class SolveReport(ct.Structure):
@@ -96,6 +97,7 @@ def load_library(path: str | None = None):
     L.cgs_set_weights.argtypes = [P, P]
     L.cgs_set_sensitivities.argtypes = [P, P]
     L.cgs_set_sensitivities_device.argtypes = [P, P]
+    L.cgs_gridding_kind.argtypes = [P]
     L.cgs_nfft_forward.argtypes = [P, I, P, P, I]
     L.cgs_nfft_adjoint.argtypes = [P, I, P, P, I]
     L.cgs_nfft_inverse_ref.argtypes = [P, I, P, P, I, I, D]
@@ -350,6 +352,13 @@ class Plan:
                   "max_iters": rep.max_iters, "status": rep.status,
                   "done_ms": list(rep.done_ms[:F]), "total_ms": rep.total_ms}
         return x, report
+
+    def gridding_kind(self) -> str:
+        """The scatter the plan runs now: "tiles", "sweep" or "binned"."""
+        k = load_library().cgs_gridding_kind(self._h)
+        if k < 0:
+            _check(-k)
+        return {0: "tiles", 1: "sweep", 2: "binned"}[k]
 
     def autotune(self, reps: int = 5, use_cache: bool = False) -> dict:
         """Time the gridding/gather launch shapes on this trajectory, keep the fastest."""

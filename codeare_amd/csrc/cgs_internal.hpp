@@ -258,6 +258,27 @@ struct GridArgs {
   int64_t ntot, tilecells;
 };
 
+// Binned gridding (kernels_bin.hip): every sample once, in the bin (TB^d cells)
+// of its base cell; a work item adds its samples' footprints into the bin's
+// region (E = TB + taps - 1 cells per axis) in LDS and writes the region to its
+// slot of `reg`; a second pass sums, for every grid cell, the region cells that
+// map onto it.
+struct BinArgs {
+  const int* ids;         // sorted sample ids grouped by (frame, bin)
+  const int4* items;      // {frame * nbins + bin, first id, end id, region slot}, longest first
+  const int2* range;      // per (frame, bin): {first slot, number of slots} (a bin's slots are consecutive)
+  const float* brec;      // per id: its footprint offsets in the region (bits: r0 | r1 << 8 | r2 << 16)
+                          // and its d x taps window values
+  const float* w;         // density weights (of the sorted samples) or null
+  const float2* y;        // Y[s * C + c]
+  float2* reg;            // [item][group][E^d][CG]
+  float2* grid;
+  const int* fdone;
+  int d, taps, C, CG, ngroups, TB, E, P1;
+  int n0, n1, n2, nb0, nb1, nb2, F, nitems;
+  int64_t ntot, ecells, nbins;
+};
+
 constexpr int kSweepPad = 24;   // padding records after the line-sweep entries (>= 3 x the chunk size)
 
 // 2D gridding as a sweep along grid lines (kernels_sweep.hip): one wave per
@@ -360,6 +381,17 @@ struct Plan {
   int64_t n_sw_ent = 0;
   float* psi_rot = nullptr;
   std::vector<int4> h_sw_ent;         // host copy (the weighted field is refilled by cgs_set_weights)
+  // binned gridding tables
+  bool use_grid_bin = false;          // binned LDS scatter (not with CGS_DETERMINISTIC)
+  int bin_TB = 0, bin_cg = 0, bin_item_max = 0, bin_threads = 1024;
+  int* bin_ids = nullptr;
+  int4* bin_items = nullptr;
+  int2* bin_range = nullptr;
+  float* bin_rec = nullptr;
+  float2* bin_reg = nullptr;
+  int n_bin_items = 0;
+  int64_t bin_nbins = 0;
+  int bin_nb[3] = {1, 1, 1};
 
   // work buffers
   float2* grid = nullptr;             // F*ntot*C
@@ -541,7 +573,25 @@ void launch_b0_gather(const float2* X3, const float2* phase, const float* nu, fl
                       int m, float b3, hipStream_t s);
 void launch_b0_scale(const float2* y, const float* scale, float2* out, int64_t M, int C, hipStream_t s);
 void launch_zero(void* dst, size_t bytes, hipStream_t s);   // bytes a multiple of 4
+void launch_grid_bin(const BinArgs& a, int threads, hipStream_t s);
+void launch_grid_bin_sum(const BinArgs& a, hipStream_t s);
+size_t grid_bin_lds(int d, int taps, int CG, int E, int P1);
+int bin_row_stride(int E);
 void launch_maps_load(const float2* S, float2* Sd, float* I, int64_t Ntot, int64_t nm, int C, hipStream_t s);
+
+// read-only tables that a kernel reads at wave-uniform addresses: through the
+// constant address space the compiler issues scalar loads (s_load) for them
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using const_ptr = const __attribute__((address_space(4))) T*;
+#else   // the host pass only needs the kernels' signatures
+template <class T>
+using const_ptr = const T*;
+#endif
+template <class T>
+__device__ __forceinline__ const_ptr<T> as_const(const T* p) {
+  return (const_ptr<T>)(p);
+}
 
 // errors
 void set_error(const std::string& msg);

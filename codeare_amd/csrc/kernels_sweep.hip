@@ -26,20 +26,6 @@
 
 namespace cgs {
 
-// read-only tables that the kernel reads at wave-uniform addresses: through the
-// constant address space the compiler issues scalar loads (s_load) for them
-#if defined(__HIP_DEVICE_COMPILE__)
-template <class T>
-using const_ptr = const __attribute__((address_space(4))) T*;
-#else   // the host pass only needs the kernels' signatures
-template <class T>
-using const_ptr = const T*;
-#endif
-template <class T>
-__device__ __forceinline__ const_ptr<T> as_const(const T* p) {
-  return (const_ptr<T>)(p);
-}
-
 // Entries are taken P at a time: the chunk's loads (records, window values,
 // data) are all issued before its first FMA, so a wave keeps P loads in
 // flight.  Before an entry that starts at cell r0 every cell below r0 is

@@ -420,6 +420,10 @@ static int interp(Plan& p, float2* y, const int* fl) {
   return CGS_OK;
 }
 
+static void bin_args(const Plan& p, const float2* y, bool weighted, const int* fl, BinArgs& a);
+static int build_bins(Plan& p, const std::vector<int>& hbase, const std::vector<float>& hpsi,
+                      const std::vector<int>& hframe);
+
 static int gridding(Plan& p, const float2* y, bool weighted, const int* fl) {
   GridArgs a{};
   a.base = p.base; a.psi = p.psi; a.w = weighted ? p.w : nullptr;
@@ -437,6 +441,13 @@ static int gridding(Plan& p, const float2* y, bool weighted, const int* fl) {
   const double bytes = (double)p.Mtot * p.d * 4 + (double)p.Mtot * 4 + (double)p.C * p.Mtot * 8 +
                        (double)p.C * p.ntot * p.F * 8;
   Prof pr(p, CGS_K_GRID, bytes);
+  if (p.use_grid_bin && p.bin_ids) {
+    BinArgs ba;
+    bin_args(p, y, weighted, fl, ba);
+    launch_grid_bin(ba, p.bin_threads, p.stream);
+    launch_grid_bin_sum(ba, p.stream);
+    return CGS_OK;
+  }
   if (p.d == 2 && p.use_grid_sweep && p.sw_items) {
     SweepArgs sa{};
     sa.items = p.sw_items; sa.ent = p.sw_ent; sa.psi_rot = p.psi_rot;
@@ -587,9 +598,14 @@ static void free_sweep(Plan& p) {
   p.h_sw_ent.shrink_to_fit();
 }
 
-// frees the tile-sorted sample tables (and, with sweep, the line-sweep tables)
+static void free_bins(Plan& p) {
+  dfree(p.bin_ids); dfree(p.bin_items); dfree(p.bin_range); dfree(p.bin_rec); dfree(p.bin_reg);
+  p.n_bin_items = 0; p.bin_nbins = 0; p.bin_TB = 0;
+}
+
+// frees the tile-sorted sample tables (and, with sweep, the line-sweep and bin tables)
 static void free_samples(Plan& p, bool sweep = true) {
-  if (sweep) free_sweep(p);
+  if (sweep) { free_sweep(p); free_bins(p); }
   dfree(p.base); dfree(p.psi); dfree(p.perm); dfree(p.sframe);
   dfree(p.interp_items); dfree(p.tile_items); dfree(p.grid_items);
   p.n_tile_items = 0; dfree(p.reduce_items); dfree(p.visits); dfree(p.vrec); dfree(p.partial);
@@ -978,6 +994,7 @@ static int build_samples(Plan& p, const float* k, bool sweep = true) {
     HIPCHK(hipMemcpy(p.reduce_items, ritems.data(), sizeof(ReduceItem) * ritems.size(), hipMemcpyHostToDevice));
   if (!vis.empty()) HIPCHK(hipMemcpy(p.visits, vis.data(), sizeof(int) * vis.size(), hipMemcpyHostToDevice));
   if (sweep) CHK(build_sweep(p, hbase, hpsi, hframe));
+  if (sweep) CHK(build_bins(p, hbase, hpsi, hframe));
   p.n_interp_items = (int)items.size();
   p.n_grid_items = (int)gitems.size();
   p.n_reduce_items = (int)ritems.size();
@@ -985,6 +1002,109 @@ static int build_samples(Plan& p, const float* k, bool sweep = true) {
   p.n_slots = slots;
   p.h_perm = std::move(perm);
   return CGS_OK;
+}
+
+// Binned gridding tables (kernels_bin.hip).  The bin side TB (2D: 32, else 16;
+// 3D: 16, else 8) must divide every grid axis and leave the region, TB + taps - 1
+// cells, no longer than the axis; the region of one coil group must fit in LDS.
+// Otherwise (and with CGS_DETERMINISTIC) the plan has no bins.  Items hold at
+// most bin_item_max samples (2D 2048, 3D 32768; a heavy bin's chunks are equal)
+// and run longest first; each owns a region slot, a bin's slots consecutive.
+static int build_bins(Plan& p, const std::vector<int>& hbase, const std::vector<float>& hpsi,
+                      const std::vector<int>& hframe) {
+  free_bins(p);
+  if (p.flags & CGS_DETERMINISTIC) return CGS_OK;
+  const int d = p.d, taps = p.taps;
+  std::vector<int> cands = (d == 2) ? std::vector<int>{32, 16} : std::vector<int>{16, 8};
+  if (const char* e = std::getenv("CGS_BIN_TB")) cands = {std::atoi(e)};
+  int CG = p.bin_cg;
+  if (d == 2 && CG != 4 && CG != 8) CG = 8;
+  if (d == 3 && CG != 1 && CG != 2) CG = 1;
+  int TB = 0;
+  for (int tb : cands) {
+    if (tb < 8 || tb > 64) continue;
+    bool ok = grid_bin_lds(d, taps, CG, tb + taps - 1, bin_row_stride(tb + taps - 1)) <= 160 * 1024;
+    for (int a = 0; a < d; ++a) ok = ok && p.n[a] % tb == 0 && p.n[a] >= tb + taps - 1;
+    if (ok) { TB = tb; break; }
+  }
+  if (!TB) return CGS_OK;
+  const int E = TB + taps - 1;
+  int nb[3] = {1, 1, 1};
+  int64_t nbins = 1, ecells = 1;
+  for (int a = 0; a < d; ++a) { nb[a] = p.n[a] / TB; nbins *= nb[a]; ecells *= E; }
+  const int64_t M = p.Mtot, nkeys = nbins * p.F;
+  if (nkeys > INT32_MAX) return CGS_OK;
+  std::vector<int64_t> cnt((size_t)nkeys + 1, 0);
+  std::vector<int64_t> key((size_t)M);
+  for (int64_t s = 0; s < M; ++s) {
+    int64_t b = 0, mul = 1;
+    for (int a = 0; a < d; ++a) { b += (int64_t)(hbase[s * d + a] / TB) * mul; mul *= nb[a]; }
+    key[s] = (int64_t)hframe[s] * nbins + b;
+    cnt[key[s] + 1]++;
+  }
+  for (int64_t q = 0; q < nkeys; ++q) cnt[q + 1] += cnt[q];
+  std::vector<int> ids((size_t)M);
+  {
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t s = 0; s < M; ++s) ids[pos[key[s]]++] = (int)s;
+  }
+  // per id: packed offsets of the footprint in its bin's region, then the window values
+  const int R = 1 + d * taps;
+  std::vector<float> rec((size_t)M * R);
+  for (int64_t k = 0; k < M; ++k) {
+    const int64_t s = ids[k];
+    uint32_t off = 0;
+    for (int a = 0; a < d; ++a) off |= (uint32_t)(hbase[s * d + a] % TB) << (8 * a);
+    std::memcpy(&rec[k * R], &off, 4);
+    for (int q = 0; q < d * taps; ++q) rec[k * R + 1 + q] = hpsi[s * d * taps + q];
+  }
+  const int K = p.bin_item_max > 0 ? p.bin_item_max : (d == 2 ? 2048 : 32768);
+  std::vector<int4> items;
+  std::vector<int2> range((size_t)nkeys, make_int2(0, 0));
+  int slot = 0;
+  for (int64_t kk = 0; kk < nkeys; ++kk) {
+    const int64_t b0 = cnt[kk], n = cnt[kk + 1] - b0;
+    if (n == 0) continue;
+    const int nch = (int)((n + K - 1) / K);
+    range[kk] = make_int2(slot, nch);
+    for (int c = 0; c < nch; ++c)
+      items.push_back(make_int4((int)kk, (int)(b0 + n * c / nch), (int)(b0 + n * (c + 1) / nch), slot + c));
+    slot += nch;
+  }
+  std::stable_sort(items.begin(), items.end(), [](const int4& x, const int4& y) { return (x.z - x.y) > (y.z - y.y); });
+  const int ngroups = (p.C + CG - 1) / CG;
+  const size_t regn = (size_t)slot * ngroups * (size_t)ecells * CG;
+  if (regn * sizeof(float2) > ((size_t)16 << 30)) return CGS_OK;   // region buffer above 16 GB: no bins
+  CHK(dalloc(&p.bin_ids, ids.size()));
+  CHK(dalloc(&p.bin_items, std::max<size_t>(1, items.size())));
+  CHK(dalloc(&p.bin_range, range.size()));
+  CHK(dalloc(&p.bin_rec, rec.size()));
+  CHK(dalloc(&p.bin_reg, std::max<size_t>(1, regn)));
+  HIPCHK(hipMemcpy(p.bin_ids, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice));
+  if (!items.empty()) HIPCHK(hipMemcpy(p.bin_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.bin_range, range.data(), sizeof(int2) * range.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.bin_rec, rec.data(), sizeof(float) * rec.size(), hipMemcpyHostToDevice));
+  p.bin_TB = TB;
+  p.bin_cg = CG;
+  p.bin_nbins = nbins;
+  for (int a = 0; a < 3; ++a) p.bin_nb[a] = nb[a];
+  p.n_bin_items = (int)items.size();
+  return CGS_OK;
+}
+
+static void bin_args(const Plan& p, const float2* y, bool weighted, const int* fl, BinArgs& a) {
+  a = BinArgs{};
+  a.ids = p.bin_ids; a.items = p.bin_items; a.range = p.bin_range; a.brec = p.bin_rec;
+  a.w = weighted ? p.w : nullptr; a.y = y; a.reg = p.bin_reg; a.grid = p.grid; a.fdone = fl;
+  a.d = p.d; a.taps = p.taps; a.C = p.C; a.CG = p.bin_cg; a.ngroups = (p.C + p.bin_cg - 1) / p.bin_cg;
+  a.TB = p.bin_TB; a.E = p.bin_TB + p.taps - 1; a.P1 = bin_row_stride(a.E);
+  a.n0 = p.n[0]; a.n1 = p.n[1]; a.n2 = p.n[2];
+  a.nb0 = p.bin_nb[0]; a.nb1 = p.bin_nb[1]; a.nb2 = p.bin_nb[2];
+  a.F = p.F; a.nitems = p.n_bin_items;
+  a.ntot = p.ntot;
+  a.ecells = 1;
+  for (int k = 0; k < p.d; ++k) a.ecells *= a.E;
+  a.nbins = p.bin_nbins;
 }
 
 // ---- off-resonance (B0) model ------------------------------------------------------------
@@ -1219,6 +1339,11 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
     if (g == 8 || g == 16 || g == 32) p.grid_cgr = g;
   }
   if (const char* e = std::getenv("CGS_GRID_ATOMIC")) p.use_grid_atomic = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CGS_GRID_BATCHED")) p.use_grid_batched = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CGS_GRID_BIN")) p.use_grid_bin = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CGS_BIN_CG")) p.bin_cg = std::atoi(e);
+  if (const char* e = std::getenv("CGS_BIN_K")) p.bin_item_max = std::max(64, std::atoi(e));
+  if (const char* e = std::getenv("CGS_BIN_THREADS")) p.bin_threads = std::atoi(e);
   if (const char* e = std::getenv("CGS_GRID_SWEEP")) p.use_grid_sweep = p.sweep_allowed = std::atoi(e) != 0;
   if (const char* e = std::getenv("CGS_SWEEP_W")) p.sweep_W = std::max(4, std::atoi(e));
   if (const char* e = std::getenv("CGS_SWEEP_K")) p.sweep_K = std::max(16, std::atoi(e));
@@ -1929,6 +2054,7 @@ struct TuneChoice {
   double grid_ms, interp_ms, def_grid_ms, def_interp_ms;
   bool mfma;
   bool sweep;
+  bool bin;
 };
 static std::mutex g_tune_mu;
 static std::vector<std::pair<std::string, TuneChoice>> g_tune_cache;
@@ -1997,17 +2123,26 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
     const int vpi0 = p.visits_per_item, cgr0 = p.grid_cgr, icg0 = p.interp_cg;
     const bool mfma0 = p.use_grid_mfma;
     const bool sweep0 = p.use_grid_sweep && p.sw_items != nullptr;
+    const bool bin0 = p.use_grid_bin && p.bin_ids != nullptr;
     double g0 = 0, i0 = 0;
     CHK(time_kernel(p, reps, true, &g0));
     CHK(time_kernel(p, reps, false, &i0));
-    best = TuneChoice{vpi0, cgr0, icg0, g0, i0, g0, i0, mfma0, sweep0};
+    best = TuneChoice{vpi0, cgr0, icg0, g0, i0, g0, i0, mfma0, sweep0, bin0};
+    p.use_grid_bin = false;
     if (p.sw_items) {   // the line sweep (independent of the visit-list chunking)
       p.use_grid_sweep = true;
       double t = 0;
       CHK(time_kernel(p, reps, true, &t));
-      if (t < best.grid_ms) { best.grid_ms = t; best.sweep = true; }
+      if (t < best.grid_ms) { best.grid_ms = t; best.sweep = true; best.bin = false; }
     }
-    p.use_grid_sweep = false;   // the tile kernels below
+    p.use_grid_sweep = false;
+    if (p.bin_ids) {    // the binned scatter (likewise)
+      p.use_grid_bin = true;
+      double t = 0;
+      CHK(time_kernel(p, reps, true, &t));
+      if (t < best.grid_ms) { best.grid_ms = t; best.bin = true; best.sweep = false; }
+    }
+    p.use_grid_bin = false;   // the tile kernels below
     std::vector<int> kinds = tiles8 ? std::vector<int>{1, 0} : std::vector<int>{0};   // 1: matrix cores
     for (int vpi : vpis) {
       CHK(apply_vpi(p, vpi));
@@ -2019,6 +2154,7 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
           CHK(time_kernel(p, reps, true, &t));
           if (t < best.grid_ms) {
             best.grid_ms = t; best.vpi = vpi; best.cgr = cgr; best.mfma = kind == 1; best.sweep = false;
+            best.bin = false;
           }
         }
       }
@@ -2037,6 +2173,7 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
   p.interp_cg = best.icg;
   p.use_grid_mfma = best.mfma;
   p.use_grid_sweep = best.sweep && p.sw_items != nullptr;
+  p.use_grid_bin = best.bin && p.bin_ids != nullptr;
   // captured graphs hold the old launch shapes
   if (p.graph_exec) (void)hipGraphExecDestroy(p.graph_exec);
   p.graph_exec = nullptr;
@@ -2052,6 +2189,7 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
     rep->default_interp_ms = best.def_interp_ms;
     rep->grid_mfma = best.mfma ? 1 : 0;
     rep->grid_sweep = best.sweep ? 1 : 0;
+    rep->grid_bin = best.bin ? 1 : 0;
   }
   return CGS_OK;
 }
@@ -2544,6 +2682,14 @@ int cgs_plan_info(cgs_plan plan, int* n, int* tile, int64_t* visits, int* grid_i
   if (grid_items) *grid_items = p.n_grid_items;
   return CGS_OK;
   GUARD_END
+}
+
+int cgs_gridding_kind(cgs_plan plan) {
+  if (!plan) return -CGS_ERR_ARG;
+  const Plan& p = plan->p;
+  if (p.use_grid_bin && p.bin_ids) return CGS_GRID_BINNED;
+  if (p.d == 2 && p.use_grid_sweep && p.sw_items) return CGS_GRID_SWEEP_LINES;
+  return CGS_GRID_TILES;
 }
 
 }  // extern "C"

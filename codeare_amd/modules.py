@@ -1,6 +1,7 @@
 """codeare's reconstruction modules for this path, as host-side strategies over
-the GPU operators: `CGSENSE` (src/modules/CGSENSE.cpp:52-178) and `NuFFT`
-(src/modules/NuFFT.cpp:49-142).
+the GPU operators: `CGSENSE` (src/modules/CGSENSE.cpp:52-178), `NuFFT`
+(src/modules/NuFFT.cpp:49-142) and `EstimateSensitivities`
+(src/modules/EstimateSensitivities.cpp:49-172).
 
 A strategy follows codeare's ReconStrategy life cycle (Init / Prepare /
 Process, src/core/ReconStrategy.hpp) over a workspace of named matrices (a
@@ -105,8 +106,139 @@ class NuFFT(ReconStrategy):
         return ws["img"]
 
 
+NYUGA = 111.246117975   # the golden angle in degrees (EstimateSensitivities.cpp:29-31)
+
+
+def natural_cspline(x, y, xi):
+    """Natural cubic spline of the columns of y (len(x) rows) at xi -- the
+    interpolant of GSL's cspline type, codeare's interp1 default
+    (Interpolate.hpp:18-38, PolyVal.hpp:261-274): second derivatives zero at
+    both ends, continuous first and second derivatives at the knots; xi within
+    [x[0], x[-1]].  Solved in FP64 with the tridiagonal (Thomas) recurrence."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    y2 = y.reshape(len(x), -1)
+    n = len(x)
+    xi = np.asarray(xi, np.float64)
+    if n < 3:   # GSL needs 3 points for a cspline; fall back to the line
+        out = np.stack([np.interp(xi, x, y2[:, j]) for j in range(y2.shape[1])], axis=1)
+        return out.reshape((len(xi),) + y.shape[1:])
+    h = np.diff(x)
+    # interior second derivatives M_1..M_{n-2}:
+    # h_{i-1} M_{i-1} + 2 (h_{i-1} + h_i) M_i + h_i M_{i+1} = 6 (d_i - d_{i-1})
+    d = np.diff(y2, axis=0) / h[:, None]
+    rhs = 6.0 * (d[1:] - d[:-1])
+    a = h[:-1].copy()              # sub-diagonal
+    b = 2.0 * (h[:-1] + h[1:])     # diagonal
+    c = h[1:].copy()               # super-diagonal
+    k = len(b)
+    cp = np.zeros(k)
+    dp = np.zeros_like(rhs)
+    cp[0] = c[0] / b[0]
+    dp[0] = rhs[0] / b[0]
+    for i in range(1, k):
+        den = b[i] - a[i] * cp[i - 1]
+        cp[i] = c[i] / den if i < k - 1 else 0.0
+        dp[i] = (rhs[i] - a[i] * dp[i - 1]) / den
+    Mi = np.zeros_like(rhs)
+    Mi[-1] = dp[-1]
+    for i in range(k - 2, -1, -1):
+        Mi[i] = dp[i] - cp[i] * Mi[i + 1]
+    M = np.zeros_like(y2)
+    M[1:-1] = Mi
+    j = np.clip(np.searchsorted(x, xi, side="right") - 1, 0, n - 2)
+    t = xi - x[j]
+    hj = h[j][:, None]
+    yl, yr, Ml, Mr = y2[j], y2[j + 1], M[j], M[j + 1]
+    out = (yl + t[:, None] * ((yr - yl) / hj - hj * (2.0 * Ml + Mr) / 6.0)
+           + t[:, None] ** 2 * Ml / 2.0 + t[:, None] ** 3 * (Mr - Ml) / (6.0 * hj))
+    return out.reshape((len(xi),) + y.shape[1:])
+
+
+class EstimateSensitivities(ReconStrategy):
+    """Coil images from multi-channel golden-angle radial data
+    (src/modules/EstimateSensitivities.cpp:49-172): workspace "meas" (nk, nc,
+    nl, nv) complex and "sync" (the gradient-raster trajectory, (kx, ky)
+    interleaved in its first column) -> "sensitivities" (image_size..., nc)
+    complex and "kspace" (2, nk * nv).
+
+    Steps as the module takes them: meas *= 1e6; the raster trajectory is
+    interpolated 10x finer with interp1's natural cubic spline and cut to nk
+    points; volume v is that spoke rotated by v golden angles; the spokes are
+    scaled to max |k| = 1/2; the NFFT (ftalpha, ftm, ftiter; unit weights)
+    inverts the coil's data with its iterative adjoint (ITrafo).  Kept quirks:
+    only coil 0 is reconstructed (the coil loop runs `i < 1`, :147-150; the
+    other coils stay zero unless all_coils=1), and the sum of squares is formed
+    but neither divides nor normalises the maps (:153-164), so neither is done.
+    One departure: the module writes the rotation as cpolar(1, exp(v * NYUGA))
+    (:123), whose angle overflows FP32 for every v >= 1 and makes those spokes
+    NaN; the rotation here is the golden angle the constant names, v * NYUGA
+    degrees."""
+
+    def init(self, **attrs):
+        self.attrs = {"ftalpha": 1.0, "ftm": 1, "ftiter": 3, "cart_3rd_dim": 0, "all_coils": 0}
+        self.attrs.update(attrs)
+        for key in ("nk", "nl", "image_size"):
+            if key not in self.attrs:
+                raise KeyError(f"EstimateSensitivities needs the {key} attribute")
+        imsz = self.attrs["image_size"]
+        if isinstance(imsz, str):
+            imsz = [int(v) for v in imsz.replace(",", " ").split()]
+        self.imsz = [int(v) for v in imsz]
+        self.nk_attr = int(self.attrs["nk"]) * int(self.attrs["nl"])
+        self.ft = NFFT({"alpha": float(self.attrs["ftalpha"]), "m": int(self.attrs["ftm"]),
+                        "nk": self.nk_attr, "imsz": self.imsz, "maxit": int(self.attrs["ftiter"]),
+                        "3rd_dim_cart": bool(int(self.attrs["cart_3rd_dim"]))})
+        self.initialised = True
+
+    def prepare(self, ws: dict):
+        sync = np.asarray(ws["sync"], dtype=np.float32)
+        col = sync.reshape(sync.shape[0], -1, order="F")[:, 0]     # kspace(CR(), CR(0)), squeezed
+        self.raster = col.reshape(2, -1, order="F").T.copy()        # resize to (2, n/2), permute: (n/2, 2)
+
+    def trajectory(self, nk: int, nv: int) -> np.ndarray:
+        """(2, nk * nv) float32: the interpolated spoke, rotated by v golden
+        angles for volume v, scaled to max |k| = 1/2."""
+        r = self.raster.astype(np.float32)
+        nr = r.shape[0]
+        xi = np.linspace(0.0, 1.0, nr, dtype=np.float32)
+        XI = np.linspace(0.0, 1.0, nr * 10, dtype=np.float32)
+        ki = natural_cspline(xi, r, XI).astype(np.float32)[:nk]
+        spoke = (ki[:, 0] + 1j * ki[:, 1]).astype(np.complex64)
+        ang = np.deg2rad(np.arange(nv, dtype=np.float64) * NYUGA)
+        kt = (spoke[:, None] * np.exp(1j * ang)[None, :].astype(np.complex64)).astype(np.complex64)
+        kt /= np.float32(2.0) * np.max(np.abs(kt))
+        kt = kt.reshape(-1, order="F")
+        return np.asfortranarray(np.stack([kt.real, kt.imag]).astype(np.float32))
+
+    def process(self, ws: dict):
+        data = np.asarray(ws["meas"], dtype=np.complex64) * np.float32(1e6)
+        data = np.squeeze(data)
+        while data.ndim < 4:
+            data = data[..., None]
+        nk, nc, nl, nv = data.shape[:4]
+        cart = bool(int(self.attrs["cart_3rd_dim"]))
+        if nk * nv != self.nk_attr or nl != (self.imsz[-1] if cart else 1):
+            raise ValueError(f"meas holds {nk} samples x {nv} volumes x {nl} lines per coil; the NFFT was made "
+                             f"for nk * nl = {self.nk_attr} samples" + (" per slice" if cart else ""))
+        k = self.trajectory(nk, nv)
+        self.ft.KSpace(k)
+        self.ft.Weights(np.ones(nk * nv, np.float32))
+        tmp = np.asfortranarray(np.transpose(data, (0, 3, 2, 1)))   # (nk, nv, nl, nc)
+        sens = np.zeros(tuple(self.imsz) + (nc,), dtype=np.complex64, order="F")
+        for i in range(nc if int(self.attrs["all_coils"]) else 1):
+            f = tmp[..., i].reshape(-1, order="F")
+            if cart:
+                f = f.reshape((nk * nv, nl), order="F")
+            sens[..., i] = np.asarray(self.ft.Adjoint(f)).reshape(self.imsz, order="F")
+        ws["sensitivities"] = sens
+        ws["kspace"] = k
+        return sens
+
+
 STRATEGIES = {"CGSENSE": (CGSENSE, ("sensitivities", "kspace", "weights", "signals"), "image"),
-              "NuFFT": (NuFFT, ("kspace", "weights", "data"), "img")}
+              "NuFFT": (NuFFT, ("kspace", "weights", "data"), "img"),
+              "EstimateSensitivities": (EstimateSensitivities, ("meas", "sync"), "sensitivities")}
 
 
 def main(argv=None):

@@ -224,6 +224,15 @@ def evaluate(net: UNet, pipe: Pipeline, items, lam, tol=1e-5, maxit=3000, log=pr
     rows = []
     xin = t.empty((1, N, N), dtype=t.complex64, device=pipe.dev)
     x0 = t.empty_like(xin)
+    b = t.empty(N * N, dtype=t.complex64, device=pipe.dev)
+    q = t.empty_like(b)
+
+    def r0(v):
+        """||b - (A + lambda) v|| / ||b||: the relative residual CG starts from."""
+        pipe.P.normal(v.reshape(-1).contiguous(), lam=lam, out=q, toeplitz=True)
+        pipe.P.sync()
+        return float(t.linalg.vector_norm(b - q) / t.linalg.vector_norm(b))
+
     for it, prob in _problems(items):
         xt = pipe.load(prob, it)
         pipe.solve(lam, tol, 16)                                   # warm caches
@@ -245,6 +254,9 @@ def evaluate(net: UNet, pipe: Pipeline, items, lam, tol=1e-5, maxit=3000, log=pr
         xw, rw = pipe.solve(lam, tol, maxit, x0=x0)
         t.cuda.synchronize()
         tw = time.perf_counter() - t0
+        pipe.P.EH(pipe.y, out=b)
+        pipe.P.sync()
+        r0_warm, r0_xa = r0(x0), r0(xa)
 
         def rel(a, b):
             return float(t.linalg.vector_norm(a.reshape(-1) - b.reshape(-1)) / t.linalg.vector_norm(b.reshape(-1)))
@@ -255,10 +267,12 @@ def evaluate(net: UNet, pipe: Pipeline, items, lam, tol=1e-5, maxit=3000, log=pr
             "rel_plain": rp["final_rel"][0], "rel_warm": rw["final_rel"][0],
             "warm_vs_plain": rel(xw, xp), "x0_vs_plain": rel(x0, xp), "xa_vs_plain": rel(xa, xp),
             "plain_vs_phantom": rel(xp, xt), "warm_vs_phantom": rel(xw, xt),
+            "r0_warm": r0_warm, "r0_xa": r0_xa,
         })
         r = rows[-1]
         log(f"  eval item {it}: iters plain {r['iters_plain']} warm {r['iters_warm']}; "
-            f"x0 err {r['x0_vs_plain']:.3e} (x_a {r['xa_vs_plain']:.3e}); warm vs plain {r['warm_vs_plain']:.2e}")
+            f"x0 err {r['x0_vs_plain']:.3e} (x_a {r['xa_vs_plain']:.3e}); warm vs plain {r['warm_vs_plain']:.2e}; "
+            f"starting residual warm {r0_warm:.3e} (x_a {r0_xa:.3e}, zero 1)")
     return rows
 
 
@@ -267,9 +281,15 @@ def summarize(rows):
     iw = np.array([r["iters_warm"] for r in rows], float)
     sp = np.array([r["s_plain"] for r in rows])
     sw = np.array([r["s_warm"] for r in rows])
+    q = [0, 5, 25, 50, 75, 95, 100]
     return {
         "n": len(rows),
         "iters_plain_mean": float(ip.mean()), "iters_warm_mean": float(iw.mean()),
+        "iters_plain_percentiles": dict(zip(map(str, q), np.percentile(ip, q).tolist())),
+        "iters_warm_percentiles": dict(zip(map(str, q), np.percentile(iw, q).tolist())),
+        "iters_saved_histogram": {str(int(k)): int(v) for k, v in zip(*np.unique(ip - iw, return_counts=True))},
+        "r0_warm_median": float(np.median([r["r0_warm"] for r in rows])) if "r0_warm" in rows[0] else None,
+        "r0_xa_median": float(np.median([r["r0_xa"] for r in rows])) if "r0_xa" in rows[0] else None,
         "iteration_ratio": float(ip.mean() / iw.mean()),
         "solves_per_s_plain": float(len(rows) / sp.sum()), "solves_per_s_warm": float(len(rows) / sw.sum()),
         "x0_err_mean": float(np.mean([r["x0_vs_plain"] for r in rows])),

@@ -208,9 +208,11 @@ typedef struct {
   double default_grid_ms, default_interp_ms;  /* the plan's defaults              */
   int grid_mfma;         /* gridding: 1 matrix-core scatter, 0 VALU registers    */
   int grid_sweep;        /* gridding: 1 line sweep (2D), 0 the tile kernels      */
+  int grid_bin;          /* gridding: 1 binned LDS scatter (each sample once)    */
 } cgs_tune_report;
-/* Times the gridding scatter (2D: line sweep, or the tile kernels -- matrix-core
- * or VALU-register, visits per work item, coils per pass) and the window gather of this plan's
+/* Times the gridding scatter (the binned LDS scatter; 2D: the line sweep; the tile
+ * kernels -- matrix-core or VALU-register, visits per work item, coils per pass) and the
+ * window gather of this plan's
  * trajectory over a small grid of launch shapes (`reps` timed runs each) and
  * keeps the fastest.  Results are cached per process under (d, n, C, frames, samples
  * per frame rounded to a power of two); with use_cache != 0 a cached bucket is
@@ -244,6 +246,12 @@ int cgs_profile_mask(cgs_plan plan, unsigned mask);
 /* Sizes the plan resolved: n[3] oversampled sides, tile side, sorted sample
  * count, gridding work items. */
 int cgs_plan_info(cgs_plan plan, int* n, int* tile, int64_t* visits, int* grid_items);
+/* The gridding scatter the plan runs now (after CGS_GRID_* switches or
+ * cgs_autotune): CGS_GRID_TILES (tile owners: matrix-core or VALU register
+ * kernels), CGS_GRID_SWEEP_LINES (2D line sweep) or CGS_GRID_BINNED (each sample
+ * once into an LDS region per bin).  No counterpart in the reference. */
+enum { CGS_GRID_TILES = 0, CGS_GRID_SWEEP_LINES = 1, CGS_GRID_BINNED = 2 };
+int cgs_gridding_kind(cgs_plan plan);
 
 const char* cgs_status_string(int status);
 const char* cgs_last_error(void);

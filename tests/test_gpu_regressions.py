@@ -141,3 +141,50 @@ def test_device_maps_load_matches_host_load(cgs, per_frame):
         out[mode] = (y.cpu().numpy(), xb.cpu().numpy())
     assert np.array_equal(out["host"][0], out["device"][0])
     assert np.array_equal(out["host"][1], out["device"][1])
+
+
+@pytest.mark.parametrize("d,C,env", [
+    (2, 32, {"CGS_GRID_MFMA": "0"}),                                              # register tiles
+    (2, 32, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0"}),                         # batched visits
+    (2, 32, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0", "CGS_GRID_ATOMIC": "1"}),  # LDS atomics
+    (2, 32, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0", "CGS_GRID_BATCHED": "0"}),  # one visit at a time
+    (2, 32, {"CGS_INTERP_LDS": "0"}),                                             # gather from global
+    (2, 12, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0", "CGS_GRID_BATCHED": "0"}),
+    (3, 16, {"CGS_GRID_MFMA": "0"}),                                              # 3D register tiles
+    (3, 16, {"CGS_INTERP_LDS": "0"}),
+    (3, 6, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0"}),
+    (2, 32, {"CGS_GRID_BIN": "1"}),                                               # binned LDS scatter
+    (2, 32, {"CGS_GRID_BIN": "1", "CGS_BIN_K": "64"}),                            # ... heavy bins split
+    (2, 12, {"CGS_GRID_BIN": "1", "CGS_BIN_CG": "4", "CGS_BIN_THREADS": "256"}),  # 4-coil groups, padded
+    (2, 32, {"CGS_GRID_BIN": "1", "CGS_BIN_TB": "16", "CGS_BIN_THREADS": "512"}),
+    (3, 16, {"CGS_GRID_BIN": "1"}),
+    (3, 6, {"CGS_GRID_BIN": "1", "CGS_BIN_K": "256", "CGS_BIN_CG": "2"}),
+    (3, 16, {"CGS_GRID_BIN": "1", "CGS_BIN_TB": "8"}),
+])
+def test_kernel_variants_match_default(cgs, d, C, env):
+    """Every alternative gridding / gather kernel that the plan's switches
+    select gives E and E^H of the default kernels (radial / koosh trajectories
+    with dense centre tiles that are split across work items)."""
+    from codeare_amd import data
+    from helpers import rand_img
+    rng = np.random.default_rng(7 * d + C)
+    if d == 2:
+        N = 64
+        k = data.radial_2d(90, 2 * N, True)
+    else:
+        N = 32
+        k = data.koosh_3d(400, 2 * N)
+    M = k.shape[1]
+    w = rng.uniform(0.1, 1.0, M).astype(np.float32)
+    S = data.coil_maps(N, d, C, rng)
+    x = rand_img(rng, (N,) * d)
+    y = rand_img(rng, (M * C,))
+    Pv = _plan_env(cgs, env, (N,) * d, M, 2.0 if d == 2 else 1.5, 3, C, k, w, S, None)
+    Pd = _plan_env(cgs, {}, (N,) * d, M, 2.0 if d == 2 else 1.5, 3, C, k, w, S, None)
+    if env.get("CGS_GRID_BIN") == "1":
+        assert Pv.gridding_kind() == "binned"
+    assert Pd.gridding_kind() == "tiles"
+    e1 = rel(Pv.EH(y), Pd.EH(y))
+    e2 = rel(Pv.E(x), Pd.E(x))
+    print(f"d={d} C={C} {env}: E^H {e1:.2e}, E {e2:.2e}", flush=True)
+    assert e1 <= 1e-6 and e2 <= 1e-6

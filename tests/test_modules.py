@@ -162,3 +162,95 @@ def test_nufft_module_stack_of_stars(ora):
         e = rel(img[..., z], io)
         print(f"NuFFT module slice {z}: {e:.2e}")
         assert e <= 1e-5
+
+
+def test_estimate_sensitivities_spline_is_gsl_cspline():
+    """interp1's default (GSL cspline, PolyVal.hpp:261-274) is the natural
+    cubic spline: pinned against scipy's CubicSpline(bc_type="natural"), and
+    exact on straight lines."""
+    import scipy.interpolate as si
+    from codeare_amd.modules import natural_cspline
+    rng = np.random.default_rng(3)
+    for n in (3, 5, 37):
+        x = np.linspace(0.0, 1.0, n)
+        y = rng.normal(size=(n, 2))
+        xi = np.linspace(0.0, 1.0, 10 * n)
+        assert np.abs(natural_cspline(x, y, xi) - si.CubicSpline(x, y, bc_type="natural")(xi)).max() <= 1e-12
+    x = np.sort(rng.uniform(0, 2, 11))
+    xi = np.linspace(x[0], x[-1], 57)
+    assert np.abs(natural_cspline(x, 3 * x - 1, xi) - (3 * xi - 1)).max() <= 1e-12
+
+
+def test_estimate_sensitivities_trajectory():
+    """The module's k-space: "sync" (kx, ky interleaved in its first column)
+    parsed to the raster spoke, interpolated 10x and cut to nk samples; volume
+    v is the spoke turned by v golden angles (111.246... degrees); max |k| is
+    1/2 (EstimateSensitivities.cpp:74-131)."""
+    from codeare_amd import modules
+    nr, nk, nv = 40, 256, 5
+    t = np.linspace(-1.0, 1.0, nr)
+    raster = np.stack([0.7 * t, 0.2 * t], axis=1)                # a straight spoke
+    sync = np.zeros((2 * nr, 3), np.float32)
+    sync[:, 0] = raster.reshape(-1)                              # kx0, ky0, kx1, ky1, ...
+    s = modules.EstimateSensitivities.__new__(modules.EstimateSensitivities)
+    s.prepare({"sync": sync})
+    assert np.allclose(s.raster, raster, atol=1e-7)
+    k = s.trajectory(nk, nv)
+    assert k.shape == (2, nk * nv) and k.dtype == np.float32
+    kc = (k[0] + 1j * k[1]).reshape((nk, nv), order="F")
+    assert abs(np.abs(kc).max() - 0.5) <= 1e-6
+    # the interpolated spoke: a line from -1 towards the first 10 % of the raster span
+    XI = np.linspace(0.0, 1.0, 10 * nr)[:nk]
+    line = (-1.0 + 2.0 * XI)
+    ref0 = (0.7 * line + 1j * 0.2 * line)
+    scale = 0.5 / np.abs(ref0).max()
+    assert np.abs(kc[:, 0] - ref0 * scale).max() <= 1e-5
+    for v in range(1, nv):
+        turn = np.angle(np.vdot(kc[:, 0], kc[:, v]))
+        want = np.angle(np.exp(1j * np.deg2rad(v * modules.NYUGA)))
+        assert abs(np.angle(np.exp(1j * (turn - want)))) <= 1e-5
+
+
+def test_estimate_sensitivities_rejects_mismatched_sizes():
+    from codeare_amd import modules
+    s = modules.EstimateSensitivities.__new__(modules.EstimateSensitivities)
+    s.attrs = {"cart_3rd_dim": 0, "all_coils": 0}
+    s.imsz = [16, 16]
+    s.nk_attr = 100
+    s.raster = np.zeros((10, 2), np.float32)
+    with pytest.raises(ValueError):
+        s.process({"meas": np.zeros((30, 2, 1, 3), np.complex64)})
+
+
+@pytest.mark.gpu
+def test_estimate_sensitivities_module_matches_oracle(ora):
+    """Coil 0 of the module's output is the NFFT's iterative adjoint (ftiter
+    CGNR steps, unit weights) of 1e6 x coil 0's data on the golden-angle
+    trajectory -- against the oracle's ITrafo; the other coils stay zero
+    (the module's `i < 1` loop) unless all_coils=1."""
+    from codeare_amd import modules
+    rng = np.random.default_rng(21)
+    N, nr, nk, nv, nc = 32, 12, 64, 24, 3
+    t = np.linspace(-1.0, 1.0, nr)
+    sync = np.zeros((2 * nr, 1), np.float32)
+    sync[:, 0] = np.stack([t, 0.3 * t ** 3], axis=1).reshape(-1)
+    meas = ((rng.normal(size=(nk, nc, 1, nv)) + 1j * rng.normal(size=(nk, nc, 1, nv))) * 1e-6).astype(np.complex64)
+    for all_coils in (0, 1):
+        s = modules.EstimateSensitivities()
+        s.init(nk=nk, nl=nv, image_size=f"{N},{N}", ftalpha=2.0, ftm=3, ftiter=3, all_coils=all_coils)
+        ws = {"meas": meas, "sync": sync}
+        s.prepare(ws)
+        sens = s.process(ws)
+        assert sens.shape == (N, N, nc)
+        k = ws["kspace"]
+        O = ora.NFFT((N, N), nk * nv, 2.0, 3)
+        O.kspace(k)
+        for c in range(nc):
+            if c > 0 and not all_coils:
+                assert not np.any(sens[..., c])
+                continue
+            f = (np.transpose(meas, (0, 3, 2, 1))[..., c].reshape(-1, order="F") * np.float32(1e6))
+            io, it, _ = O.itrafo(f.astype(np.complex64), np.ones(nk * nv, np.float32), maxit=3, eps=7e-4)
+            e = rel(sens[..., c], io)
+            print(f"EstimateSensitivities all_coils={all_coils} coil {c}: {e:.2e} (oracle steps {it})")
+            assert e <= 1e-5
