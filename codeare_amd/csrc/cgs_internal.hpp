@@ -383,6 +383,7 @@ struct Plan {
   std::vector<int4> h_sw_ent;         // host copy (the weighted field is refilled by cgs_set_weights)
   // binned gridding tables
   bool use_grid_bin = false;          // binned LDS scatter (not with CGS_DETERMINISTIC)
+  bool bin_allowed = false;           // build the bin tables (CGS_GRID_BIN=1)
   int bin_TB = 0, bin_cg = 0, bin_item_max = 0, bin_threads = 1024;
   int* bin_ids = nullptr;
   int4* bin_items = nullptr;

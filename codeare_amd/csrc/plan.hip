@@ -1004,7 +1004,10 @@ static int build_samples(Plan& p, const float* k, bool sweep = true) {
   return CGS_OK;
 }
 
-// Binned gridding tables (kernels_bin.hip).  The bin side TB (2D: 32, else 16;
+// Binned gridding tables (kernels_bin.hip), built only when the binned scatter
+// is switched on (CGS_GRID_BIN=1): on gfx950 its LDS float atomics take ~128
+// cycles per 64-lane ds_add_f32 (LDS 80-90% busy, no bank conflicts), which
+// makes it 30-40x slower than the tile kernels.  The bin side TB (2D: 32, else 16;
 // 3D: 16, else 8) must divide every grid axis and leave the region, TB + taps - 1
 // cells, no longer than the axis; the region of one coil group must fit in LDS.
 // Otherwise (and with CGS_DETERMINISTIC) the plan has no bins.  Items hold at
@@ -1013,7 +1016,7 @@ static int build_samples(Plan& p, const float* k, bool sweep = true) {
 static int build_bins(Plan& p, const std::vector<int>& hbase, const std::vector<float>& hpsi,
                       const std::vector<int>& hframe) {
   free_bins(p);
-  if (p.flags & CGS_DETERMINISTIC) return CGS_OK;
+  if (!p.bin_allowed || (p.flags & CGS_DETERMINISTIC)) return CGS_OK;
   const int d = p.d, taps = p.taps;
   std::vector<int> cands = (d == 2) ? std::vector<int>{32, 16} : std::vector<int>{16, 8};
   if (const char* e = std::getenv("CGS_BIN_TB")) cands = {std::atoi(e)};
@@ -1340,7 +1343,7 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   }
   if (const char* e = std::getenv("CGS_GRID_ATOMIC")) p.use_grid_atomic = std::atoi(e) != 0;
   if (const char* e = std::getenv("CGS_GRID_BATCHED")) p.use_grid_batched = std::atoi(e) != 0;
-  if (const char* e = std::getenv("CGS_GRID_BIN")) p.use_grid_bin = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CGS_GRID_BIN")) p.use_grid_bin = p.bin_allowed = std::atoi(e) != 0;
   if (const char* e = std::getenv("CGS_BIN_CG")) p.bin_cg = std::atoi(e);
   if (const char* e = std::getenv("CGS_BIN_K")) p.bin_item_max = std::max(64, std::atoi(e));
   if (const char* e = std::getenv("CGS_BIN_THREADS")) p.bin_threads = std::atoi(e);

@@ -161,10 +161,13 @@ def test_device_maps_load_matches_host_load(cgs, per_frame):
     (3, 6, {"CGS_GRID_BIN": "1", "CGS_BIN_K": "256", "CGS_BIN_CG": "2"}),
     (3, 16, {"CGS_GRID_BIN": "1", "CGS_BIN_TB": "8"}),
 ])
-def test_kernel_variants_match_default(cgs, d, C, env):
+def test_kernel_variants_match_default(cgs, ora, d, C, env):
     """Every alternative gridding / gather kernel that the plan's switches
     select gives E and E^H of the default kernels (radial / koosh trajectories
-    with dense centre tiles that are split across work items)."""
+    with dense centre tiles that are split across work items), and is as close
+    to the FP64 oracle as the default.  Kernels that sum in another order differ
+    from the default by FP32 rounding, most where thousands of samples meet
+    (the koosh-ball centre: ~1e-6)."""
     from codeare_amd import data
     from helpers import rand_img
     rng = np.random.default_rng(7 * d + C)
@@ -184,7 +187,13 @@ def test_kernel_variants_match_default(cgs, d, C, env):
     if env.get("CGS_GRID_BIN") == "1":
         assert Pv.gridding_kind() == "binned"
     assert Pd.gridding_kind() == "tiles"
-    e1 = rel(Pv.EH(y), Pd.EH(y))
+    xv, xd = Pv.EH(y), Pd.EH(y)
+    e1 = rel(xv, xd)
     e2 = rel(Pv.E(x), Pd.E(x))
-    print(f"d={d} C={C} {env}: E^H {e1:.2e}, E {e2:.2e}", flush=True)
-    assert e1 <= 1e-6 and e2 <= 1e-6
+    O = ora.NFFT((N,) * d, M, 2.0 if d == 2 else 1.5, 3)
+    O.kspace(k)
+    xo = O.EH_true(S, w, np.asfortranarray(y.reshape((M, C), order="F")))
+    ev, ed = rel(xv, xo), rel(xd, xo)
+    print(f"d={d} C={C} {env}: E^H {e1:.2e}, E {e2:.2e}; E^H vs oracle {ev:.2e} (default {ed:.2e})", flush=True)
+    assert e1 <= 3e-6 and e2 <= 1e-6
+    assert ev <= 1e-5 and ev <= 2.0 * ed + 1e-6

@@ -230,18 +230,21 @@ def test_estimate_sensitivities_module_matches_oracle(ora):
     (the module's `i < 1` loop) unless all_coils=1."""
     from codeare_amd import modules
     rng = np.random.default_rng(21)
-    N, nr, nk, nv, nc = 32, 12, 64, 24, 3
+    # two partitions (cart_3rd_dim): the module squeezes meas before it reads
+    # (nk, nc, nl, nv), so a single-line acquisition would lose its nl axis
+    N, nr, nk, nv, nc, nl = 32, 12, 64, 24, 3, 2
     t = np.linspace(-1.0, 1.0, nr)
     sync = np.zeros((2 * nr, 1), np.float32)
     sync[:, 0] = np.stack([t, 0.3 * t ** 3], axis=1).reshape(-1)
-    meas = ((rng.normal(size=(nk, nc, 1, nv)) + 1j * rng.normal(size=(nk, nc, 1, nv))) * 1e-6).astype(np.complex64)
+    meas = ((rng.normal(size=(nk, nc, nl, nv)) + 1j * rng.normal(size=(nk, nc, nl, nv))) * 1e-6).astype(np.complex64)
     for all_coils in (0, 1):
         s = modules.EstimateSensitivities()
-        s.init(nk=nk, nl=nv, image_size=f"{N},{N}", ftalpha=2.0, ftm=3, ftiter=3, all_coils=all_coils)
+        s.init(nk=nk, nl=nv, image_size=f"{N},{N},{nl}", ftalpha=2.0, ftm=3, ftiter=3, cart_3rd_dim=1,
+               all_coils=all_coils)
         ws = {"meas": meas, "sync": sync}
         s.prepare(ws)
         sens = s.process(ws)
-        assert sens.shape == (N, N, nc)
+        assert sens.shape == (N, N, nl, nc)
         k = ws["kspace"]
         O = ora.NFFT((N, N), nk * nv, 2.0, 3)
         O.kspace(k)
@@ -249,8 +252,11 @@ def test_estimate_sensitivities_module_matches_oracle(ora):
             if c > 0 and not all_coils:
                 assert not np.any(sens[..., c])
                 continue
-            f = (np.transpose(meas, (0, 3, 2, 1))[..., c].reshape(-1, order="F") * np.float32(1e6))
-            io, it, _ = O.itrafo(f.astype(np.complex64), np.ones(nk * nv, np.float32), maxit=3, eps=7e-4)
-            e = rel(sens[..., c], io)
-            print(f"EstimateSensitivities all_coils={all_coils} coil {c}: {e:.2e} (oracle steps {it})")
-            assert e <= 1e-5
+            for z in range(nl):
+                # (nk, nv) samples of coil c, partition z, times 1e6
+                f = np.asfortranarray(meas[:, c, z, :]).reshape(-1, order="F") * np.float32(1e6)
+                io, it, _ = O.itrafo(f.astype(np.complex64), np.ones(nk * nv, np.float32), maxit=3, eps=7e-4)
+                e = rel(sens[:, :, z, c], io)
+                print(f"EstimateSensitivities all_coils={all_coils} coil {c} partition {z}: {e:.2e} "
+                      f"(oracle steps {it})")
+                assert e <= 1e-5
