@@ -221,15 +221,19 @@ __global__ void k_grid_reduce(const ReduceItem* items, const float2* partial, fl
 // coil group of CG coils per blockIdx.y.  The extended region of the tile,
 // (T + taps - 1)^2 cells, and the items' window values are staged once; every
 // tap is then an LDS read.
-template <int CG, int TAPS>
-__global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* items, int nitems, int T,
-                                                    int rowstride, int maxs, int nt0) {
+template <int CG, int TAPS, int TT>
+__global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* items, int nitems, int T_,
+                                                    int rowstride_, int maxs, int nt0) {
   constexpr int RS = 64 / CG;
   extern __shared__ float2 sh[];
   const int item = blockIdx.x;
   const int4 it = items[item];             // frame, tile, sbeg, send
   if (a.fdone && a.fdone[it.x]) return;
+  // TT > 0: the tile side as a constant, so that the staging's divisions by the
+  // region width are multiplications
+  const int T = TT > 0 ? TT : T_;
   const int E = T + TAPS - 1;
+  const int rowstride = TT > 0 ? (TT + TAPS - 1) * CG + ((CG == 8) ? 8 : 0) : rowstride_;
   const int m = TAPS / 2 - 1;
   float2* ext = sh;                                          // [E][rowstride]
   float* wpsi = (float*)(sh + (size_t)E * rowstride);        // [maxs][2*TAPS]
@@ -1133,16 +1137,23 @@ size_t interp_lds_bytes(int T, int taps, int CG, int rowstride, int maxs) {
          sizeof(int) * (size_t)maxs * 2;
 }
 
+template <int CG, int TT>
+static void interp_lds_tt(const SampArgs& a, const int4* items, int nitems, int T, int rowstride, int maxs,
+                          int nt0, dim3 g, size_t shm, hipStream_t s) {
+  switch (a.taps) {
+    case 4: hipLaunchKernelGGL((k_interp_lds<CG, 4, TT>), g, dim3(256), shm, s, a, items, nitems, T, rowstride, maxs, nt0); break;
+    case 6: hipLaunchKernelGGL((k_interp_lds<CG, 6, TT>), g, dim3(256), shm, s, a, items, nitems, T, rowstride, maxs, nt0); break;
+    case 8: hipLaunchKernelGGL((k_interp_lds<CG, 8, TT>), g, dim3(256), shm, s, a, items, nitems, T, rowstride, maxs, nt0); break;
+    case 10: hipLaunchKernelGGL((k_interp_lds<CG, 10, TT>), g, dim3(256), shm, s, a, items, nitems, T, rowstride, maxs, nt0); break;
+    default: break;
+  }
+}
+
 template <int CG>
 static void interp_lds_taps(const SampArgs& a, const int4* items, int nitems, int T, int rowstride, int maxs,
                             int nt0, dim3 g, size_t shm, hipStream_t s) {
-  switch (a.taps) {
-    case 4: hipLaunchKernelGGL((k_interp_lds<CG, 4>), g, dim3(256), shm, s, a, items, nitems, T, rowstride, maxs, nt0); break;
-    case 6: hipLaunchKernelGGL((k_interp_lds<CG, 6>), g, dim3(256), shm, s, a, items, nitems, T, rowstride, maxs, nt0); break;
-    case 8: hipLaunchKernelGGL((k_interp_lds<CG, 8>), g, dim3(256), shm, s, a, items, nitems, T, rowstride, maxs, nt0); break;
-    case 10: hipLaunchKernelGGL((k_interp_lds<CG, 10>), g, dim3(256), shm, s, a, items, nitems, T, rowstride, maxs, nt0); break;
-    default: break;
-  }
+  if (T == 8) interp_lds_tt<CG, 8>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, s);
+  else interp_lds_tt<CG, 0>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, s);
 }
 
 void launch_interp_lds(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int cg, int maxs,

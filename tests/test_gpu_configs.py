@@ -220,11 +220,14 @@ def test_c5_warm_start_held_out():
         # 4.8e-3 at 1e-3, 3.7e-3 at 1e-4
         assert max(r["warm_vs_plain"] for r in rows) <= 1e3 * tol
     # The network (residual loss, 5000 phantoms) starts CG at a relative
-    # residual of ~1.3e-2 (x_a: 0.13), so the loose tolerance is reached in fewer
-    # iterations; at 1e-5 the count is set by the spectrum of what is left, and
-    # the warm start costs 0-5 iterations more (500 held-out seeds: mean 58.5
-    # warm vs 56.1 plain, at most +5).
+    # residual of ~1.3e-2 (x_a: 0.13), so loose tolerances are reached in fewer
+    # iterations (measured on these seeds: 1e-3 plain 4 / warm 2, 1e-4 plain
+    # 17-18 / warm 15-17); at 1e-5 the count is set by the spectrum of what is
+    # left, and the warm start costs 0-5 iterations more (500 held-out seeds:
+    # mean 58.5 warm vs 56.1 plain, at most +5).
     ip, iw = res[1e-3]
-    assert iw.mean() < ip.mean()
+    assert iw.mean() <= 0.5 * ip.mean()
+    ip, iw = res[1e-4]
+    assert np.all(iw <= ip)
     ip, iw = res[1e-5]
     assert np.all(iw <= ip + 6) and np.all(iw >= ip - 2)
