@@ -293,6 +293,10 @@ NUFFT_VARIANTS = [
     ("2d_grid_atomic", 2, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0", "CGS_GRID_ATOMIC": "1"}),
     ("2d_grid_plain", 2, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0", "CGS_GRID_BATCHED": "0"}),
     ("2d_interp_global", 2, {"CGS_INTERP_LDS": "0"}),
+    ("2d_interp_cg32", 2, {"CGS_INTERP_CG": "32"}),
+    ("2d_interp_cg8", 2, {"CGS_INTERP_CG": "8"}),
+    ("2d_interp_item256", 2, {"CGS_TILE_ITEM_MAX": "256"}),
+    ("2d_grid_vpi256", 2, {"CGS_VISITS_PER_ITEM": "256"}),
     ("3d_grid_reg3", 3, {"CGS_GRID_MFMA": "0"}),
     ("3d_interp_global", 3, {"CGS_INTERP_LDS": "0"}),
 ]
@@ -345,6 +349,10 @@ def nufft_leg(args):
         per = per_kernel(plan.kernel_stats(reset=True))
         out[label or wl.name] = {"applications": args.steps, "kernels": per,
                                  "frac_hbm": {k: round(v["GBps"] / HBM_PEAK_GBS, 4) for k, v in per.items()}}
+        if label is None and args.variants:
+            # what the autotuner picks on this trajectory, and its timings
+            rep = plan.autotune(reps=5)
+            out[wl.name]["autotune"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in rep.items()}
         del plan
         torch.cuda.empty_cache()
     return out
