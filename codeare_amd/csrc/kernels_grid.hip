@@ -242,7 +242,28 @@ __global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* item
   const int t0 = it.y % nt0, t1 = it.y / nt0;
   const int org0 = t0 * T - m, org1 = t1 * T - m;
   const int64_t gbase = (int64_t)it.x * a.ntot;
-  {
+  if constexpr (TT > 0) {
+    // a thread owns one (column e0, coil) of the region and loads it from all
+    // E rows (E loads in flight); the row index is the loop's, so the row
+    // address is wave-uniform and the per-load work is one add
+    constexpr int EC = (TT + TAPS - 1), NQ = EC * CG;
+    for (int q = threadIdx.x; q < NQ; q += 256) {
+      const int e0 = q / CG, c = q - e0 * CG;
+      const bool cok = cbase + c < a.C;
+      int l0 = org0 + e0;
+      if (l0 < 0) l0 += a.n0; else if (l0 >= a.n0) l0 -= a.n0;
+      const float2* src = a.grid + (gbase + l0) * a.C + cbase + c;
+      float2 v[EC];
+#pragma unroll
+      for (int e1 = 0; e1 < EC; ++e1) {
+        int l1 = org1 + e1;
+        if (l1 < 0) l1 += a.n1; else if (l1 >= a.n1) l1 -= a.n1;
+        v[e1] = cok ? src[(int64_t)a.n0 * l1 * a.C] : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int e1 = 0; e1 < EC; ++e1) ext[e1 * rowstride + e0 * CG + c] = v[e1];
+    }
+  } else {
     // eight global loads in flight per thread before any is waited on
     constexpr int U = 8, STEP = 256 / CG;
     const int c = threadIdx.x % CG;
@@ -745,9 +766,11 @@ __global__ void __launch_bounds__(512) k_grid_reg3(GridArgs a, const int4* __res
 template <int TAPS>
 __global__ void __launch_bounds__(512) k_grid_mfma3(GridArgs a, const int4* __restrict__ vrec, int nitems) {
   constexpr int B = 64, NPSI = 3 * TAPS, CC = 16;
-  __shared__ float2 sy[B * CC];
-  __shared__ float spsi[B * NPSI];
-  __shared__ int4 srec[B];
+  // one more visit slot, B: a footprint outside every tile with zero data,
+  // for the odd lane of a wave's last pair (so the pair loop has no branch)
+  __shared__ float2 sy[(B + 1) * CC];
+  __shared__ float spsi[(B + 1) * NPSI];
+  __shared__ int4 srec[B + 1];
   __shared__ unsigned char slist[8][B];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const GridItem it = a.items[blockIdx.x];
@@ -761,10 +784,13 @@ __global__ void __launch_bounds__(512) k_grid_mfma3(GridArgs a, const int4* __re
   gf32x16 acc0, acc1;
 #pragma unroll
   for (int r = 0; r < 16; ++r) { acc0[r] = 0.f; acc1[r] = 0.f; }
+  if (tid < CC) sy[B * CC + tid] = make_float2(0.f, 0.f);
+  if (tid < NPSI) spsi[B * NPSI + tid] = 0.f;
+  if (tid == 0) srec[B] = make_int4(0, 0x40004000, 0x4000, 0x4000);
 
   for (int vb = it.vbeg; vb < it.vend; vb += B) {
     const int nb = min(B, it.vend - vb);
-    lds_barrier();   // the previous batch is consumed
+    lds_barrier();   // the previous batch is consumed (and the pad slot written)
     if (tid < nb) srec[tid] = vrec[vb + tid];
     lds_barrier();
     {
@@ -816,21 +842,20 @@ __global__ void __launch_bounds__(512) k_grid_mfma3(GridArgs a, const int4* __re
     if (hit) slist[w][__popcll(mask & ((1ull << lane) - 1ull))] = (unsigned char)lane;
     lds_barrier();   // staged data and this wave's list are visible
     for (int p = 0; p < cnt; p += 2) {
-      float w0 = 0.f, w1 = 0.f, bv = 0.f;
-      if (p + kk < cnt) {
-        const int v = slist[w][p + kk];
-        const int4 rc = srec[v];
-        const int rel0 = (int)(short)(rc.y & 0xffff), rel1 = (int)(short)(rc.y >> 16);
-        const int dz = w - (int)(short)(rc.w & 0xffff);
-        const float* ps = spsi + v * NPSI;
-        const int dx = cx - rel0, dy0 = cy0 - rel1, dy1 = cy1 - rel1;
-        if ((unsigned)dx < (unsigned)TAPS) {
-          const float pxz = ps[dx] * ps[2 * TAPS + dz];
-          if ((unsigned)dy0 < (unsigned)TAPS) w0 = pxz * ps[TAPS + dy0];
-          if ((unsigned)dy1 < (unsigned)TAPS) w1 = pxz * ps[TAPS + dy1];
-        }
-        bv = ((const float*)sy)[v * 2 * CC + i];
-      }
+      // branch-free: past cnt the pad slot B (zero data, weights selected to 0)
+      const int v = (p + kk < cnt) ? slist[w][p + kk] : B;
+      const int4 rc = srec[v];
+      const int rel0 = (int)(short)(rc.y & 0xffff), rel1 = (int)(short)(rc.y >> 16);
+      const int dz = w - (int)(short)(rc.w & 0xffff);   // in [0, TAPS) for listed visits
+      const float* ps = spsi + v * NPSI;
+      const int dx = cx - rel0, dy0 = cy0 - rel1, dy1 = cy1 - rel1;
+      const bool okx = (unsigned)dx < (unsigned)TAPS && (unsigned)dz < (unsigned)TAPS;
+      const bool ok0 = okx && (unsigned)dy0 < (unsigned)TAPS, ok1 = okx && (unsigned)dy1 < (unsigned)TAPS;
+      const float px = ps[okx ? dx : 0], pz = ps[2 * TAPS + (okx ? dz : 0)];
+      const float q0 = ps[TAPS + (ok0 ? dy0 : 0)], q1 = ps[TAPS + (ok1 ? dy1 : 0)];
+      const float bv = ((const float*)sy)[v * 2 * CC + i];
+      const float pxz = px * pz;
+      const float w0 = ok0 ? pxz * q0 : 0.f, w1 = ok1 ? pxz * q1 : 0.f;
       acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(w0, bv, acc0, 0, 0, 0);
       acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(w1, bv, acc1, 0, 0, 0);
     }
@@ -946,7 +971,9 @@ __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs a, const int4* __res
 
   for (int vb = it.vbeg; vb < it.vend; vb += B) {
     const int nb = min(B, it.vend - vb);
-    if (lane < nb) wrec[lane] = vrec[vb + lane];
+    // visits past nb (the odd one of the last pair) get a footprint outside
+    // every tile and zero data, so the pair loop below needs no branch
+    if (lane < B) wrec[lane] = (lane < nb) ? vrec[vb + lane] : make_int4(0, 0x40004000, 0, 0);
     __builtin_amdgcn_wave_barrier();
     {
       // every load of the batch in flight before the first LDS write
@@ -983,29 +1010,26 @@ __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs a, const int4* __res
 #pragma unroll
       for (int u = 0; u < UY; ++u) {
         const int idx = lane + 64 * u;
-        if (idx < nb * CC) ((float2*)wy)[idx] = yv[u];
+        if (idx < B * CC) ((float2*)wy)[idx] = yv[u];   // zeros past nb
       }
     }
     __builtin_amdgcn_wave_barrier();
+    // branch-free: the four LDS reads of a pair depend only on its record, and
+    // out-of-footprint weights are selected to zero (reads stay in the batch)
     for (int v = 0; v < nb; v += 2) {
-      const int vk = v + kk;   // this lane's visit of the pair (K index)
-      float w0 = 0.f, w1 = 0.f;
+      const int vk = v + kk;   // this lane's visit of the pair (K index); vk <= nb < B or vk < B
+      const int4 rc = wrec[vk];
+      const int rel0 = (int)(short)(rc.y & 0xffff), rel1 = (int)(short)(rc.y >> 16);
+      const float* ps = wpsi + vk * NPSI;
+      const int dx = cx - rel0, dy0 = cy0 - rel1, dy1 = cy1 - rel1;
+      const bool okx = (unsigned)dx < (unsigned)TAPS;
+      const bool ok0 = okx && (unsigned)dy0 < (unsigned)TAPS, ok1 = okx && (unsigned)dy1 < (unsigned)TAPS;
+      const float px = ps[okx ? dx : 0];
+      const float q0 = ps[TAPS + (ok0 ? dy0 : 0)], q1 = ps[TAPS + (ok1 ? dy1 : 0)];
       float bv[NT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) bv[t] = 0.f;
-      if (vk < nb) {
-        const int4 rc = wrec[vk];
-        const int rel0 = (int)(short)(rc.y & 0xffff), rel1 = (int)(short)(rc.y >> 16);
-        const float* ps = wpsi + vk * NPSI;
-        const int dx = cx - rel0, dy0 = cy0 - rel1, dy1 = cy1 - rel1;
-        if ((unsigned)dx < (unsigned)TAPS) {
-          const float px = ps[dx];
-          if ((unsigned)dy0 < (unsigned)TAPS) w0 = px * ps[TAPS + dy0];
-          if ((unsigned)dy1 < (unsigned)TAPS) w1 = px * ps[TAPS + dy1];
-        }
-#pragma unroll
-        for (int t = 0; t < NT; ++t) bv[t] = wy[vk * NCOL + 32 * t + i];
-      }
+      for (int t = 0; t < NT; ++t) bv[t] = wy[vk * NCOL + 32 * t + i];
+      const float w0 = ok0 ? px * q0 : 0.f, w1 = ok1 ? px * q1 : 0.f;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(w0, bv[t], acc[0][t], 0, 0, 0);
