@@ -45,7 +45,7 @@ EXPORTED = [
     "cgs_E", "cgs_EH", "cgs_normal", "cgs_solve", "cgs_solve_noalloc",
     "cgs_kernel_stats", "cgs_plan_info", "cgs_status_string", "cgs_last_error",
     "cgs_residual_history", "cgs_autotune", "cgs_profile_mask", "cgs_set_b0", "cgs_b0_info",
-    "cgs_gridding_kind",
+    "cgs_gridding_kind", "cgs_time_toeplitz_conv",
 ]
 
 
@@ -98,6 +98,7 @@ def load_library(path: str | None = None):
     L.cgs_set_sensitivities.argtypes = [P, P]
     L.cgs_set_sensitivities_device.argtypes = [P, P]
     L.cgs_gridding_kind.argtypes = [P]
+    L.cgs_time_toeplitz_conv.argtypes = [P, I, ct.POINTER(D)]
     L.cgs_nfft_forward.argtypes = [P, I, P, P, I]
     L.cgs_nfft_adjoint.argtypes = [P, I, P, P, I]
     L.cgs_nfft_inverse_ref.argtypes = [P, I, P, P, I, I, D]
@@ -352,6 +353,12 @@ class Plan:
                   "max_iters": rep.max_iters, "status": rep.status,
                   "done_ms": list(rep.done_ms[:F]), "total_ms": rep.total_ms}
         return x, report
+
+    def time_toeplitz_conv(self, reps: int = 200) -> float:
+        """Average µs of the Toeplitz convolution pass, `reps` back-to-back launches."""
+        us = ct.c_double()
+        _check(load_library().cgs_time_toeplitz_conv(self._h, int(reps), ct.byref(us)))
+        return us.value
 
     def gridding_kind(self) -> str:
         """The scatter the plan runs now: "tiles", "sweep" or "binned"."""

@@ -2687,6 +2687,45 @@ int cgs_plan_info(cgs_plan plan, int* n, int* tile, int64_t* visits, int* grid_i
   GUARD_END
 }
 
+int cgs_time_toeplitz_conv(cgs_plan plan, int reps, double* us_per_launch) {
+  GUARD_BEGIN
+  if (!plan || !us_per_launch || reps < 1 || reps > 100000) { set_error("bad argument"); return CGS_ERR_ARG; }
+  Plan& p = plan->p;
+  if (p.d != 2 || !p.have_toep || !toep_lean_enabled() || !toep_conv_supported((int)p.N[0], (int)p.N[1], p.C)) {
+    set_error("needs a 2D plan whose Toeplitz spectrum is prepared, on the lean conv kernel");
+    return CGS_ERR_STATE;
+  }
+  // a scratch copy of the conv's in-place buffer (zeros): the plan's own
+  // buffers and CG state are not touched
+  const size_t n = (size_t)p.C * 2 * p.N[0] * p.N[1] * p.F;
+  float2* buf = nullptr;
+  CHK(dalloc(&buf, n));
+  launch_zero(buf, sizeof(float2) * n, p.stream);
+  ToepArgs a{};
+  a.C = p.C; a.N0 = (int)p.N[0]; a.N1 = (int)p.N[1]; a.F = p.F; a.Ntot = p.Ntot;
+  a.S = p.S; a.Sfs = p.Sfs; a.fdone = nullptr; a.row = 0;
+  a.tw = p.ft[1].tw; a.out = buf; a.That = p.That;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
+  float ms = 0.f;
+  if (he == hipSuccess) {
+    launch_toep_conv(a, p.stream);   // warm-up
+    (void)hipEventRecord(e0, p.stream);
+    for (int k = 0; k < reps; ++k) launch_toep_conv(a, p.stream);
+    (void)hipEventRecord(e1, p.stream);
+    he = hipEventSynchronize(e1);
+    if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  dfree(buf);
+  if (he != hipSuccess) { set_error(std::string("timing failed: ") + hipGetErrorString(he)); return CGS_ERR_HIP; }
+  *us_per_launch = 1e3 * (double)ms / reps;
+  return CGS_OK;
+  GUARD_END
+}
+
 int cgs_gridding_kind(cgs_plan plan) {
   if (!plan) return -CGS_ERR_ARG;
   const Plan& p = plan->p;

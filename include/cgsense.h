@@ -252,6 +252,12 @@ int cgs_plan_info(cgs_plan plan, int* n, int* tile, int64_t* visits, int* grid_i
  * once into an LDS region per bin).  No counterpart in the reference. */
 enum { CGS_GRID_TILES = 0, CGS_GRID_SWEEP_LINES = 1, CGS_GRID_BINNED = 2 };
 int cgs_gridding_kind(cgs_plan plan);
+/* Average duration of the Toeplitz convolution pass (the CG iteration's
+ * dominant kernel) over `reps` back-to-back launches between two events, on a
+ * zeroed scratch copy of its buffer (the plan's state is untouched); needs a 2D
+ * plan whose Toeplitz spectrum is prepared.  No counterpart in the reference:
+ * a measurement aid for roofline figures. */
+int cgs_time_toeplitz_conv(cgs_plan plan, int reps, double* us_per_launch);
 
 const char* cgs_status_string(int status);
 const char* cgs_last_error(void);

@@ -197,3 +197,20 @@ def test_kernel_variants_match_default(cgs, ora, d, C, env):
     print(f"d={d} C={C} {env}: E^H {e1:.2e}, E {e2:.2e}; E^H vs oracle {ev:.2e} (default {ed:.2e})", flush=True)
     assert e1 <= 3e-6 and e2 <= 1e-6
     assert ev <= 1e-5 and ev <= 2.0 * ed + 1e-6
+
+
+def test_time_toeplitz_conv_leaves_the_plan_alone(cgs):
+    """cgs_time_toeplitz_conv times the convolution pass on a scratch buffer:
+    a positive duration, and the next solve equals the one before it bit for
+    bit; without a prepared spectrum it refuses."""
+    prob, P = _plan(cgs, 128, 16, 64, 256)   # a shape of the lean conv kernel
+    from codeare_amd import data
+    y = data.make_data(prob, P.E)
+    with pytest.raises(cgs.CgsError):
+        P.time_toeplitz_conv(reps=5)
+    x1, r1 = P.solve(y, maxit=20, tol=0.0, lam=1e-2, toeplitz=True)
+    us = P.time_toeplitz_conv(reps=20)
+    x2, r2 = P.solve(y, maxit=20, tol=0.0, lam=1e-2, toeplitz=True)
+    print(f"Toeplitz conv at 128^2 x 16 coils: {us:.2f} us per launch")
+    assert 0.0 < us < 1e4
+    assert np.array_equal(x1, x2) and r1["iters"][0] == r2["iters"][0] == 20
