@@ -1196,13 +1196,16 @@ void launch_interp_lds(const SampArgs& a, const int4* items, int nitems, int T, 
 // coils) staged in LDS.  Work item = (frame, tile, samples whose floor(n x)
 // lies in the tile); lanes = (coil, tap row (s1, s2)), a shuffle reduction
 // over the rows.
-template <int CG, int TAPS, int NT>
-__global__ void __launch_bounds__(NT, NT / 128) k_interp_lds3(SampArgs a, const int4* items, int T, int maxs,
+template <int CG, int TAPS, int NT, int TT>
+__global__ void __launch_bounds__(NT, NT / 128) k_interp_lds3(SampArgs a, const int4* items, int T_, int maxs,
                                                               int nt0, int nt1) {
   constexpr int RS = 64 / CG;
   extern __shared__ float2 sh[];
   const int4 it = items[blockIdx.x];            // frame, tile, sbeg, send
   if (a.fdone && a.fdone[it.x]) return;
+  // TT > 0: the tile side as a constant (the region width E too), so that the
+  // tap-row addressing and the staging's index splits are shifts and adds
+  const int T = TT > 0 ? TT : T_;
   const int E = T + TAPS - 1;
   const int m = TAPS / 2 - 1;
   const int E3 = E * E * E;
@@ -1305,8 +1308,11 @@ size_t interp_lds3_bytes(int T, int taps, int cg, int maxs) {
 template <int NT>
 static void interp_lds3_launch(const SampArgs& a, const int4* items, dim3 g, size_t shm, int T, int nt0, int nt1,
                                int cg, int maxs, hipStream_t s) {
-#define CGS_IL3(CG_, TP) \
-  hipLaunchKernelGGL((k_interp_lds3<CG_, TP, NT>), g, dim3(NT), shm, s, a, items, T, maxs, nt0, nt1)
+#define CGS_IL3(CG_, TP)                                                                                  \
+  do {                                                                                                  \
+    if (T == 8) hipLaunchKernelGGL((k_interp_lds3<CG_, TP, NT, 8>), g, dim3(NT), shm, s, a, items, T, maxs, nt0, nt1); \
+    else hipLaunchKernelGGL((k_interp_lds3<CG_, TP, NT, 0>), g, dim3(NT), shm, s, a, items, T, maxs, nt0, nt1);        \
+  } while (0)
   if (cg == 2) {
     switch (a.taps) { case 4: CGS_IL3(2, 4); break; case 6: CGS_IL3(2, 6); break;
                       case 8: CGS_IL3(2, 8); break; case 10: CGS_IL3(2, 10); break; default: break; }

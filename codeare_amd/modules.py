@@ -27,6 +27,55 @@ from .cgsense import NCSENSE, NFFT
 from .codfile import read_cod, write_cod
 
 
+# Shepp-Logan ellipses as codeare's phantom() tabulates them (Creators.hpp:454-489):
+# semi-axes (a0, a1) and centre offsets (c0, c1) in half-widths, angle, value
+_SL2 = [(.69, .92, .0, .0, .0, 1.), (.6624, .874, .0, -.0184, .0, -.8), (.11, .31, -.22, .0, -.3, -.2),
+        (.16, .41, .22, .0, .3, -.2), (.21, .25, .0, .35, .0, .1), (.046, .046, .00, .1, .0, .1),
+        (.046, .046, .0, -.1, .0, .1), (.046, .023, .08, -.605, .0, .1), (.023, .023, .0, -.606, .0, .1),
+        (.023, .046, -.06, -.605, .0, .1)]
+# Koay's 3D ellipsoids (Creators.hpp:510-533): semi-axes, centre offsets, angle
+_SL3 = [(.69, .92, .9, .0, .0, .0, .0), (.662, .874, .88, .0, .0, .0, .0), (.11, .31, .22, -.22, .0, -.25, -.3),
+        (.16, .41, .21, .22, .0, -.25, .3), (.21, .25, .5, .0, .35, -.25, .0), (.046, .046, .046, .0, .1, -.25, .0),
+        (.046, .023, .02, .08, -.65, -.25, .0), (.046, .023, .02, .06, -.65, -.25, .0),
+        (.056, .04, .1, -.06, -.105, .625, .0), (.056, .056, .1, .0, .1, .625, .0)]
+
+
+def phantom(n: int, d: int = 2) -> np.ndarray:
+    """codeare's phantom<cxfl>(n) (2D Shepp-Logan, Creators.hpp:369-489) or
+    phantom3D (Creators.hpp:408-536) as a complex64 column-major array.  The
+    ellipse test runs in FP32 as codeare's does: element (c, r) lies inside when
+    x^2 + y^2 <= 1 with x = ((c - m1) cos + (r - m0) sin) / a1 and y = ((r - m0)
+    cos - (c - m1) sin) / a0, m = (1 - centre) n / 2, a = semi-axis n / 2.  In 3D
+    codeare's ellipsoid writes the slice index s where its value belongs (its
+    loop variable shadows the value parameter, Creators.hpp:427-433); that is
+    kept, so the 3D phantom holds slice indices inside its ellipsoids."""
+    f = np.float32
+    n_ = f(n)
+    c = np.arange(n, dtype=f)
+    if d == 2:
+        res = np.zeros((n, n), dtype=np.complex64)
+        C, R = np.meshgrid(c, c, indexing="ij")           # res(c, r): first index c
+        for a0, a1, p2, p3, ang, val in _SL2:
+            A0, A1 = f(a0) * n_ / f(2), f(a1) * n_ / f(2)
+            M0, M1 = (f(1) - f(p2)) * n_ / f(2), (f(1) - f(p3)) * n_ / f(2)
+            cs, sn = f(np.cos(f(ang))), f(np.sin(f(ang)))
+            x = ((C - M1) * cs + (R - M0) * sn) / A1
+            y = ((R - M0) * cs - (C - M1) * sn) / A0
+            res += np.where(x * x + y * y <= f(1), np.complex64(val), np.complex64(0))
+        return np.asfortranarray(res)
+    res = np.zeros((n, n, n), dtype=np.complex64)
+    C, R, S = np.meshgrid(c, c, c, indexing="ij")          # res(c, r, s)
+    for a0, a1, a2, p3, p4, p5, ang in _SL3:
+        A0, A1, A2 = f(a0) * n_ / f(2), f(a1) * n_ / f(2), f(a2) * n_ / f(2)
+        M0, M1, M2 = (f(1) - f(p3)) * n_ / f(2), (f(1) - f(p4)) * n_ / f(2), (f(1) - f(p5)) * n_ / f(2)
+        cs, sn = f(np.cos(f(ang))), f(np.sin(f(ang)))
+        x = ((C - M1) * cs + (R - M0) * sn) / A1
+        y = ((R - M0) * cs - (C - M1) * sn) / A0
+        z = (S - M2) / A2
+        res += np.where(x * x + y * y + z * z <= f(1), S.astype(np.complex64), np.complex64(0))
+    return np.asfortranarray(res)
+
+
 class ReconStrategy:
     """Init(attributes) / Prepare(workspace) / Process(workspace)."""
 
@@ -68,8 +117,11 @@ class CGSENSE(ReconStrategy):
 
     def process(self, ws: dict):
         if int(self.attrs["testcase"]):
-            raise NotImplementedError("the built-in phantom test case is not built; pass signals")
-        data = ws["signals"]
+            # E applied to codeare's Shepp-Logan phantom of the maps' side (CGSENSE.cpp:171-172)
+            S = np.asarray(ws["sensitivities"])
+            data = self.ncs * phantom(S.shape[0], 2)
+        else:
+            data = ws["signals"]
         noise = float(self.attrs["noise"])
         if noise:
             rng = np.random.default_rng()
@@ -101,8 +153,13 @@ class NuFFT(ReconStrategy):
 
     def process(self, ws: dict):
         if int(self.attrs["test_case"]):
-            raise NotImplementedError("the built-in phantom test case is not built; pass data")
-        ws["img"] = self.ft.Adjoint(np.asarray(ws["data"], dtype=np.complex64).reshape(-1, order="F"))
+            # NFFT applied to codeare's phantom of the NFFT's image side, 3D for dim 3
+            # (NuFFT.cpp:138-140; ImageSize() is N[0], NFFT.hpp:445)
+            d = int(self.attrs["dim"])
+            data = self.ft * phantom(self.ft.N[0], 3 if d == 3 else 2)
+        else:
+            data = ws["data"]
+        ws["img"] = self.ft.Adjoint(np.asarray(data, dtype=np.complex64).reshape(-1, order="F"))
         return ws["img"]
 
 

@@ -260,3 +260,69 @@ def test_estimate_sensitivities_module_matches_oracle(ora):
                 print(f"EstimateSensitivities all_coils={all_coils} coil {c} partition {z}: {e:.2e} "
                       f"(oracle steps {it})")
                 assert e <= 1e-5
+
+
+def test_codeare_phantom():
+    """codeare's phantom<cxfl>(n) (Creators.hpp:369-489): the centre lies in the
+    two outer ellipses only (1 - 0.8); the bright ring between them covers the
+    area the two ellipses' semi-axes give; the 3D phantom keeps codeare's
+    ellipsoid quirk (the slice index where the value belongs)."""
+    from codeare_amd.modules import phantom
+    n = 512
+    P = phantom(n)
+    assert P.shape == (n, n) and P.dtype == np.complex64 and not np.any(P.imag)
+    assert abs(P[n // 2, n // 2] - 0.2) < 1e-6
+    ring = np.count_nonzero(np.isclose(P.real, 1.0))
+    want = np.pi * (0.69 * 0.92 - 0.6624 * 0.874) * n * n / 4   # the second ellipse sits 0.0184 off centre
+    assert abs(ring - want) / want < 0.08
+    # the dark ellipses (-0.2, centres offset -/+0.22 along the second index:
+    # m0 = (1 + 0.22) n / 2 = 312, (1 - 0.22) n / 2 = 200) hold 0; the same
+    # offsets along the first index fall in the 0.2 interior (pins which index is which)
+    assert abs(P[256, 312]) < 1e-6 and abs(P[256, 200]) < 1e-6
+    assert abs(P[312, 256] - 0.2) < 1e-6 and P[200, 256].real > 0.1
+    Q = phantom(32, 3)
+    assert Q.shape == (32, 32, 32)
+    # voxel (16, 16, 16): inside ellipsoids 1 and 2 (offset 0), so 2 x its slice index
+    assert abs(Q[16, 16, 16] - 32.0) < 1e-6
+    # every value inside is a multiple of its slice index
+    s = np.arange(32)[None, None, :]
+    k = np.divide(Q.real, s, out=np.zeros_like(Q.real), where=s > 0)
+    assert np.allclose(k, np.round(k))
+
+
+@pytest.mark.gpu
+def test_module_testcases_match_oracle(ora):
+    """CGSENSE testcase=1 (CGSENSE.cpp:171-172) and NuFFT test_case=1
+    (NuFFT.cpp:138-140) take E / the NFFT of codeare's Shepp-Logan phantom as
+    their data; against the oracle on the same phantom."""
+    from codeare_amd import data, modules
+    from codeare_amd.modules import phantom
+    rng = np.random.default_rng(5)
+    N, C, R = 64, 4, 128
+    k = data.radial_2d(48, R, True)
+    M = k.shape[1]
+    w = data.density(k, "k", R)
+    S = data.coil_maps(N, 2, C, rng)
+    s = modules.CGSENSE()
+    s.init(nk=M, m=3, testcase=1, cgmaxit=6, cgeps=1e-7, ftmaxit=3, **{"lambda": 5e-2})
+    ws = {"sensitivities": S, "kspace": k, "weights": w}
+    s.prepare(ws)
+    img = s.process(ws)
+    O = ora.NFFT((N, N), M, 1.0, 3)
+    O.kspace(k)
+    y = O.E(S, phantom(N))
+    xo, it, _ = O.cgls_ref(S, w, y, mode=1, ftiter=3, fteps=1e-3, maxit=6, cgeps=1e-7, lam=5e-2)
+    e = rel(img, xo)
+    print(f"CGSENSE testcase: {e:.2e} vs oracle ({it} iterations)")
+    assert e <= 1e-4
+    u = modules.NuFFT()
+    u.init(dim=2, Nx=N, Ny=N, M=M, shots=1, m=3, alpha=2.0, test_case=1)
+    ws = {"kspace": k, "weights": w}
+    u.prepare(ws)
+    img = u.process(ws)
+    O2 = ora.NFFT((N, N), M, 2.0, 3)
+    O2.kspace(k)
+    io, it, _ = O2.itrafo(O2.trafo(phantom(N)).astype(np.complex64), w, maxit=3, eps=7e-4)
+    e = rel(img, io)
+    print(f"NuFFT test_case: {e:.2e} vs oracle ITrafo")
+    assert e <= 1e-5
