@@ -24,6 +24,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libcgsense_hip.so")
 
 CGS_OK = 0
+CGS_ERR_ARG = 1
+CGS_ERR_STATE = 2
+CGS_ERR_HIP = 3
+CGS_ERR_ALLOC = 4
+CGS_ERR_UNSUPPORTED = 5
 CGS_ERR_NAN = 6
 CGS_DEVICE_IO = 0x1
 CGS_PROFILE = 0x2

@@ -227,6 +227,24 @@ struct ToepArgs {
   int* fdone_w;
   int C, N0, N1, F;
   int64_t Ntot;
+  // CG tail folded into the passes (2D row kernels): the crop forms
+  // q = A p + lambda p and the per-row <p, q>, <r, q>, |q|^2 and ||r||^2 of
+  // the stored residual R; the next pad forms alpha = R / <p, q>,
+  // ||r_new||^2 = R - 2 alpha <r, q> + alpha^2 |q|^2, beta and the stop test,
+  // and updates x and r of its row before p = r + beta p
+  int tailfold;
+  const double* frr;      // pad: the crop's ||r||^2 row partials
+  double* rr_part;        // crop output, per (f, i1)
+  float lam;
+  float2* xs;             // pad: the solution, updated in place
+  float2* rs;             // pad: the residual, updated in place
+  const float2* qv;       // pad: q of the previous iteration
+  const double* fpq;      // pad: the crop's per-row partials (nrpart per frame)
+  const double* frq;
+  const double* fqq;
+  const float2* rdot;     // crop: r (for <r, q>)
+  double* rq_part;        // crop outputs, per (f, i1)
+  double* qq_part;
 };
 
 struct SampArgs {
@@ -462,6 +480,13 @@ struct Plan {
   bool beta_folded = false;           // the captured / last CG iteration forms beta in the pad pass
   bool graph_folded = false;          // beta_folded of the iterations captured in graph_exec
   double* tail_pq = nullptr;
+  // CG tail folded into the next pad pass (2D row kernels; CGS_CG_TAILFOLD=0 turns it off)
+  bool tailfold = true;
+  bool tailfold_used = false;         // the last solve's iterations ran folded
+  bool graph_tailfold = false;        // the captured graph's iterations are folded
+  double* fold_rq = nullptr;          // crop partials <r, q>, |q|^2 per (frame, row)
+  double* fold_qq = nullptr;
+  double* fold_rr = nullptr;          // crop partials ||r||^2 per (frame, row)
   struct Pending { hipEvent_t a, b; int kind; double bytes; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
@@ -578,6 +603,10 @@ void launch_grid_bin(const BinArgs& a, int threads, hipStream_t s);
 void launch_grid_bin_sum(const BinArgs& a, hipStream_t s);
 size_t grid_bin_lds(int d, int taps, int CG, int E, int P1);
 int bin_row_stride(int E);
+void launch_cg_fold_final(const Plan& p, float2* x, float2* r, const float2* pv, const float2* qv, const double* fpq,
+                          const double* frq, const double* fqq, const double* frr, int nrpart, int tolkind,
+                          double tol, int maxit,
+                          hipStream_t s);
 void launch_maps_load(const float2* S, float2* Sd, float* I, int64_t Ntot, int64_t nm, int C, hipStream_t s);
 
 // read-only tables that a kernel reads at wave-uniform addresses: through the

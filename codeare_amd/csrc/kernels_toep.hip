@@ -664,6 +664,21 @@ __global__ void __launch_bounds__(1024) k_toep_pad_row(ToepArgs a) {
   const int f = (int)(o / a.N1);
   const int64_t i1 = o - (int64_t)f * a.N1;
   if (a.fdone && a.fdone[f]) return;
+  // the folded tail's per-row partials (<p, q>, <r, q>, |q|^2, ||r||^2 of rows
+  // tid + 64 j), loaded first so that their latency overlaps the loads below
+  double tfp[4][4];
+  if (a.tailfold && tid < 64) {
+    const int64_t fo = (int64_t)f * a.nrpart;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int u = tid + 64 * j;
+      const bool in = u < a.nrpart;
+      tfp[0][j] = in ? a.fpq[fo + u] : 0.0;
+      tfp[1][j] = in ? a.frq[fo + u] : 0.0;
+      tfp[2][j] = in ? a.fqq[fo + u] : 0.0;
+      tfp[3][j] = in ? a.frr[fo + u] : 0.0;
+    }
+  }
   fill_r8_tables<NP, SG>(a.tw, twA, twB);
 
   const int64_t ib = (int64_t)f * a.Ntot + (int64_t)a.N0 * i1;
@@ -683,15 +698,92 @@ __global__ void __launch_bounds__(1024) k_toep_pad_row(ToepArgs a) {
 #pragma unroll
       for (int u = 0; u < 8; ++u) rt[u] = tid + 64 * u < a.nrpart ? rpp[tid + 64 * u] : 0.0;
     }
-    float2 rv[H], pv[H];
+    float2 rv[H], pv[H], qrow[H], xrow[H];
 #pragma unroll
     for (int k = 0; k < H; ++k) {
       rv[k] = bld2(rr, vi, k * 512);
       pv[k] = bld2(rp, vi, k * 512);
     }
+    if (a.tailfold) {   // block-uniform; q and x of the row, issued with r and p
+      const Rsrc rq_ = rsrc(a.qv + ib, (int64_t)a.N0 * 8), rx_ = rsrc(a.xs + ib, (int64_t)a.N0 * 8);
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        qrow[k] = bld2(rq_, vi, k * 512);
+        xrow[k] = bld2(rx_, vi, k * 512);
+      }
+    }
     float be;
     const CgState& cs = a.cgst[f];
-    if (a.rpart && cs.bpend) {
+    if (a.tailfold && cs.bpend) {
+      // the previous iteration's CG tail: alpha from the crop's <p, q>,
+      // ||r_new||^2 = R - 2 alpha <r, q> + alpha^2 |q|^2 (r_new = r - alpha q),
+      // beta and the stop test; x and r of this row updated here
+      __shared__ double tf_s[3];
+      if (tid < 64) {
+        const int64_t fo = (int64_t)f * a.nrpart;
+        double sp = 0.0, sr = 0.0, sq = 0.0, R = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {   // the same order in every block
+          sp += tfp[0][j];
+          sr += tfp[1][j];
+          sq += tfp[2][j];
+          R += tfp[3][j];
+        }
+        for (int u = tid + 256; u < a.nrpart; u += 64) {
+          sp += a.fpq[fo + u];
+          sr += a.frq[fo + u];
+          sq += a.fqq[fo + u];
+          R += a.frr[fo + u];
+        }
+        sp = wave_sum_d(sp);
+        sr = wave_sum_d(sr);
+        sq = wave_sum_d(sq);
+        R = wave_sum_d(R);
+        if (tid == 0) {
+          // R: ||r||^2 of the stored residual, so that rounding in the
+          // recurrence does not accumulate over iterations
+          const double al = R / sp;   // CGLS.hpp:50
+          const double g = fmax(R - 2.0 * al * sr + al * al * sq, 0.0);
+          const int it = cs.it_snap + 1;
+          const double rel = (a.tolkind == CGS_TOL_SQUARED_RATIO) ? g / cs.xn : sqrt(g / cs.bn);
+          const int nan = (cs.nan || isnan(rel) || isnan(al)) ? 1 : 0;
+          const int done = (nan || rel <= a.tol || it >= a.maxit) ? 1 : 0;
+          const double beta = g / R;
+          if (i1 == 0) {   // the frame's first block commits (the crop passes `done` on)
+            CgState* w = a.cgw + f;
+            w->pq = sp;
+            w->alpha = al;
+            w->beta = beta;
+            w->rn = g;
+            w->it = it;
+            w->rel = rel;
+            w->nan = nan;
+            w->done = done;
+            if (a.hist && it < a.hcap) a.hist[(int64_t)f * a.hcap + it] = rel;
+          }
+          tf_s[0] = al;
+          tf_s[1] = beta;
+          tf_s[2] = done ? 1.0 : 0.0;
+        }
+      }
+      lds_barrier();
+      const float al = (float)tf_s[0];
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        xrow[k] = make_float2(xrow[k].x + al * pv[k].x, xrow[k].y + al * pv[k].y);
+        rv[k] = make_float2(rv[k].x - al * qrow[k].x, rv[k].y - al * qrow[k].y);
+      }
+      if (q == 0) {
+        const Rsrc wx = rsrc(a.xs + ib, (int64_t)a.N0 * 8), wr = rsrc(a.rs + ib, (int64_t)a.N0 * 8);
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+          bst2(wx, xrow[k], vi, k * 512);
+          bst2(wr, rv[k], vi, k * 512);
+        }
+      }
+      if (tf_s[2] != 0.0) return;   // block-uniform
+      be = (float)tf_s[1];
+    } else if (a.rpart && cs.bpend) {
       // beta and the stop test of the finished iteration, as in k_toep_pad
       __shared__ double fold_s[2];
       if (tid < 64) {
@@ -787,6 +879,10 @@ __global__ void __launch_bounds__(1024) k_toep_crop_row(ToepArgs a) {
   const int f = (int)(o / a.N1);
   const int64_t i1 = o - (int64_t)f * a.N1;
   if (a.fdone && a.fdone[f]) return;
+  if (a.tailfold && a.cgst[f].done) {   // the pad stopped this frame: pass the flag on
+    if (i1 == 0 && tid == 0) a.fdone_w[f] = 1;
+    return;
+  }
   fill_r8_tables<NP, SG>(a.tw, twA, twB);
   const Rsrc rin = rsrc(a.in + o * n * a.C, (int64_t)n * a.C * 8);
   const int vs = (t * a.C + q) * 8, ss = 64 * a.C * 8;
@@ -805,6 +901,11 @@ __global__ void __launch_bounds__(1024) k_toep_crop_row(ToepArgs a) {
   float2 pd[4];
 #pragma unroll
   for (int m2 = 0; m2 < 4; ++m2) pd[m2] = bld2(rd, vp, m2 * sp);
+  // r at the same positions, for <r, q> and ||r||^2 when the CG tail is folded (zeros otherwise)
+  const Rsrc rrd = rsrc(a.rdot ? a.rdot + rowb : a.in, a.rdot ? (int64_t)a.N0 * 8 : 0);
+  float2 rdv[4];
+#pragma unroll
+  for (int m2 = 0; m2 < 4; ++m2) rdv[m2] = bld2(rrd, vp, m2 * sp);
   float2 c[4];
 #pragma unroll
   for (int m2 = 0; m2 < 4; ++m2) c[m2] = make_float2(0.f, 0.f);
@@ -837,18 +938,55 @@ __global__ void __launch_bounds__(1024) k_toep_crop_row(ToepArgs a) {
     c[m2].x = row16_sum(c[m2].x);
     c[m2].y = row16_sum(c[m2].y);
   }
-  double pq = 0.0;
+  double pq = 0.0, rq = 0.0, qq = 0.0, rr = 0.0;
   if (q == 0 && act) {
     const Rsrc ro = rsrc(a.out + rowb, (int64_t)a.N0 * 8);
 #pragma unroll
     for (int m2 = 0; m2 < 4; ++m2) {
-      bst2(ro, c[m2], vp, m2 * sp);
-      pq += (double)pd[m2].x * c[m2].x + (double)pd[m2].y * c[m2].y;
+      float2 qv = c[m2];
+      if (a.tailfold) {   // q = (E^H W E) p + lambda p
+        qv.x += a.lam * pd[m2].x;
+        qv.y += a.lam * pd[m2].y;
+        rq += (double)rdv[m2].x * qv.x + (double)rdv[m2].y * qv.y;
+        qq += (double)qv.x * qv.x + (double)qv.y * qv.y;
+        rr += (double)rdv[m2].x * rdv[m2].x + (double)rdv[m2].y * rdv[m2].y;
+      }
+      bst2(ro, qv, vp, m2 * sp);
+      pq += (double)pd[m2].x * qv.x + (double)pd[m2].y * qv.y;
     }
   }
-  if (a.pq_part) {   // block-uniform
+  if (a.tailfold) {   // block-uniform: the four sums over one exchange
+    __shared__ double red4[4 * 16];
+    pq = wave_sum_d(pq);
+    rq = wave_sum_d(rq);
+    qq = wave_sum_d(qq);
+    rr = wave_sum_d(rr);
+    if ((tid & 63) == 0) {
+      red4[(tid >> 6)] = pq;
+      red4[16 + (tid >> 6)] = rq;
+      red4[32 + (tid >> 6)] = qq;
+      red4[48 + (tid >> 6)] = rr;
+    }
+    lds_barrier();
+    if (tid < 4) {
+      double v = 0.0;
+#pragma unroll
+      for (int w = 0; w < 16; ++w) v += red4[16 * tid + w];
+      double* dst = tid == 0 ? a.pq_part : tid == 1 ? a.rq_part : tid == 2 ? a.qq_part : a.rr_part;
+      dst[o] = v;
+    }
+  } else if (a.pq_part) {   // block-uniform
     const double s = block_sum<16>(pq, red);
     if (tid == 0) a.pq_part[o] = s;
+  }
+  if (a.tailfold) {
+    // the iteration's rn and it for the next pad, whose first block rewrites them
+    if (i1 == 0 && tid == 0) {
+      CgState* w = a.cgw + f;
+      w->rn_snap = w->rn;
+      w->it_snap = w->it;
+      w->bpend = 1;
+    }
   }
 }
 

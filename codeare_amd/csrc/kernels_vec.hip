@@ -467,6 +467,10 @@ __global__ void k_cg_init(CgState* st, const double* pbn, const double* prn, int
     double rel;
     if (tolkind == CGS_TOL_SQUARED_RATIO) { c.xn = bn; rel = bn > 0 ? rn / bn : 0.0; }
     else rel = bn > 0 ? sqrt(rn / bn) : 0.0;
+    // a NaN in E^H y (bn) or in r0 stops the frame with the NaN flag, as
+    // codeare's isnan(rn/xn) test does (CGLS.hpp:41); bn == 0 is a zero
+    // right-hand side, solved by x = x0
+    if (isnan(bn) || isnan(rn)) rel = bn + rn;
     c.rel = rel;
     c.nan = isnan(rel) ? 1 : 0;
     c.done = (c.nan || rel <= tol || maxit <= 0) ? 1 : 0;
@@ -888,6 +892,92 @@ void launch_zero(void* dst, size_t bytes, hipStream_t s) {
   if (n <= 0) return;
   const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t*)dst, n);
+}
+
+// The CG tail of the last iteration when the loop ends with it still pending
+// (the folded tail runs in the next pad pass, and there is none): per frame not
+// yet stopped, alpha and ||r_new||^2 from the crop's per-row partials in the
+// pad's order, x += alpha p, r -= alpha q (k_cg_fold_update), then the frame's
+// scalars and stop state (k_cg_fold_commit, one block per frame).
+__device__ __forceinline__ void fold_scalars(const double* fp, const double* fr, const double* fq, const double* frr,
+                                             int nrpart, double* al, double* g, double* R) {
+  // lanes 0..63 of the calling wave; the result in every lane
+  double sp = 0.0, sr = 0.0, sq = 0.0, sr2 = 0.0;
+  for (int u = (int)(threadIdx.x & 63); u < nrpart; u += 64) {
+    sp += fp[u];
+    sr += fr[u];
+    sq += fq[u];
+    sr2 += frr[u];
+  }
+  sp = wave_sum_d(sp);
+  sr = wave_sum_d(sr);
+  sq = wave_sum_d(sq);
+  sr2 = wave_sum_d(sr2);
+  *R = sr2;
+  *al = sr2 / sp;
+  *g = fmax(sr2 - 2.0 * (*al) * sr + (*al) * (*al) * sq, 0.0);
+}
+
+__global__ void __launch_bounds__(256) k_cg_fold_update(float2* x, float2* r, const float2* pv, const float2* qv,
+                                                        const CgState* st, const double* fpq, const double* frq,
+                                                        const double* fqq, const double* frr, int nrpart,
+                                                        int64_t Ntot) {
+  const int f = blockIdx.y;
+  const CgState& cs = st[f];
+  if (cs.done || !cs.bpend) return;
+  __shared__ float al_s;
+  if (threadIdx.x < 64) {
+    double al, g, R;
+    const int64_t o = (int64_t)f * nrpart;
+    fold_scalars(fpq + o, frq + o, fqq + o, frr + o, nrpart, &al, &g, &R);
+    if (threadIdx.x == 0) al_s = (float)al;
+  }
+  __syncthreads();
+  const float al = al_s;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Ntot; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = (int64_t)f * Ntot + i;
+    const float2 pp = pv[o], qq = qv[o];
+    float2 xx = x[o], rr = r[o];
+    xx.x += al * pp.x; xx.y += al * pp.y;
+    rr.x -= al * qq.x; rr.y -= al * qq.y;
+    x[o] = xx;
+    r[o] = rr;
+  }
+}
+
+__global__ void k_cg_fold_commit(CgState* st, const double* fpq, const double* frq, const double* fqq,
+                                 const double* frr, int nrpart, int tolkind, double tol, int maxit, double* hist,
+                                 int hcap) {
+  const int f = blockIdx.x;
+  CgState& cs = st[f];
+  if (cs.done || !cs.bpend) return;
+  double al, g, R;
+  const int64_t o = (int64_t)f * nrpart;
+  fold_scalars(fpq + o, frq + o, fqq + o, frr + o, nrpart, &al, &g, &R);
+  if (threadIdx.x == 0) {
+    const int it = cs.it_snap + 1;
+    const double rel = (tolkind == CGS_TOL_SQUARED_RATIO) ? g / cs.xn : sqrt(g / cs.bn);
+    const int nan = (cs.nan || isnan(rel) || isnan(al)) ? 1 : 0;
+    cs.alpha = al;
+    cs.beta = g / R;
+    cs.rn = g;
+    cs.it = it;
+    cs.rel = rel;
+    cs.nan = nan;
+    cs.done = (nan || rel <= tol || it >= maxit) ? 1 : 0;
+    cs.bpend = 0;
+    if (hist && it < hcap) hist[(int64_t)f * hcap + it] = rel;
+  }
+}
+
+void launch_cg_fold_final(const Plan& p, float2* x, float2* r, const float2* pv, const float2* qv, const double* fpq,
+                          const double* frq, const double* fqq, const double* frr, int nrpart, int tolkind,
+                          double tol, int maxit, hipStream_t s) {
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((p.Ntot + 255) / 256, 1024));
+  hipLaunchKernelGGL(k_cg_fold_update, dim3((unsigned)blocks, (unsigned)p.F), dim3(256), 0, s, x, r, pv, qv, p.cgst,
+                     fpq, frq, fqq, frr, nrpart, p.Ntot);
+  hipLaunchKernelGGL(k_cg_fold_commit, dim3((unsigned)p.F), dim3(64), 0, s, p.cgst, fpq, frq, fqq, frr, nrpart,
+                     tolkind, tol, maxit, p.hist, p.hist_cap);
 }
 
 // Maps already on the device, (Ntot, nm, C) column-major -> [z][pixel][coil],

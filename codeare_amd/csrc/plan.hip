@@ -1344,6 +1344,7 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   if (const char* e = std::getenv("CGS_GRID_ATOMIC")) p.use_grid_atomic = std::atoi(e) != 0;
   if (const char* e = std::getenv("CGS_GRID_BATCHED")) p.use_grid_batched = std::atoi(e) != 0;
   if (const char* e = std::getenv("CGS_GRID_BIN")) p.use_grid_bin = p.bin_allowed = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CGS_CG_TAILFOLD")) p.tailfold = std::atoi(e) != 0;
   if (const char* e = std::getenv("CGS_BIN_CG")) p.bin_cg = std::atoi(e);
   if (const char* e = std::getenv("CGS_BIN_K")) p.bin_item_max = std::max(64, std::atoi(e));
   if (const char* e = std::getenv("CGS_BIN_THREADS")) p.bin_threads = std::atoi(e);
@@ -1516,7 +1517,10 @@ static int toeplitz_prepare(Plan& p) {
     const size_t rows = (size_t)(p.Ntot / p.N[0]) * F;
     const size_t groups = (size_t)(p.C + 15) / 16;
     if (hipMalloc((void**)&p.tail_pp, sizeof(double) * rows) != hipSuccess ||
-        hipMalloc((void**)&p.tail_pq, sizeof(double) * rows * groups) != hipSuccess) {
+        hipMalloc((void**)&p.tail_pq, sizeof(double) * rows * groups) != hipSuccess ||
+        hipMalloc((void**)&p.fold_rq, sizeof(double) * rows) != hipSuccess ||
+        hipMalloc((void**)&p.fold_qq, sizeof(double) * rows) != hipSuccess ||
+        hipMalloc((void**)&p.fold_rr, sizeof(double) * rows) != hipSuccess) {
       destroy_plan(q);
       set_error("allocation of the CG tail partials failed");
       return CGS_ERR_ALLOC;
@@ -1546,6 +1550,9 @@ static void toep_drop(Plan& p) {
   dfree(p.That);
   dfree(p.tail_pp);
   dfree(p.tail_pq);
+  dfree(p.fold_rq);
+  dfree(p.fold_qq);
+  dfree(p.fold_rr);
   if (!p.tt_alias) { dfree(p.tt1); dfree(p.tt2); }
   p.tt1 = p.tt2 = nullptr;
   p.have_toep = false;
@@ -1754,11 +1761,70 @@ static int op_normal(Plan& p, int normal_op, const float2* x, float2* qo, float 
   return op_EH_true(p, p.ybuf[1], qo, x, lambda, partials, fl);
 }
 
+// The folded CG iteration (2D, the row pad/crop kernels): three passes, no
+// tail kernel.  The pad applies the previous iteration's tail to its row (alpha
+// from the crop's <p, q>, x += alpha p, r -= alpha q) with ||r_new||^2 from the
+// recurrence rn - 2 alpha <r, q> + alpha^2 |q|^2, forms beta and the stop test,
+// then p = r + beta p and the first FFT pass; the crop writes q = A p + lambda p
+// with its per-row partials and passes the pad's stop flag on.
+static bool tailfold_ok(const Plan& p, const cgs_solve_opts& o) {
+  const int C = p.C;
+  return p.tailfold && p.d == 2 && o.normal_op == CGS_NORMAL_TOEPLITZ && o.eh_mode == CGS_EH_TRUE &&
+         toep_lean_enabled() && toep_pad_crop_supported((int)p.N[0], (int)p.N[1], C) &&
+         toep_conv_supported((int)p.N[0], (int)p.N[1], C) && toep_pad_row_enabled(C) && toep_row_enabled(C) &&
+         p.fold_rq && p.fold_qq && p.fold_rr;
+}
+
+static int cg_iteration_tailfold(Plan& p, float2* x, float2* r, const float2* pold, float2* pnew, float2* q,
+                                 const cgs_solve_opts& o) {
+  const int C = p.C, F = p.F;
+  const int64_t N0 = p.N[0], N1 = p.N[1], M0 = 2 * N0;
+  ToepArgs ta{};
+  ta.C = C; ta.N0 = (int)N0; ta.N1 = (int)N1; ta.F = F; ta.Ntot = p.Ntot;
+  ta.S = p.S; ta.Sfs = p.Sfs; ta.fdone = p.fdone;
+  {
+    ToepArgs a = ta;
+    a.row = 1;
+    a.tw = p.ft[0].tw; a.out = p.tt1;
+    a.r = r; a.pold = pold; a.pnew = pnew; a.cgst = p.cgst;
+    a.tailfold = 1; a.xs = x; a.rs = r; a.qv = q;
+    a.fpq = p.tail_pq; a.frq = p.fold_rq; a.fqq = p.fold_qq; a.frr = p.fold_rr; a.nrpart = (int)N1;
+    a.tolkind = o.tol_kind; a.tol = o.tol; a.maxit = o.maxit;
+    a.cgw = p.cgst; a.hist = p.hist; a.hcap = p.hist_cap;
+    // reads r, p_old, q, x and the maps, writes x, r, p_new and the grid rows
+    const double bytes = ((double)(7 + C) * N0 * N1 + (double)C * M0 * N1) * 8;
+    Prof pr(p, CGS_K_TOEP_PAD, bytes * F);
+    launch_toep_pad(a, p.stream);
+  }
+  {
+    ToepArgs a = ta;
+    a.tw = p.ft[1].tw; a.out = p.tt1; a.That = p.That;
+    Prof pr(p, CGS_K_TOEP_CONV, (2.0 * C * M0 * N1 * 8 + 4.0 * M0 * 2 * N1) * F);
+    launch_toep_conv(a, p.stream);
+  }
+  {
+    ToepArgs a = ta;
+    a.row = 1;
+    a.tw = p.ft[0].tw; a.in = p.tt1; a.out = q;
+    a.pq_part = p.tail_pq; a.pdot = pnew;
+    a.tailfold = 1; a.lam = (float)o.lambda; a.rdot = r; a.rq_part = p.fold_rq; a.qq_part = p.fold_qq;
+    a.rr_part = p.fold_rr;
+    a.cgst = p.cgst; a.cgw = p.cgst; a.fdone_w = p.fdone;
+    const double bytes = ((double)C * M0 * N1 + (double)C * N0 * N1 + 3.0 * N0 * N1) * 8;
+    Prof pr(p, CGS_K_TOEP_CROP, bytes * F);
+    launch_toep_crop(a, p.stream);
+  }
+  p.beta_folded = false;
+  p.tailfold_used = true;
+  return CGS_OK;
+}
+
 // One fused CG iteration of the true-adjoint solve: p_new = r + beta p_old
 // inside the first FFT pass, q = (E^H W E + lambda) p_new with alpha formed by
 // the coil combine's last block, then x, r, beta and the stop test.
 static int cg_iteration_fused(Plan& p, int normal_op, float2* x, float2* r, const float2* pold, float2* pnew,
                               float2* q, const cgs_solve_opts& o) {
+  if (tailfold_ok(p, o)) return cg_iteration_tailfold(p, x, r, pold, pnew, q, o);
   PUpd pu;
   pu.r = r;
   pu.pold = pold;
@@ -1825,6 +1891,7 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
   const int F = p.F;
   const bool joint = o.joint != 0 && F > 1;
   p.beta_folded = false;
+  p.tailfold_used = false;
   std::vector<double>& done_ms = p.done_ms;
   std::fill(done_ms.begin(), done_ms.end(), -1.0);
   const size_t ib = sizeof(float2) * (size_t)F * p.Ntot;
@@ -1872,9 +1939,13 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
     // capture an even number of fused iterations once and replay the graph
     step = o.graph_iters + (o.graph_iters & 1);
     char key[256];
-    snprintf(key, sizeof(key), "%d|%d|%.17g|%d|%.17g|%d|%p|%p|%d|%p|%d", step, o.normal_op, o.lambda, o.tol_kind,
-             o.tol, o.maxit, (void*)p.That, (void*)p.stream, joint ? 1 : 0, (void*)p.hist, p.hist_cap);
-    if (p.graph_exec && p.graph_key == key) p.beta_folded = p.graph_folded;
+    snprintf(key, sizeof(key), "%d|%d|%.17g|%d|%.17g|%d|%p|%p|%d|%p|%d|%d", step, o.normal_op, o.lambda, o.tol_kind,
+             o.tol, o.maxit, (void*)p.That, (void*)p.stream, joint ? 1 : 0, (void*)p.hist, p.hist_cap,
+             tailfold_ok(p, o) ? 1 : 0);
+    if (p.graph_exec && p.graph_key == key) {
+      p.beta_folded = p.graph_folded;
+      p.tailfold_used = p.graph_tailfold;
+    }
     if (!p.graph_exec || p.graph_key != key) {
       if (p.graph_exec) (void)hipGraphExecDestroy(p.graph_exec);
       p.graph_exec = nullptr;
@@ -1894,9 +1965,12 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
       if (ei != hipSuccess) { p.graph_exec = nullptr; set_error(std::string("graph instantiate failed: ") + hipGetErrorString(ei)); return CGS_ERR_HIP; }
       p.graph_key = key;
       p.graph_folded = p.beta_folded;
+      p.graph_tailfold = p.tailfold_used;
     }
   }
+  int launched = 0;   // iterations issued (p_new of the last one is pbuf[launched & 1])
   for (int it = 0; it < o.maxit; it += step) {
+    launched += step;
     if (step > 1) {
       HIPCHK(hipGraphLaunch(p.graph_exec, p.stream));
     } else if (o.eh_mode == CGS_EH_TRUE && !joint) {
@@ -1926,6 +2000,11 @@ static int solve(Plan& p, const float2* y_ext, const float2* x0, float2* xout, c
   // are skipped)
   if (p.beta_folded && o.eh_mode == CGS_EH_TRUE && o.normal_op == CGS_NORMAL_TOEPLITZ)
     launch_cg_beta_after_xr(p, cg_tail_blocks(p), o.tol_kind, o.tol, o.maxit, p.stream);
+  // with the whole tail folded, the last iteration's x, r update and stop test
+  // are pending for every frame still running
+  if (p.tailfold_used && !joint)
+    launch_cg_fold_final(p, x, r, pbuf[launched & 1], q, p.tail_pq, p.fold_rq, p.fold_qq, p.fold_rr, (int)p.N[1],
+                         o.tol_kind, o.tol, o.maxit, p.stream);
   HIPCHK(hipMemcpyAsync(xout, x, ib, hipMemcpyDeviceToDevice, p.stream));
   HIPCHK(hipMemcpyAsync(p.h_cgst, p.cgst, sizeof(CgState) * F, hipMemcpyDeviceToHost, p.stream));
   launch_fill_int(p.fdone, 0, F, p.stream);

@@ -69,13 +69,18 @@ def test_c2_against_fp64_nudft():
     e_qn = rel(P.normal(prob.image, lam=wl.lam), q)
     print(f"C2 vs FP64 NUDFT: E^H W y {e_b:.2e}, normal (Toeplitz) {e_q:.2e}, normal (NFFT) {e_qn:.2e}")
     assert e_b <= 2e-5 and e_q <= 2e-5 and e_qn <= 2e-5
-    for it in (10, 50):
+    # CG iterates in FP32 follow the FP64 ones to rounding, amplified where
+    # the residual plateaus: on this problem it stalls near 8e-5 from
+    # iteration ~40 to ~55, where FP32 variants that differ only in the order
+    # of rounding (Toeplitz or NFFT operator, folded or separate CG tail) part
+    # by up to 2e-4 and then meet again (5e-7 at iteration 70)
+    for it, bound in ((10, 1e-5), (50, 4e-4)):
         xg, rep = P.solve(y, maxit=it, tol=0.0, lam=wl.lam, toeplitz=True)
         xo, n, _ = T.cg(S, Kh, b, wl.lam, maxit=it, tol=0.0)
         e = rel(xg, xo.cpu().numpy())
         print(f"  CG({it}) image vs FP64 NUDFT CG({n}): {e:.2e}")
         assert rep["iters"][0] == n == it
-        assert e <= 1e-4
+        assert e <= bound
     # lambda = 1e-3 lambda_max (FP64 power iteration): a well-posed problem,
     # whose converged images must agree
     v = torch.randn(N, N, dtype=torch.complex128, device=dev)
