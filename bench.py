@@ -317,6 +317,8 @@ NUFFT_VARIANTS = [
     ("2d_grid_atomic", 2, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0", "CGS_GRID_ATOMIC": "1"}),
     ("2d_grid_plain", 2, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0", "CGS_GRID_BATCHED": "0"}),
     ("2d_interp_global", 2, {"CGS_INTERP_LDS": "0"}),
+    ("2d_interp_rows", 2, {"CGS_INTERP_CS": "0"}),
+    ("2d_interp_x512", 2, {"CGS_INTERP_CS": "2"}),
     ("2d_interp_cg32", 2, {"CGS_INTERP_CG": "32"}),
     ("2d_interp_cg8", 2, {"CGS_INTERP_CG": "8"}),
     ("2d_interp_item256", 2, {"CGS_TILE_ITEM_MAX": "256"}),

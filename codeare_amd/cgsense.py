@@ -71,7 +71,8 @@ class TuneReport(ct.Structure):
     _fields_ = [("visits_per_item", ct.c_int), ("grid_coils", ct.c_int), ("interp_coils", ct.c_int),
                 ("from_cache", ct.c_int), ("grid_ms", ct.c_double), ("interp_ms", ct.c_double),
                 ("default_grid_ms", ct.c_double), ("default_interp_ms", ct.c_double),
-                ("grid_mfma", ct.c_int), ("grid_sweep", ct.c_int), ("grid_bin", ct.c_int)]
+                ("grid_mfma", ct.c_int), ("grid_sweep", ct.c_int), ("grid_bin", ct.c_int),
+                ("interp_lanes", ct.c_int)]
 
 
 class SolveReport(ct.Structure):

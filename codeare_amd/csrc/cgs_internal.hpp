@@ -339,6 +339,7 @@ struct Plan {
   int interp_cg = 32;
   int visits_per_item = 256;
   bool use_interp_lds = true;         // gather from the tile staged in LDS
+  int interp_lanes = 0;               // 2D LDS gather lanes: 0 (coil, tap row); (coil, sample) in blocks of 1: 256, 2: 512
   bool use_grid_batched = true;       // batched visit lists (else one visit at a time)
   bool use_grid_reg = true;           // 2D: register accumulation, lane = cell of an 8x8 tile
   bool use_grid_mfma = false;         // 2D: scatter on the matrix cores (k_grid_mfma)
@@ -529,7 +530,7 @@ void launch_grid(const GridArgs& a, int cg, int waves, int nitems, int ncgroups,
 void launch_grid_reduce(const Plan& p, const float2* partial, float2* grid, const int* fl, hipStream_t s);
 void launch_grid_batched(const GridArgs& a, int cg, int waves, int nitems, int ncgroups, hipStream_t s);
 void launch_interp_lds(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int cg, int maxs,
-                       int ncgroups, hipStream_t s);
+                       int ncgroups, int lanes, hipStream_t s);
 size_t grid_batched_lds(int d, int T, int taps, int CG, int rowstride);
 void launch_grid_atomic(const GridArgs& a, const int4* vrec, int cg, int nitems, int ncgroups, hipStream_t s);
 void launch_grid_reg(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s);

@@ -216,29 +216,14 @@ __global__ void k_grid_reduce(const ReduceItem* items, const float2* partial, fl
 }
 
 // ---------------------------------------------------------------------------
-// forward B with the grid staged in LDS (2D).  Work item = (frame, tile,
-// samples [sbeg, send) whose base cell lies in the tile); block = 4 waves; one
-// coil group of CG coils per blockIdx.y.  The extended region of the tile,
-// (T + taps - 1)^2 cells, and the items' window values are staged once; every
-// tap is then an LDS read.
-template <int CG, int TAPS, int TT>
-__global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* items, int nitems, int T_,
-                                                    int rowstride_, int maxs, int nt0) {
-  constexpr int RS = 64 / CG;
-  extern __shared__ float2 sh[];
-  const int item = blockIdx.x;
-  const int4 it = items[item];             // frame, tile, sbeg, send
-  if (a.fdone && a.fdone[it.x]) return;
-  // TT > 0: the tile side as a constant, so that the staging's divisions by the
-  // region width are multiplications
-  const int T = TT > 0 ? TT : T_;
-  const int E = T + TAPS - 1;
-  const int rowstride = TT > 0 ? (TT + TAPS - 1) * CG + ((CG == 8) ? 8 : 0) : rowstride_;
+// The 2D gathers' LDS staging: the tile's extended region, (T + taps - 1)^2
+// cells x CG coils at ext[e1 * rowstride + e0 * CG + c], and the item's window
+// values (wpsi, 2 taps per sample) and base cells relative to the region
+// origin (wrel).  Returns the item's sample count.
+template <int CG, int TAPS, int TT, int NTH = 256, bool WIN = true>
+__device__ __forceinline__ int stage_region_2d(const SampArgs& a, const int4 it, int T, int E, int rowstride,
+                                               int nt0, int cbase, float2* ext, float* wpsi, int* wrel) {
   const int m = TAPS / 2 - 1;
-  float2* ext = sh;                                          // [E][rowstride]
-  float* wpsi = (float*)(sh + (size_t)E * rowstride);        // [maxs][2*TAPS]
-  int* wrel = (int*)(wpsi + (size_t)maxs * 2 * TAPS);        // [maxs][2]
-  const int cbase = blockIdx.y * CG;
   const int t0 = it.y % nt0, t1 = it.y / nt0;
   const int org0 = t0 * T - m, org1 = t1 * T - m;
   const int64_t gbase = (int64_t)it.x * a.ntot;
@@ -247,7 +232,7 @@ __global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* item
     // E rows (E loads in flight); the row index is the loop's, so the row
     // address is wave-uniform and the per-load work is one add
     constexpr int EC = (TT + TAPS - 1), NQ = EC * CG;
-    for (int q = threadIdx.x; q < NQ; q += 256) {
+    for (int q = threadIdx.x; q < NQ; q += NTH) {
       const int e0 = q / CG, c = q - e0 * CG;
       const bool cok = cbase + c < a.C;
       int l0 = org0 + e0;
@@ -265,7 +250,7 @@ __global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* item
     }
   } else {
     // eight global loads in flight per thread before any is waited on
-    constexpr int U = 8, STEP = 256 / CG;
+    constexpr int U = 8, STEP = NTH / CG;
     const int c = threadIdx.x % CG;
     const bool cok = cbase + c < a.C;
     for (int e00 = threadIdx.x / CG; e00 < E * E; e00 += STEP * U) {
@@ -291,6 +276,7 @@ __global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* item
     }
   }
   const int ns = it.w - it.z;
+  if constexpr (!WIN) return ns;
   for (int idx = threadIdx.x; idx < ns * 2 * TAPS; idx += blockDim.x)
     wpsi[idx] = a.psi[(int64_t)it.z * 2 * TAPS + idx];
   for (int k = threadIdx.x; k < ns; k += blockDim.x) {
@@ -301,6 +287,34 @@ __global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* item
     wrel[2 * k] = r0;
     wrel[2 * k + 1] = r1;
   }
+  return ns;
+}
+
+// ---------------------------------------------------------------------------
+// forward B with the grid staged in LDS (2D).  Work item = (frame, tile,
+// samples [sbeg, send) whose base cell lies in the tile); block = 4 waves; one
+// coil group of CG coils per blockIdx.y.  The extended region of the tile,
+// (T + taps - 1)^2 cells, and the items' window values are staged once; every
+// tap is then an LDS read.  Lanes = (coil, tap row slot), one sample per wave
+// at a time, a lane sum over the row slots.
+template <int CG, int TAPS, int TT>
+__global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* items, int nitems, int T_,
+                                                    int rowstride_, int maxs, int nt0) {
+  constexpr int RS = 64 / CG;
+  extern __shared__ float2 sh[];
+  const int item = blockIdx.x;
+  const int4 it = items[item];             // frame, tile, sbeg, send
+  if (a.fdone && a.fdone[it.x]) return;
+  // TT > 0: the tile side as a constant, so that the staging's divisions by the
+  // region width are multiplications
+  const int T = TT > 0 ? TT : T_;
+  const int E = T + TAPS - 1;
+  const int rowstride = TT > 0 ? (TT + TAPS - 1) * CG + ((CG == 8) ? 8 : 0) : rowstride_;
+  float2* ext = sh;                                          // [E][rowstride]
+  float* wpsi = (float*)(sh + (size_t)E * rowstride);        // [maxs][2*TAPS]
+  int* wrel = (int*)(wpsi + (size_t)maxs * 2 * TAPS);        // [maxs][2]
+  const int cbase = blockIdx.y * CG;
+  const int ns = stage_region_2d<CG, TAPS, TT>(a, it, T, E, rowstride, nt0, cbase, ext, wpsi, wrel);
   lds_barrier();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cl = lane % CG, rs = lane / CG;
@@ -326,6 +340,83 @@ __global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* item
     }
     acc = make_float2(class_sum<CG>(acc.x), class_sum<CG>(acc.y));   // over the row slots
     if (rs == 0 && c < a.C) a.y[(int64_t)(it.z + k) * a.C + c] = acc;
+  }
+}
+
+// The same gather with lanes = (coil, sample): a wave takes 64 / CG samples
+// at a time and every lane sums all taps of its sample for its coil, so no
+// lane sum is needed (about 0.75 of the VALU instructions per sample of
+// k_interp_lds); the lanes of one sample read CG consecutive coils of a region
+// row.  Only the region is staged (its LDS, 29 KB at 16 coils, lets four
+// 512-thread blocks share a CU); window values and base cells are read from
+// global memory, one address per sample (a broadcast to its CG lanes).
+template <int CG, int TAPS, int TT, int NTH>
+__global__ void __launch_bounds__(NTH) k_interp_cs(SampArgs a, const int4* items, int nitems, int T_,
+                                                   int rowstride_, int maxs, int nt0) {
+  constexpr int SPW = 64 / CG, NW = NTH / 64;
+  extern __shared__ float2 sh[];
+  const int item = blockIdx.x;
+  const int4 it = items[item];             // frame, tile, sbeg, send
+  if (a.fdone && a.fdone[it.x]) return;
+  const int T = TT > 0 ? TT : T_;
+  const int E = T + TAPS - 1;
+  const int rowstride = TT > 0 ? (TT + TAPS - 1) * CG + ((CG == 8) ? 8 : 0) : rowstride_;
+  float2* ext = sh;                                          // [E][rowstride]
+  const int cbase = blockIdx.y * CG;
+  const int ns = stage_region_2d<CG, TAPS, TT, NTH, false>(a, it, T, E, rowstride, nt0, cbase, ext, nullptr,
+                                                           nullptr);
+  const int m = TAPS / 2 - 1;
+  const int org0 = (it.y % nt0) * T - m, org1 = (it.y / nt0) * T - m;
+  lds_barrier();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cl = lane % CG, sl = lane / CG;
+  const int c = cbase + cl;
+  // the window values and base cell of the lane's next sample are loaded while
+  // the current one is summed (lanes past the item's end load sample sbeg)
+  float nx[TAPS], ny[TAPS];
+  int nb0, nb1;
+  auto load_next = [&](int k) {
+    const int smp = it.z + (k < ns ? k : 0);
+    const float* ps = a.psi + (int64_t)smp * 2 * TAPS;
+#pragma unroll
+    for (int s0 = 0; s0 < TAPS; ++s0) {
+      nx[s0] = ps[s0];
+      ny[s0] = ps[TAPS + s0];
+    }
+    nb0 = a.base[2 * smp];
+    nb1 = a.base[2 * smp + 1];
+  };
+  load_next(wave * SPW + sl);
+  for (int k0 = wave * SPW; k0 < ns; k0 += NW * SPW) {
+    const int k = k0 + sl;
+    float px[TAPS], py[TAPS];
+#pragma unroll
+    for (int s0 = 0; s0 < TAPS; ++s0) {
+      px[s0] = nx[s0];
+      py[s0] = ny[s0];
+    }
+    int r0 = nb0 - org0, r1 = nb1 - org1;
+    load_next(k + NW * SPW);
+    if (k >= ns) continue;   // the item's last step: lanes past its end stay idle
+    const int smp = it.z + k;
+    if (r0 < 0) r0 += a.n0; else if (r0 >= a.n0) r0 -= a.n0;
+    if (r1 < 0) r1 += a.n1; else if (r1 >= a.n1) r1 -= a.n1;
+    const float2* base = ext + r1 * rowstride + r0 * CG + cl;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int s1 = 0; s1 < TAPS; ++s1) {
+      const float2* row = base + s1 * rowstride;
+      float2 racc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int s0 = 0; s0 < TAPS; ++s0) {
+        const float2 v = row[s0 * CG];
+        racc.x += px[s0] * v.x;
+        racc.y += px[s0] * v.y;
+      }
+      acc.x += py[s1] * racc.x;
+      acc.y += py[s1] * racc.y;
+    }
+    if (c < a.C) a.y[(int64_t)smp * a.C + c] = acc;
   }
 }
 
@@ -1180,12 +1271,58 @@ static void interp_lds_taps(const SampArgs& a, const int4* items, int nitems, in
   else interp_lds_tt<CG, 0>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, s);
 }
 
+template <int CG, int TAPS, int TT>
+static void interp_cs_go(const SampArgs& a, const int4* items, int nitems, int T, int rowstride, int maxs,
+                         int nt0, dim3 g, size_t shm, int nth, hipStream_t s) {
+  if (shm > 65536) {   // more than 64 KB of dynamic LDS (32 coils) needs the kernel's limit raised (once)
+    static const bool raised = [] {
+      return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_interp_cs<CG, TAPS, TT, 256>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess &&
+             hipFuncSetAttribute(reinterpret_cast<const void*>(&k_interp_cs<CG, TAPS, TT, 512>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess;
+    }();
+    (void)raised;
+  }
+  if (nth == 512)
+    hipLaunchKernelGGL((k_interp_cs<CG, TAPS, TT, 512>), g, dim3(512), shm, s, a, items, nitems, T, rowstride, maxs, nt0);
+  else
+    hipLaunchKernelGGL((k_interp_cs<CG, TAPS, TT, 256>), g, dim3(256), shm, s, a, items, nitems, T, rowstride, maxs, nt0);
+}
+
+template <int CG, int TT>
+static void interp_cs_tt(const SampArgs& a, const int4* items, int nitems, int T, int rowstride, int maxs,
+                         int nt0, dim3 g, size_t shm, int nth, hipStream_t s) {
+  switch (a.taps) {
+    case 4: interp_cs_go<CG, 4, TT>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, s); break;
+    case 6: interp_cs_go<CG, 6, TT>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, s); break;
+    case 8: interp_cs_go<CG, 8, TT>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, s); break;
+    case 10: interp_cs_go<CG, 10, TT>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, s); break;
+    default: break;
+  }
+}
+
+template <int CG>
+static void interp_cs_taps(const SampArgs& a, const int4* items, int nitems, int T, int rowstride, int maxs,
+                           int nt0, dim3 g, size_t shm, int nth, hipStream_t s) {
+  if (T == 8) interp_cs_tt<CG, 8>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, s);
+  else interp_cs_tt<CG, 0>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, s);
+}
+
+// lanes: 0 = (coil, tap row slot) k_interp_lds; 1, 2 = (coil, sample) k_interp_cs in blocks of 256, 512
 void launch_interp_lds(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int cg, int maxs,
-                       int ncgroups, hipStream_t s) {
+                       int ncgroups, int lanes, hipStream_t s) {
   if (nitems == 0) return;
   const int rowstride = (T + a.taps - 1) * cg + ((cg == 8) ? 8 : 0);
   const size_t shm = interp_lds_bytes(T, a.taps, cg, rowstride, maxs);
   const dim3 g((unsigned)nitems, (unsigned)ncgroups);
+  if (lanes >= 1) {   // 1: 256-thread blocks, 2: 512; the region alone in LDS
+    const size_t rshm = sizeof(float2) * (size_t)(T + a.taps - 1) * rowstride;
+    const int nth = lanes == 2 ? 512 : 256;
+    if (cg == 8) interp_cs_taps<8>(a, items, nitems, T, rowstride, maxs, nt0, g, rshm, nth, s);
+    else if (cg == 16) interp_cs_taps<16>(a, items, nitems, T, rowstride, maxs, nt0, g, rshm, nth, s);
+    else interp_cs_taps<32>(a, items, nitems, T, rowstride, maxs, nt0, g, rshm, nth, s);
+    return;
+  }
   if (cg == 8) interp_lds_taps<8>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, s);
   else if (cg == 16) interp_lds_taps<16>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, s);
   else interp_lds_taps<32>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, s);

@@ -407,7 +407,8 @@ static int interp(Plan& p, float2* y, const int* fl) {
   Prof pr(p, CGS_K_INTERP, bytes);
   if (p.d == 2 && p.use_interp_lds) {
     const int ng = (p.C + p.interp_cg - 1) / p.interp_cg;
-    launch_interp_lds(a, p.tile_items, p.n_tile_items, p.T, p.nt[0], p.interp_cg, p.tile_item_max, ng, p.stream);
+    launch_interp_lds(a, p.tile_items, p.n_tile_items, p.T, p.nt[0], p.interp_cg, p.tile_item_max, ng,
+                      p.interp_lanes, p.stream);
   } else if (p.d == 3 && p.use_interp_lds && p.T == 8 && p.taps <= 8 &&
              interp_lds3_bytes(p.T, p.taps, 2, kInterp3Batch) <= 65536) {
     // (8 + taps - 1)^3 cells x 2 coils staged once per item (54 KB at taps
@@ -1310,6 +1311,7 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   // up to 2048 samples amortise the staging (dense koosh-ball tiles hold
   // thousands of samples)
   p.use_interp_lds = true;
+  p.interp_lanes = (d == 2) ? 1 : 0;
   p.tile_item_max = (d == 3) ? 2048 : 128;
   p.visits_per_item = (d == 3) ? 4096 : 512;
   p.grid_cgr = (C <= 8) ? 8 : 16;
@@ -1333,6 +1335,7 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   if (const char* e = std::getenv("CGS_VISITS_PER_ITEM")) p.visits_per_item = std::max(16, std::atoi(e));
   if (const char* e = std::getenv("CGS_TILE_ITEM_MAX")) p.tile_item_max = std::max(8, std::atoi(e));
   if (const char* e = std::getenv("CGS_INTERP_LDS")) p.use_interp_lds = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CGS_INTERP_CS")) p.interp_lanes = std::max(0, std::min(2, std::atoi(e)));
   if (const char* e = std::getenv("CGS_INTERP3_THREADS")) p.interp3_threads = std::atoi(e);
   if (const char* e = std::getenv("CGS_GRID_REG")) p.use_grid_reg = std::atoi(e) != 0;
   p.use_grid_mfma = true;
@@ -2137,6 +2140,7 @@ struct TuneChoice {
   bool mfma;
   bool sweep;
   bool bin;
+  int ilanes;
 };
 static std::mutex g_tune_mu;
 static std::vector<std::pair<std::string, TuneChoice>> g_tune_cache;
@@ -2202,14 +2206,14 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
     std::vector<int> vpis = (p.d == 2) ? std::vector<int>{256, 512, 1024} : std::vector<int>{1024, 4096, 16384};
     std::vector<int> cgrs = (reg && p.C > 8) ? std::vector<int>{8, 16} : std::vector<int>{p.grid_cgr};
     std::vector<int> icgs = (p.C > 8) ? std::vector<int>{8, 16} : std::vector<int>{p.interp_cg};
-    const int vpi0 = p.visits_per_item, cgr0 = p.grid_cgr, icg0 = p.interp_cg;
+    const int vpi0 = p.visits_per_item, cgr0 = p.grid_cgr, icg0 = p.interp_cg, il0 = p.interp_lanes;
     const bool mfma0 = p.use_grid_mfma;
     const bool sweep0 = p.use_grid_sweep && p.sw_items != nullptr;
     const bool bin0 = p.use_grid_bin && p.bin_ids != nullptr;
     double g0 = 0, i0 = 0;
     CHK(time_kernel(p, reps, true, &g0));
     CHK(time_kernel(p, reps, false, &i0));
-    best = TuneChoice{vpi0, cgr0, icg0, g0, i0, g0, i0, mfma0, sweep0, bin0};
+    best = TuneChoice{vpi0, cgr0, icg0, g0, i0, g0, i0, mfma0, sweep0, bin0, il0};
     p.use_grid_bin = false;
     if (p.sw_items) {   // the line sweep (independent of the visit-list chunking)
       p.use_grid_sweep = true;
@@ -2241,18 +2245,24 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
         }
       }
     }
-    for (int icg : icgs) {   // the gather does not depend on the visit lists
-      p.interp_cg = icg;
-      double t = 0;
-      CHK(time_kernel(p, reps, false, &t));
-      if (t < best.interp_ms) { best.interp_ms = t; best.icg = icg; }
-    }
+    // the gather does not depend on the visit lists; 2D LDS gathers try both
+    // lane layouts
+    const std::vector<int> lanes = (p.d == 2 && p.use_interp_lds) ? std::vector<int>{1, 0} : std::vector<int>{il0};
+    for (int il : lanes)
+      for (int icg : icgs) {
+        p.interp_cg = icg;
+        p.interp_lanes = il;
+        double t = 0;
+        CHK(time_kernel(p, reps, false, &t));
+        if (t < best.interp_ms) { best.interp_ms = t; best.icg = icg; best.ilanes = il; }
+      }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache.emplace_back(key, best);
   }
   CHK(apply_vpi(p, best.vpi));
   p.grid_cgr = best.cgr;
   p.interp_cg = best.icg;
+  p.interp_lanes = best.ilanes;
   p.use_grid_mfma = best.mfma;
   p.use_grid_sweep = best.sweep && p.sw_items != nullptr;
   p.use_grid_bin = best.bin && p.bin_ids != nullptr;
@@ -2272,6 +2282,7 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
     rep->grid_mfma = best.mfma ? 1 : 0;
     rep->grid_sweep = best.sweep ? 1 : 0;
     rep->grid_bin = best.bin ? 1 : 0;
+    rep->interp_lanes = best.ilanes;
   }
   return CGS_OK;
 }

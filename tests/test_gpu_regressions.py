@@ -149,6 +149,12 @@ def test_device_maps_load_matches_host_load(cgs, per_frame):
     (2, 32, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0", "CGS_GRID_ATOMIC": "1"}),  # LDS atomics
     (2, 32, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0", "CGS_GRID_BATCHED": "0"}),  # one visit at a time
     (2, 32, {"CGS_INTERP_LDS": "0"}),                                             # gather from global
+    (2, 32, {"CGS_INTERP_CS": "0"}),                                              # lanes = (coil, tap row)
+    (2, 12, {"CGS_INTERP_CS": "0", "CGS_INTERP_CG": "8"}),
+    (2, 32, {"CGS_INTERP_CS": "1", "CGS_INTERP_CG": "32"}),                       # lanes = (coil, sample)
+    (2, 12, {"CGS_INTERP_CS": "1", "CGS_INTERP_CG": "8"}),
+    (2, 32, {"CGS_INTERP_CS": "2"}),                                              # ... in 512-thread blocks
+    (2, 12, {"CGS_INTERP_CS": "2", "CGS_INTERP_CG": "8"}),
     (2, 12, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0", "CGS_GRID_BATCHED": "0"}),
     (3, 16, {"CGS_GRID_MFMA": "0"}),                                              # 3D register tiles
     (3, 16, {"CGS_INTERP_LDS": "0"}),
