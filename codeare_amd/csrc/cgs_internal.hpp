@@ -339,7 +339,8 @@ struct Plan {
   int interp_cg = 32;
   int visits_per_item = 256;
   bool use_interp_lds = true;         // gather from the tile staged in LDS
-  int interp_lanes = 0;               // 2D LDS gather lanes: 0 (coil, tap row); (coil, sample) in blocks of 1: 256, 2: 512
+  int interp_lanes = 0;               // LDS gather lanes: 0 (coil, tap row); 2D (coil, sample) in blocks of
+                                      // 1: 256, 2: 512 threads; 3D 1: (sample, plane), 2: (sample, plane, parity)
   bool use_grid_batched = true;       // batched visit lists (else one visit at a time)
   bool use_grid_reg = true;           // 2D: register accumulation, lane = cell of an 8x8 tile
   bool use_grid_mfma = false;         // 2D: scatter on the matrix cores (k_grid_mfma)
@@ -541,6 +542,8 @@ bool grid_sweep_supported(int taps, int C);
 void launch_grid_sweep(const SweepArgs& a, hipStream_t s);
 void launch_grid_sweep_reduce(const SweepArgs& a, hipStream_t s);
 size_t interp_lds3_bytes(int T, int taps, int cg, int maxs);
+void launch_interp_cs3(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int nt1, int threads,
+                       int split, hipStream_t s);
 void launch_interp_lds3(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int nt1, int cg, int maxs,
                         int threads, hipStream_t s);
 void launch_grid_reg3(const GridArgs& a, const int4* vrec, int cgr, int nitems, int ncgroups, hipStream_t s);

@@ -1437,6 +1437,122 @@ __global__ void __launch_bounds__(NT, NT / 128) k_interp_lds3(SampArgs a, const 
   }
 }
 
+// forward B in 3D with the region of 2 coils staged as in k_interp_lds3 (one
+// 16-byte cell = both coils) and lanes over samples and tap planes.
+// SPLIT = 0: a wave takes 8 samples, lane (s, z) sums the TAPS^2 taps of plane
+// z of sample s for both coils; SPLIT = 1: a wave takes 4 samples, lane
+// (s, z, h) the taps of plane z with s0 = h mod 2, and the region's planes are
+// PS = E^2 + 1 cells apart, so that the 16 lanes of a sample read 16 distinct
+// 4-bank groups (all 64 banks) at every step.  The lanes of a sample are summed
+// by DPP steps (three or four, in place of five over 32 tap-row lanes).  Window
+// values and base cells come from global memory, one address per sample
+// broadcast to its lanes, so only the region sits in LDS.
+template <int TAPS, int NT, int TT, int SPLIT>
+__global__ void __launch_bounds__(NT, NT / 128) k_interp_cs3(SampArgs a, const int4* items, int T_, int nt0,
+                                                             int nt1) {
+  static_assert(TAPS == 8, "8 planes per sample: one per lane group");
+  constexpr int CG = 2, NW = NT / 64, LPS = SPLIT ? 16 : 8, SPW = 64 / LPS;
+  extern __shared__ float2 sh[];
+  const int4 it = items[blockIdx.x];            // frame, tile, sbeg, send
+  if (a.fdone && a.fdone[it.x]) return;
+  const int T = TT > 0 ? TT : T_;
+  const int E = T + TAPS - 1;
+  const int PS = E * E + SPLIT;                  // plane stride in cells
+  const int m = TAPS / 2 - 1;
+  const int E3 = E * E * E;
+  float2* ext = sh;                                     // [E][PS][2]
+  const int cbase = blockIdx.y * CG;
+  const int t0 = it.y % nt0, t1 = (it.y / nt0) % nt1, t2 = it.y / (nt0 * nt1);
+  const int org0 = t0 * T - m, org1 = t1 * T - m, org2 = t2 * T - m;
+  const int64_t gbase = (int64_t)it.x * a.ntot;
+  {
+    constexpr int U = 8, STEP = NT / CG;
+    const int c = threadIdx.x % CG;
+    const bool cok = cbase + c < a.C;
+    for (int e00 = threadIdx.x / CG; e00 < E3; e00 += STEP * U) {
+      float2 v[U];
+      int dst[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e00 + u * STEP;
+        v[u] = make_float2(0.f, 0.f);
+        dst[u] = -1;
+        if (e < E3) {
+          const int e2 = e / (E * E), r = e - e2 * E * E, e1 = r / E, e0 = r - e1 * E;
+          int l0 = org0 + e0, l1 = org1 + e1, l2 = org2 + e2;   // one wrap suffices
+          if (l0 < 0) l0 += a.n0; else if (l0 >= a.n0) l0 -= a.n0;
+          if (l1 < 0) l1 += a.n1; else if (l1 >= a.n1) l1 -= a.n1;
+          if (l2 < 0) l2 += a.n2; else if (l2 >= a.n2) l2 -= a.n2;
+          if (cok) v[u] = a.grid[(gbase + l0 + (int64_t)a.n0 * (l1 + (int64_t)a.n1 * l2)) * a.C + cbase + c];
+          dst[u] = (e2 * PS + r) * CG + c;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (dst[u] >= 0) ext[dst[u]] = v[u];
+    }
+  }
+  lds_barrier();
+  const float4* ext4 = reinterpret_cast<const float4*>(ext);   // cell -> (coil 0, coil 1)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = SPLIT ? (lane & 1) : 0;
+  const int pl = SPLIT ? ((lane >> 1) & 7) : (lane & 7);
+  const int sl = lane / LPS;
+  const int ns = it.w - it.z;
+  for (int k0 = wave * SPW; k0 < ns; k0 += NW * SPW) {
+    const int k = k0 + sl;
+    const int smp = it.z + (k < ns ? k : 0);   // lanes past the item's end repeat its first sample
+    const float* ps = a.psi + (int64_t)smp * 3 * TAPS;
+    constexpr int NX = SPLIT ? TAPS / 2 : TAPS;
+    float px[NX], py[TAPS];
+#pragma unroll
+    for (int j = 0; j < NX; ++j) px[j] = ps[SPLIT ? 2 * j + h : j];
+#pragma unroll
+    for (int s1 = 0; s1 < TAPS; ++s1) py[s1] = ps[TAPS + s1];
+    const float pz = ps[2 * TAPS + pl];
+    int r0 = a.base[3 * smp] - org0, r1 = a.base[3 * smp + 1] - org1, r2 = a.base[3 * smp + 2] - org2;
+    if (r0 < 0) r0 += a.n0; else if (r0 >= a.n0) r0 -= a.n0;
+    if (r1 < 0) r1 += a.n1; else if (r1 >= a.n1) r1 -= a.n1;
+    if (r2 < 0) r2 += a.n2; else if (r2 >= a.n2) r2 -= a.n2;
+    const float4* pp = ext4 + (r2 + pl) * PS + r1 * E + r0 + h;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int s1 = 0; s1 < TAPS; ++s1) {
+      float4 racc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int j = 0; j < NX; ++j) {
+        const float4 v = pp[s1 * E + (SPLIT ? 2 * j : j)];
+        racc.x += px[j] * v.x;
+        racc.y += px[j] * v.y;
+        racc.z += px[j] * v.z;
+        racc.w += px[j] * v.w;
+      }
+      acc.x += py[s1] * racc.x;
+      acc.y += py[s1] * racc.y;
+      acc.z += py[s1] * racc.z;
+      acc.w += py[s1] * racc.w;
+    }
+    float2 a0 = make_float2(pz * acc.x, pz * acc.y), a1 = make_float2(pz * acc.z, pz * acc.w);
+    // over the sample's lanes: l ^ 1, l ^ 2 (quad permutations), l + 4 (row
+    // shift); SPLIT: also l + 8 -- the sample's first lane holds the sums
+    a0 = dpp_add2<0xB1>(a0);
+    a1 = dpp_add2<0xB1>(a1);
+    a0 = dpp_add2<0x4E>(a0);
+    a1 = dpp_add2<0x4E>(a1);
+    a0 = dpp_add2<0x104>(a0);
+    a1 = dpp_add2<0x104>(a1);
+    if (SPLIT) {
+      a0 = dpp_add2<0x108>(a0);
+      a1 = dpp_add2<0x108>(a1);
+    }
+    if ((lane % LPS) == 0 && k < ns) {
+      float2* dst = a.y + (int64_t)smp * a.C + cbase;
+      dst[0] = a0;
+      if (cbase + 1 < a.C) dst[1] = a1;
+    }
+  }
+}
+
 size_t interp_lds3_bytes(int T, int taps, int cg, int maxs) {
   const size_t E = (size_t)T + taps - 1;
   return sizeof(float2) * E * E * E * cg + sizeof(float) * (size_t)maxs * 3 * taps + sizeof(int) * (size_t)maxs * 3;
@@ -1464,6 +1580,31 @@ static void interp_lds3_launch(const SampArgs& a, const int4* items, dim3 g, siz
 // CU, so 1024-thread blocks give 8 waves per SIMD to cover the LDS latency of
 // the tap reads (256-thread blocks: 2 waves per SIMD, 1.34x slower on the
 // 128^3 koosh ball)
+// the region alone in LDS (54 KB at taps 8: two blocks per CU); split: the
+// 16-lanes-per-sample layout with padded planes
+void launch_interp_cs3(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int nt1, int threads,
+                       int split, hipStream_t s) {
+  if (nitems == 0 || a.taps != 8) return;
+  const size_t E = (size_t)T + a.taps - 1;
+  const size_t shm = sizeof(float2) * E * (E * E + (split ? 1 : 0)) * 2;
+  const dim3 g((unsigned)nitems, (unsigned)((a.C + 1) / 2));
+#define CGS_IC3(NT_, SP_)                                                                                        \
+  do {                                                                                                         \
+    if (T == 8) hipLaunchKernelGGL((k_interp_cs3<8, NT_, 8, SP_>), g, dim3(NT_), shm, s, a, items, T, nt0, nt1); \
+    else hipLaunchKernelGGL((k_interp_cs3<8, NT_, 0, SP_>), g, dim3(NT_), shm, s, a, items, T, nt0, nt1);        \
+  } while (0)
+  if (split) {
+    if (threads >= 1024) CGS_IC3(1024, 1);
+    else if (threads >= 512) CGS_IC3(512, 1);
+    else CGS_IC3(256, 1);
+  } else {
+    if (threads >= 1024) CGS_IC3(1024, 0);
+    else if (threads >= 512) CGS_IC3(512, 0);
+    else CGS_IC3(256, 0);
+  }
+#undef CGS_IC3
+}
+
 void launch_interp_lds3(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int nt1, int cg, int maxs,
                         int threads, hipStream_t s) {
   if (nitems == 0) return;

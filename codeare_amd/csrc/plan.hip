@@ -409,6 +409,10 @@ static int interp(Plan& p, float2* y, const int* fl) {
     const int ng = (p.C + p.interp_cg - 1) / p.interp_cg;
     launch_interp_lds(a, p.tile_items, p.n_tile_items, p.T, p.nt[0], p.interp_cg, p.tile_item_max, ng,
                       p.interp_lanes, p.stream);
+  } else if (p.d == 3 && p.use_interp_lds && p.interp_lanes >= 1 && p.taps == 8 && p.T == 8) {
+    // (8 + 7)^3 cells x 2 coils staged once per item (54 KB), lanes = (sample, tap plane[, tap parity])
+    launch_interp_cs3(a, p.tile_items, p.n_tile_items, p.T, p.nt[0], p.nt[1], p.interp3_threads,
+                      p.interp_lanes == 2 ? 1 : 0, p.stream);
   } else if (p.d == 3 && p.use_interp_lds && p.T == 8 && p.taps <= 8 &&
              interp_lds3_bytes(p.T, p.taps, 2, kInterp3Batch) <= 65536) {
     // (8 + taps - 1)^3 cells x 2 coils staged once per item (54 KB at taps
@@ -1311,7 +1315,7 @@ static int create_plan(Plan& p, int d, const int* N, int64_t Mtot, double sigma,
   // up to 2048 samples amortise the staging (dense koosh-ball tiles hold
   // thousands of samples)
   p.use_interp_lds = true;
-  p.interp_lanes = (d == 2) ? 1 : 0;
+  p.interp_lanes = (d == 2) ? 1 : 2;   // 2D: (coil, sample); 3D: (sample, plane, tap parity) where taps = 8, T = 8
   p.tile_item_max = (d == 3) ? 2048 : 128;
   p.visits_per_item = (d == 3) ? 4096 : 512;
   p.grid_cgr = (C <= 8) ? 8 : 16;
@@ -2245,9 +2249,11 @@ static int autotune(Plan& p, int reps, bool use_cache, cgs_tune_report* rep) {
         }
       }
     }
-    // the gather does not depend on the visit lists; 2D LDS gathers try both
+    // the gather does not depend on the visit lists; the LDS gathers try both
     // lane layouts
-    const std::vector<int> lanes = (p.d == 2 && p.use_interp_lds) ? std::vector<int>{1, 0} : std::vector<int>{il0};
+    const bool cs3 = p.d == 3 && p.use_interp_lds && p.taps == 8 && p.T == 8;
+    const std::vector<int> lanes = (p.d == 2 && p.use_interp_lds) ? std::vector<int>{1, 0}
+                                   : cs3 ? std::vector<int>{2, 0} : std::vector<int>{il0};
     for (int il : lanes)
       for (int icg : icgs) {
         p.interp_cg = icg;

@@ -158,6 +158,10 @@ def test_device_maps_load_matches_host_load(cgs, per_frame):
     (2, 12, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0", "CGS_GRID_BATCHED": "0"}),
     (3, 16, {"CGS_GRID_MFMA": "0"}),                                              # 3D register tiles
     (3, 16, {"CGS_INTERP_LDS": "0"}),
+    (3, 16, {"CGS_INTERP_CS": "1"}),                                              # lanes = (sample, tap plane)
+    (3, 6, {"CGS_INTERP_CS": "1", "CGS_INTERP3_THREADS": "256"}),
+    (3, 16, {"CGS_INTERP_CS": "0"}),                                              # lanes = (coil, tap row)
+    (3, 6, {"CGS_INTERP_CS": "2", "CGS_INTERP3_THREADS": "512"}),
     (3, 6, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0"}),
     (2, 32, {"CGS_GRID_BIN": "1"}),                                               # binned LDS scatter
     (2, 32, {"CGS_GRID_BIN": "1", "CGS_BIN_K": "64"}),                            # ... heavy bins split
