@@ -1449,7 +1449,7 @@ __global__ void __launch_bounds__(NT, NT / 128) k_interp_lds3(SampArgs a, const 
 // broadcast to its lanes, so only the region sits in LDS.
 template <int TAPS, int NT, int TT, int SPLIT>
 __global__ void __launch_bounds__(NT, NT / 128) k_interp_cs3(SampArgs a, const int4* items, int T_, int nt0,
-                                                             int nt1) {
+                                                             int nt1, int ppad) {
   static_assert(TAPS == 8, "8 planes per sample: one per lane group");
   constexpr int CG = 2, NW = NT / 64, LPS = SPLIT ? 16 : 8, SPW = 64 / LPS;
   extern __shared__ float2 sh[];
@@ -1457,7 +1457,7 @@ __global__ void __launch_bounds__(NT, NT / 128) k_interp_cs3(SampArgs a, const i
   if (a.fdone && a.fdone[it.x]) return;
   const int T = TT > 0 ? TT : T_;
   const int E = T + TAPS - 1;
-  const int PS = E * E + SPLIT;                  // plane stride in cells
+  const int PS = E * E + (SPLIT ? ppad : 0);     // plane stride in cells
   const int m = TAPS / 2 - 1;
   const int E3 = E * E * E;
   float2* ext = sh;                                     // [E][PS][2]
@@ -1585,13 +1585,15 @@ static void interp_lds3_launch(const SampArgs& a, const int4* items, dim3 g, siz
 void launch_interp_cs3(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int nt1, int threads,
                        int split, hipStream_t s) {
   if (nitems == 0 || a.taps != 8) return;
+  // cells of padding after each E^2-cell plane (split layout; CGS_INTERP3_PAD, default 1)
+  static const int ppad = [] { const char* e = std::getenv("CGS_INTERP3_PAD"); return e ? std::max(0, std::min(15, std::atoi(e))) : 1; }();
   const size_t E = (size_t)T + a.taps - 1;
-  const size_t shm = sizeof(float2) * E * (E * E + (split ? 1 : 0)) * 2;
+  const size_t shm = sizeof(float2) * E * (E * E + (split ? ppad : 0)) * 2;
   const dim3 g((unsigned)nitems, (unsigned)((a.C + 1) / 2));
 #define CGS_IC3(NT_, SP_)                                                                                        \
   do {                                                                                                         \
-    if (T == 8) hipLaunchKernelGGL((k_interp_cs3<8, NT_, 8, SP_>), g, dim3(NT_), shm, s, a, items, T, nt0, nt1); \
-    else hipLaunchKernelGGL((k_interp_cs3<8, NT_, 0, SP_>), g, dim3(NT_), shm, s, a, items, T, nt0, nt1);        \
+    if (T == 8) hipLaunchKernelGGL((k_interp_cs3<8, NT_, 8, SP_>), g, dim3(NT_), shm, s, a, items, T, nt0, nt1, ppad); \
+    else hipLaunchKernelGGL((k_interp_cs3<8, NT_, 0, SP_>), g, dim3(NT_), shm, s, a, items, T, nt0, nt1, ppad);        \
   } while (0)
   if (split) {
     if (threads >= 1024) CGS_IC3(1024, 1);
