@@ -690,6 +690,7 @@ __global__ void __launch_bounds__(1024) k_toep_pad_row(ToepArgs a) {
 #pragma unroll
   for (int k = 0; k < H; ++k) sv[k] = bld2(rS, vs, k * ss);
   float pp = 0.f;
+  bool fold = false;   // folded CG tail: x, r and p_new leave through the LDS staging below
   if (a.pold) {   // p_new = r + beta p_old (CGLS.hpp:55-56), written by the q = 0 lanes
     const Rsrc rr = rsrc(a.r + ib, (int64_t)a.N0 * 8), rp = rsrc(a.pold + ib, (int64_t)a.N0 * 8);
     double rt[8];
@@ -773,16 +774,19 @@ __global__ void __launch_bounds__(1024) k_toep_pad_row(ToepArgs a) {
         xrow[k] = make_float2(xrow[k].x + al * pv[k].x, xrow[k].y + al * pv[k].y);
         rv[k] = make_float2(rv[k].x - al * qrow[k].x, rv[k].y - al * qrow[k].y);
       }
-      if (q == 0) {
-        const Rsrc wx = rsrc(a.xs + ib, (int64_t)a.N0 * 8), wr = rsrc(a.rs + ib, (int64_t)a.N0 * 8);
+      if (tf_s[2] != 0.0) {   // block-uniform: the frame stops here with these x and r
+        if (q == 0) {
+          const Rsrc wx = rsrc(a.xs + ib, (int64_t)a.N0 * 8), wr = rsrc(a.rs + ib, (int64_t)a.N0 * 8);
 #pragma unroll
-        for (int k = 0; k < H; ++k) {
-          bst2(wx, xrow[k], vi, k * 512);
-          bst2(wr, rv[k], vi, k * 512);
+          for (int k = 0; k < H; ++k) {
+            bst2(wx, xrow[k], vi, k * 512);
+            bst2(wr, rv[k], vi, k * 512);
+          }
         }
+        return;
       }
-      if (tf_s[2] != 0.0) return;   // block-uniform
       be = (float)tf_s[1];
+      fold = true;
     } else if (a.rpart && cs.bpend) {
       // beta and the stop test of the finished iteration, as in k_toep_pad
       __shared__ double fold_s[2];
@@ -822,6 +826,26 @@ __global__ void __launch_bounds__(1024) k_toep_pad_row(ToepArgs a) {
       xv[k] = make_float2(rv[k].x + be * pv[k].x, rv[k].y + be * pv[k].y);
       if (q == 0) pp += xv[k].x * xv[k].x + xv[k].y * xv[k].y;
     }
+    if (fold) {   // block-uniform
+      // x, r and p_new of the row: a thread holds positions t + 64 k, so
+      // they go through LDS (the exchange buffer, unused until the FFT) and
+      // leave in runs of 64 consecutive positions per wave
+      constexpr int N0 = 32 * NP;
+      if (q == 0) {
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+          ex[t + 64 * k] = xrow[k];
+          ex[N0 + t + 64 * k] = rv[k];
+          ex[2 * N0 + t + 64 * k] = xv[k];
+        }
+      }
+      lds_barrier();
+      for (int e = tid; e < 3 * N0; e += 1024) {
+        const int arr = e / N0, pos = e - arr * N0;
+        float2* dst = arr == 0 ? a.xs : arr == 1 ? a.rs : a.pnew;
+        dst[ib + pos] = ex[e];
+      }
+    }
   } else {
     const Rsrc rx = rsrc(a.x + ib, (int64_t)a.N0 * 8);
 #pragma unroll
@@ -833,7 +857,7 @@ __global__ void __launch_bounds__(1024) k_toep_pad_row(ToepArgs a) {
   }
   // p_new is stored after the reduction: at the merge of the two paths above
   // the compiler waits for every outstanding memory operation
-  if (a.pold && q == 0) {
+  if (a.pold && q == 0 && !fold) {
     const Rsrc rn = rsrc(a.pnew + ib, (int64_t)a.N0 * 8);
 #pragma unroll
     for (int k = 0; k < H; ++k) bst2(rn, xv[k], vi, k * 512);
