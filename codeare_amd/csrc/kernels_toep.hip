@@ -700,17 +700,29 @@ __global__ void __launch_bounds__(1024) k_toep_pad_row(ToepArgs a) {
       for (int u = 0; u < 8; ++u) rt[u] = tid + 64 * u < a.nrpart ? rpp[tid + 64 * u] : 0.0;
     }
     float2 rv[H], pv[H], qrow[H], xrow[H];
-#pragma unroll
-    for (int k = 0; k < H; ++k) {
-      rv[k] = bld2(rr, vi, k * 512);
-      pv[k] = bld2(rp, vi, k * 512);
-    }
-    if (a.tailfold) {   // block-uniform; q and x of the row, issued with r and p
-      const Rsrc rq_ = rsrc(a.qv + ib, (int64_t)a.N0 * 8), rx_ = rsrc(a.xs + ib, (int64_t)a.N0 * 8);
+    if (a.tailfold) {   // block-uniform
+      // r, p_old, q and x of the row: one coalesced load per thread into LDS
+      // (the exchange buffer, unused until the FFT), then each thread takes
+      // its positions t + 64 k
+      constexpr int N0 = 32 * NP;
+      for (int e = tid; e < 4 * N0; e += 1024) {
+        const int arr = e / N0, pos = e - arr * N0;
+        const float2* src = arr == 0 ? a.r : arr == 1 ? a.pold : arr == 2 ? a.qv : a.xs;
+        ex[e] = src[ib + pos];
+      }
+      lds_barrier();
 #pragma unroll
       for (int k = 0; k < H; ++k) {
-        qrow[k] = bld2(rq_, vi, k * 512);
-        xrow[k] = bld2(rx_, vi, k * 512);
+        rv[k] = ex[t + 64 * k];
+        pv[k] = ex[N0 + t + 64 * k];
+        qrow[k] = ex[2 * N0 + t + 64 * k];
+        xrow[k] = ex[3 * N0 + t + 64 * k];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        rv[k] = bld2(rr, vi, k * 512);
+        pv[k] = bld2(rp, vi, k * 512);
       }
     }
     float be;
