@@ -55,7 +55,7 @@ def _true_residual(P, y, x, lam):
     return float(np.linalg.norm(b - q) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("N,C,frames", [(128, 32, 2), (256, 32, 1), (192, 48, 1)])
+@pytest.mark.parametrize("N,C,frames", [(128, 32, 2), (256, 32, 1), (192, 48, 1), (128, 32, 3)])
 @pytest.mark.parametrize("graph", [0, 8])
 def test_tailfold_matches_fused_iteration(cgs, N, C, frames, graph):
     from helpers import rand_img
@@ -91,11 +91,19 @@ def test_tailfold_matches_fused_iteration(cgs, N, C, frames, graph):
                 # CG on an ill-conditioned system: rounding differences grow
                 # with the iterations, so the histories agree closely at first
                 assert np.allclose(hf[:30], hd[:30], rtol=1e-2, atol=0), (hf[:30], hd[:30])
-                assert rf["final_rel"][f] <= tol and rd["final_rel"][f] <= tol
+                if rf["iters"][f] < maxit:   # stopped at the tolerance
+                    assert rf["final_rel"][f] <= tol and rd["final_rel"][f] <= tol
+                else:                        # both ran out of iterations: the same residual
+                    assert abs(rf["final_rel"][f] / rd["final_rel"][f] - 1.0) <= 0.05
         if tol > 0.0:
             tf, td = _true_residual(Pd, y, xf, lam), _true_residual(Pd, y, xd, lam)
+            tf, td = np.atleast_1d(tf), np.atleast_1d(td)
             print(f"  true residual {tf} / fused {td}", flush=True)
-            assert np.all(np.asarray(tf) <= 2 * tol) and np.all(np.asarray(td) <= 2 * tol)
+            for f in range(frames):
+                if rf["iters"][f] < maxit:
+                    assert tf[f] <= 2 * tol and td[f] <= 2 * tol
+                else:
+                    assert abs(tf[f] / td[f] - 1.0) <= 0.05
         assert e <= (1e-4 if tol == 0.0 else 2e-3)
         assert rf["status"] == rd["status"] == 0
 
