@@ -161,6 +161,8 @@ def test_device_maps_load_matches_host_load(cgs, per_frame):
     (3, 16, {"CGS_INTERP_CS": "1"}),                                              # lanes = (sample, tap plane)
     (3, 6, {"CGS_INTERP_CS": "1", "CGS_INTERP3_THREADS": "256"}),
     (3, 16, {"CGS_INTERP_CS": "0"}),                                              # lanes = (coil, tap row)
+    (3, 5, {"CGS_INTERP_CS": "0"}),                                               # odd coil count: a lone last coil
+    (2, 5, {"CGS_INTERP_CS": "0"}),
     (3, 6, {"CGS_INTERP_CS": "2", "CGS_INTERP3_THREADS": "512"}),
     (3, 6, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0"}),
     (2, 32, {"CGS_GRID_BIN": "1"}),                                               # binned LDS scatter
