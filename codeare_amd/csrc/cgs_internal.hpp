@@ -374,6 +374,7 @@ struct Plan {
   float* psi = nullptr;
   float* w = nullptr;
   int* perm = nullptr;                // sorted -> original sample index
+  int* iperm = nullptr;               // original -> sorted sample index
   int* sframe = nullptr;              // frame of each sorted sample
   std::vector<int> h_perm;
   int4* interp_items = nullptr;

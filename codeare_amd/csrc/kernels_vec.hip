@@ -632,11 +632,89 @@ __global__ void k_y_int2ext(const float2* in, float2* ext, const int* perm, cons
   }
 }
 
+// The same conversions over all frames as a transpose through LDS: a block
+// takes 64 consecutive samples of codeare's order, reads or writes each one's
+// C coils as one row of the internal layout (C x 8 contiguous bytes) and each
+// coil's 64 samples as one run of codeare's column (512 contiguous bytes).
+constexpr int kYT = 64;   // samples per block
+// per sample of the block: its internal index, its column base in codeare's
+// layout and its frame's sample count (the column stride)
+__device__ __forceinline__ void y_tile_index(const int* iperm, const int* sframe, const int64_t* foff, int batch,
+                                             int64_t j0, int64_t Mtot, int* si, int64_t* eb, int64_t* mf) {
+  const int jj = threadIdx.x;
+  if (jj < kYT) {
+    const int64_t j = j0 + jj;
+    if (j < Mtot) {
+      const int s = iperm[j];
+      const int f = sframe[s];
+      const int64_t o = foff[f];
+      si[jj] = s;
+      eb[jj] = (int64_t)batch * o + (j - o);
+      mf[jj] = foff[f + 1] - o;
+    }
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_y_ext2int_t(const float2* ext, float2* in, const int* iperm,
+                                                     const int* sframe, const int64_t* foff, int C, int batch,
+                                                     int64_t Mtot) {
+  extern __shared__ float2 yt[];                 // [kYT][C + 1]
+  __shared__ int si[kYT];
+  __shared__ int64_t eb[kYT], mf[kYT];
+  const int64_t j0 = (int64_t)blockIdx.x * kYT;
+  const int nj = (int)min<int64_t>(kYT, Mtot - j0);
+  const int ld = C + 1;
+  y_tile_index(iperm, sframe, foff, batch, j0, Mtot, si, eb, mf);
+  for (int e = threadIdx.x; e < kYT * batch; e += blockDim.x) {   // coil-major: runs of codeare's column
+    const int c = e / kYT, jj = e - c * kYT;
+    if (jj < nj) yt[jj * ld + c] = ext[eb[jj] + mf[jj] * c];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kYT * C; e += blockDim.x) {       // sample-major: rows of the internal layout
+    const int jj = e / C, c = e - jj * C;
+    if (jj < nj) in[(int64_t)si[jj] * C + c] = c < batch ? yt[jj * ld + c] : make_float2(0.f, 0.f);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_y_int2ext_t(const float2* in, float2* ext, const int* iperm,
+                                                     const int* sframe, const int64_t* foff, int C, int batch,
+                                                     int64_t Mtot) {
+  extern __shared__ float2 yt[];                 // [kYT][C + 1]
+  __shared__ int si[kYT];
+  __shared__ int64_t eb[kYT], mf[kYT];
+  const int64_t j0 = (int64_t)blockIdx.x * kYT;
+  const int nj = (int)min<int64_t>(kYT, Mtot - j0);
+  const int ld = C + 1;
+  y_tile_index(iperm, sframe, foff, batch, j0, Mtot, si, eb, mf);
+  for (int e = threadIdx.x; e < kYT * batch; e += blockDim.x) {   // sample-major reads
+    const int jj = e / batch, c = e - jj * batch;
+    if (jj < nj) yt[jj * ld + c] = in[(int64_t)si[jj] * C + c];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kYT * batch; e += blockDim.x) {   // coil-major writes
+    const int c = e / kYT, jj = e - c * kYT;
+    if (jj < nj) ext[eb[jj] + mf[jj] * c] = yt[jj * ld + c];
+  }
+}
+
 void launch_y_ext2int(const Plan& p, const float2* ext, float2* in, int batch, int only_frame, hipStream_t s) {
+  if (only_frame < 0 && p.iperm && batch <= p.C && p.C < 128) {   // its LDS tile within 64 KB
+    const size_t shm = sizeof(float2) * kYT * (size_t)(p.C + 1);
+    hipLaunchKernelGGL(k_y_ext2int_t, dim3((unsigned)((p.Mtot + kYT - 1) / kYT)), dim3(256), shm, s, ext, in,
+                       p.iperm, p.sframe, p.d_foff, p.C, batch, p.Mtot);
+    return;
+  }
   hipLaunchKernelGGL(k_y_ext2int, dim3(2048), dim3(256), 0, s, ext, in, p.perm, p.sframe, p.d_foff, p.C, batch,
                      only_frame, p.Mtot);
 }
 void launch_y_int2ext(const Plan& p, const float2* in, float2* ext, int batch, int only_frame, hipStream_t s) {
+  if (only_frame < 0 && p.iperm && batch <= p.C && p.C < 128) {
+    const size_t shm = sizeof(float2) * kYT * (size_t)(p.C + 1);
+    hipLaunchKernelGGL(k_y_int2ext_t, dim3((unsigned)((p.Mtot + kYT - 1) / kYT)), dim3(256), shm, s, in, ext,
+                       p.iperm, p.sframe, p.d_foff, p.C, batch, p.Mtot);
+    return;
+  }
   hipLaunchKernelGGL(k_y_int2ext, dim3(2048), dim3(256), 0, s, in, ext, p.perm, p.sframe, p.d_foff, p.C, batch,
                      only_frame, p.Mtot);
 }

@@ -611,7 +611,7 @@ static void free_bins(Plan& p) {
 // frees the tile-sorted sample tables (and, with sweep, the line-sweep and bin tables)
 static void free_samples(Plan& p, bool sweep = true) {
   if (sweep) { free_sweep(p); free_bins(p); }
-  dfree(p.base); dfree(p.psi); dfree(p.perm); dfree(p.sframe);
+  dfree(p.base); dfree(p.psi); dfree(p.perm); dfree(p.iperm); dfree(p.sframe);
   dfree(p.interp_items); dfree(p.tile_items); dfree(p.grid_items);
   p.n_tile_items = 0; dfree(p.reduce_items); dfree(p.visits); dfree(p.vrec); dfree(p.partial);
   p.n_interp_items = p.n_grid_items = p.n_reduce_items = 0;
@@ -974,6 +974,7 @@ static int build_samples(Plan& p, const float* k, bool sweep = true) {
   CHK(dalloc(&p.base, hbase.size()));
   CHK(dalloc(&p.psi, hpsi.size()));
   CHK(dalloc(&p.perm, perm.size()));
+  CHK(dalloc(&p.iperm, perm.size()));
   CHK(dalloc(&p.sframe, hframe.size()));
   CHK(dalloc(&p.interp_items, items.size()));
   CHK(dalloc(&p.tile_items, titems.size()));
@@ -990,6 +991,11 @@ static int build_samples(Plan& p, const float* k, bool sweep = true) {
   HIPCHK(hipMemcpy(p.base, hbase.data(), sizeof(int) * hbase.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p.psi, hpsi.data(), sizeof(float) * hpsi.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p.perm, perm.data(), sizeof(int) * perm.size(), hipMemcpyHostToDevice));
+  {
+    std::vector<int> iperm(perm.size());
+    for (size_t s = 0; s < perm.size(); ++s) iperm[(size_t)perm[s]] = (int)s;
+    HIPCHK(hipMemcpy(p.iperm, iperm.data(), sizeof(int) * iperm.size(), hipMemcpyHostToDevice));
+  }
   HIPCHK(hipMemcpy(p.sframe, hframe.data(), sizeof(int) * hframe.size(), hipMemcpyHostToDevice));
   if (!items.empty())
     HIPCHK(hipMemcpy(p.interp_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
