@@ -637,35 +637,40 @@ __global__ void k_y_int2ext(const float2* in, float2* ext, const int* perm, cons
 // C coils as one row of the internal layout (C x 8 contiguous bytes) and each
 // coil's 64 samples as one run of codeare's column (512 contiguous bytes).
 constexpr int kYT = 64;   // samples per block
-// per sample of the block: its internal index, its column base in codeare's
-// layout and its frame's sample count (the column stride)
+// per sample of the block (codeare order j = j0 + jj, j < jend): its internal
+// index, its column base in codeare's layout and its frame's sample count (the
+// column stride).  all_frames: codeare's frame blocks one after the other
+// (batch columns each); else the one frame's block alone.
 __device__ __forceinline__ void y_tile_index(const int* iperm, const int* sframe, const int64_t* foff, int batch,
-                                             int64_t j0, int64_t Mtot, int* si, int64_t* eb, int64_t* mf) {
+                                             int64_t j0, int64_t jend, bool all_frames, int* si, int64_t* eb,
+                                             int64_t* mf) {
   const int jj = threadIdx.x;
   if (jj < kYT) {
     const int64_t j = j0 + jj;
-    if (j < Mtot) {
+    if (j < jend) {
       const int s = iperm[j];
       const int f = sframe[s];
       const int64_t o = foff[f];
       si[jj] = s;
-      eb[jj] = (int64_t)batch * o + (j - o);
+      eb[jj] = (all_frames ? (int64_t)batch * o : 0) + (j - o);
       mf[jj] = foff[f + 1] - o;
     }
   }
   __syncthreads();
 }
 
+// blocks cover codeare's samples [jbeg, jend): all of them (only_frame < 0) or
+// one frame's
 __global__ void __launch_bounds__(256) k_y_ext2int_t(const float2* ext, float2* in, const int* iperm,
                                                      const int* sframe, const int64_t* foff, int C, int batch,
-                                                     int64_t Mtot) {
+                                                     int64_t jbeg, int64_t jend, int only_frame) {
   extern __shared__ float2 yt[];                 // [kYT][C + 1]
   __shared__ int si[kYT];
   __shared__ int64_t eb[kYT], mf[kYT];
-  const int64_t j0 = (int64_t)blockIdx.x * kYT;
-  const int nj = (int)min<int64_t>(kYT, Mtot - j0);
+  const int64_t j0 = jbeg + (int64_t)blockIdx.x * kYT;
+  const int nj = (int)min<int64_t>(kYT, jend - j0);
   const int ld = C + 1;
-  y_tile_index(iperm, sframe, foff, batch, j0, Mtot, si, eb, mf);
+  y_tile_index(iperm, sframe, foff, batch, j0, jend, only_frame < 0, si, eb, mf);
   for (int e = threadIdx.x; e < kYT * batch; e += blockDim.x) {   // coil-major: runs of codeare's column
     const int c = e / kYT, jj = e - c * kYT;
     if (jj < nj) yt[jj * ld + c] = ext[eb[jj] + mf[jj] * c];
@@ -679,14 +684,14 @@ __global__ void __launch_bounds__(256) k_y_ext2int_t(const float2* ext, float2* 
 
 __global__ void __launch_bounds__(256) k_y_int2ext_t(const float2* in, float2* ext, const int* iperm,
                                                      const int* sframe, const int64_t* foff, int C, int batch,
-                                                     int64_t Mtot) {
+                                                     int64_t jbeg, int64_t jend, int only_frame) {
   extern __shared__ float2 yt[];                 // [kYT][C + 1]
   __shared__ int si[kYT];
   __shared__ int64_t eb[kYT], mf[kYT];
-  const int64_t j0 = (int64_t)blockIdx.x * kYT;
-  const int nj = (int)min<int64_t>(kYT, Mtot - j0);
+  const int64_t j0 = jbeg + (int64_t)blockIdx.x * kYT;
+  const int nj = (int)min<int64_t>(kYT, jend - j0);
   const int ld = C + 1;
-  y_tile_index(iperm, sframe, foff, batch, j0, Mtot, si, eb, mf);
+  y_tile_index(iperm, sframe, foff, batch, j0, jend, only_frame < 0, si, eb, mf);
   for (int e = threadIdx.x; e < kYT * batch; e += blockDim.x) {   // sample-major reads
     const int jj = e / batch, c = e - jj * batch;
     if (jj < nj) yt[jj * ld + c] = in[(int64_t)si[jj] * C + c];
@@ -698,21 +703,32 @@ __global__ void __launch_bounds__(256) k_y_int2ext_t(const float2* in, float2* e
   }
 }
 
+// the samples of codeare's order a conversion covers
+static void y_range(const Plan& p, int only_frame, int64_t* jbeg, int64_t* jend) {
+  *jbeg = only_frame < 0 ? 0 : p.foff[only_frame];
+  *jend = only_frame < 0 ? p.Mtot : p.foff[only_frame + 1];
+}
 void launch_y_ext2int(const Plan& p, const float2* ext, float2* in, int batch, int only_frame, hipStream_t s) {
-  if (only_frame < 0 && p.iperm && batch <= p.C && p.C < 128) {   // its LDS tile within 64 KB
+  if (p.iperm && batch <= p.C && p.C < 128) {   // its LDS tile within 64 KB
+    int64_t jb, je;
+    y_range(p, only_frame, &jb, &je);
+    if (je <= jb) return;
     const size_t shm = sizeof(float2) * kYT * (size_t)(p.C + 1);
-    hipLaunchKernelGGL(k_y_ext2int_t, dim3((unsigned)((p.Mtot + kYT - 1) / kYT)), dim3(256), shm, s, ext, in,
-                       p.iperm, p.sframe, p.d_foff, p.C, batch, p.Mtot);
+    hipLaunchKernelGGL(k_y_ext2int_t, dim3((unsigned)((je - jb + kYT - 1) / kYT)), dim3(256), shm, s, ext, in,
+                       p.iperm, p.sframe, p.d_foff, p.C, batch, jb, je, only_frame);
     return;
   }
   hipLaunchKernelGGL(k_y_ext2int, dim3(2048), dim3(256), 0, s, ext, in, p.perm, p.sframe, p.d_foff, p.C, batch,
                      only_frame, p.Mtot);
 }
 void launch_y_int2ext(const Plan& p, const float2* in, float2* ext, int batch, int only_frame, hipStream_t s) {
-  if (only_frame < 0 && p.iperm && batch <= p.C && p.C < 128) {
+  if (p.iperm && batch <= p.C && p.C < 128) {
+    int64_t jb, je;
+    y_range(p, only_frame, &jb, &je);
+    if (je <= jb) return;
     const size_t shm = sizeof(float2) * kYT * (size_t)(p.C + 1);
-    hipLaunchKernelGGL(k_y_int2ext_t, dim3((unsigned)((p.Mtot + kYT - 1) / kYT)), dim3(256), shm, s, in, ext,
-                       p.iperm, p.sframe, p.d_foff, p.C, batch, p.Mtot);
+    hipLaunchKernelGGL(k_y_int2ext_t, dim3((unsigned)((je - jb + kYT - 1) / kYT)), dim3(256), shm, s, in, ext,
+                       p.iperm, p.sframe, p.d_foff, p.C, batch, jb, je, only_frame);
     return;
   }
   hipLaunchKernelGGL(k_y_int2ext, dim3(2048), dim3(256), 0, s, in, ext, p.perm, p.sframe, p.d_foff, p.C, batch,
@@ -736,10 +752,56 @@ __global__ void k_img_coil2ext(const float2* X, float2* ext, int C, int batch, i
     if (c < batch) ext[i + Ntot * c] = X[(int64_t)f * Ntot * C + idx];
   }
 }
+// the same as transposes through LDS: 64 pixels per block, each coil's run of
+// pixels and each pixel's row of coils contiguous
+__global__ void __launch_bounds__(256) k_img_ext2coil_t(const float2* ext, float2* X, int C, int batch,
+                                                        int64_t Ntot, int f) {
+  extern __shared__ float2 it_[];                // [kYT][C + 1]
+  const int64_t i0 = (int64_t)blockIdx.x * kYT;
+  const int ni = (int)min<int64_t>(kYT, Ntot - i0), ld = C + 1;
+  for (int e = threadIdx.x; e < kYT * batch; e += blockDim.x) {
+    const int c = e / kYT, ii = e - c * kYT;
+    if (ii < ni) it_[ii * ld + c] = ext[i0 + ii + Ntot * c];
+  }
+  __syncthreads();
+  float2* Xf = X + ((int64_t)f * Ntot + i0) * C;
+  for (int e = threadIdx.x; e < kYT * C; e += blockDim.x) {
+    const int ii = e / C, c = e - ii * C;
+    if (ii < ni) Xf[(int64_t)ii * C + c] = c < batch ? it_[ii * ld + c] : make_float2(0.f, 0.f);
+  }
+}
+__global__ void __launch_bounds__(256) k_img_coil2ext_t(const float2* X, float2* ext, int C, int batch,
+                                                        int64_t Ntot, int f) {
+  extern __shared__ float2 it_[];                // [kYT][C + 1]
+  const int64_t i0 = (int64_t)blockIdx.x * kYT;
+  const int ni = (int)min<int64_t>(kYT, Ntot - i0), ld = C + 1;
+  const float2* Xf = X + ((int64_t)f * Ntot + i0) * C;
+  for (int e = threadIdx.x; e < kYT * batch; e += blockDim.x) {
+    const int ii = e / batch, c = e - ii * batch;
+    if (ii < ni) it_[ii * ld + c] = Xf[(int64_t)ii * C + c];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kYT * batch; e += blockDim.x) {
+    const int c = e / kYT, ii = e - c * kYT;
+    if (ii < ni) ext[i0 + ii + Ntot * c] = it_[ii * ld + c];
+  }
+}
 void launch_img_ext2coil(const Plan& p, const float2* ext, float2* X, int batch, int frame, hipStream_t s) {
+  if (p.C < 128) {
+    const size_t shm = sizeof(float2) * kYT * (size_t)(p.C + 1);
+    hipLaunchKernelGGL(k_img_ext2coil_t, dim3((unsigned)((p.Ntot + kYT - 1) / kYT)), dim3(256), shm, s, ext, X, p.C,
+                       batch, p.Ntot, frame);
+    return;
+  }
   hipLaunchKernelGGL(k_img_ext2coil, dim3(2048), dim3(256), 0, s, ext, X, p.C, batch, p.Ntot, frame);
 }
 void launch_img_coil2ext(const Plan& p, const float2* X, float2* ext, int batch, int frame, hipStream_t s) {
+  if (p.C < 128) {
+    const size_t shm = sizeof(float2) * kYT * (size_t)(p.C + 1);
+    hipLaunchKernelGGL(k_img_coil2ext_t, dim3((unsigned)((p.Ntot + kYT - 1) / kYT)), dim3(256), shm, s, X, ext, p.C,
+                       batch, p.Ntot, frame);
+    return;
+  }
   hipLaunchKernelGGL(k_img_coil2ext, dim3(2048), dim3(256), 0, s, X, ext, p.C, batch, p.Ntot, frame);
 }
 
