@@ -131,6 +131,9 @@ __device__ __forceinline__ void dft(float2 (&v)[N]) {
   } else if constexpr (N == 6) {
     stg<6, 1, 2, SG>(v, t);
     stg<6, 2, 3, SG>(t, v);
+  } else if constexpr (N == 12) {
+    stg<12, 1, 4, SG>(v, t);
+    stg<12, 4, 3, SG>(t, v);
   } else if constexpr (N == 4) {
     stg<4, 1, 4, SG>(v, t);
 #pragma unroll
@@ -217,7 +220,8 @@ constexpr int kRow = 144;          // pad/crop exchange row (float2): 8 columns 
 // j + 16 k < N1 are formed and stored.
 template <int P, int SG>
 __global__ void __launch_bounds__(256) k_toep_conv(ToepArgs a) {
-  constexpr int n = 16 * P, H = P / 2, RPT = P / 16;
+  // P = 24 (n = 384): the second exchange round holds rows 16..23 only
+  constexpr int n = 16 * P, H = P / 2, RPT = (P + 15) / 16;
   __shared__ float2 ex[16 * kXS];
   __shared__ float2 twT[16 * P];   // [j][k1] = exp(SG 2 pi i j k1 / n)
   const int tid = threadIdx.x;
@@ -247,34 +251,40 @@ __global__ void __launch_bounds__(256) k_toep_conv(ToepArgs a) {
   for (int k1 = 1; k1 < P; ++k1) v[k1] = cmul(v[k1], twT[j * P + k1]);
   float2 X[RPT][16];
 #pragma unroll
-  for (int r = 0; r < RPT; ++r) {   // round r: rows k1 = 16 r .. 16 r + 15
+  for (int r = 0; r < RPT; ++r) {   // round r: rows k1 = 16 r .. 16 r + 15 (< P)
     if (r) lds_barrier();
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) ex[kk * kXS + j * 16 + q] = v[16 * r + kk];
+    for (int kk = 0; kk < 16; ++kk)
+      if (16 * r + kk < P) ex[kk * kXS + j * 16 + q] = v[16 * r + kk];
     lds_barrier();
 #pragma unroll
     for (int jj = 0; jj < 16; ++jj) X[r][jj] = ex[j * kXS + jj * 16 + q];
-    dft<16, SG>(X[r]);
+    dft<16, SG>(X[r]);   // rows past P (a partial round's threads j >= P - 16 r) are never read back
   }
   // thread j holds bins (j + 16 r) + P k2
   const int vt = j * 4;
 #pragma unroll
   for (int r = 0; r < RPT; ++r)
+    if (16 * r + j < P) {
 #pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) {
-      const float t = bld1(rt, vt + (16 * r + P * k2) * 4, 0);
-      X[r][k2] = make_float2(X[r][k2].x * t, X[r][k2].y * t);
+      for (int k2 = 0; k2 < 16; ++k2) {
+        const float t = bld1(rt, vt + (16 * r + P * k2) * 4, 0);
+        X[r][k2] = make_float2(X[r][k2].x * t, X[r][k2].y * t);
+      }
     }
 #pragma unroll
   for (int r = 0; r < RPT; ++r) dft<16, -SG>(X[r]);   // over k2 -> column j'
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
     lds_barrier();   // the previous round's (or the forward exchange's) reads are done
+    if (16 * r + j < P) {
 #pragma unroll
-    for (int jj = 0; jj < 16; ++jj) ex[j * kXS + jj * 16 + q] = cmulc(X[r][jj], twT[jj * P + j + 16 * r]);
+      for (int jj = 0; jj < 16; ++jj) ex[j * kXS + jj * 16 + q] = cmulc(X[r][jj], twT[jj * P + j + 16 * r]);
+    }
     lds_barrier();
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) v[16 * r + kk] = ex[kk * kXS + j * 16 + q];
+    for (int kk = 0; kk < 16; ++kk)
+      if (16 * r + kk < P) v[16 * r + kk] = ex[kk * kXS + j * 16 + q];
   }
   dft_low_half<P, -SG>(v);   // over k1 -> k < H: position j + 16 k
 #pragma unroll
@@ -1127,7 +1137,11 @@ void launch_toep_conv(const ToepArgs& a, hipStream_t s) {
   static const size_t extra = occ_pad_lds("CGS_TOEP_OCC_CONV", 37888);
   if (a.N1 == 256) CGS_LAUNCH((k_toep_conv<32, -1>), dim3((unsigned)(lines / 16)), dim3(256), extra, s, a);
   else if (a.N1 == 128) CGS_LAUNCH((k_toep_conv<16, -1>), dim3((unsigned)(lines / 16)), dim3(256), extra, s, a);
-  else CGS_LAUNCH((k_toep_conv_r8<6, -1>), dim3((unsigned)(lines / 16)), dim3(1024), 0, s, a);
+  else {   // N1 = 192: 16 threads per line (P = 24) unless CGS_TOEP_CONV24=0 (64 threads per line)
+    static const bool c24 = [] { const char* e = std::getenv("CGS_TOEP_CONV24"); return !(e && std::atoi(e) == 0); }();
+    if (c24) CGS_LAUNCH((k_toep_conv<24, -1>), dim3((unsigned)(lines / 16)), dim3(256), 0, s, a);
+    else CGS_LAUNCH((k_toep_conv_r8<6, -1>), dim3((unsigned)(lines / 16)), dim3(1024), 0, s, a);
+  }
 }
 
 template <int NP>
