@@ -1584,11 +1584,12 @@ static bool toep_lean_enabled() {
 }
 
 // per-coil images X_c = crop(IFFT_2N(That .* FFT_2N(pad(S_c x)))) (Toeplitz embedding)
-// crop (and, opt-in, pad) as one block per grid row over the row's 2..4 coil
-// groups, loading group g + 1 while group g is transformed: the crop then sums
-// all coils (one partial image for the CG tail).  Measured at 256^2 x 32 coils:
-// crop 13.65 -> 13.03 us and tail 4.62 -> 4.57 us; the row pad 13.58 -> 13.91 us,
-// so the pad keeps a block per group unless CGS_TOEP_PAD_ROW=1.
+// crop and pad as one block per grid row over the row's 2..4 coil groups,
+// loading group g + 1 while group g is transformed: the crop then sums all
+// coils (one partial image), and the row kernels carry the folded CG tail.
+// Measured at 256^2 x 32 coils before the fold: crop 13.65 -> 13.03 us, the row
+// pad 13.58 -> 13.91 us against a block per group; CGS_TOEP_ROW=0 /
+// CGS_TOEP_PAD_ROW=0 select a block per group of 16 coils.
 static bool toep_row_shape(int C) { return C % 16 == 0 && C / 16 >= 2 && C / 16 <= 4; }
 static bool toep_row_enabled(int C) {
   static const bool on = [] {
