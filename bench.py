@@ -319,6 +319,8 @@ NUFFT_VARIANTS = [
     ("2d_interp_global", 2, {"CGS_INTERP_LDS": "0"}),
     ("2d_interp_rows", 2, {"CGS_INTERP_CS": "0"}),
     ("2d_interp_x512", 2, {"CGS_INTERP_CS": "2"}),
+    ("2d_interp_cs256", 2, {"CGS_INTERP_CS": "1"}),
+    ("2d_interp_pairs256", 2, {"CGS_INTERP_CS": "3"}),
     ("2d_interp_cg32", 2, {"CGS_INTERP_CG": "32"}),
     ("2d_interp_cg8", 2, {"CGS_INTERP_CG": "8"}),
     ("2d_interp_item256", 2, {"CGS_TILE_ITEM_MAX": "256"}),

@@ -340,7 +340,8 @@ struct Plan {
   int visits_per_item = 256;
   bool use_interp_lds = true;         // gather from the tile staged in LDS
   int interp_lanes = 0;               // LDS gather lanes: 0 (coil, tap row); 2D (coil, sample) in blocks of
-                                      // 1: 256, 2: 512 threads; 3D 1: (sample, plane), 2: (sample, plane, parity)
+                                      // 1: 256, 2: 512 threads, (coil pair, sample) 3: 256, 4: 512;
+                                      // 3D 1: (sample, plane), 2: (sample, plane, parity)
   bool use_grid_batched = true;       // batched visit lists (else one visit at a time)
   bool use_grid_reg = true;           // 2D: register accumulation, lane = cell of an 8x8 tile
   bool use_grid_mfma = false;         // 2D: scatter on the matrix cores (k_grid_mfma)

@@ -350,10 +350,15 @@ __global__ void __launch_bounds__(256) k_interp_lds(SampArgs a, const int4* item
 // row.  Only the region is staged (its LDS, 29 KB at 16 coils, lets four
 // 512-thread blocks share a CU); window values and base cells are read from
 // global memory, one address per sample (a broadcast to its CG lanes).
-template <int CG, int TAPS, int TT, int NTH>
+// CPL = 2: a lane takes two adjacent coils, so that every tap is one 16-byte
+// LDS read (ds_read_b128 moves about three times the bytes per clock of
+// ds_read_b64 on gfx950); rowstride and CG are even, so the pairs are aligned.
+template <int CG, int TAPS, int TT, int NTH, int CPL>
 __global__ void __launch_bounds__(NTH) k_interp_cs(SampArgs a, const int4* items, int nitems, int T_,
                                                    int rowstride_, int maxs, int nt0) {
-  constexpr int SPW = 64 / CG, NW = NTH / 64;
+  static_assert(CPL == 1 || (CPL == 2 && CG % 2 == 0), "one or two coils per lane");
+  constexpr int LPS = CG / CPL;            // lanes per sample
+  constexpr int SPW = 64 / LPS, NW = NTH / 64;
   extern __shared__ float2 sh[];
   const int item = blockIdx.x;
   const int4 it = items[item];             // frame, tile, sbeg, send
@@ -369,8 +374,8 @@ __global__ void __launch_bounds__(NTH) k_interp_cs(SampArgs a, const int4* items
   const int org0 = (it.y % nt0) * T - m, org1 = (it.y / nt0) * T - m;
   lds_barrier();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int cl = lane % CG, sl = lane / CG;
-  const int c = cbase + cl;
+  const int cl = lane % LPS, sl = lane / LPS;
+  const int c = cbase + cl * CPL;
   // the window values and base cell of the lane's next sample are loaded while
   // the current one is summed (lanes past the item's end load sample sbeg)
   float nx[TAPS], ny[TAPS];
@@ -401,22 +406,51 @@ __global__ void __launch_bounds__(NTH) k_interp_cs(SampArgs a, const int4* items
     const int smp = it.z + k;
     if (r0 < 0) r0 += a.n0; else if (r0 >= a.n0) r0 -= a.n0;
     if (r1 < 0) r1 += a.n1; else if (r1 >= a.n1) r1 -= a.n1;
-    const float2* base = ext + r1 * rowstride + r0 * CG + cl;
-    float2 acc = make_float2(0.f, 0.f);
+    if constexpr (CPL == 1) {
+      const float2* base = ext + r1 * rowstride + r0 * CG + cl;
+      float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-    for (int s1 = 0; s1 < TAPS; ++s1) {
-      const float2* row = base + s1 * rowstride;
-      float2 racc = make_float2(0.f, 0.f);
+      for (int s1 = 0; s1 < TAPS; ++s1) {
+        const float2* row = base + s1 * rowstride;
+        float2 racc = make_float2(0.f, 0.f);
 #pragma unroll
-      for (int s0 = 0; s0 < TAPS; ++s0) {
-        const float2 v = row[s0 * CG];
-        racc.x += px[s0] * v.x;
-        racc.y += px[s0] * v.y;
+        for (int s0 = 0; s0 < TAPS; ++s0) {
+          const float2 v = row[s0 * CG];
+          racc.x += px[s0] * v.x;
+          racc.y += px[s0] * v.y;
+        }
+        acc.x += py[s1] * racc.x;
+        acc.y += py[s1] * racc.y;
       }
-      acc.x += py[s1] * racc.x;
-      acc.y += py[s1] * racc.y;
+      if (c < a.C) a.y[(int64_t)smp * a.C + c] = acc;
+    } else {
+      const float4* base = reinterpret_cast<const float4*>(ext + r1 * rowstride + r0 * CG + 2 * cl);
+      float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int s1 = 0; s1 < TAPS; ++s1) {
+        const float4* row = base + s1 * (rowstride / 2);
+        float4 racc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int s0 = 0; s0 < TAPS; ++s0) {
+          const float4 v = row[s0 * (CG / 2)];
+          racc.x += px[s0] * v.x;
+          racc.y += px[s0] * v.y;
+          racc.z += px[s0] * v.z;
+          racc.w += px[s0] * v.w;
+        }
+        acc.x += py[s1] * racc.x;
+        acc.y += py[s1] * racc.y;
+        acc.z += py[s1] * racc.z;
+        acc.w += py[s1] * racc.w;
+      }
+      float2* yo = a.y + (int64_t)smp * a.C + c;
+      if ((a.C & 1) == 0 && c + 1 < a.C) {   // (smp C + c) even: 16-byte aligned
+        *reinterpret_cast<float4*>(yo) = acc;
+      } else {
+        if (c < a.C) yo[0] = make_float2(acc.x, acc.y);
+        if (c + 1 < a.C) yo[1] = make_float2(acc.z, acc.w);
+      }
     }
-    if (c < a.C) a.y[(int64_t)smp * a.C + c] = acc;
   }
 }
 
@@ -1271,56 +1305,67 @@ static void interp_lds_taps(const SampArgs& a, const int4* items, int nitems, in
   else interp_lds_tt<CG, 0>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, s);
 }
 
-template <int CG, int TAPS, int TT>
+template <int CG, int TAPS, int TT, int CPL>
 static void interp_cs_go(const SampArgs& a, const int4* items, int nitems, int T, int rowstride, int maxs,
                          int nt0, dim3 g, size_t shm, int nth, hipStream_t s) {
   if (shm > 65536) {   // more than 64 KB of dynamic LDS (32 coils) needs the kernel's limit raised (once)
     static const bool raised = [] {
-      return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_interp_cs<CG, TAPS, TT, 256>),
+      return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_interp_cs<CG, TAPS, TT, 256, CPL>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess &&
-             hipFuncSetAttribute(reinterpret_cast<const void*>(&k_interp_cs<CG, TAPS, TT, 512>),
+             hipFuncSetAttribute(reinterpret_cast<const void*>(&k_interp_cs<CG, TAPS, TT, 512, CPL>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess;
     }();
     (void)raised;
   }
   if (nth == 512)
-    hipLaunchKernelGGL((k_interp_cs<CG, TAPS, TT, 512>), g, dim3(512), shm, s, a, items, nitems, T, rowstride, maxs, nt0);
+    hipLaunchKernelGGL((k_interp_cs<CG, TAPS, TT, 512, CPL>), g, dim3(512), shm, s, a, items, nitems, T, rowstride,
+                       maxs, nt0);
   else
-    hipLaunchKernelGGL((k_interp_cs<CG, TAPS, TT, 256>), g, dim3(256), shm, s, a, items, nitems, T, rowstride, maxs, nt0);
+    hipLaunchKernelGGL((k_interp_cs<CG, TAPS, TT, 256, CPL>), g, dim3(256), shm, s, a, items, nitems, T, rowstride,
+                       maxs, nt0);
+}
+
+template <int CG, int TAPS, int TT>
+static void interp_cs_cpl(const SampArgs& a, const int4* items, int nitems, int T, int rowstride, int maxs,
+                          int nt0, dim3 g, size_t shm, int nth, int cpl, hipStream_t s) {
+  if (cpl == 2) interp_cs_go<CG, TAPS, TT, 2>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, s);
+  else interp_cs_go<CG, TAPS, TT, 1>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, s);
 }
 
 template <int CG, int TT>
 static void interp_cs_tt(const SampArgs& a, const int4* items, int nitems, int T, int rowstride, int maxs,
-                         int nt0, dim3 g, size_t shm, int nth, hipStream_t s) {
+                         int nt0, dim3 g, size_t shm, int nth, int cpl, hipStream_t s) {
   switch (a.taps) {
-    case 4: interp_cs_go<CG, 4, TT>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, s); break;
-    case 6: interp_cs_go<CG, 6, TT>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, s); break;
-    case 8: interp_cs_go<CG, 8, TT>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, s); break;
-    case 10: interp_cs_go<CG, 10, TT>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, s); break;
+    case 4: interp_cs_cpl<CG, 4, TT>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, cpl, s); break;
+    case 6: interp_cs_cpl<CG, 6, TT>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, cpl, s); break;
+    case 8: interp_cs_cpl<CG, 8, TT>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, cpl, s); break;
+    case 10: interp_cs_cpl<CG, 10, TT>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, cpl, s); break;
     default: break;
   }
 }
 
 template <int CG>
 static void interp_cs_taps(const SampArgs& a, const int4* items, int nitems, int T, int rowstride, int maxs,
-                           int nt0, dim3 g, size_t shm, int nth, hipStream_t s) {
-  if (T == 8) interp_cs_tt<CG, 8>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, s);
-  else interp_cs_tt<CG, 0>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, s);
+                           int nt0, dim3 g, size_t shm, int nth, int cpl, hipStream_t s) {
+  if (T == 8) interp_cs_tt<CG, 8>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, cpl, s);
+  else interp_cs_tt<CG, 0>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, nth, cpl, s);
 }
 
-// lanes: 0 = (coil, tap row slot) k_interp_lds; 1, 2 = (coil, sample) k_interp_cs in blocks of 256, 512
+// lanes: 0 = (coil, tap row slot) k_interp_lds; 1, 2 = (coil, sample) k_interp_cs in blocks of 256, 512;
+// 3, 4 = (coil pair, sample) in blocks of 256, 512
 void launch_interp_lds(const SampArgs& a, const int4* items, int nitems, int T, int nt0, int cg, int maxs,
                        int ncgroups, int lanes, hipStream_t s) {
   if (nitems == 0) return;
   const int rowstride = (T + a.taps - 1) * cg + ((cg == 8) ? 8 : 0);
   const size_t shm = interp_lds_bytes(T, a.taps, cg, rowstride, maxs);
   const dim3 g((unsigned)nitems, (unsigned)ncgroups);
-  if (lanes >= 1) {   // 1: 256-thread blocks, 2: 512; the region alone in LDS
+  if (lanes >= 1) {   // the region alone in LDS
     const size_t rshm = sizeof(float2) * (size_t)(T + a.taps - 1) * rowstride;
-    const int nth = lanes == 2 ? 512 : 256;
-    if (cg == 8) interp_cs_taps<8>(a, items, nitems, T, rowstride, maxs, nt0, g, rshm, nth, s);
-    else if (cg == 16) interp_cs_taps<16>(a, items, nitems, T, rowstride, maxs, nt0, g, rshm, nth, s);
-    else interp_cs_taps<32>(a, items, nitems, T, rowstride, maxs, nt0, g, rshm, nth, s);
+    const int nth = (lanes == 2 || lanes == 4) ? 512 : 256;
+    const int cpl = lanes >= 3 ? 2 : 1;
+    if (cg == 8) interp_cs_taps<8>(a, items, nitems, T, rowstride, maxs, nt0, g, rshm, nth, cpl, s);
+    else if (cg == 16) interp_cs_taps<16>(a, items, nitems, T, rowstride, maxs, nt0, g, rshm, nth, cpl, s);
+    else interp_cs_taps<32>(a, items, nitems, T, rowstride, maxs, nt0, g, rshm, nth, cpl, s);
     return;
   }
   if (cg == 8) interp_lds_taps<8>(a, items, nitems, T, rowstride, maxs, nt0, g, shm, s);

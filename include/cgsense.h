@@ -209,7 +209,8 @@ typedef struct {
   int grid_mfma;         /* gridding: 1 matrix-core scatter, 0 VALU registers    */
   int grid_sweep;        /* gridding: 1 line sweep (2D), 0 the tile kernels      */
   int grid_bin;          /* gridding: 1 binned LDS scatter (each sample once)    */
-  int interp_lanes;      /* gather lanes: 0 (coil, tap row); 2D 1 (coil, sample); 3D 2 (sample, tap plane, tap parity) */
+  int interp_lanes;      /* gather lanes: 0 (coil, tap row); 2D 1, 2 (coil, sample), 3, 4 (coil pair, sample);
+                            3D 1 (sample, tap plane), 2 (sample, tap plane, tap parity) */
 } cgs_tune_report;
 /* Times the gridding scatter (the binned LDS scatter; 2D: the line sweep; the tile
  * kernels -- matrix-core or VALU-register, visits per work item, coils per pass) and the

@@ -151,10 +151,16 @@ def test_device_maps_load_matches_host_load(cgs, per_frame):
     (2, 32, {"CGS_INTERP_LDS": "0"}),                                             # gather from global
     (2, 32, {"CGS_INTERP_CS": "0"}),                                              # lanes = (coil, tap row)
     (2, 12, {"CGS_INTERP_CS": "0", "CGS_INTERP_CG": "8"}),
-    (2, 32, {"CGS_INTERP_CS": "1", "CGS_INTERP_CG": "32"}),                       # lanes = (coil, sample)
+    (2, 32, {"CGS_INTERP_CS": "1"}),                                              # lanes = (coil, sample)
+    (2, 32, {"CGS_INTERP_CS": "1", "CGS_INTERP_CG": "32"}),
     (2, 12, {"CGS_INTERP_CS": "1", "CGS_INTERP_CG": "8"}),
     (2, 32, {"CGS_INTERP_CS": "2"}),                                              # ... in 512-thread blocks
     (2, 12, {"CGS_INTERP_CS": "2", "CGS_INTERP_CG": "8"}),
+    (2, 32, {"CGS_INTERP_CS": "3"}),                                              # lanes = (coil pair, sample)
+    (2, 12, {"CGS_INTERP_CS": "3", "CGS_INTERP_CG": "8"}),
+    (2, 5, {"CGS_INTERP_CS": "3"}),                                               # odd coil count: a half pair
+    (2, 32, {"CGS_INTERP_CS": "4", "CGS_INTERP_CG": "32"}),                       # ... in 512-thread blocks
+    (2, 5, {"CGS_INTERP_CS": "4", "CGS_INTERP_CG": "16"}),
     (2, 12, {"CGS_GRID_MFMA": "0", "CGS_GRID_REG": "0", "CGS_GRID_BATCHED": "0"}),
     (3, 16, {"CGS_GRID_MFMA": "0"}),                                              # 3D register tiles
     (3, 16, {"CGS_INTERP_LDS": "0"}),
