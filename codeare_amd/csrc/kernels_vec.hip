@@ -824,25 +824,37 @@ __global__ void __launch_bounds__(256) k_cnr_partials(const float2* v, const flo
   const int64_t r0 = image ? (int64_t)f * Ntot : foff[f];
   const int64_t r1 = image ? r0 + Ntot : foff[f + 1];
   __shared__ double sh[256];
+  auto weight = [&](int64_t r) -> float {
+    if (!image) return w[r];
+    const int64_t i = r - r0;
+    const int i0 = (int)(i % N0);
+    const int64_t rest = i / N0;
+    const int i1 = (int)(rest % N1);
+    int64_t q = (int64_t)(i0 - mask_nh) * (i0 - mask_nh) + (int64_t)(i1 - mask_nh) * (i1 - mask_nh);
+    if (d == 3) { const int i2 = (int)(rest / N1); q += (int64_t)(i2 - mask_nh) * (i2 - mask_nh); }
+    return (q <= (int64_t)mask_nh * mask_nh) ? 1.f : 0.f;
+  };
+  const int64_t stride = (int64_t)gridDim.x * RPB;
   for (int c0 = 0; c0 < C; c0 += CP) {
     const int c = c0 + cl;
     double acc = 0.0;
     if (c < C) {
-      for (int64_t r = r0 + (int64_t)blockIdx.x * RPB + g; r < r1; r += (int64_t)gridDim.x * RPB) {
-        float wt;
-        if (image) {
-          const int64_t i = r - r0;
-          const int i0 = (int)(i % N0);
-          const int64_t rest = i / N0;
-          const int i1 = (int)(rest % N1);
-          int64_t q = (int64_t)(i0 - mask_nh) * (i0 - mask_nh) + (int64_t)(i1 - mask_nh) * (i1 - mask_nh);
-          if (d == 3) { const int i2 = (int)(rest / N1); q += (int64_t)(i2 - mask_nh) * (i2 - mask_nh); }
-          wt = (q <= (int64_t)mask_nh * mask_nh) ? 1.f : 0.f;
-        } else {
-          wt = w[r];
+      // rows r, r + stride, ... summed in that order, four rows' loads in flight
+      int64_t r = r0 + (int64_t)blockIdx.x * RPB + g;
+      for (; r + 3 * stride < r1; r += 4 * stride) {
+        float2 a[4];
+        float wt[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          a[u] = v[(r + u * stride) * C + c];
+          wt[u] = weight(r + u * stride);
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += (double)wt[u] * ((double)a[u].x * a[u].x + (double)a[u].y * a[u].y);
+      }
+      for (; r < r1; r += stride) {
         const float2 a = v[r * C + c];
-        acc += (double)wt * ((double)a.x * a.x + (double)a.y * a.y);
+        acc += (double)weight(r) * ((double)a.x * a.x + (double)a.y * a.y);
       }
     }
     sh[threadIdx.x] = acc;
@@ -856,15 +868,33 @@ __global__ void __launch_bounds__(256) k_cnr_partials(const float2* v, const flo
   }
 }
 
-__global__ void k_cnr_reduce(const double* partials, int nb, int C, int F, double* cnr, int slot, const int* fmask) {
-  // one thread per (frame, coil), sums nb partials in order
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= F * C) return;
-  const int f = idx / C, c = idx % C;
+// one block per frame: thread (coil cl, chunk j) sums partials b = j, j + 8, ...
+// in order, then chunk sums j = 0..7 are added in order (a fixed order: the
+// result does not depend on timing)
+__global__ void __launch_bounds__(256) k_cnr_reduce(const double* partials, int nb, int C, int F, double* cnr,
+                                                    int slot, const int* fmask) {
+  constexpr int CB = 32, NJ = 256 / CB;
+  __shared__ double sh[256];
+  const int f = blockIdx.x;
   if (fmask && fmask[f]) return;
-  double t = 0.0;
-  for (int b = 0; b < nb; ++b) t += partials[((int64_t)f * nb + b) * C + c];
-  cnr[(int64_t)slot * F * C + idx] = t;
+  const int cl = threadIdx.x % CB, j = threadIdx.x / CB;
+  for (int c0 = 0; c0 < C; c0 += CB) {
+    const int c = c0 + cl;
+    double t = 0.0;
+    if (c < C) {
+#pragma unroll 8
+      for (int b = j; b < nb; b += NJ) t += partials[((int64_t)f * nb + b) * C + c];
+    }
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    if (j == 0 && c < C) {
+      double u = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < NJ; ++jj) u += sh[jj * CB + cl];
+      cnr[(int64_t)slot * F * C + (int64_t)f * C + c] = u;
+    }
+    __syncthreads();
+  }
 }
 
 static void cnr_dot(const Plan& p, const float2* v, int image, double* dst_cnr, int slot, hipStream_t s) {
@@ -876,8 +906,8 @@ static void cnr_dot(const Plan& p, const float2* v, int image, double* dst_cnr, 
     hipLaunchKernelGGL(k_cnr_partials<16>, g, dim3(256), 0, s, v, p.w, p.d_foff, p.Ntot, nh, p.d, p.N[0], p.N[1], p.C, image, p.partials, p.fmask);
   else
     hipLaunchKernelGGL(k_cnr_partials<32>, g, dim3(256), 0, s, v, p.w, p.d_foff, p.Ntot, nh, p.d, p.N[0], p.N[1], p.C, image, p.partials, p.fmask);
-  const int n = p.F * p.C;
-  hipLaunchKernelGGL(k_cnr_reduce, dim3((n + 255) / 256), dim3(256), 0, s, p.partials, kCnrBlocks, p.C, p.F, dst_cnr, slot, p.fmask);
+  hipLaunchKernelGGL(k_cnr_reduce, dim3((unsigned)p.F), dim3(256), 0, s, p.partials, kCnrBlocks, p.C, p.F, dst_cnr,
+                     slot, p.fmask);
 }
 
 void launch_cnr_dot_samples(const Plan& p, const float2* v, double* cnr_slot_base, hipStream_t s) {
