@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--volume", dest="volume", action="store_true", default=None,
                     help="time the 3D koosh-ball workload (default: with the secondary measurements)")
     ap.add_argument("--no-volume", dest="volume", action="store_false", help="skip the 3D koosh-ball workload")
+    ap.add_argument("--reference-only", action="store_true",
+                    help="only codeare's own algorithm on the headline data (CGLS with the reference E^H), one JSON line")
     ap.add_argument("--nufft-only", action="store_true",
                     help="only time the NuFFT operators (E, then E^H W) at 256^2 x 32 coils and the 128^3 x 16-coil "
                          "koosh ball, `steps` applications each (for kernel profiles)")
@@ -134,6 +136,9 @@ def main():
 
     tol = wl.tol if wl.tol > 0 else 1e-5
     lam = wl.lam
+    if args.reference_only:
+        print(json.dumps({"workload": wl.name, "reference_algorithm": {"gpu": ref_compat_leg(args, plan, y, x, lam)}}))
+        return
 
     def solve(toeplitz, lam_=lam, graph=args.graph_iters):
         _, rep = plan.solve(y, maxit=args.maxit, tol=tol, lam=lam_, out=x, toeplitz=toeplitz,
