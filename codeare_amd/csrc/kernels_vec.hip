@@ -839,18 +839,19 @@ __global__ void __launch_bounds__(256) k_cnr_partials(const float2* v, const flo
     const int c = c0 + cl;
     double acc = 0.0;
     if (c < C) {
-      // rows r, r + stride, ... summed in that order, four rows' loads in flight
+      // rows r, r + stride, ... summed in that order, eight rows' loads in flight
+      constexpr int U = 8;
       int64_t r = r0 + (int64_t)blockIdx.x * RPB + g;
-      for (; r + 3 * stride < r1; r += 4 * stride) {
-        float2 a[4];
-        float wt[4];
+      for (; r + (U - 1) * stride < r1; r += U * stride) {
+        float2 a[U];
+        float wt[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
           a[u] = v[(r + u * stride) * C + c];
           wt[u] = weight(r + u * stride);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc += (double)wt[u] * ((double)a[u].x * a[u].x + (double)a[u].y * a[u].y);
+        for (int u = 0; u < U; ++u) acc += (double)wt[u] * ((double)a[u].x * a[u].x + (double)a[u].y * a[u].y);
       }
       for (; r < r1; r += stride) {
         const float2 a = v[r * C + c];
