@@ -869,6 +869,64 @@ __global__ void __launch_bounds__(256) k_cnr_partials(const float2* v, const flo
   }
 }
 
+// r -= alpha v over the frame's samples, fused with the next dot_r partials:
+// the rows, lanes and summation order of k_cnr_partials over the updated r
+// (same block partials), so the pass over r is read once
+template <int CP>
+__global__ void __launch_bounds__(256) k_cnr_update_samples_dot(float2* r, const float2* v, const float* w,
+                                                                const int64_t* foff, const double* cnr, int F,
+                                                                int C, double* partials, const int* fmask) {
+  const int f = blockIdx.y;
+  if (fmask && fmask[f]) return;
+  constexpr int RPB = 256 / CP, U = 8;
+  const int g = threadIdx.x / CP, cl = threadIdx.x % CP;
+  const int64_t r0 = foff[f], r1 = foff[f + 1];
+  const int64_t stride = (int64_t)gridDim.x * RPB;
+  __shared__ double sh[256];
+  for (int c0 = 0; c0 < C; c0 += CP) {
+    const int c = c0 + cl;
+    double acc = 0.0;
+    if (c < C) {
+      const float al = (float)cnr[(int64_t)3 * F * C + (int64_t)f * C + c];
+      int64_t row = r0 + (int64_t)blockIdx.x * RPB + g;
+      for (; row + (U - 1) * stride < r1; row += U * stride) {
+        float2 a[U], b[U];
+        float wt[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          a[u] = r[(row + u * stride) * C + c];
+          b[u] = v[(row + u * stride) * C + c];
+          wt[u] = w[row + u * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          a[u].x -= al * b[u].x;
+          a[u].y -= al * b[u].y;
+          r[(row + u * stride) * C + c] = a[u];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc += (double)wt[u] * ((double)a[u].x * a[u].x + (double)a[u].y * a[u].y);
+      }
+      for (; row < r1; row += stride) {
+        float2 a = r[row * C + c];
+        const float2 b = v[row * C + c];
+        a.x -= al * b.x;
+        a.y -= al * b.y;
+        r[row * C + c] = a;
+        acc += (double)w[row] * ((double)a.x * a.x + (double)a.y * a.y);
+      }
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < CP) {
+      double t = 0.0;
+      for (int k = 0; k < RPB; ++k) t += sh[k * CP + threadIdx.x];
+      if (c0 + threadIdx.x < C) partials[((int64_t)f * gridDim.x + blockIdx.x) * C + c0 + threadIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
 // one block per frame: thread (coil cl, chunk j) sums partials b = j, j + 8, ...
 // in order, then chunk sums j = 0..7 are added in order (a fixed order: the
 // result does not depend on timing)
@@ -1017,6 +1075,22 @@ __global__ void k_cnr_update_samples(float2* r, const float2* v, const double* c
     rr.y -= al * vv.y;
     r[idx] = rr;
   }
+}
+
+// r -= alpha v, then dot_r (slot 0) from the fused pass's partials
+void launch_cnr_update_samples_dot(const Plan& p, float2* r, const float2* v, hipStream_t s) {
+  const dim3 g(kCnrBlocks, p.F);
+  if (p.C <= 8)
+    hipLaunchKernelGGL(k_cnr_update_samples_dot<8>, g, dim3(256), 0, s, r, v, p.w, p.d_foff, p.cnr, p.F, p.C,
+                       p.partials, p.fmask);
+  else if (p.C <= 16)
+    hipLaunchKernelGGL(k_cnr_update_samples_dot<16>, g, dim3(256), 0, s, r, v, p.w, p.d_foff, p.cnr, p.F, p.C,
+                       p.partials, p.fmask);
+  else
+    hipLaunchKernelGGL(k_cnr_update_samples_dot<32>, g, dim3(256), 0, s, r, v, p.w, p.d_foff, p.cnr, p.F, p.C,
+                       p.partials, p.fmask);
+  hipLaunchKernelGGL(k_cnr_reduce, dim3((unsigned)p.F), dim3(256), 0, s, p.partials, kCnrBlocks, p.C, p.F, p.cnr, 0,
+                     p.fmask);
 }
 
 void launch_cnr_update_samples(const Plan& p, float2* r, const float2* v, hipStream_t s) {

@@ -559,8 +559,7 @@ static int cgnr(Plan& p, const float2* y, int maxit, double eps, const int* fl, 
       launch_cnr_dot_samples(p, v, p.cnr + 2 * FC, p.stream);                  // dot_v
       launch_cnr_scalars(p, 1, eps, p.stream);                                 // alpha
       launch_cnr_update_image(p, p.Xc2, p.Xc3, nullptr, 0, p.stream);          // f_hat += a w_hat p
-      launch_cnr_update_samples(p, r, v, p.stream);                            // r -= a v
-      launch_cnr_dot_samples(p, r, p.cnr + 0 * FC, p.stream);                  // dot_r
+      launch_cnr_update_samples_dot(p, r, v, p.stream);                        // r -= a v; dot_r
     }
     CHK(adj_w(r));                                                              // z = A^H W r
     {
