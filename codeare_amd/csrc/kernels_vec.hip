@@ -1058,25 +1058,6 @@ void launch_cnr_update_image(const Plan& p, float2* fhat, float2* phat, const fl
                      (int)(0.5 * p.N[0]), p.d, p.N[0], p.N[1], step, p.fmask);
 }
 
-// r -= alpha v over the sorted samples
-__global__ void k_cnr_update_samples(float2* r, const float2* v, const double* cnr, const int* sframe, int F, int C,
-                                     int64_t Mtot, const int* fmask) {
-  const int64_t tot = Mtot * C;
-  const int FC = F * C;
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t s = idx / C;
-    const int c = (int)(idx - s * C);
-    const int f = sframe[s];
-    if (fmask && fmask[f]) continue;
-    const float al = (float)cnr[3 * FC + f * C + c];
-    const float2 vv = v[idx];
-    float2 rr = r[idx];
-    rr.x -= al * vv.x;
-    rr.y -= al * vv.y;
-    r[idx] = rr;
-  }
-}
-
 // r -= alpha v, then dot_r (slot 0) from the fused pass's partials
 void launch_cnr_update_samples_dot(const Plan& p, float2* r, const float2* v, hipStream_t s) {
   const dim3 g(kCnrBlocks, p.F);
@@ -1090,11 +1071,6 @@ void launch_cnr_update_samples_dot(const Plan& p, float2* r, const float2* v, hi
     hipLaunchKernelGGL(k_cnr_update_samples_dot<32>, g, dim3(256), 0, s, r, v, p.w, p.d_foff, p.cnr, p.F, p.C,
                        p.partials, p.fmask);
   hipLaunchKernelGGL(k_cnr_reduce, dim3((unsigned)p.F), dim3(256), 0, s, p.partials, kCnrBlocks, p.C, p.F, p.cnr, 0,
-                     p.fmask);
-}
-
-void launch_cnr_update_samples(const Plan& p, float2* r, const float2* v, hipStream_t s) {
-  hipLaunchKernelGGL(k_cnr_update_samples, dim3(2048), dim3(256), 0, s, r, v, p.cnr, p.sframe, p.F, p.C, p.Mtot,
                      p.fmask);
 }
 
