@@ -811,6 +811,18 @@ void launch_img_coil2ext(const Plan& p, const float2* X, float2* ext, int batch,
 //   cnr[k*F*C + f*C + c], k: 0 dot_r, 1 dot_z, 2 dot_v, 3 alpha, 4 beta, 5 active, 6 new value
 constexpr int kCnrBlocks = 256;
 
+// inside NFFT3's damping ball (PRECOMPUTE_DAMP): pixel i of a frame (i < 2^32),
+// |(i0, i1[, i2]) - nh|^2 <= nh^2, with 32-bit index arithmetic
+__device__ __forceinline__ bool damp_in(uint32_t i, int N0, int N1, int d, int nh) {
+  const uint32_t rest = i / (uint32_t)N0;
+  const int i0 = (int)(i - rest * (uint32_t)N0);
+  const uint32_t r2 = rest / (uint32_t)N1;
+  const int i1 = (int)(rest - r2 * (uint32_t)N1);
+  int64_t q = (int64_t)(i0 - nh) * (i0 - nh) + (int64_t)(i1 - nh) * (i1 - nh);
+  if (d == 3) q += (int64_t)((int)r2 - nh) * ((int)r2 - nh);
+  return q <= (int64_t)nh * nh;
+}
+
 // per-coil weighted sums over rows of a [rows][C] array; rows are samples of
 // each frame (weights w[s]) or pixels of each frame (damping mask)
 template <int CP>
@@ -826,13 +838,7 @@ __global__ void __launch_bounds__(256) k_cnr_partials(const float2* v, const flo
   __shared__ double sh[256];
   auto weight = [&](int64_t r) -> float {
     if (!image) return w[r];
-    const int64_t i = r - r0;
-    const int i0 = (int)(i % N0);
-    const int64_t rest = i / N0;
-    const int i1 = (int)(rest % N1);
-    int64_t q = (int64_t)(i0 - mask_nh) * (i0 - mask_nh) + (int64_t)(i1 - mask_nh) * (i1 - mask_nh);
-    if (d == 3) { const int i2 = (int)(rest / N1); q += (int64_t)(i2 - mask_nh) * (i2 - mask_nh); }
-    return (q <= (int64_t)mask_nh * mask_nh) ? 1.f : 0.f;
+    return damp_in((uint32_t)(r - r0), N0, N1, d, mask_nh) ? 1.f : 0.f;
   };
   const int64_t stride = (int64_t)gridDim.x * RPB;
   for (int c0 = 0; c0 < C; c0 += CP) {
@@ -1014,48 +1020,61 @@ void launch_cnr_scalars(const Plan& p, int step, double eps, hipStream_t s) {
   hipLaunchKernelGGL(k_cnr_scalars, dim3((n + 255) / 256), dim3(256), 0, s, p.cnr, n, step, eps, p.fmask, p.C);
 }
 
-// step 0: f_hat += alpha w_hat p_hat ; step 1: p_hat = active ? beta p_hat + z : p_hat
-__global__ void k_cnr_update_image(float2* fhat, float2* phat, const float2* z, const double* cnr, int F, int C,
-                                   int64_t Ntot, int mask_nh, int d, int N0, int N1, int step, const int* fmask) {
-  const int64_t tot = (int64_t)F * Ntot * C;
-  const int FC = F * C;
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(idx % C);
-    const int64_t fi = idx / C;
-    const int f = (int)(fi / Ntot);
-    if (fmask && fmask[f]) continue;
-    const int fc = f * C + c;
+// step 0: f_hat += alpha w_hat p_hat ; step 1: p_hat = active ? beta p_hat + z : p_hat.
+// Blocks (pixel range, frame), lanes (pixel, coil): no index division per element.
+template <int CP>
+__global__ void __launch_bounds__(256) k_cnr_update_image(float2* fhat, float2* phat, const float2* z,
+                                                          const double* cnr, int F, int C, int64_t Ntot, int mask_nh,
+                                                          int d, int N0, int N1, int step, const int* fmask) {
+  const int f = blockIdx.y;
+  if (fmask && fmask[f]) return;
+  constexpr int RPB = 256 / CP;
+  const int g = threadIdx.x / CP, cl = threadIdx.x % CP;
+  const int64_t stride = (int64_t)gridDim.x * RPB;
+  const int64_t FC = (int64_t)F * C;
+  for (int c0 = 0; c0 < C; c0 += CP) {
+    const int c = c0 + cl;
+    if (c >= C) continue;
+    const int64_t fc = (int64_t)f * C + c;
+    float2* fh = fhat + (int64_t)f * Ntot * C + c;
+    float2* ph = phat + (int64_t)f * Ntot * C + c;
     if (step == 0) {
       const float al = (float)cnr[3 * FC + fc];
-      const int64_t i = fi - (int64_t)f * Ntot;
-      const int i0 = (int)(i % N0);
-      const int64_t rest = i / N0;
-      const int i1 = (int)(rest % N1);
-      int64_t q = (int64_t)(i0 - mask_nh) * (i0 - mask_nh) + (int64_t)(i1 - mask_nh) * (i1 - mask_nh);
-      if (d == 3) { const int i2 = (int)(rest / N1); q += (int64_t)(i2 - mask_nh) * (i2 - mask_nh); }
-      if (q <= (int64_t)mask_nh * mask_nh) {
-        const float2 pp = phat[idx];
-        float2 ff = fhat[idx];
+      for (int64_t i = (int64_t)blockIdx.x * RPB + g; i < Ntot; i += stride) {
+        if (!damp_in((uint32_t)i, N0, N1, d, mask_nh)) continue;
+        const float2 pp = ph[i * C];
+        float2 ff = fh[i * C];
         ff.x += al * pp.x;
         ff.y += al * pp.y;
-        fhat[idx] = ff;
+        fh[i * C] = ff;
       }
     } else {
-      if (cnr[5 * FC + fc] != 0.0) {
-        const float be = (float)cnr[4 * FC + fc];
-        const float2 zz = z[idx];
-        float2 pp = phat[idx];
-        pp.x = be * pp.x + zz.x;
-        pp.y = be * pp.y + zz.y;
-        phat[idx] = pp;
+      if (cnr[5 * FC + fc] == 0.0) continue;
+      const float be = (float)cnr[4 * FC + fc];
+      const float2* zz = z + (int64_t)f * Ntot * C + c;
+      for (int64_t i = (int64_t)blockIdx.x * RPB + g; i < Ntot; i += stride) {
+        const float2 zv = zz[i * C];
+        float2 pp = ph[i * C];
+        pp.x = be * pp.x + zv.x;
+        pp.y = be * pp.y + zv.y;
+        ph[i * C] = pp;
       }
     }
   }
 }
 
 void launch_cnr_update_image(const Plan& p, float2* fhat, float2* phat, const float2* z, int step, hipStream_t s) {
-  hipLaunchKernelGGL(k_cnr_update_image, dim3(2048), dim3(256), 0, s, fhat, phat, z, p.cnr, p.F, p.C, p.Ntot,
-                     (int)(0.5 * p.N[0]), p.d, p.N[0], p.N[1], step, p.fmask);
+  const dim3 g((unsigned)std::max(64, 2048 / std::max(1, p.F)), (unsigned)p.F);
+  const int nh = (int)(0.5 * p.N[0]);
+  if (p.C <= 8)
+    hipLaunchKernelGGL(k_cnr_update_image<8>, g, dim3(256), 0, s, fhat, phat, z, p.cnr, p.F, p.C, p.Ntot, nh, p.d,
+                       p.N[0], p.N[1], step, p.fmask);
+  else if (p.C <= 16)
+    hipLaunchKernelGGL(k_cnr_update_image<16>, g, dim3(256), 0, s, fhat, phat, z, p.cnr, p.F, p.C, p.Ntot, nh, p.d,
+                       p.N[0], p.N[1], step, p.fmask);
+  else
+    hipLaunchKernelGGL(k_cnr_update_image<32>, g, dim3(256), 0, s, fhat, phat, z, p.cnr, p.F, p.C, p.Ntot, nh, p.d,
+                       p.N[0], p.N[1], step, p.fmask);
 }
 
 // r -= alpha v, then dot_r (slot 0) from the fused pass's partials
