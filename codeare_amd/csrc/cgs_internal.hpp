@@ -587,9 +587,10 @@ void launch_img_ext2coil(const Plan& p, const float2* ext, float2* X, int batch,
 void launch_img_coil2ext(const Plan& p, const float2* X, float2* ext, int batch, int frame,
                          hipStream_t s);
 // CGNR (reference E^H) kernels
-void launch_cnr_dot_samples(const Plan& p, const float2* v, double* out_slot, hipStream_t s);
+// step 1 / 2: also take the CGNR scalar step that reads this dot product (alpha / beta)
+void launch_cnr_dot_samples(const Plan& p, const float2* v, double* out_slot, hipStream_t s, int step = -1);
 void launch_cnr_dot_image(const Plan& p, const float2* v, bool masked, double* out_slot,
-                          hipStream_t s);
+                          hipStream_t s, int step = -1);
 void launch_cnr_scalars(const Plan& p, int step, double eps, hipStream_t s);
 void launch_cnr_update_image(const Plan& p, float2* fhat, float2* phat, const float2* z,
                              int step, hipStream_t s);

@@ -936,8 +936,10 @@ __global__ void __launch_bounds__(256) k_cnr_update_samples_dot(float2* r, const
 // one block per frame: thread (coil cl, chunk j) sums partials b = j, j + 8, ...
 // in order, then chunk sums j = 0..7 are added in order (a fixed order: the
 // result does not depend on timing)
+// step 1 / 2 (else -1): the scalar step that reads this sum, taken here for the
+// same (frame, coil) -- alpha after dot_v, beta after the new dot_z (as k_cnr_scalars)
 __global__ void __launch_bounds__(256) k_cnr_reduce(const double* partials, int nb, int C, int F, double* cnr,
-                                                    int slot, const int* fmask) {
+                                                    int slot, const int* fmask, int step) {
   constexpr int CB = 32, NJ = 256 / CB;
   __shared__ double sh[256];
   const int f = blockIdx.x;
@@ -956,13 +958,22 @@ __global__ void __launch_bounds__(256) k_cnr_reduce(const double* partials, int 
       double u = 0.0;
 #pragma unroll
       for (int jj = 0; jj < NJ; ++jj) u += sh[jj * CB + cl];
-      cnr[(int64_t)slot * F * C + (int64_t)f * C + c] = u;
+      const int64_t FC = (int64_t)F * C, idx = (int64_t)f * C + c;
+      cnr[slot * FC + idx] = u;
+      if (step == 1) {          // alpha = dot_z / dot_v (u = dot_v)
+        cnr[3 * FC + idx] = (cnr[5 * FC + idx] != 0.0) ? cnr[1 * FC + idx] / u : 0.0;
+      } else if (step == 2) {   // beta = new dot_z / dot_z; dot_z = new (u = new dot_z)
+        if (cnr[5 * FC + idx] != 0.0) {
+          cnr[4 * FC + idx] = u / cnr[1 * FC + idx];
+          cnr[1 * FC + idx] = u;
+        }
+      }
     }
     __syncthreads();
   }
 }
 
-static void cnr_dot(const Plan& p, const float2* v, int image, double* dst_cnr, int slot, hipStream_t s) {
+static void cnr_dot(const Plan& p, const float2* v, int image, double* dst_cnr, int slot, int step, hipStream_t s) {
   const dim3 g(kCnrBlocks, p.F);
   const int nh = (int)(0.5 * p.N[0]);
   if (p.C <= 8)
@@ -972,18 +983,19 @@ static void cnr_dot(const Plan& p, const float2* v, int image, double* dst_cnr, 
   else
     hipLaunchKernelGGL(k_cnr_partials<32>, g, dim3(256), 0, s, v, p.w, p.d_foff, p.Ntot, nh, p.d, p.N[0], p.N[1], p.C, image, p.partials, p.fmask);
   hipLaunchKernelGGL(k_cnr_reduce, dim3((unsigned)p.F), dim3(256), 0, s, p.partials, kCnrBlocks, p.C, p.F, dst_cnr,
-                     slot, p.fmask);
+                     slot, p.fmask, step);
 }
 
-void launch_cnr_dot_samples(const Plan& p, const float2* v, double* cnr_slot_base, hipStream_t s) {
+void launch_cnr_dot_samples(const Plan& p, const float2* v, double* cnr_slot_base, hipStream_t s, int step) {
   // cnr_slot_base encodes the slot: pointer p.cnr + slot*F*C
   const int slot = (int)((cnr_slot_base - p.cnr) / (p.F * p.C));
-  cnr_dot(p, v, 0, p.cnr, slot, s);
+  cnr_dot(p, v, 0, p.cnr, slot, step, s);
 }
-void launch_cnr_dot_image(const Plan& p, const float2* v, bool masked, double* cnr_slot_base, hipStream_t s) {
+void launch_cnr_dot_image(const Plan& p, const float2* v, bool masked, double* cnr_slot_base, hipStream_t s,
+                          int step) {
   (void)masked;
   const int slot = (int)((cnr_slot_base - p.cnr) / (p.F * p.C));
-  cnr_dot(p, v, 1, p.cnr, slot, s);
+  cnr_dot(p, v, 1, p.cnr, slot, step, s);
 }
 
 // step 0: active &= dot_r >= eps (NFFTTraits.hpp:131); step 1: alpha; step 2: beta,
@@ -1090,7 +1102,7 @@ void launch_cnr_update_samples_dot(const Plan& p, float2* r, const float2* v, hi
     hipLaunchKernelGGL(k_cnr_update_samples_dot<32>, g, dim3(256), 0, s, r, v, p.w, p.d_foff, p.cnr, p.F, p.C,
                        p.partials, p.fmask);
   hipLaunchKernelGGL(k_cnr_reduce, dim3((unsigned)p.F), dim3(256), 0, s, p.partials, kCnrBlocks, p.C, p.F, p.cnr, 0,
-                     p.fmask);
+                     p.fmask, -1);
 }
 
 // Toeplitz spectrum: grid order Khat[f][r + cells*pos] -> conv-pass layout

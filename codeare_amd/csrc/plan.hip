@@ -556,16 +556,14 @@ static int cgnr(Plan& p, const float2* y, int maxit, double eps, const int* fl, 
     CHK(fwd_masked(p.Xc3, v));                                                  // v = A(w_hat p)
     {
       Prof pr(p, CGS_K_CG, 4.0 * xbytes + 4.0 * ybytes);
-      launch_cnr_dot_samples(p, v, p.cnr + 2 * FC, p.stream);                  // dot_v
-      launch_cnr_scalars(p, 1, eps, p.stream);                                 // alpha
+      launch_cnr_dot_samples(p, v, p.cnr + 2 * FC, p.stream, 1);               // dot_v; alpha
       launch_cnr_update_image(p, p.Xc2, p.Xc3, nullptr, 0, p.stream);          // f_hat += a w_hat p
       launch_cnr_update_samples_dot(p, r, v, p.stream);                        // r -= a v; dot_r
     }
     CHK(adj_w(r));                                                              // z = A^H W r
     {
       Prof pr(p, CGS_K_CG, 4.0 * xbytes);
-      launch_cnr_dot_image(p, p.Xc, true, p.cnr + 6 * FC, p.stream);           // dot_z (new)
-      launch_cnr_scalars(p, 2, eps, p.stream);                                 // beta
+      launch_cnr_dot_image(p, p.Xc, true, p.cnr + 6 * FC, p.stream, 2);        // dot_z (new); beta
       launch_cnr_update_image(p, nullptr, p.Xc3, p.Xc, 1, p.stream);           // p = b p + z
     }
   }
