@@ -594,7 +594,7 @@ void launch_cnr_dot_image(const Plan& p, const float2* v, bool masked, double* o
 void launch_cnr_scalars(const Plan& p, int step, double eps, hipStream_t s);
 void launch_cnr_update_image(const Plan& p, float2* fhat, float2* phat, const float2* z,
                              int step, hipStream_t s);
-void launch_cnr_update_samples_dot(const Plan& p, float2* r, const float2* v, hipStream_t s);
+void launch_cnr_update_samples_dot(const Plan& p, const float2* rin, float2* r, const float2* v, hipStream_t s);
 void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 void launch_toep_layout(const float2* Khat, float* That, int64_t cells, int nl, int F, float scale,
                         hipStream_t s);

@@ -879,9 +879,10 @@ __global__ void __launch_bounds__(256) k_cnr_partials(const float2* v, const flo
 // the rows, lanes and summation order of k_cnr_partials over the updated r
 // (same block partials), so the pass over r is read once
 template <int CP>
-__global__ void __launch_bounds__(256) k_cnr_update_samples_dot(float2* r, const float2* v, const float* w,
-                                                                const int64_t* foff, const double* cnr, int F,
-                                                                int C, double* partials, const int* fmask) {
+__global__ void __launch_bounds__(256) k_cnr_update_samples_dot(const float2* rin, float2* r, const float2* v,
+                                                                const float* w, const int64_t* foff,
+                                                                const double* cnr, int F, int C, double* partials,
+                                                                const int* fmask) {
   const int f = blockIdx.y;
   if (fmask && fmask[f]) return;
   constexpr int RPB = 256 / CP, U = 8;
@@ -900,7 +901,7 @@ __global__ void __launch_bounds__(256) k_cnr_update_samples_dot(float2* r, const
         float wt[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          a[u] = r[(row + u * stride) * C + c];
+          a[u] = rin[(row + u * stride) * C + c];
           b[u] = v[(row + u * stride) * C + c];
           wt[u] = w[row + u * stride];
         }
@@ -914,7 +915,7 @@ __global__ void __launch_bounds__(256) k_cnr_update_samples_dot(float2* r, const
         for (int u = 0; u < U; ++u) acc += (double)wt[u] * ((double)a[u].x * a[u].x + (double)a[u].y * a[u].y);
       }
       for (; row < r1; row += stride) {
-        float2 a = r[row * C + c];
+        float2 a = rin[row * C + c];
         const float2 b = v[row * C + c];
         a.x -= al * b.x;
         a.y -= al * b.y;
@@ -1089,17 +1090,17 @@ void launch_cnr_update_image(const Plan& p, float2* fhat, float2* phat, const fl
                        p.N[0], p.N[1], step, p.fmask);
 }
 
-// r -= alpha v, then dot_r (slot 0) from the fused pass's partials
-void launch_cnr_update_samples_dot(const Plan& p, float2* r, const float2* v, hipStream_t s) {
+// r = rin - alpha v (rin == r: in place), then dot_r (slot 0) from the fused pass's partials
+void launch_cnr_update_samples_dot(const Plan& p, const float2* rin, float2* r, const float2* v, hipStream_t s) {
   const dim3 g(kCnrBlocks, p.F);
   if (p.C <= 8)
-    hipLaunchKernelGGL(k_cnr_update_samples_dot<8>, g, dim3(256), 0, s, r, v, p.w, p.d_foff, p.cnr, p.F, p.C,
+    hipLaunchKernelGGL(k_cnr_update_samples_dot<8>, g, dim3(256), 0, s, rin, r, v, p.w, p.d_foff, p.cnr, p.F, p.C,
                        p.partials, p.fmask);
   else if (p.C <= 16)
-    hipLaunchKernelGGL(k_cnr_update_samples_dot<16>, g, dim3(256), 0, s, r, v, p.w, p.d_foff, p.cnr, p.F, p.C,
+    hipLaunchKernelGGL(k_cnr_update_samples_dot<16>, g, dim3(256), 0, s, rin, r, v, p.w, p.d_foff, p.cnr, p.F, p.C,
                        p.partials, p.fmask);
   else
-    hipLaunchKernelGGL(k_cnr_update_samples_dot<32>, g, dim3(256), 0, s, r, v, p.w, p.d_foff, p.cnr, p.F, p.C,
+    hipLaunchKernelGGL(k_cnr_update_samples_dot<32>, g, dim3(256), 0, s, rin, r, v, p.w, p.d_foff, p.cnr, p.F, p.C,
                        p.partials, p.fmask);
   hipLaunchKernelGGL(k_cnr_reduce, dim3((unsigned)p.F), dim3(256), 0, s, p.partials, kCnrBlocks, p.C, p.F, p.cnr, 0,
                      p.fmask, -1);

@@ -529,10 +529,10 @@ static int cgnr(Plan& p, const float2* y, int maxit, double eps, const int* fl, 
   float2* r = p.ybuf[2];
   float2* v = p.ybuf[3];
   {
-    Prof pr(p, CGS_K_CG, 2.0 * ybytes);
-    HIPCHK(hipMemcpyAsync(r, y, ybytes, hipMemcpyDeviceToDevice, p.stream));   // r = y - A 0
+    // r = y - A 0 = y: the first step reads y and writes r, so y is not copied
+    Prof pr(p, CGS_K_CG, 1.0 * ybytes);
     launch_cnr_scalars(p, 3, eps, p.stream);
-    launch_cnr_dot_samples(p, r, p.cnr + 0 * FC, p.stream);                    // dot_r
+    launch_cnr_dot_samples(p, y, p.cnr + 0 * FC, p.stream);                    // dot_r
   }
   auto adj_w = [&](const float2* rr) -> int {                                   // z = A^H W r
     if (ops) return ops->adj_w(rr);
@@ -544,7 +544,7 @@ static int cgnr(Plan& p, const float2* y, int maxit, double eps, const int* fl, 
     CHK(fft_forward(p, 3, xx, m));
     return interp(p, vv, m);
   };
-  CHK(adj_w(r));
+  CHK(adj_w(y));
   {
     Prof pr(p, CGS_K_CG, 3.0 * xbytes);
     launch_cnr_dot_image(p, p.Xc, true, p.cnr + 1 * FC, p.stream);             // dot_z
@@ -558,7 +558,7 @@ static int cgnr(Plan& p, const float2* y, int maxit, double eps, const int* fl, 
       Prof pr(p, CGS_K_CG, 4.0 * xbytes + 4.0 * ybytes);
       launch_cnr_dot_samples(p, v, p.cnr + 2 * FC, p.stream, 1);               // dot_v; alpha
       launch_cnr_update_image(p, p.Xc2, p.Xc3, nullptr, 0, p.stream);          // f_hat += a w_hat p
-      launch_cnr_update_samples_dot(p, r, v, p.stream);                        // r -= a v; dot_r
+      launch_cnr_update_samples_dot(p, l == 0 ? y : r, r, v, p.stream);        // r -= a v; dot_r
     }
     CHK(adj_w(r));                                                              // z = A^H W r
     {
