@@ -92,7 +92,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                          "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+                          "(sharding the path over several GPUs is not implemented)"}))
         sys.exit(2)
     import numpy as np
     import torch
