@@ -27,6 +27,24 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return t;   // valid in thread 0
 }
 
+// part[t], part[t + blockDim], ... added in that order by thread t, eight loads
+// issued before they are added (the order, and so the sum, is that of a plain loop)
+__device__ __forceinline__ double strided_sum(const double* part, int nb) {
+  double acc = 0.0;
+  const int st = (int)blockDim.x;
+  int b = threadIdx.x;
+  for (; b + 7 * st < nb; b += 8 * st) {
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = part[b + u * st];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += t[u];
+  }
+  for (; b < nb; b += st) acc += part[b];
+  return acc;
+}
+
+
 // ---------------------------------------------------------------------------
 // x[f][i] = (sum_c conj(S[i][c]) X[f][i][c]) * I[i] + lambda p[f][i]; partial <p, x>
 // CP lanes per pixel (power of two >= C, at most 64).
@@ -334,9 +352,7 @@ __global__ void __launch_bounds__(256) k_cg_xr_alpha(float2* x, float2* r, const
   const int f = blockIdx.y;
   if (fdone[f]) return;
   {
-    double a = 0.0;
-    for (int b = threadIdx.x; b < npq; b += blockDim.x) a += pq_part[(int64_t)f * npq + b];
-    const double pq = block_sum(a, sh);
+    const double pq = block_sum(strided_sum(pq_part + (int64_t)f * npq, npq), sh);
     if (threadIdx.x == 0) {
       const double al = st[f].rn / pq;   // CGLS.hpp:50
       alpha_s = (float)al;
@@ -440,9 +456,7 @@ void launch_sqnorm_partials(const Plan& p, const float2* v, double* partials, hi
 }
 
 __device__ double sum_partials(const double* part, int nb, double* sh) {
-  double acc = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += part[b];
-  return block_sum(acc, sh);
+  return block_sum(strided_sum(part, nb), sh);
 }
 
 // The scalar kernels below run one block per frame, or, for a joint CG over
